@@ -1,0 +1,225 @@
+/*
+ * mfixdem.h -- C ABI of the MI355X (gfx950) DEM particle hot path.
+ *
+ * This library replaces the body of MFiX's DEM time step
+ * (model/des/des_time_march.f:293-460, DES_TIME_STEP) and the argument-less
+ * module routines it calls.  MFiX keeps those routines in Fortran modules
+ * that read and write module globals; the drop-in is a set of Fortran
+ * UDF-override files with the same module and subroutine names whose bodies
+ * call the entry points below through ISO_C_BINDING (see the Fortran shim in
+ * mfix-cfd-dem_amd/fortran/).  Every entry point is plain C: opaque handle,
+ * plain pointers, explicit leading dimensions, int status return.
+ *
+ * Conventions
+ *   - Host particle arrays use the reference layout: Fortran column-major
+ *     (MAX_PIP,3) arrays, i.e. component c of particle p is at [c*ld + p]
+ *     (model/des/discretelement_mod.f:579-583,630-631).  ld >= n.
+ *   - Particle p (0-based) on the host has global id p+1; results handed
+ *     back to the host are always in that order, whatever order the device
+ *     keeps internally.
+ *   - Index arrays exported in reference layout are 1-based
+ *     (NEIGHBOR_INDEX, NEIGHBORS, DG_PIJK, PIJK).
+ *   - Gas arrays use MFiX's serial cell linearisation (J fastest, then I,
+ *     then K; cells 1..IMAX+2 etc. including the ghost layer):
+ *     ijk0 = (j-1) + (i-1)*(JMAX+2) + (k-1)*(JMAX+2)*(IMAX+2)
+ *     (model/include/functions.inc:37-42, dmp_modules/gridmap_mod.f:474-477).
+ *   - The handle owns device memory only; host arrays are never retained.
+ *   - A handle is single-threaded: one host thread, one HIP stream.
+ *   - Every function returns MFIXDEM_OK (0) or a positive error code.
+ */
+#ifndef MFIXDEM_H
+#define MFIXDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFIXDEM_ABI_VERSION 1
+#define MFIXDEM_MAX_PHASES 4
+#define MFIXDEM_MAX_PAIRS_IN (MFIXDEM_MAX_PHASES * (MFIXDEM_MAX_PHASES + 1) / 2)
+/* MFiX's UNDEFINED sentinel (model/param1_mod.f:20). */
+#define MFIXDEM_UNDEFINED 9.87654321e31
+
+/* status codes */
+enum {
+  MFIXDEM_OK = 0,
+  MFIXDEM_ERR_ARG = 1,           /* bad argument / size / state            */
+  MFIXDEM_ERR_HIP = 2,           /* a HIP runtime call failed              */
+  MFIXDEM_ERR_OOM = 3,           /* device allocation failed               */
+  MFIXDEM_ERR_ROGUE = 4,         /* normal particle in a ghost DES cell and
+                                    remove_rogue == 0 (desgrid_mod.f:1074) */
+  MFIXDEM_ERR_EPG = 5,           /* EP_G <= 0 or > 1 in a fluid cell
+                                    (calc_epg_des.f:146-203)               */
+  MFIXDEM_ERR_ZERO_DISTANCE = 6, /* two particle centres coincide
+                                    (calc_force_dem.f:267-271)             */
+  MFIXDEM_ERR_CAPACITY = 7,      /* caller buffer too small                */
+  MFIXDEM_ERR_NOT_READY = 8,     /* mfixdem_init not called / no gas field */
+  MFIXDEM_ERR_UNSUPPORTED = 9    /* option outside this build's scope      */
+};
+
+enum { MFIXDEM_LSD = 0, MFIXDEM_HERTZ = 1 };                 /* DES_COLL_MODEL   */
+enum { MFIXDEM_EULER = 0, MFIXDEM_ADAMS_BASHFORTH = 1 };     /* DES_INTG_METHOD  */
+enum { MFIXDEM_FP64 = 0, MFIXDEM_MIXED = 1 };                /* arithmetic mode  */
+enum { MFIXDEM_DRAG_GIDASPOW = 0, MFIXDEM_DRAG_WEN_YU = 1 }; /* DRAG_TYPE        */
+enum { MFIXDEM_INTERP_GARG_2012 = 0 };                       /* DES_INTERP_SCHEME */
+
+/* debug flags for mfixdem_set_debug */
+enum { MFIXDEM_DEBUG_STORE_FORCES = 1 };
+
+/*
+ * Run parameters.  Names follow the MFiX namelist keywords they stand for
+ * (model/des/des_init_namelist.f, model/init_namelist.f).  Fill with
+ * mfixdem_params_default() first: it sets the reference defaults, including
+ * the default-REAL literals widened to FP64 (FACTOR_RLM = 1.2 at
+ * des_init_namelist.f:1169, KT_FAC / KT_W_FAC = 0.28571428571429 at :1215,1237).
+ * 32-bit fields come first, in an even number, so that every double is
+ * 8-byte aligned without padding.
+ */
+typedef struct mfixdem_params {
+  int32_t abi_version;        /* = MFIXDEM_ABI_VERSION                      */
+  int32_t coll_model;         /* MFIXDEM_LSD | MFIXDEM_HERTZ                */
+  int32_t integrator;         /* MFIXDEM_EULER | MFIXDEM_ADAMS_BASHFORTH    */
+  int32_t precision;          /* MFIXDEM_FP64 | MFIXDEM_MIXED               */
+  int32_t nphases;            /* DES_MMAX, 1..MFIXDEM_MAX_PHASES            */
+  int32_t neighbor_search_n;  /* NEIGHBOR_SEARCH_N (25)                     */
+  int32_t coupled;            /* DES_CONTINUUM_COUPLED with a frozen gas    */
+  int32_t drag_type;          /* MFIXDEM_DRAG_*                             */
+  int32_t interp_scheme;      /* MFIXDEM_INTERP_GARG_2012                   */
+  int32_t model_b;            /* MODEL_B (0 = Model A)                      */
+  int32_t max_pip;            /* particle capacity                          */
+  int32_t imax, jmax, kmax;   /* fluid grid cells (IMAX, JMAX, KMAX)        */
+  int32_t desgridsearch_imax; /* DES grid; 0 = derive (check_geometry.f:148) */
+  int32_t desgridsearch_jmax;
+  int32_t desgridsearch_kmax;
+  int32_t remove_rogue;       /* REMOVE_ROGUE_PARTICLES (1)                 */
+  int32_t reserved0, reserved1;
+  double x_min, y_min, z_min;        /* X_MIN ... (0)                       */
+  double xlength, ylength, zlength;  /* XLENGTH ...                         */
+  double dtsolid;             /* DTSOLID; UNDEFINED = TCOLL/DTSOLID_FACTOR  */
+  double dtsolid_factor;      /* DTSOLID_FACTOR (50)                        */
+  double grav[3];             /* GRAVITY_X/Y/Z (0, -9.80665, 0)             */
+  double factor_rlm;          /* FACTOR_RLM                                 */
+  double kn, kn_w;            /* KN, KN_W (LSD)                             */
+  double kt_fac, kt_w_fac;    /* KT_FAC, KT_W_FAC                           */
+  double des_etat_fac, des_etat_w_fac; /* DES_ETAT_FAC, DES_ETAT_W_FAC (UNDEFINED -> 1/2) */
+  double mew, mew_w;          /* MEW, MEW_W                                 */
+  double des_en_input[MFIXDEM_MAX_PAIRS_IN];  /* upper triangle, M then L>=M */
+  double des_et_input[MFIXDEM_MAX_PAIRS_IN];
+  double des_en_wall_input[MFIXDEM_MAX_PHASES];
+  double des_et_wall_input[MFIXDEM_MAX_PHASES];
+  double e_young[MFIXDEM_MAX_PHASES], v_poisson[MFIXDEM_MAX_PHASES];
+  double ew_young, vw_poisson;
+  double d_p0[MFIXDEM_MAX_PHASES], ro_s0[MFIXDEM_MAX_PHASES];
+  double max_radius;          /* MAX_RADIUS of the particle set             */
+  double ro_g0, mu_g0;        /* gas density and viscosity (frozen field)   */
+  double des_epg_clip;        /* DES_EPG_CLIP (UNDEFINED)                   */
+} mfixdem_params;
+
+/* Quantities derived once at set-up (check_solids_dem.f:519-1122,
+   check_geometry.f:134-188, desgrid_mod.f:506-875). */
+typedef struct mfixdem_derived {
+  double dtsolid, tcoll;
+  double kt, kt_w;
+  double des_etan[MFIXDEM_MAX_PHASES][MFIXDEM_MAX_PHASES];
+  double des_etat[MFIXDEM_MAX_PHASES][MFIXDEM_MAX_PHASES];
+  double hert_kn[MFIXDEM_MAX_PHASES][MFIXDEM_MAX_PHASES];
+  double hert_kt[MFIXDEM_MAX_PHASES][MFIXDEM_MAX_PHASES];
+  double des_etan_wall[MFIXDEM_MAX_PHASES], des_etat_wall[MFIXDEM_MAX_PHASES];
+  double hert_kwn[MFIXDEM_MAX_PHASES], hert_kwt[MFIXDEM_MAX_PHASES];
+  double dg_xstart, dg_ystart, dg_zstart;
+  double dg_dxinv, dg_dyinv, dg_dzinv;
+  int32_t dg_imax, dg_jmax, dg_kmax;   /* interior DES cells per axis */
+  int32_t n_facets;                    /* wall triangles (6 for a box) */
+} mfixdem_derived;
+
+typedef struct mfixdem_status {
+  int64_t steps;              /* DES_TIME_STEP calls since mfixdem_init     */
+  int64_t rebuilds;           /* neighbour-list builds                      */
+  int64_t n_particles;        /* existing particles                          */
+  int64_t n_pairs;            /* half-list pairs (reference NEIGHBORS size) */
+  int64_t n_contacts;         /* pairs in contact at the last step          */
+  int64_t rogue_deleted;      /* particles removed as rogue (cumulative)    */
+  int64_t epg_errors;         /* fluid cells with EP_G out of (0,1]         */
+  int64_t zero_distance;      /* coincident contact pairs seen              */
+  int64_t out_of_domain;      /* particles deleted by PARTICLES_IN_CELL     */
+} mfixdem_status;
+
+typedef struct mfixdem_handle mfixdem_handle;
+
+/* --- lifetime ----------------------------------------------------------- */
+int32_t mfixdem_abi_version(void);
+const char* mfixdem_error_string(int32_t code);
+/* Reference defaults (des_init_namelist.f, init_namelist.f, constant_mod.f). */
+int32_t mfixdem_params_default(mfixdem_params* p);
+/* Allocate device state for p->max_pip particles and derive the collision
+   parameters, DES grid and wall facets.  Replaces the set-up done by
+   CHECK_SOLIDS_DEM (check_solids_dem.f:519-1122), DESGRID_INIT
+   (desgrid_mod.f:506) and DES_STL_PREPROCESSING (stl_preproc_des_mod.f:80). */
+int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out);
+int32_t mfixdem_destroy(mfixdem_handle* h);
+int32_t mfixdem_get_derived(const mfixdem_handle* h, mfixdem_derived* out);
+
+/* --- state in / out (the only host<->device traffic) --------------------- */
+/* Upload n particles (reference SoA layout, leading dimension ld).  phase is
+   1-based PIJK(:,5).  Per-particle PVOL/PMASS/OMOI follow
+   make_arrays_des.f:288-291.  All particles start NORMAL_PARTICLE. */
+int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos,
+                              const double* vel, const double* omega,
+                              const double* radius, const double* ro_sol,
+                              const int32_t* phase, int64_t ld);
+/* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal). */
+int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
+                              double* omega, int8_t* state, int64_t ld);
+/* Frozen gas field: staggered U_G/V_G/W_G and P_FORCE(3,IJK) (= -grad p,
+   calc_pg_grad.f:20), MFiX cell layout described above. */
+int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
+                        const double* w_g, const double* p_force);
+/* EP_G and ROP_S (solids phase summed) after the last deposition. */
+int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s);
+
+/* --- the time step -------------------------------------------------------- */
+/* The set-up tail of MAKE_ARRAYS_DES (make_arrays_des.f:343-432): DESGRID_PIC,
+   PARTICLES_IN_CELL, NEIGHBOUR, CALC_INTERP_WEIGHTS, COMP_MEAN_FIELDS. */
+int32_t mfixdem_init(mfixdem_handle* h);
+/* One DES_TIME_STEP(NN) body (des_time_march.f:293-460) for the configured
+   path: CALC_FORCE_DEM, CALC_DRAG_DES, CFNEWVALUES, the rebuild when
+   NN==1 or MOD(NN,NEIGHBOR_SEARCH_N)==0, then PARTICLES_IN_CELL and
+   COMP_MEAN_FIELDS when coupled.  Runs the fused kernels. */
+int32_t mfixdem_step(mfixdem_handle* h, int32_t nn);
+/* nsteps consecutive DES_TIME_STEP calls, NN = nn_first .. nn_first+nsteps-1;
+   the runs of steps between rebuilds are replayed from a captured hipGraph. */
+int32_t mfixdem_run(mfixdem_handle* h, int32_t nn_first, int32_t nsteps);
+int32_t mfixdem_synchronize(mfixdem_handle* h);
+
+/* --- finer entry points (one per reference routine, unfused kernels) ----- */
+int32_t mfixdem_calc_force_dem(mfixdem_handle* h);   /* calc_force_dem.f:18  */
+int32_t mfixdem_calc_drag_des(mfixdem_handle* h);    /* calc_drag_des.f:13   */
+int32_t mfixdem_cfnewvalues(mfixdem_handle* h);      /* cfnewvalues.f:641    */
+int32_t mfixdem_desgrid_pic(mfixdem_handle* h);      /* desgrid_mod.f:877    */
+int32_t mfixdem_neighbour(mfixdem_handle* h);        /* neighbour.f:21       */
+int32_t mfixdem_particles_in_cell(mfixdem_handle* h);/* particles_in_cell.f:20 */
+int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
+
+/* --- inspection for parity and restart ------------------------------------ */
+int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags);
+/* FC and TOW as they stood just before the integrator of the last step
+   (requires MFIXDEM_DEBUG_STORE_FORCES, or the unfused entry points). */
+int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t ld);
+/* DG_PIJK (1-based DES cell) and PIJK(:,1:4) (fluid I,J,K,IJK; 1-based). */
+int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk,
+                          int64_t ld);
+/* Neighbour half-list in reference layout (neighbour.f, calc_force_dem.f:203):
+   neighbor_index[p] = 1-based exclusive end of row p (rows in global-id order),
+   neighbors[0..npairs) 1-based global ids, pft[3*c + d] = PFT_NEIGHBOR(d,c).
+   Pass capacity 0 to query *npairs only. */
+int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index,
+                          int32_t* neighbors, double* pft, int64_t capacity,
+                          int64_t* npairs);
+int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFIXDEM_H */

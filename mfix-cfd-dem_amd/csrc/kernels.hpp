@@ -1,0 +1,910 @@
+// kernels.hpp -- gfx950 kernels of the DEM time step.
+//
+// Layout in HBM (all per-particle arrays in the current Morton order of the
+// neighbour-search grid; component c of particle i at [c*stride + i]):
+//   pos/vel/omega      double, double-buffered (read one, write the other)
+//   radius/pmass/omoi/pvol, gid, state, phase, dgc (DES cell), fcell (fluid
+//   cell), wall mask + per-facet wall history, AB2 accelerations
+//   neighbour rows     full rows: [incoming (owner has lower id) | own]
+//                      nbr_start[i] .. nbr_split[i] .. nbr_start[i+1]
+//                      nbr = neighbour slot, nbr_pid = pair id (history slot)
+//   pair history       pft[pid*3 + d], double-buffered per step
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device_physics.hpp"
+
+namespace mfixdem {
+
+constexpr int kBS = 256;
+
+struct StepPtrs {
+  const double *pos, *vel, *omega;
+  double *pos_o, *vel_o, *omega_o;
+  const double *radius, *pmass, *omoi, *pvol;
+  const int* gid;
+  uint8_t* state;
+  const uint8_t* phase;
+  const int* dgc;
+  const uint16_t* cell_facets;
+  uint8_t* wmask;
+  double* wpft;                 // [(f*3 + d)*stride + i]
+  const int *nbr_start, *nbr_split, *nbr, *nbr_pid;
+  const double* pft_in;
+  double* pft_out;
+  double *fc, *tow;
+  double *acc_old, *rot_acc_old;
+  int* fcell;
+  const double *xe, *yn, *zt;
+  const double *vnode, *p_force, *ep_g;
+  const uint8_t* fluid;
+  int* flags;
+};
+
+enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
+       kFlagOutOfDomain, kFlagContacts, kFlagCount };
+
+enum { kModeFused = 0, kModeForce = 1, kModeDrag = 2, kModeIntegrate = 3 };
+
+__device__ __forceinline__ void load3(const double* a, int S, int i, double v[3]) {
+  v[0] = a[i];
+  v[1] = a[S + i];
+  v[2] = a[2 * S + i];
+}
+__device__ __forceinline__ void store3(double* a, int S, int i, const double v[3]) {
+  a[i] = v[0];
+  a[S + i] = v[1];
+  a[2 * S + i] = v[2];
+}
+
+// GARG_2012 drag on one particle (DRAG_GS_DES0 :140-212 with
+// DRAG_INTERPOLATION :507-566); adds D_FORCE and P_FORCE*PVOL to fc.
+template <bool MIXED>
+__device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, int i,
+                                         const double pos[3], const double vel[3], double fc[3]) {
+  const int c = a.fcell[i];
+  if (c < 0 || !a.fluid[c]) return;
+  const int cj = c % P.nj + 1;
+  const int t = c / P.nj;
+  const int ci = t % P.ni + 1;
+  const int ck = t / P.ni + 1;
+  const double x0 = a.xe[ci - 1], y0 = a.yn[cj - 1], z0 = a.zt[ck - 1];
+  const double dxx = a.xe[ci] - x0, dyy = a.yn[cj] - y0, dzz = a.zt[ck] - z0;
+  double zx = pos[0] - x0, zy = pos[1] - y0;
+  zx = zx / dxx;
+  zy = zy / dyy;
+  double zz = pos[2] - z0;
+  zz = zz / dzz;
+  const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  double velfp[3] = {0.0, 0.0, 0.0};
+  const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;  // node (ci-1,cj-1,ck-1)
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const double w = xv[ii] * yv[jj] * zv[kk];
+        const double* vn = a.vnode + 3 * (base + jj + ii * P.nj + kk * P.nj * P.ni);
+        for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + vn[d] * w;
+      }
+  const double pvol = a.pvol[i];
+  if (!MIXED) {
+    const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+    const double vrel = sqrt(dot3(vs, vs));
+    const double fgp = drag_fgp_fp64(P.drag_type, a.ep_g[c], P.mu_g0, P.ro_g0, vrel,
+                                     2.0 * a.radius[i], pvol);
+    for (int d = 0; d < 3; d++) fc[d] = fc[d] + fgp * (velfp[d] - vel[d]);
+  } else {
+    const float vs[3] = {(float)(velfp[0] - vel[0]), (float)(velfp[1] - vel[1]),
+                         (float)(velfp[2] - vel[2])};
+    const float vrel = fsqrt(dot3f(vs, vs));
+    const float fgp = drag_fgp_fp32(P.drag_type, (float)a.ep_g[c], (float)P.mu_g0,
+                                    (float)P.ro_g0, vrel, (float)(2.0 * a.radius[i]), (float)pvol);
+    for (int d = 0; d < 3; d++) fc[d] = fc[d] + (double)(fgp * vs[d]);
+  }
+  for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
+}
+
+// One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
+// unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES).
+template <bool MIXED, int MODE>
+__global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= P.n) return;
+  const int S = P.stride;
+  const int st = a.state[i];
+  if (st == 0) {
+    if (MODE == kModeFused || MODE == kModeIntegrate) {
+      for (int c = 0; c < 3; c++) {
+        a.pos_o[c * S + i] = a.pos[c * S + i];
+        a.vel_o[c * S + i] = a.vel[c * S + i];
+        a.omega_o[c * S + i] = a.omega[c * S + i];
+      }
+    }
+    if (MODE == kModeFused || MODE == kModeForce)  // rows of a removed particle keep their history
+      for (int e = a.nbr_split[i]; e < a.nbr_start[i + 1]; e++) {
+        const int pid = a.nbr_pid[e];
+        for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+      }
+    return;
+  }
+  double pos[3], vel[3], om[3];
+  load3(a.pos, S, i, pos);
+  load3(a.vel, S, i, vel);
+  load3(a.omega, S, i, om);
+  const double ri = a.radius[i], mi = a.pmass[i];
+  const int phi = a.phase[i];
+  double fc[3] = {0.0, 0.0, 0.0}, tow[3] = {0.0, 0.0, 0.0};
+
+  if (MODE == kModeFused || MODE == kModeForce) {
+    // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
+    const uint16_t fm = a.cell_facets[a.dgc[i]];
+    uint8_t wm = a.wmask[i];
+    const uint8_t wm0 = wm;
+    if (fm == 0) {
+      wm = 0;
+    } else {
+      for (int f = 0; f < P.nf; f++) {
+        if (!((fm >> f) & 1)) continue;
+        double hist[3];
+        const bool set = (wm >> f) & 1;
+        if (set)
+          for (int d = 0; d < 3; d++) hist[d] = a.wpft[(f * 3 + d) * S + i];
+        const int r = wall_contact<MIXED>(P, P.facets[f], pos, vel, om, ri, mi, phi, hist, set,
+                                          fc, tow);
+        if (r == 0) {
+          wm &= (uint8_t)~(1u << f);
+        } else if (r == 1) {
+          wm |= (uint8_t)(1u << f);
+          for (int d = 0; d < 3; d++) a.wpft[(f * 3 + d) * S + i] = hist[d];
+        }
+      }
+    }
+    if (wm != wm0) a.wmask[i] = wm;
+
+    // ---- pairs (calc_force_dem.f:180-412): incoming rows first (this
+    // particle is I of an owner with a lower id, ascending owner id), then
+    // own rows in the reference's stencil order.
+    const int e0 = a.nbr_start[i], es = a.nbr_split[i], e1 = a.nbr_start[i + 1];
+    for (int e = e0; e < e1; e++) {
+      const int j = a.nbr[e];
+      const bool owner = e >= es;
+      const int pid = a.nbr_pid[e];
+      if (a.state[j] == 0) {  // IF(IS_NONEXISTENT(I)) CYCLE: history kept
+        if (owner)
+          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+        continue;
+      }
+      double xj[3];
+      load3(a.pos, S, j, xj);
+      const double rj = a.radius[j];
+      // contact test (orientation independent: |x_I - x_L|^2 and r_L + r_I)
+      const double r_lm = owner ? ri + rj : rj + ri;
+      const double d0 = owner ? xj[0] - pos[0] : pos[0] - xj[0];
+      const double d1 = owner ? xj[1] - pos[1] : pos[1] - xj[1];
+      const double d2 = owner ? xj[2] - pos[2] : pos[2] - xj[2];
+      const double dm = d0 * d0 + d1 * d1 + d2 * d2;
+      const double rs = r_lm - kSmall;
+      if (dm > rs * rs) {
+        if (owner) {
+          a.pft_out[3 * pid] = 0.0;
+          a.pft_out[3 * pid + 1] = 0.0;
+          a.pft_out[3 * pid + 2] = 0.0;
+        }
+        continue;
+      }
+      PairIn in;
+      double vj[3], wj[3];
+      load3(a.vel, S, j, vj);
+      load3(a.omega, S, j, wj);
+      const double mj = a.pmass[j];
+      const int phj = a.phase[j];
+      if (owner) {
+        for (int c = 0; c < 3; c++) {
+          in.xL[c] = pos[c]; in.xI[c] = xj[c];
+          in.vL[c] = vel[c]; in.vI[c] = vj[c];
+          in.wL[c] = om[c];  in.wI[c] = wj[c];
+        }
+        in.rL = ri; in.rI = rj; in.mL = mi; in.mI = mj; in.phL = phi; in.phI = phj;
+      } else {
+        for (int c = 0; c < 3; c++) {
+          in.xL[c] = xj[c];  in.xI[c] = pos[c];
+          in.vL[c] = vj[c];  in.vI[c] = vel[c];
+          in.wL[c] = wj[c];  in.wI[c] = om[c];
+        }
+        in.rL = rj; in.rI = ri; in.mL = mj; in.mI = mi; in.phL = phj; in.phI = phi;
+      }
+      const double hin[3] = {a.pft_in[3 * pid], a.pft_in[3 * pid + 1], a.pft_in[3 * pid + 2]};
+      PairOut o;
+      const int r = MIXED ? pair_force_mixed(P, in, hin, o) : pair_force_fp64(P, in, hin, o);
+      if (r == 2) {
+        atomicAdd(&a.flags[kFlagZeroDist], 1);
+        if (owner)
+          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = hin[c];
+        continue;
+      }
+      if (owner) {
+        for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = o.pft[c];
+        for (int c = 0; c < 3; c++) fc[c] = fc[c] + o.fct[c];
+        for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.tl[c];
+        if (P.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
+      } else {
+        for (int c = 0; c < 3; c++) fc[c] = fc[c] - o.fct[c];
+        for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.ti[c];
+      }
+    }
+    if (MODE == kModeForce) {
+      store3(a.fc, S, i, fc);
+      store3(a.tow, S, i, tow);
+      return;
+    }
+  }
+  if (MODE == kModeDrag) {
+    load3(a.fc, S, i, fc);
+    if (P.coupled) add_drag<MIXED>(P, a, i, pos, vel, fc);
+    store3(a.fc, S, i, fc);
+    return;
+  }
+  if (MODE == kModeFused && P.coupled) add_drag<MIXED>(P, a, i, pos, vel, fc);
+  if (MODE == kModeIntegrate) {
+    load3(a.fc, S, i, fc);
+    load3(a.tow, S, i, tow);
+  }
+  if (MODE == kModeFused && P.store_forces) {
+    store3(a.fc, S, i, fc);
+    store3(a.tow, S, i, tow);
+  }
+
+  // ---- CFNEWVALUES (cfnewvalues.f:700-811)
+  const double dt = P.dt;
+  const double omoi = a.omoi[i];
+  if (!P.ab2) {
+    for (int c = 0; c < 3; c++) {
+      vel[c] = vel[c] + dt * (fc[c] / mi + P.grav[c]);
+      pos[c] = pos[c] + vel[c] * dt;
+    }
+    for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
+  } else {
+    double acc[3], racc[3];
+    if (P.first_pass) {
+      for (int c = 0; c < 3; c++) acc[c] = fc[c] / mi + P.grav[c];
+      for (int c = 0; c < 3; c++) racc[c] = tow[c];
+    } else {
+      load3(a.acc_old, S, i, acc);
+      load3(a.rot_acc_old, S, i, racc);
+    }
+    for (int c = 0; c < 3; c++) {
+      const double f = fc[c] / mi + P.grav[c];
+      const double vn = vel[c] + 0.5 * (3.0 * f - acc[c]) * dt;
+      acc[c] = f;
+      pos[c] = pos[c] + 0.5 * (vel[c] + vn) * dt;
+      vel[c] = vn;
+    }
+    for (int c = 0; c < 3; c++) {
+      om[c] = om[c] + 0.5 * (3.0 * tow[c] * omoi - racc[c]) * dt;
+      racc[c] = tow[c] * omoi;
+    }
+    store3(a.acc_old, S, i, acc);
+    store3(a.rot_acc_old, S, i, racc);
+  }
+  store3(a.pos_o, S, i, pos);
+  store3(a.vel_o, S, i, vel);
+  store3(a.omega_o, S, i, om);
+
+  // ---- PARTICLES_IN_CELL locate, fused (particles_in_cell.f:87-154)
+  if (MODE == kModeFused && P.coupled) {
+    const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
+    const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
+    const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
+    a.fcell[i] = (I && J && K) ? (J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni : -1;
+  }
+}
+
+// ---------------------------------------------------------------- scans
+__device__ __forceinline__ int block_excl_scan(int v, int* total, int* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < kBS / 64; w++) {
+      const int t = lds[w];
+      lds[w] = s;
+      s += t;
+    }
+    lds[kBS / 64] = s;
+  }
+  __syncthreads();
+  const int res = x - v + lds[wid];
+  *total = lds[kBS / 64];
+  __syncthreads();
+  return res;
+}
+
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kBS * kScanPer;
+
+// exclusive scan of in[0..n) into out[0..n], out[n] = total (3 launches)
+__global__ __launch_bounds__(kBS) void k_scan_tiles(const int* in, int* out, int n, int* bsum) {
+  __shared__ int lds[kBS / 64 + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  int v[kScanPer], s = 0;
+  for (int q = 0; q < kScanPer; q++) {
+    v[q] = (base + q < n) ? in[base + q] : 0;
+    s += v[q];
+  }
+  int tot;
+  int pre = block_excl_scan(s, &tot, lds);
+  for (int q = 0; q < kScanPer; q++) {
+    if (base + q < n) out[base + q] = pre;
+    pre += v[q];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_sums(int* bsum, int nb) {
+  __shared__ int lds[kBS / 64 + 1];
+  const int per = (nb + kBS - 1) / kBS;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) s += bsum[b0 + q];
+  int tot;
+  int pre = block_excl_scan(s, &tot, lds);
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) {
+      const int t = bsum[b0 + q];
+      bsum[b0 + q] = pre;
+      pre += t;
+    }
+  if (threadIdx.x == 0) bsum[nb] = tot;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_add(int* out, int n, const int* bsum, int nb) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) out[i] += bsum[i / kScanTile];
+  if (i == 0) out[n] = bsum[nb];
+}
+
+__global__ void k_fill_i32(int* a, int n, int v) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_f64(double* a, int n, double v) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ------------------------------------------------------------- rebuild
+struct GridDev {
+  // reference DES grid (ghost layer included): cells 1..dg_n*
+  int dg_ni, dg_nj, dg_nk;
+  int coarse[3];
+  double dg_lo[3], dg_inv[3];
+  double factor_rlm;
+  // neighbour-search grid
+  int fn[3];
+  double f_lo[3], f_inv;
+  int gid_bits;
+};
+
+__device__ __forceinline__ int floor_sat_d(double v) {
+  if (!(v > -1.0e9)) return -1000000000;
+  if (v > 1.0e9) return 1000000000;
+  return (int)floor(v);
+}
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+__device__ __forceinline__ void dg_decode(const GridDev& G, int c, int& i, int& j, int& k) {
+  j = c % G.dg_nj + 1;
+  const int t = c / G.dg_nj;
+  i = t % G.dg_ni + 1;
+  k = t / G.dg_ni + 1;
+}
+__device__ __forceinline__ bool dg_ghost(const GridDev& G, int i, int j, int k) {
+  return i < 2 || i > G.dg_ni - 1 || j < 2 || j > G.dg_nj - 1 || k < 2 || k > G.dg_nk - 1;
+}
+// coarse half-stencil offsets (desgrid_mod.f:1111-1152): lo/hi per axis
+__device__ __forceinline__ void stencil_off(const GridDev& G, const double x[3], int lo[3],
+                                            int hi[3]) {
+  for (int d = 0; d < 3; d++) {
+    lo[d] = 1;
+    hi[d] = 1;
+    if (G.coarse[d]) {
+      const double t = (x[d] - G.dg_lo[d]) * G.dg_inv[d];
+      if (t - floor(t) >= 0.5) lo[d] = 0;
+      else hi[d] = 0;
+    }
+  }
+}
+
+// DESGRID_PIC locate (desgrid_mod.f:900-915) + sort key on the search grid
+__global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G, const double* pos,
+                                             const uint8_t* state, const int* gid, int* dgc,
+                                             uint64_t* keys, int* vals, int* flags) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  vals[i] = i;
+  if (state[i] == 0) {
+    keys[i] = ~0ull;
+    dgc[i] = 0;
+    return;
+  }
+  const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+  const int nn[3] = {G.dg_ni, G.dg_nj, G.dg_nk};
+  int c3[3], f3[3];
+  for (int d = 0; d < 3; d++) {
+    int v = floor_sat_d((x[d] - G.dg_lo[d]) * G.dg_inv[d]) + 2;
+    c3[d] = v < 1 ? 1 : (v > nn[d] ? nn[d] : v);
+    int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
+    f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
+  }
+  dgc[i] = (c3[1] - 1) + (c3[0] - 1) * G.dg_nj + (c3[2] - 1) * G.dg_nj * G.dg_ni;
+  if (dg_ghost(G, c3[0], c3[1], c3[2])) {
+    atomicMin(&flags[kFlagRogueGid], gid[i]);
+    atomicAdd(&flags[kFlagRogueCount], 1);
+  }
+  const uint64_t m = (uint64_t)(spread10(f3[0]) | (spread10(f3[1]) << 1) | (spread10(f3[2]) << 2));
+  keys[i] = (m << G.gid_bits) | (uint64_t)gid[i];
+}
+
+struct PermPtrs {
+  const int* perm;
+  const double *pos, *vel, *omega;
+  double *pos_o, *vel_o, *omega_o;
+  const double *radius, *pmass, *omoi, *pvol, *acc, *racc;
+  double *radius_o, *pmass_o, *omoi_o, *pvol_o, *acc_o, *racc_o;
+  const int *gid, *fcell, *dgc;
+  int *gid_o, *fcell_o, *dgc_o;
+  const uint8_t *state, *phase, *wmask;
+  uint8_t *state_o, *phase_o, *wmask_o;
+  const double* wpft;
+  double* wpft_o;
+  int nf, ab2;
+};
+
+__global__ __launch_bounds__(kBS) void k_permute(int n, int S, PermPtrs p) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int o = p.perm[i];
+  for (int c = 0; c < 3; c++) {
+    p.pos_o[c * S + i] = p.pos[c * S + o];
+    p.vel_o[c * S + i] = p.vel[c * S + o];
+    p.omega_o[c * S + i] = p.omega[c * S + o];
+  }
+  p.radius_o[i] = p.radius[o];
+  p.pmass_o[i] = p.pmass[o];
+  p.omoi_o[i] = p.omoi[o];
+  p.pvol_o[i] = p.pvol[o];
+  p.gid_o[i] = p.gid[o];
+  p.fcell_o[i] = p.fcell[o];
+  p.dgc_o[i] = p.dgc[o];
+  p.state_o[i] = p.state[o];
+  p.phase_o[i] = p.phase[o];
+  const uint8_t wm = p.wmask[o];
+  p.wmask_o[i] = wm;
+  for (int f = 0; f < p.nf; f++)
+    if ((wm >> f) & 1)
+      for (int d = 0; d < 3; d++) p.wpft_o[(f * 3 + d) * S + i] = p.wpft[(f * 3 + d) * S + o];
+  if (p.ab2)
+    for (int c = 0; c < 3; c++) {
+      p.acc_o[c * S + i] = p.acc[c * S + o];
+      p.racc_o[c * S + i] = p.racc[c * S + o];
+    }
+}
+
+__device__ __forceinline__ uint32_t compact10(uint64_t v) {
+  v &= 0x09249249;
+  v = (v | (v >> 2)) & 0x030C30C3;
+  v = (v | (v >> 4)) & 0x0300F00F;
+  v = (v | (v >> 8)) & 0x030000FF;
+  v = (v | (v >> 16)) & 0x3ff;
+  return (uint32_t)v;
+}
+
+// search-grid cell ranges of the sorted particles (from the Morton part of
+// the sort key)
+__global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G, const uint64_t* keys,
+                                                     const uint8_t* state, int* cstart,
+                                                     int* cend) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n || state[i] == 0) return;
+  const uint64_t m = keys[i] >> G.gid_bits;
+  const bool first = (i == 0) || (keys[i - 1] >> G.gid_bits) != m;
+  const bool last = (i + 1 == n) || (keys[i + 1] >> G.gid_bits) != m;
+  const int c = (int)compact10(m) + G.fn[0] * ((int)compact10(m >> 1) + G.fn[1] * (int)compact10(m >> 2));
+  if (first) cstart[c] = i;
+  if (last) cend[c] = i + 1;
+}
+
+struct BuildPtrs {
+  const double *pos, *radius;
+  const int *gid, *dgc;
+  const uint8_t* state;
+  const int *fstart, *fend;     // search-grid cell ranges
+  int* own_cnt;
+  int* own_tmp;                 // [i*maxrow + k]
+  uint64_t* own_key;
+  int maxrow;
+  int* flags;
+};
+
+// Own rows of DESGRID_NEIGH_BUILD (desgrid_mod.f:1057-1214): j in the
+// reference stencil of i, j after i (ids), within FACTOR_RLM*(ri+rj).  The
+// search walks the 27 search-grid cells (cell >= the largest cut-off), the
+// reference stencil is applied as a filter, and the row is put in the
+// reference's order (stencil lk,lj,li then ascending id).
+__global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, BuildPtrs b) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  if (b.state[i] != 1) {
+    b.own_cnt[i] = 0;
+    return;
+  }
+  const int gi = b.gid[i];
+  int ci, cj, ck;
+  dg_decode(G, b.dgc[i], ci, cj, ck);
+  const int g_r = b.flags[kFlagRogueGid];
+  if (dg_ghost(G, ci, cj, ck) || gi >= g_r) {
+    b.own_cnt[i] = 0;
+    return;
+  }
+  const double x[3] = {b.pos[i], b.pos[S + i], b.pos[2 * S + i]};
+  const double r = b.radius[i];
+  int lo[3], hi[3];
+  stencil_off(G, x, lo, hi);
+  int f3[3];
+  for (int d = 0; d < 3; d++) {
+    const int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
+    f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
+  }
+  int cnt = 0;
+  int* row = b.own_tmp + (size_t)i * b.maxrow;
+  uint64_t* key = b.own_key + (size_t)i * b.maxrow;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int fz = f3[2] + dz;
+    if (fz < 0 || fz >= G.fn[2]) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int fy = f3[1] + dy;
+      if (fy < 0 || fy >= G.fn[1]) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        const int fx = f3[0] + dx;
+        if (fx < 0 || fx >= G.fn[0]) continue;
+        const int fc = fx + G.fn[0] * (fy + G.fn[1] * fz);
+        const int t1 = b.fend[fc];
+        for (int t = b.fstart[fc]; t < t1; t++) {
+          if (t == i || b.state[t] == 0) continue;
+          const int gj = b.gid[t];
+          if (gj < gi) continue;
+          const double sr = G.factor_rlm * (r + b.radius[t]);
+          const double d0 = x[0] - b.pos[t], d1 = x[1] - b.pos[S + t], d2 = x[2] - b.pos[2 * S + t];
+          const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+          if (ds > sr * sr) continue;
+          int li, lj, lk;
+          dg_decode(G, b.dgc[t], li, lj, lk);
+          if (li < ci - lo[0] || li > ci + hi[0] || lj < cj - lo[1] || lj > cj + hi[1] ||
+              lk < ck - lo[2] || lk > ck + hi[2])
+            continue;
+          if (cnt < b.maxrow) {
+            const uint64_t sk = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
+            int q = cnt;
+            while (q > 0 && key[q - 1] > sk) {
+              key[q] = key[q - 1];
+              row[q] = row[q - 1];
+              q--;
+            }
+            key[q] = sk;
+            row[q] = t;
+          }
+          cnt++;
+        }
+      }
+    }
+  }
+  if (cnt > b.maxrow) atomicMax(&b.flags[kFlagOwnOverflow], cnt);
+  b.own_cnt[i] = cnt;
+}
+
+__global__ __launch_bounds__(kBS) void k_count_in(int n, const int* own_cnt, const int* own_tmp,
+                                                  int maxrow, int* in_cnt) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int c = own_cnt[i];
+  for (int k = 0; k < c; k++) atomicAdd(&in_cnt[own_tmp[(size_t)i * maxrow + k]], 1);
+}
+
+__global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const int* b, int* o) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) o[i] = a[i] + b[i];
+}
+
+// place own entries and scatter the transposed (incoming) entries
+__global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, const int* own_tmp,
+                                               int maxrow, const int* pair_base,
+                                               const int* nbr_start, const int* in_cnt,
+                                               int* nbr_split, int* in_cur, int* nbr,
+                                               int* nbr_pid) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int split = nbr_start[i] + in_cnt[i];
+  nbr_split[i] = split;
+  const int c = own_cnt[i];
+  const int pb = pair_base[i];
+  for (int k = 0; k < c; k++) {
+    const int j = own_tmp[(size_t)i * maxrow + k];
+    nbr[split + k] = j;
+    nbr_pid[split + k] = pb + k;
+    const int slot = atomicAdd(&in_cur[j], 1);
+    nbr[nbr_start[j] + slot] = i;
+    nbr_pid[nbr_start[j] + slot] = pb + k;
+  }
+}
+
+// incoming entries in ascending owner id
+__global__ __launch_bounds__(kBS) void k_sort_in(int n, const int* gid, const int* nbr_start,
+                                                 const int* nbr_split, int* nbr, int* nbr_pid) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int s = nbr_start[i], e = nbr_split[i];
+  for (int a = s + 1; a < e; a++) {
+    const int j = nbr[a], pj = nbr_pid[a], gj = gid[j];
+    int q = a;
+    while (q > s && gid[nbr[q - 1]] > gj) {
+      nbr[q] = nbr[q - 1];
+      nbr_pid[q] = nbr_pid[q - 1];
+      q--;
+    }
+    nbr[q] = j;
+    nbr_pid[q] = pj;
+  }
+}
+
+// history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
+struct CarryPtrs {
+  const int* perm;              // new slot -> old slot
+  const int* gid;               // new order
+  const int *nbr_start, *nbr_split, *nbr, *pair_base;
+  const int* old_gid;           // old order
+  const int *old_start, *old_split, *old_nbr, *old_pair_base;
+  const double* pft_old;
+  double* pft_new;
+};
+__global__ __launch_bounds__(kBS) void k_carry(int n, CarryPtrs c) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int s = c.nbr_split[i], e = c.nbr_start[i + 1];
+  if (s == e) return;
+  const int o = c.perm[i];
+  const int os = c.old_split[o], oe = c.old_start[o + 1];
+  const int pb = c.pair_base[i], opb = c.old_pair_base[o];
+  for (int q = s; q < e; q++) {
+    const int gj = c.gid[c.nbr[q]];
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    for (int r = os; r < oe; r++) {
+      if (c.old_gid[c.old_nbr[r]] == gj) {
+        const int op = opb + (r - os);
+        h0 = c.pft_old[3 * op];
+        h1 = c.pft_old[3 * op + 1];
+        h2 = c.pft_old[3 * op + 2];
+        break;
+      }
+    }
+    const int p = pb + (q - s);
+    c.pft_new[3 * p] = h0;
+    c.pft_new[3 * p + 1] = h1;
+    c.pft_new[3 * p + 2] = h2;
+  }
+}
+
+// DELETE_PARTICLE of rogue particles (mass_outflow_dem.f:201-280)
+__global__ __launch_bounds__(kBS) void k_delete_rogue(int n, int S, GridDev G, const int* dgc,
+                                                      uint8_t* state, uint8_t* wmask, double* pos,
+                                                      double* vel, double* omega, double* radius) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n || state[i] != 1) return;
+  int ci, cj, ck;
+  dg_decode(G, dgc[i], ci, cj, ck);
+  if (!dg_ghost(G, ci, cj, ck)) return;
+  state[i] = 0;
+  wmask[i] = 0;
+  for (int c = 0; c < 3; c++) {
+    pos[c * S + i] = 0.0;
+    vel[c * S + i] = 0.0;
+    omega[c * S + i] = 0.0;
+  }
+  radius[i] = 0.0;
+}
+
+// ---------------------------------------------------------- gas coupling
+__global__ __launch_bounds__(kBS) void k_locate(int n, int S, int ni, int nj, int nk,
+                                                const double* xe, const double* yn,
+                                                const double* zt, const double* pos,
+                                                const uint8_t* state, int* fcell) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  if (state[i] == 0) {
+    fcell[i] = -1;
+    return;
+  }
+  const int I = locate_cell(xe, ni, 1.0 / (xe[1] - xe[0]), pos[i]);
+  const int J = locate_cell(yn, nj, 1.0 / (yn[1] - yn[0]), pos[S + i]);
+  const int K = locate_cell(zt, nk, 1.0 / (zt[1] - zt[0]), pos[2 * S + i]);
+  fcell[i] = (I && J && K) ? (J - 1) + (I - 1) * nj + (K - 1) * nj * ni : -1;
+}
+
+// PIC counts; deletes particles outside the fluid grid (particles_in_cell.f:175-184)
+__global__ __launch_bounds__(kBS) void k_pic_count(int n, int S, const int* fcell,
+                                                   uint8_t* state, uint8_t* wmask, double* pos,
+                                                   double* vel, double* omega, double* radius,
+                                                   int* cnt, int* flags) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n || state[i] == 0) return;
+  const int c = fcell[i];
+  if (c < 0) {
+    state[i] = 0;
+    wmask[i] = 0;
+    for (int d = 0; d < 3; d++) {
+      pos[d * S + i] = 0.0;
+      vel[d * S + i] = 0.0;
+      omega[d * S + i] = 0.0;
+    }
+    radius[i] = 0.0;
+    atomicAdd(&flags[kFlagOutOfDomain], 1);
+    return;
+  }
+  atomicAdd(&cnt[c], 1);
+}
+
+__global__ __launch_bounds__(kBS) void k_pic_scatter(int n, const int* fcell, const uint8_t* state,
+                                                     const int* off, int* cur, int* pic) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n || state[i] == 0) return;
+  const int c = fcell[i];
+  const int s = atomicAdd(&cur[c], 1);
+  pic[off[c] + s] = i;
+}
+
+// PIC(IJK)%p in ascending particle id (serial fill order of the reference)
+__global__ __launch_bounds__(kBS) void k_pic_sort(int ncell, const int* off, const int* gid,
+                                                  int* pic) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  if (c >= ncell) return;
+  const int s = off[c], e = off[c + 1];
+  for (int a = s + 1; a < e; a++) {
+    const int p = pic[a], g = gid[p];
+    int q = a;
+    while (q > s && gid[pic[q - 1]] > g) {
+      pic[q] = pic[q - 1];
+      q--;
+    }
+    pic[q] = p;
+  }
+}
+
+// COMP_MEAN_FIELDS0 particle -> node deposition (comp_mean_fields0.f:167-210),
+// gathered per node in the reference's summation order: the 8 cells sharing
+// the node in ascending IJK (K, then I, then J), particles in ascending id.
+__global__ __launch_bounds__(kBS) void k_deposit_nodes(int imax, int jmax, int kmax, int ni,
+                                                       int nj, int nph, int S,
+                                                       const double* xe, const double* yn,
+                                                       const double* zt, const uint8_t* fluid,
+                                                       const int* off, const int* pic,
+                                                       const double* pos, const double* pmass,
+                                                       const uint8_t* phase, double* rops_node,
+                                                       int ncell) {
+  const int t = blockIdx.x * kBS + threadIdx.x;
+  const int nI = imax + 1, nJ = jmax + 1;
+  if (t >= nI * nJ * (kmax + 1)) return;
+  const int J = t % nJ + 1;
+  const int I = (t / nJ) % nI + 1;
+  const int K = t / (nJ * nI) + 1;
+  const int node = (J - 1) + (I - 1) * nj + (K - 1) * nj * ni;
+  double sum[kMaxPh] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < 2; c++)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        const int ci = I + a, cj = J + b, ck = K + c;
+        const int cell = (cj - 1) + (ci - 1) * nj + (ck - 1) * nj * ni;
+        if (!fluid[cell]) continue;
+        const int s = off[cell], e = off[cell + 1];
+        if (s == e) continue;
+        const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
+        const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
+        for (int q = s; q < e; q++) {
+          const int p = pic[q];
+          double zx = pos[p] - x0, zy = pos[S + p] - y0;
+          zx = zx / dxx;
+          zy = zy / dyy;
+          double zz = pos[2 * S + p] - z0;
+          zz = zz / dzz;
+          // this node is corner (1-a, 1-b, 1-c) of the cell
+          const double xv = a ? 1.0 - zx : zx;
+          const double yv = b ? 1.0 - zy : zy;
+          const double zv = c ? 1.0 - zz : zz;
+          const double w = xv * yv * zv;
+          const int m = phase[p];
+          sum[m] = sum[m] + w * pmass[p];
+        }
+      }
+  for (int m = 0; m < nph; m++) rops_node[(size_t)m * ncell + node] = sum[m];
+}
+
+// node -> cell ROP_S (comp_mean_fields0.f:298-345), ROP_S/VOL (:347-370) and
+// CALC_EPG_DES (calc_epg_des.f:95-150).  Nodes visited K, J, I ascending.
+__global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kmax, int ni, int nj,
+                                                     int nk, int nph, const double* rops_node,
+                                                     const double* vol, const double* vol_surr,
+                                                     const uint8_t* fluid, const double* ro_s,
+                                                     double packed_eps, double* rop_s,
+                                                     double* ep_g, int* flags) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  const int ncell = ni * nj * nk;
+  if (c >= ncell) return;
+  const int cj = c % nj + 1;
+  const int ci = (c / nj) % ni + 1;
+  const int ck = c / (nj * ni) + 1;
+  double acc[kMaxPh] = {0.0, 0.0, 0.0, 0.0};
+  for (int K = ck - 1; K <= ck; K++) {
+    if (K < 1 || K > kmax + 1) continue;
+    for (int J = cj - 1; J <= cj; J++) {
+      if (J < 1 || J > jmax + 1) continue;
+      for (int I = ci - 1; I <= ci; I++) {
+        if (I < 1 || I > imax + 1) continue;
+        const int node = (J - 1) + (I - 1) * nj + (K - 1) * nj * ni;
+        const double vs = vol_surr[node];
+        if (vs == 0.0) continue;
+        for (int m = 0; m < nph; m++) {
+          const double dens = rops_node[(size_t)m * ncell + node] / vs;
+          acc[m] = acc[m] + dens * vol[c];
+        }
+      }
+    }
+  }
+  if (fluid[c])
+    for (int m = 0; m < nph; m++)
+      if (acc[m] > 0.0) acc[m] = acc[m] / vol[c];
+  for (int m = 0; m < nph; m++) rop_s[(size_t)m * ncell + c] = acc[m];
+  if (fluid[c]) {
+    double sum_eps = 0.0;
+    for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / ro_s[m];
+    const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);
+    ep_g[c] = e;
+    if (e <= 0.0 || e > 1.0) atomicAdd(&flags[kFlagEpg], 1);
+  }
+}
+
+// ------------------------------------------------------------- exports
+__global__ void k_scatter_gid_d3(int n, int S, int ld, const int* gid, const uint8_t* state,
+                                 const double* src, double* dst) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int g = gid[i];
+  for (int c = 0; c < 3; c++) dst[(size_t)c * ld + g] = src[c * S + i];
+}
+__global__ void k_scatter_gid_i(int n, const int* gid, const int* src, int add, int* dst) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  dst[gid[i]] = src[i] + add;
+}
+__global__ void k_scatter_gid_u8(int n, const int* gid, const uint8_t* src, int8_t* dst) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  dst[gid[i]] = (int8_t)src[i];
+}
+
+}  // namespace mfixdem

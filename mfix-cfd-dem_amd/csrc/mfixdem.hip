@@ -1,0 +1,1005 @@
+// mfixdem.hip -- C ABI of the gfx950 DEM hot path (include/mfixdem.h).
+//
+// One handle = one DEM system resident on the current HIP device, driven on
+// one stream.  The time step is a fused particle kernel (walls, pairs, drag,
+// integrator, fluid-cell locate) plus, when gas-coupled, the deposition
+// kernels; every NEIGHBOR_SEARCH_N steps the particles are re-sorted and the
+// neighbour list rebuilt.  Runs of steps between rebuilds are replayed from
+// captured hipGraphs.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mfixdem.h"
+#include "device_physics.hpp"
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+using namespace mfixdem;
+
+#define HIPOK(x)                                   \
+  do {                                             \
+    hipError_t e_ = (x);                           \
+    if (e_ != hipSuccess) {                        \
+      std::fprintf(stderr, "mfixdem: %s failed: %s (%s:%d)\n", #x, \
+                   hipGetErrorString(e_), __FILE__, __LINE__);     \
+      return MFIXDEM_ERR_HIP;                      \
+    }                                              \
+  } while (0)
+
+#define CHECKRC(x)            \
+  do {                        \
+    int32_t r_ = (x);         \
+    if (r_ != MFIXDEM_OK) return r_; \
+  } while (0)
+
+enum { kTimeStep = 0, kTimeRebuild, kTimeGas, kTimeSlots };
+enum { MFIXDEM_DEBUG_TIME_KERNELS_ = 2 };
+
+struct mfixdem_handle {
+  Setup S;
+  DevParams P{};
+  GridDev G{};
+  hipStream_t stream = nullptr;
+  int n = 0, stride = 0;
+  int cur = 0, pcur = 0, tw = 0, lt = 0;
+  // particles (cur / tw twins)
+  double *pos[2]{}, *vel[2]{}, *omega[2]{};
+  double *radius[2]{}, *pmass[2]{}, *omoi[2]{}, *pvol[2]{};
+  double *acc[2]{}, *racc[2]{};
+  int *gid[2]{}, *fcell[2]{}, *dgc[2]{};
+  uint8_t *state[2]{}, *phase[2]{}, *wmask[2]{};
+  double* wpft[2]{};
+  double *fc = nullptr, *tow = nullptr;
+  // rebuild
+  uint64_t* keys[2]{};
+  int* vals[2]{};
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int *fstart = nullptr, *fend = nullptr;
+  int n_fine = 0;
+  int *own_cnt = nullptr, *own_tmp = nullptr, *in_cnt = nullptr, *in_cur = nullptr,
+      *tot_cnt = nullptr;
+  uint64_t* own_key = nullptr;
+  int maxrow = 48;
+  int *nbr_start[2]{}, *nbr_split[2]{}, *pair_base[2]{};
+  int *nbr[2]{}, *nbr_pid[2]{};
+  int64_t nbr_cap[2]{0, 0};
+  double* pft[2]{};
+  int64_t pft_cap[2]{0, 0};
+  int64_t n_pairs = 0;
+  int* scan_bsum = nullptr;
+  int* flags = nullptr;
+  // gas
+  double *xe = nullptr, *yn = nullptr, *zt = nullptr, *vnode = nullptr, *p_force = nullptr,
+         *ep_g = nullptr, *rop_s = nullptr, *rops_node = nullptr, *vol = nullptr,
+         *vol_surr = nullptr, *ro_s = nullptr;
+  uint8_t* fluid = nullptr;
+  int *pic_cnt = nullptr, *pic_off = nullptr, *pic_cur = nullptr, *pic = nullptr;
+  uint16_t* cell_facets = nullptr;
+  double* stage = nullptr;  // export staging, 3*n doubles
+  // status
+  mfixdem_status st{};
+  bool inited = false, gas_set = false, first_pass = true, binned = false;
+  int debug = 0;
+  // graphs: [cur parity]
+  hipGraphExec_t graph[2]{nullptr, nullptr};
+  // kernel timing
+  hipEvent_t ev[2]{nullptr, nullptr};
+  double t_ms[kTimeSlots]{};
+  int64_t t_calls[kTimeSlots]{};
+};
+
+namespace {
+
+inline int nblk(int64_t n) { return (int)((n + kBS - 1) / kBS); }
+
+template <typename T>
+int32_t dalloc(T** p, int64_t count) {
+  *p = nullptr;
+  if (count <= 0) count = 1;
+  if (hipMalloc((void**)p, sizeof(T) * (size_t)count) != hipSuccess) return MFIXDEM_ERR_OOM;
+  return MFIXDEM_OK;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+int32_t fill_i32(hipStream_t s, int* a, int64_t n, int v) {
+  if (n <= 0) return MFIXDEM_OK;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(kBS), 0, s, a, (int)n, v);
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+// exclusive scan of in[0..n) -> out[0..n]; out[n] = total
+int32_t scan_i32(mfixdem_handle* h, const int* in, int* out, int64_t n) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(std::max(nb, 1)), dim3(kBS), 0, h->stream, in, out,
+                     (int)n, h->scan_bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBS), 0, h->stream, h->scan_bsum, std::max(nb, 1));
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(n + 1)), dim3(kBS), 0, h->stream, out, (int)n,
+                     h->scan_bsum, std::max(nb, 1));
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+StepPtrs step_ptrs(mfixdem_handle* h) {
+  StepPtrs a{};
+  const int c = h->cur, t = h->tw, l = h->lt, p = h->pcur;
+  a.pos = h->pos[c]; a.vel = h->vel[c]; a.omega = h->omega[c];
+  a.pos_o = h->pos[1 - c]; a.vel_o = h->vel[1 - c]; a.omega_o = h->omega[1 - c];
+  a.radius = h->radius[t]; a.pmass = h->pmass[t]; a.omoi = h->omoi[t]; a.pvol = h->pvol[t];
+  a.gid = h->gid[t]; a.state = h->state[t]; a.phase = h->phase[t]; a.dgc = h->dgc[t];
+  a.cell_facets = h->cell_facets;
+  a.wmask = h->wmask[t]; a.wpft = h->wpft[t];
+  a.nbr_start = h->nbr_start[l]; a.nbr_split = h->nbr_split[l];
+  a.nbr = h->nbr[l]; a.nbr_pid = h->nbr_pid[l];
+  a.pft_in = h->pft[p]; a.pft_out = h->pft[1 - p];
+  a.fc = h->fc; a.tow = h->tow;
+  a.acc_old = h->acc[t]; a.rot_acc_old = h->racc[t];
+  a.fcell = h->fcell[t];
+  a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
+  a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
+  a.flags = h->flags;
+  return a;
+}
+
+void invalidate_graphs(mfixdem_handle* h) {
+  for (auto& g : h->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS_) != 0; }
+void t_begin(mfixdem_handle* h) {
+  if (timing(h)) (void)hipEventRecord(h->ev[0], h->stream);
+}
+void t_end(mfixdem_handle* h, int slot) {
+  if (!timing(h)) return;
+  (void)hipEventRecord(h->ev[1], h->stream);
+  (void)hipEventSynchronize(h->ev[1]);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+  h->t_ms[slot] += ms;
+  h->t_calls[slot] += 1;
+}
+
+template <int MODE>
+int32_t launch_step(mfixdem_handle* h) {
+  DevParams P = h->P;
+  P.first_pass = h->first_pass ? 1 : 0;
+  P.store_forces = (h->debug & MFIXDEM_DEBUG_STORE_FORCES) ? 1 : 0;
+  StepPtrs a = step_ptrs(h);
+  if (h->S.p.precision == MFIXDEM_MIXED)
+    hipLaunchKernelGGL((k_step<true, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
+  else
+    hipLaunchKernelGGL((k_step<false, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+int32_t ensure_list_capacity(mfixdem_handle* h, int nl, int64_t entries) {
+  if (h->nbr_cap[nl] >= entries) return MFIXDEM_OK;
+  const int64_t cap = std::max<int64_t>(entries + entries / 4, 1024);
+  dfree(h->nbr[nl]);
+  dfree(h->nbr_pid[nl]);
+  CHECKRC(dalloc(&h->nbr[nl], cap));
+  CHECKRC(dalloc(&h->nbr_pid[nl], cap));
+  h->nbr_cap[nl] = cap;
+  return MFIXDEM_OK;
+}
+
+// history buffer b must hold `pairs` entries; its contents are not kept
+int32_t ensure_pft_capacity(mfixdem_handle* h, int b, int64_t pairs) {
+  if (h->pft_cap[b] >= pairs && h->pft[b]) return MFIXDEM_OK;
+  const int64_t cap = std::max<int64_t>(pairs + pairs / 4, 1024);
+  dfree(h->pft[b]);
+  CHECKRC(dalloc(&h->pft[b], 3 * cap));
+  h->pft_cap[b] = cap;
+  return MFIXDEM_OK;
+}
+
+// DESGRID_PIC(.TRUE.) (desgrid_mod.f:877): DES cells, search-grid sort,
+// permutation of every per-particle array into the new order.
+int32_t do_bin(mfixdem_handle* h) {
+  const int n = h->n, S = h->stride;
+  int init[kFlagCount] = {INT_MAX, 0, 0, 0, 0, 0, 0};
+  // only the rebuild flags are reset here
+  HIPOK(hipMemcpyAsync(h->flags + kFlagRogueGid, init, 2 * sizeof(int), hipMemcpyHostToDevice,
+                       h->stream));
+  HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, init + kFlagOwnOverflow, sizeof(int),
+                       hipMemcpyHostToDevice, h->stream));
+  const int t = h->tw, c = h->cur;
+  hipLaunchKernelGGL(k_bin, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, h->pos[c],
+                     h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0], h->flags);
+  HIPOK(hipGetLastError());
+  size_t tb = h->sort_tmp_bytes;
+  HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
+                                           h->vals[1], n, 0, 30 + h->G.gid_bits, h->stream));
+  PermPtrs p{};
+  p.perm = h->vals[1];
+  p.pos = h->pos[c]; p.vel = h->vel[c]; p.omega = h->omega[c];
+  p.pos_o = h->pos[1 - c]; p.vel_o = h->vel[1 - c]; p.omega_o = h->omega[1 - c];
+  p.radius = h->radius[t]; p.pmass = h->pmass[t]; p.omoi = h->omoi[t]; p.pvol = h->pvol[t];
+  p.acc = h->acc[t]; p.racc = h->racc[t];
+  p.radius_o = h->radius[1 - t]; p.pmass_o = h->pmass[1 - t]; p.omoi_o = h->omoi[1 - t];
+  p.pvol_o = h->pvol[1 - t]; p.acc_o = h->acc[1 - t]; p.racc_o = h->racc[1 - t];
+  p.gid = h->gid[t]; p.fcell = h->fcell[t]; p.dgc = h->dgc[t];
+  p.gid_o = h->gid[1 - t]; p.fcell_o = h->fcell[1 - t]; p.dgc_o = h->dgc[1 - t];
+  p.state = h->state[t]; p.phase = h->phase[t]; p.wmask = h->wmask[t];
+  p.state_o = h->state[1 - t]; p.phase_o = h->phase[1 - t]; p.wmask_o = h->wmask[1 - t];
+  p.wpft = h->wpft[t]; p.wpft_o = h->wpft[1 - t];
+  p.nf = h->P.nf;
+  p.ab2 = h->P.ab2;
+  hipLaunchKernelGGL(k_permute, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, p);
+  HIPOK(hipGetLastError());
+  h->cur = 1 - c;
+  h->tw = 1 - t;
+  CHECKRC(fill_i32(h->stream, h->fstart, h->n_fine, 0));
+  CHECKRC(fill_i32(h->stream, h->fend, h->n_fine, 0));
+  hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G, h->keys[1],
+                     h->state[h->tw], h->fstart, h->fend);
+  HIPOK(hipGetLastError());
+  h->binned = true;
+  invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+// NEIGHBOUR (neighbour.f:21-126) on the binned order.
+int32_t do_neighbour(mfixdem_handle* h) {
+  if (!h->binned) CHECKRC(do_bin(h));
+  const int n = h->n, S = h->stride, t = h->tw, c = h->cur;
+  BuildPtrs b{};
+  b.pos = h->pos[c]; b.radius = h->radius[t]; b.gid = h->gid[t]; b.dgc = h->dgc[t];
+  b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
+  b.own_cnt = h->own_cnt; b.flags = h->flags;
+  int fl[kFlagCount];
+  for (;;) {
+    b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow;
+    hipLaunchKernelGGL(k_own_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
+    HIPOK(hipStreamSynchronize(h->stream));
+    if (fl[kFlagOwnOverflow] <= h->maxrow) break;
+    const int mr = fl[kFlagOwnOverflow] + 16;
+    dfree(h->own_tmp);
+    dfree(h->own_key);
+    CHECKRC(dalloc(&h->own_tmp, (int64_t)n * mr));
+    CHECKRC(dalloc(&h->own_key, (int64_t)n * mr));
+    h->maxrow = mr;
+    int z = 0;
+    HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, &z, sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  }
+  if (fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
+  const int nl = 1 - h->lt;
+  // own rows -> pair ids
+  CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
+  int npairs = 0;
+  HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
+                       h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  CHECKRC(ensure_list_capacity(h, nl, 2 * (int64_t)npairs));
+  CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
+  CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
+  hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
+                     h->own_tmp, h->maxrow, h->in_cnt);
+  hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
+                     h->own_cnt, h->tot_cnt);
+  HIPOK(hipGetLastError());
+  CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
+  CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
+  hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
+                     h->maxrow, h->pair_base[nl], h->nbr_start[nl], h->in_cnt, h->nbr_split[nl],
+                     h->in_cur, h->nbr[nl], h->nbr_pid[nl]);
+  hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
+                     h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
+  HIPOK(hipGetLastError());
+  CarryPtrs cp{};
+  cp.perm = h->vals[1];
+  cp.gid = h->gid[t];
+  cp.nbr_start = h->nbr_start[nl]; cp.nbr_split = h->nbr_split[nl]; cp.nbr = h->nbr[nl];
+  cp.pair_base = h->pair_base[nl];
+  cp.old_gid = h->gid[1 - t];
+  cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
+  cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
+  cp.pft_old = h->pft[h->pcur];
+  cp.pft_new = h->pft[1 - h->pcur];
+  hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, cp);
+  HIPOK(hipGetLastError());
+  h->lt = nl;
+  h->pcur = 1 - h->pcur;
+  h->n_pairs = npairs;
+  // the old history buffer becomes the next step's output
+  HIPOK(hipStreamSynchronize(h->stream));
+  CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
+  if (fl[kFlagRogueCount] > 0) {
+    hipLaunchKernelGGL(k_delete_rogue, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
+                       h->dgc[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c],
+                       h->radius[t]);
+    HIPOK(hipGetLastError());
+    h->st.rogue_deleted += fl[kFlagRogueCount];
+  }
+  h->binned = false;
+  h->st.rebuilds++;
+  invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+// PARTICLES_IN_CELL lists (particles_in_cell.f:157-229) from fcell
+int32_t do_pic_lists(mfixdem_handle* h) {
+  const int n = h->n, S = h->stride, t = h->tw, c = h->cur, nc = h->S.ncell;
+  CHECKRC(fill_i32(h->stream, h->pic_cnt, nc, 0));
+  hipLaunchKernelGGL(k_pic_count, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->fcell[t],
+                     h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c], h->radius[t],
+                     h->pic_cnt, h->flags);
+  HIPOK(hipGetLastError());
+  CHECKRC(scan_i32(h, h->pic_cnt, h->pic_off, nc));
+  CHECKRC(fill_i32(h->stream, h->pic_cur, nc, 0));
+  hipLaunchKernelGGL(k_pic_scatter, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->fcell[t],
+                     h->state[t], h->pic_off, h->pic_cur, h->pic);
+  hipLaunchKernelGGL(k_pic_sort, dim3(nblk(nc)), dim3(kBS), 0, h->stream, nc, h->pic_off,
+                     h->gid[t], h->pic);
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+// COMP_MEAN_FIELDS (GARG) + CALC_EPG_DES
+int32_t do_mean_fields(mfixdem_handle* h) {
+  const mfixdem_params& p = h->S.p;
+  const int nnode = (p.imax + 1) * (p.jmax + 1) * (p.kmax + 1);
+  const int t = h->tw, c = h->cur;
+  hipLaunchKernelGGL(k_deposit_nodes, dim3(nblk(nnode)), dim3(kBS), 0, h->stream, p.imax, p.jmax,
+                     p.kmax, h->S.ni, h->S.nj, p.nphases, h->stride, h->xe, h->yn, h->zt,
+                     h->fluid, h->pic_off, h->pic, h->pos[c], h->pmass[t], h->phase[t],
+                     h->rops_node, h->S.ncell);
+  const double packed = (p.des_epg_clip != kUndef) ? 1.0 - p.des_epg_clip : 1.0;
+  hipLaunchKernelGGL(k_cell_fields, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream, p.imax,
+                     p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, h->rops_node, h->vol,
+                     h->vol_surr, h->fluid, h->ro_s, packed, h->rop_s, h->ep_g, h->flags);
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+int32_t gas_tail(mfixdem_handle* h) {
+  CHECKRC(do_pic_lists(h));
+  return do_mean_fields(h);
+}
+
+// the part of a step that is identical between rebuilds (graph-capturable)
+int32_t plain_step_body(mfixdem_handle* h) {
+  CHECKRC(launch_step<kModeFused>(h));
+  h->cur = 1 - h->cur;
+  h->pcur = 1 - h->pcur;
+  if (h->S.p.coupled) CHECKRC(gas_tail(h));
+  return MFIXDEM_OK;
+}
+
+int32_t one_step(mfixdem_handle* h, int32_t nn) {
+  const bool rebuild = (nn == 1) || (nn % h->S.p.neighbor_search_n == 0);
+  const bool use_graph = !rebuild && !h->first_pass && !timing(h) &&
+                         !(h->debug & MFIXDEM_DEBUG_STORE_FORCES);
+  if (use_graph) {
+    hipGraphExec_t& g = h->graph[h->cur];
+    if (!g) {
+      const int c0 = h->cur, p0 = h->pcur;
+      HIPOK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      int32_t rc = plain_step_body(h);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(h->stream, &graph);
+      h->cur = c0;
+      h->pcur = p0;
+      if (rc != MFIXDEM_OK) return rc;
+      HIPOK(e);
+      HIPOK(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+    }
+    HIPOK(hipGraphLaunch(g, h->stream));
+    h->cur = 1 - h->cur;
+    h->pcur = 1 - h->pcur;
+  } else {
+    t_begin(h);
+    CHECKRC(launch_step<kModeFused>(h));
+    t_end(h, kTimeStep);
+    h->cur = 1 - h->cur;
+    h->pcur = 1 - h->pcur;
+    if (rebuild) {
+      t_begin(h);
+      CHECKRC(do_bin(h));
+      CHECKRC(do_neighbour(h));
+      t_end(h, kTimeRebuild);
+    }
+    if (h->S.p.coupled) {
+      t_begin(h);
+      CHECKRC(gas_tail(h));
+      t_end(h, kTimeGas);
+    }
+  }
+  h->first_pass = false;
+  h->st.steps++;
+  return MFIXDEM_OK;
+}
+
+int32_t scatter_d3(mfixdem_handle* h, const double* src, double* host, int64_t ld) {
+  const int n = h->n;
+  hipLaunchKernelGGL(k_scatter_gid_d3, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->stride, n,
+                     h->gid[h->tw], h->state[h->tw], src, h->stage);
+  HIPOK(hipGetLastError());
+  std::vector<double> tmp((size_t)3 * n);
+  HIPOK(hipMemcpyAsync(tmp.data(), h->stage, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                       h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  for (int c = 0; c < 3; c++) std::memcpy(host + c * ld, tmp.data() + (size_t)c * n, sizeof(double) * n);
+  return MFIXDEM_OK;
+}
+
+template <typename T>
+int32_t download(mfixdem_handle* h, std::vector<T>& v, const T* d, int64_t count) {
+  v.resize((size_t)std::max<int64_t>(count, 0));
+  if (count > 0)
+    HIPOK(hipMemcpyAsync(v.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return MFIXDEM_OK;
+}
+
+int32_t upload_setup(mfixdem_handle* h) {
+  Setup& S = h->S;
+  HIPOK(hipMemcpy(h->xe, S.xe.data(), sizeof(double) * S.xe.size(), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->yn, S.yn.data(), sizeof(double) * S.yn.size(), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->zt, S.zt.data(), sizeof(double) * S.zt.size(), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->vol, S.vol.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->vol_surr, S.vol_surr.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->fluid, S.fluid.data(), S.ncell, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->cell_facets, S.cell_facets.data(), sizeof(uint16_t) * S.dg_ncell,
+                  hipMemcpyHostToDevice));
+  double ros[MFIXDEM_MAX_PHASES];
+  for (int m = 0; m < MFIXDEM_MAX_PHASES; m++) ros[m] = S.p.ro_s0[m];
+  HIPOK(hipMemcpy(h->ro_s, ros, sizeof(ros), hipMemcpyHostToDevice));
+  std::vector<double> one(S.ncell, 1.0);
+  HIPOK(hipMemcpy(h->ep_g, one.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
+  return MFIXDEM_OK;
+}
+
+void fill_devparams(mfixdem_handle* h) {
+  const Setup& S = h->S;
+  const mfixdem_params& p = S.p;
+  DevParams& P = h->P;
+  P.n = h->n;
+  P.stride = h->stride;
+  P.hertz = p.coll_model == MFIXDEM_HERTZ;
+  P.ab2 = p.integrator == MFIXDEM_ADAMS_BASHFORTH;
+  P.coupled = p.coupled;
+  P.drag_type = p.drag_type;
+  P.nph = p.nphases;
+  P.nf = (int)S.facets.size();
+  P.dt = S.d.dtsolid;
+  for (int c = 0; c < 3; c++) P.grav[c] = p.grav[c];
+  P.kn = p.kn; P.kt = S.d.kt; P.kn_w = p.kn_w; P.kt_w = S.d.kt_w;
+  P.mew = p.mew; P.mew_w = p.mew_w;
+  for (int a = 0; a < kMaxPh; a++) {
+    for (int b = 0; b < kMaxPh; b++) {
+      P.etan[a][b] = S.d.des_etan[a][b];
+      P.etat[a][b] = S.d.des_etat[a][b];
+      P.hkn[a][b] = S.d.hert_kn[a][b];
+      P.hkt[a][b] = S.d.hert_kt[a][b];
+    }
+    P.etan_w[a] = S.d.des_etan_wall[a];
+    P.etat_w[a] = S.d.des_etat_wall[a];
+    P.hkwn[a] = S.d.hert_kwn[a];
+    P.hkwt[a] = S.d.hert_kwt[a];
+  }
+  P.ni = S.ni; P.nj = S.nj; P.nk = S.nk;
+  P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
+  for (size_t f = 0; f < S.facets.size(); f++) {
+    FacetDev& F = P.facets[f];
+    std::memcpy(F.v, S.facets[f].v, sizeof(F.v));
+    std::memcpy(F.n, S.facets[f].n, sizeof(F.n));
+    F.fmin = S.facets[f].fmin;
+    F.fmax = S.facets[f].fmax;
+    F.dir = S.facets[f].dir;
+    std::memcpy(F.ignore_edge, S.facets[f].ignore_edge, 4);
+  }
+  GridDev& G = h->G;
+  G.dg_ni = S.dg_ni; G.dg_nj = S.dg_nj; G.dg_nk = S.dg_nk;
+  for (int d = 0; d < 3; d++) G.coarse[d] = S.coarse[d];
+  G.dg_lo[0] = S.d.dg_xstart; G.dg_lo[1] = S.d.dg_ystart; G.dg_lo[2] = S.d.dg_zstart;
+  G.dg_inv[0] = S.d.dg_dxinv; G.dg_inv[1] = S.d.dg_dyinv; G.dg_inv[2] = S.d.dg_dzinv;
+  G.factor_rlm = p.factor_rlm;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int32_t mfixdem_abi_version(void) { return MFIXDEM_ABI_VERSION; }
+
+const char* mfixdem_error_string(int32_t code) {
+  switch (code) {
+    case MFIXDEM_OK: return "ok";
+    case MFIXDEM_ERR_ARG: return "invalid argument";
+    case MFIXDEM_ERR_HIP: return "HIP runtime error";
+    case MFIXDEM_ERR_OOM: return "device allocation failed";
+    case MFIXDEM_ERR_ROGUE: return "normal particle found in a ghost DES cell";
+    case MFIXDEM_ERR_EPG: return "unphysical gas volume fraction";
+    case MFIXDEM_ERR_ZERO_DISTANCE: return "distance between particles is zero";
+    case MFIXDEM_ERR_CAPACITY: return "buffer too small";
+    case MFIXDEM_ERR_NOT_READY: return "handle not initialised";
+    case MFIXDEM_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown error";
+  }
+}
+
+int32_t mfixdem_params_default(mfixdem_params* p) {
+  if (!p) return MFIXDEM_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->abi_version = MFIXDEM_ABI_VERSION;
+  p->coll_model = MFIXDEM_LSD;
+  p->integrator = MFIXDEM_EULER;
+  p->precision = MFIXDEM_FP64;
+  p->nphases = 1;
+  p->neighbor_search_n = 25;
+  p->drag_type = MFIXDEM_DRAG_GIDASPOW;
+  p->interp_scheme = MFIXDEM_INTERP_GARG_2012;
+  p->imax = p->jmax = p->kmax = 1;
+  p->remove_rogue = 1;
+  p->dtsolid = kUndef;
+  p->dtsolid_factor = 50.0;
+  p->grav[1] = -9.80665;
+  p->factor_rlm = (double)1.2f;
+  p->kt_fac = (double)0.28571428571429f;
+  p->kt_w_fac = (double)0.28571428571429f;
+  p->des_etat_fac = kUndef;
+  p->des_etat_w_fac = kUndef;
+  for (int i = 0; i < MFIXDEM_MAX_PAIRS_IN; i++) p->des_en_input[i] = p->des_et_input[i] = kUndef;
+  for (int i = 0; i < MFIXDEM_MAX_PHASES; i++) {
+    p->des_en_wall_input[i] = p->des_et_wall_input[i] = kUndef;
+    p->e_young[i] = p->v_poisson[i] = p->d_p0[i] = p->ro_s0[i] = kUndef;
+  }
+  p->ew_young = p->vw_poisson = kUndef;
+  p->des_epg_clip = kUndef;
+  p->ro_g0 = 1.2;
+  p->mu_g0 = 1.8e-5;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
+  if (!p || !out) return MFIXDEM_ERR_ARG;
+  *out = nullptr;
+  mfixdem_handle* h = new (std::nothrow) mfixdem_handle();
+  if (!h) return MFIXDEM_ERR_OOM;
+  int32_t rc = build_setup(*p, h->S);
+  if (rc != MFIXDEM_OK) {
+    delete h;
+    return rc;
+  }
+  h->n = p->max_pip;
+  h->stride = (h->n + 63) / 64 * 64;
+  fill_devparams(h);
+  // neighbour-search grid: cell >= the largest cut-off FACTOR_RLM*(2 r_max)
+  {
+    const double hcut = p->factor_rlm * (2.0 * p->max_radius) * (1.0 + 1e-9);
+    const double lo[3] = {p->x_min - hcut, p->y_min - hcut, p->z_min - hcut};
+    const double len[3] = {p->xlength, p->ylength, p->zlength};
+    for (int d = 0; d < 3; d++) {
+      h->G.f_lo[d] = lo[d];
+      h->G.fn[d] = std::max(1, (int)std::ceil((len[d] + 2.0 * hcut) / hcut));
+      if (h->G.fn[d] > 1022) {
+        delete h;
+        return MFIXDEM_ERR_UNSUPPORTED;
+      }
+    }
+    h->G.f_inv = 1.0 / hcut;
+    h->n_fine = h->G.fn[0] * h->G.fn[1] * h->G.fn[2];
+    int gb = 1;
+    while ((1ll << gb) < (int64_t)h->n) gb++;
+    h->G.gid_bits = gb;
+  }
+  const int64_t n = h->n, S = h->stride, nc = h->S.ncell;
+  auto fail = [&](int32_t r) {
+    mfixdem_destroy(h);
+    return r;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  if (hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  const int nf = std::max(1, h->P.nf);
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dalloc(&h->pos[b], 3 * S)) || (rc = dalloc(&h->vel[b], 3 * S)) ||
+        (rc = dalloc(&h->omega[b], 3 * S)) || (rc = dalloc(&h->radius[b], S)) ||
+        (rc = dalloc(&h->pmass[b], S)) || (rc = dalloc(&h->omoi[b], S)) ||
+        (rc = dalloc(&h->pvol[b], S)) || (rc = dalloc(&h->acc[b], 3 * S)) ||
+        (rc = dalloc(&h->racc[b], 3 * S)) || (rc = dalloc(&h->gid[b], S)) ||
+        (rc = dalloc(&h->fcell[b], S)) || (rc = dalloc(&h->dgc[b], S)) ||
+        (rc = dalloc(&h->state[b], S)) || (rc = dalloc(&h->phase[b], S)) ||
+        (rc = dalloc(&h->wmask[b], S)) || (rc = dalloc(&h->wpft[b], 3 * nf * S)) ||
+        (rc = dalloc(&h->keys[b], n)) || (rc = dalloc(&h->vals[b], n)) ||
+        (rc = dalloc(&h->nbr_start[b], n + 1)) || (rc = dalloc(&h->nbr_split[b], n)) ||
+        (rc = dalloc(&h->pair_base[b], n + 1)))
+      return fail(rc);
+  }
+  if ((rc = dalloc(&h->fc, 3 * S)) || (rc = dalloc(&h->tow, 3 * S)) ||
+      (rc = dalloc(&h->fstart, h->n_fine)) || (rc = dalloc(&h->fend, h->n_fine)) ||
+      (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
+      (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
+      (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
+      (rc = dalloc(&h->scan_bsum, std::max<int64_t>(n, nc) / kScanTile + 2)) ||
+      (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
+      (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
+      (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
+      (rc = dalloc(&h->ep_g, nc)) || (rc = dalloc(&h->rop_s, nc * p->nphases)) ||
+      (rc = dalloc(&h->rops_node, nc * p->nphases)) || (rc = dalloc(&h->vol, nc)) ||
+      (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
+      (rc = dalloc(&h->fluid, nc)) || (rc = dalloc(&h->pic_cnt, nc)) ||
+      (rc = dalloc(&h->pic_off, nc + 1)) || (rc = dalloc(&h->pic_cur, nc)) ||
+      (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, h->S.dg_ncell)) ||
+      (rc = dalloc(&h->stage, 3 * n)))
+    return fail(rc);
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
+                                         h->vals[0], h->vals[1], (int)n, 0, 30 + h->G.gid_bits,
+                                         h->stream) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
+    return fail(MFIXDEM_ERR_OOM);
+  if ((rc = upload_setup(h))) return fail(rc);
+  if (hipMemset(h->flags, 0, sizeof(int) * kFlagCount) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
+  *out = h;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_destroy(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  invalidate_graphs(h);
+  for (int b = 0; b < 2; b++) {
+    dfree(h->pos[b]); dfree(h->vel[b]); dfree(h->omega[b]); dfree(h->radius[b]);
+    dfree(h->pmass[b]); dfree(h->omoi[b]); dfree(h->pvol[b]); dfree(h->acc[b]);
+    dfree(h->racc[b]); dfree(h->gid[b]); dfree(h->fcell[b]); dfree(h->dgc[b]);
+    dfree(h->state[b]); dfree(h->phase[b]); dfree(h->wmask[b]); dfree(h->wpft[b]);
+    dfree(h->keys[b]); dfree(h->vals[b]); dfree(h->nbr_start[b]); dfree(h->nbr_split[b]);
+    dfree(h->pair_base[b]); dfree(h->nbr[b]); dfree(h->nbr_pid[b]); dfree(h->pft[b]);
+  }
+  dfree(h->fc); dfree(h->tow); dfree(h->sort_tmp); dfree(h->fstart); dfree(h->fend);
+  dfree(h->own_cnt); dfree(h->own_tmp); dfree(h->own_key); dfree(h->in_cnt); dfree(h->in_cur);
+  dfree(h->tot_cnt); dfree(h->scan_bsum); dfree(h->flags); dfree(h->xe); dfree(h->yn);
+  dfree(h->zt); dfree(h->vnode); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
+  dfree(h->rops_node); dfree(h->vol); dfree(h->vol_surr); dfree(h->ro_s); dfree(h->fluid);
+  dfree(h->pic_cnt); dfree(h->pic_off); dfree(h->pic_cur); dfree(h->pic);
+  dfree(h->cell_facets); dfree(h->stage);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_derived(const mfixdem_handle* h, mfixdem_derived* out) {
+  if (!h || !out) return MFIXDEM_ERR_ARG;
+  *out = h->S.d;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, const double* vel,
+                              const double* omega, const double* radius, const double* ro_sol,
+                              const int32_t* phase, int64_t ld) {
+  if (!h || n < 0 || n > h->n || ld < n || !pos || !radius || !ro_sol) return MFIXDEM_ERR_ARG;
+  const int64_t N = h->n, S = h->stride;
+  std::vector<double> P3((size_t)3 * S, 0.0), V3((size_t)3 * S, 0.0), W3((size_t)3 * S, 0.0);
+  std::vector<double> R(S, 0.0), M(S, 3.4028234663852886e38), OI(S, 0.0), PV(S, 3.4028234663852886e38);
+  std::vector<int> G(S, 0), FC(S, -1), DG(S, 0);
+  std::vector<uint8_t> ST(S, 0), PH(S, 0), WM(S, 0);
+  for (int64_t q = 0; q < N; q++) G[q] = (int)q;
+  for (int64_t q = 0; q < n; q++) {
+    for (int c = 0; c < 3; c++) {
+      P3[c * S + q] = pos[c * ld + q];
+      V3[c * S + q] = vel ? vel[c * ld + q] : 0.0;
+      W3[c * S + q] = omega ? omega[c * ld + q] : 0.0;
+    }
+    const double r = radius[q];
+    R[q] = r;
+    // make_arrays_des.f:288-291
+    PV[q] = (4.0 / 3.0) * kPi * (r * r * r);
+    M[q] = PV[q] * ro_sol[q];
+    OI[q] = 2.5 / (M[q] * (r * r));
+    const int ph = phase ? phase[q] - 1 : 0;
+    if (ph < 0 || ph >= h->S.p.nphases) return MFIXDEM_ERR_ARG;
+    PH[q] = (uint8_t)ph;
+    ST[q] = 1;
+  }
+  invalidate_graphs(h);
+  h->cur = h->pcur = h->tw = h->lt = 0;
+  HIPOK(hipMemcpy(h->pos[0], P3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->vel[0], V3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->omega[0], W3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->radius[0], R.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->pmass[0], M.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->omoi[0], OI.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->pvol[0], PV.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->gid[0], G.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->fcell[0], FC.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->dgc[0], DG.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->state[0], ST.data(), S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->phase[0], PH.data(), S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->wmask[0], WM.data(), S, hipMemcpyHostToDevice));
+  std::vector<int> Z((size_t)N + 1, 0);
+  for (int b = 0; b < 2; b++) {
+    HIPOK(hipMemcpy(h->nbr_start[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->nbr_split[b], Z.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  }
+  HIPOK(hipMemset(h->flags, 0, sizeof(int) * kFlagCount));
+  h->st = mfixdem_status{};
+  h->n_pairs = 0;
+  h->inited = false;
+  h->first_pass = true;
+  h->binned = false;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, double* omega,
+                              int8_t* state, int64_t ld) {
+  if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
+  if (pos) CHECKRC(scatter_d3(h, h->pos[h->cur], pos, ld));
+  if (vel) CHECKRC(scatter_d3(h, h->vel[h->cur], vel, ld));
+  if (omega) CHECKRC(scatter_d3(h, h->omega[h->cur], omega, ld));
+  if (state) {
+    hipLaunchKernelGGL(k_scatter_gid_u8, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->n,
+                       h->gid[h->tw], h->state[h->tw], (int8_t*)h->stage);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(state, h->stage, h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPOK(hipStreamSynchronize(h->stream));
+  }
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
+                        const double* w_g, const double* p_force) {
+  if (!h || !u_g || !v_g || !w_g) return MFIXDEM_ERR_ARG;
+  const Setup& S = h->S;
+  const mfixdem_params& p = S.p;
+  const int nc = S.ncell;
+  // node velocities of DRAG_GS_DES0 (drag_gs_des0.f:148-179) for every node
+  // (I,J,K) = (1..IMAX+1, 1..JMAX+1, 1..KMAX+1); frozen field -> computed once
+  std::vector<double> vn((size_t)3 * nc, 0.0);
+  const double af = 0.25;
+  for (int K = 1; K <= p.kmax + 1; K++)
+    for (int I = 1; I <= p.imax + 1; I++)
+      for (int J = 1; J <= p.jmax + 1; J++) {
+        const int cur = fluid_ijk(S, I, J, K), ipjk = fluid_ijk(S, I + 1, J, K);
+        const int ijpk = fluid_ijk(S, I, J + 1, K), ipjpk = fluid_ijk(S, I + 1, J + 1, K);
+        const int ijpkp = fluid_ijk(S, I, J + 1, K + 1), ipjkp = fluid_ijk(S, I + 1, J, K + 1);
+        const int ijkp = fluid_ijk(S, I, J, K + 1);
+        double vx = af * (u_g[cur] + u_g[ijpk]);
+        double vy = af * (v_g[cur] + v_g[ipjk]);
+        vx = vx + af * (u_g[ijkp] + u_g[ijpkp]);
+        vy = vy + af * (v_g[ijkp] + v_g[ipjkp]);
+        const double vz = af * (w_g[cur] + w_g[ijpk] + w_g[ipjk] + w_g[ipjpk]);
+        vn[3 * cur] = vx;
+        vn[3 * cur + 1] = vy;
+        vn[3 * cur + 2] = vz;
+      }
+  HIPOK(hipMemcpy(h->vnode, vn.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
+  std::vector<double> pf((size_t)3 * nc, 0.0);
+  if (p_force) std::memcpy(pf.data(), p_force, sizeof(double) * 3 * nc);
+  HIPOK(hipMemcpy(h->p_force, pf.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
+  h->gas_set = true;
+  invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  const int nc = h->S.ncell;
+  if (ep_g) HIPOK(hipMemcpyAsync(ep_g, h->ep_g, sizeof(double) * nc, hipMemcpyDeviceToHost, h->stream));
+  if (rop_s)
+    HIPOK(hipMemcpyAsync(rop_s, h->rop_s, sizeof(double) * nc * h->S.p.nphases,
+                         hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_init(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (h->S.p.coupled && !h->gas_set) return MFIXDEM_ERR_NOT_READY;
+  const int n = h->n;
+  HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
+  if (h->S.p.coupled) {
+    hipLaunchKernelGGL(k_locate, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->stride, h->S.ni,
+                       h->S.nj, h->S.nk, h->xe, h->yn, h->zt, h->pos[h->cur], h->state[h->tw],
+                       h->fcell[h->tw]);
+    HIPOK(hipGetLastError());
+  }
+  CHECKRC(do_bin(h));
+  CHECKRC(do_neighbour(h));
+  if (h->S.p.coupled) CHECKRC(gas_tail(h));
+  HIPOK(hipStreamSynchronize(h->stream));
+  h->inited = true;
+  h->first_pass = true;
+  h->st.steps = 0;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_step(mfixdem_handle* h, int32_t nn) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (!h->inited) return MFIXDEM_ERR_NOT_READY;
+  return one_step(h, nn);
+}
+
+int32_t mfixdem_run(mfixdem_handle* h, int32_t nn_first, int32_t nsteps) {
+  if (!h || nsteps < 0) return MFIXDEM_ERR_ARG;
+  if (!h->inited) return MFIXDEM_ERR_NOT_READY;
+  for (int32_t t = 0; t < nsteps; t++) CHECKRC(one_step(h, nn_first + t));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_synchronize(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  HIPOK(hipStreamSynchronize(h->stream));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_calc_force_dem(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  invalidate_graphs(h);
+  CHECKRC(launch_step<kModeForce>(h));
+  h->pcur = 1 - h->pcur;
+  return MFIXDEM_OK;
+}
+int32_t mfixdem_calc_drag_des(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  return launch_step<kModeDrag>(h);
+}
+int32_t mfixdem_cfnewvalues(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  CHECKRC(launch_step<kModeIntegrate>(h));
+  h->cur = 1 - h->cur;
+  h->first_pass = false;
+  return MFIXDEM_OK;
+}
+int32_t mfixdem_desgrid_pic(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  return do_bin(h);
+}
+int32_t mfixdem_neighbour(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  return do_neighbour(h);
+}
+int32_t mfixdem_particles_in_cell(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  const int n = h->n;
+  hipLaunchKernelGGL(k_locate, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->stride, h->S.ni,
+                     h->S.nj, h->S.nk, h->xe, h->yn, h->zt, h->pos[h->cur], h->state[h->tw],
+                     h->fcell[h->tw]);
+  HIPOK(hipGetLastError());
+  return do_pic_lists(h);
+}
+int32_t mfixdem_comp_mean_fields(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  return do_mean_fields(h);
+}
+
+int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  h->debug = flags;
+  invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t ld) {
+  if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
+  if (fc) CHECKRC(scatter_d3(h, h->fc, fc, ld));
+  if (tow) CHECKRC(scatter_d3(h, h->tow, tow, ld));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk, int64_t ld) {
+  if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
+  const int n = h->n;
+  std::vector<int> g, d, f;
+  CHECKRC(download(h, g, h->gid[h->tw], n));
+  CHECKRC(download(h, d, h->dgc[h->tw], n));
+  CHECKRC(download(h, f, h->fcell[h->tw], n));
+  const int nj = h->S.nj, ni = h->S.ni;
+  for (int i = 0; i < n; i++) {
+    const int q = g[i];
+    if (dg_pijk) dg_pijk[q] = d[i] + 1;
+    if (pijk) {
+      const int c = f[i];
+      if (c >= 0) {
+        pijk[q] = (c / nj) % ni + 1;
+        pijk[ld + q] = c % nj + 1;
+        pijk[2 * ld + q] = c / (nj * ni) + 1;
+        pijk[3 * ld + q] = c + 1;
+      } else {
+        pijk[q] = pijk[ld + q] = pijk[2 * ld + q] = pijk[3 * ld + q] = 0;
+      }
+    }
+  }
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index, int32_t* neighbors,
+                          double* pft, int64_t capacity, int64_t* npairs) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (npairs) *npairs = h->n_pairs;
+  if (capacity == 0) return MFIXDEM_OK;
+  if (capacity < h->n_pairs) return MFIXDEM_ERR_CAPACITY;
+  const int n = h->n, l = h->lt;
+  std::vector<int> g, st, sp, nb, pid;
+  std::vector<double> pf;
+  CHECKRC(download(h, g, h->gid[h->tw], n));
+  CHECKRC(download(h, st, h->nbr_start[l], n + 1));
+  CHECKRC(download(h, sp, h->nbr_split[l], n));
+  CHECKRC(download(h, nb, h->nbr[l], st[n]));
+  CHECKRC(download(h, pid, h->nbr_pid[l], st[n]));
+  CHECKRC(download(h, pf, h->pft[h->pcur], 3 * h->n_pairs));
+  std::vector<int> slot_of(n);
+  for (int i = 0; i < n; i++) slot_of[g[i]] = i;
+  int64_t w = 0;
+  for (int q = 0; q < n; q++) {
+    const int i = slot_of[q];
+    for (int e = sp[i]; e < st[i + 1]; e++) {
+      if (neighbors) neighbors[w] = g[nb[e]] + 1;
+      if (pft)
+        for (int d = 0; d < 3; d++) pft[3 * w + d] = pf[3 * (size_t)pid[e] + d];
+      w++;
+    }
+    if (neighbor_index) neighbor_index[q] = (int32_t)(w + 1);
+  }
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out) {
+  if (!h || !out) return MFIXDEM_ERR_ARG;
+  int fl[kFlagCount];
+  HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
+  std::vector<uint8_t> s;
+  CHECKRC(download(h, s, h->state[h->tw], h->n));
+  *out = h->st;
+  int64_t np = 0;
+  for (uint8_t v : s) np += (v != 0);
+  out->n_particles = np;
+  out->n_pairs = h->n_pairs;
+  out->n_contacts = fl[kFlagContacts];
+  out->epg_errors = fl[kFlagEpg];
+  out->zero_distance = fl[kFlagZeroDist];
+  out->out_of_domain = fl[kFlagOutOfDomain];
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_stream(mfixdem_handle* h, void** stream) {
+  if (!h || !stream) return MFIXDEM_ERR_ARG;
+  *stream = (void*)h->stream;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_kernel_times(mfixdem_handle* h, double* ms, int64_t* calls, int32_t n) {
+  if (!h || n < 0) return MFIXDEM_ERR_ARG;
+  for (int i = 0; i < n && i < kTimeSlots; i++) {
+    if (ms) ms[i] = h->t_ms[i];
+    if (calls) calls[i] = h->t_calls[i];
+  }
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_reset_kernel_times(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  for (int i = 0; i < kTimeSlots; i++) {
+    h->t_ms[i] = 0.0;
+    h->t_calls[i] = 0;
+  }
+  return MFIXDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Seeded synthetic inputs for the DEM hot path.
+
+Every workload is a (Params, particles, gas) triple built from a seed, so the
+HIP library and the CPU oracle can be driven with identical inputs.  Axis
+convention follows the reference: gravity along -Y (constant_mod.f:235), so
+the long/vertical box axis is Y.
+
+  settling_bed       20,000 monodisperse spheres, d = 1 mm, rho = 2500,
+                     0.02 x 0.08 x 0.02 m box, LSD kn = 1e3, e = 0.9,
+                     mu = 0.3, dt = 1e-5 s, no gas.
+  fluidised_bed      polydisperse spheres (log-normal d, median 1 mm,
+                     sigma 0.25, clipped to [0.5, 2] mm), Hertz E = 1e7,
+                     nu = 0.3, e = 0.9, mu = 0.3, gas grid 64 x 128 x 64 on
+                     0.128 x 0.256 x 0.128 m, frozen upflow 1.2 m/s with a
+                     5 % sinusoidal perturbation, Gidaspow drag, GARG_2012.
+  dem01              the reference test DEM01 (single particle bouncing on
+                     the south wall, tests/dem/DEM01/mfix.dat).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import abi
+
+
+@dataclasses.dataclass
+class Particles:
+    pos: np.ndarray      # (3, n) float64, reference SoA (component-major)
+    vel: np.ndarray      # (3, n)
+    omega: np.ndarray    # (3, n)
+    radius: np.ndarray   # (n,)
+    ro_sol: np.ndarray   # (n,)
+    phase: np.ndarray    # (n,) int32, 1-based
+
+    @property
+    def n(self) -> int:
+        return int(self.radius.shape[0])
+
+
+@dataclasses.dataclass
+class Gas:
+    u_g: np.ndarray
+    v_g: np.ndarray
+    w_g: np.ndarray
+    p_force: np.ndarray  # (ncell, 3) -> P_FORCE(3, IJK)
+
+
+def n_cells(p: abi.Params) -> int:
+    return (p.imax + 2) * (p.jmax + 2) * (p.kmax + 2)
+
+
+def make_particles(pos, radius, rho, vel=None, omega=None, phase=None) -> Particles:
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[1]
+    vel = np.zeros((3, n)) if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+    omega = np.zeros((3, n)) if omega is None else np.ascontiguousarray(omega, dtype=np.float64)
+    radius = np.ascontiguousarray(np.broadcast_to(radius, (n,)), dtype=np.float64)
+    rho = np.ascontiguousarray(np.broadcast_to(rho, (n,)), dtype=np.float64)
+    phase = (np.ones(n, dtype=np.int32) if phase is None
+             else np.ascontiguousarray(phase, dtype=np.int32))
+    return Particles(pos, vel, omega, radius, rho, phase)
+
+
+def _rsa_lib():
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "lib", "libmfixdem_workload.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    fn = lib.mfixdem_rsa_place
+    fn.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                   C.c_int64, C.c_void_p]
+    fn.restype = C.c_int64
+    return fn
+
+
+def rsa_positions(seed, radii, lo, hi, max_tries=1_000_000):
+    """Random sequential addition of spheres (in the order given) inside the
+    box [lo, hi]; returns (3, n) centres.  Deterministic for a given seed."""
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    n = radii.shape[0]
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    pos = np.zeros((3, n))
+    got = _rsa_lib()(n, radii.ctypes.data, lo.ctypes.data, hi.ctypes.data,
+                     int(seed), int(max_tries), pos.ctypes.data)
+    if got != n:
+        raise RuntimeError(f"RSA placed {got} of {n} spheres")
+    return pos
+
+
+def settling_bed(n=20_000, seed=1, steps=2000):
+    """Plain FP64 DEM settling bed (the reference's CPU-runnable case)."""
+    rng = np.random.default_rng(seed)
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = abi.EULER
+    p.nphases = 1
+    p.max_pip = n
+    p.xlength, p.ylength, p.zlength = 0.02, 0.08, 0.02
+    p.imax = p.jmax = p.kmax = 1
+    p.dtsolid = 1.0e-5
+    p.grav[:] = [0.0, -9.81, 0.0]
+    p.kn = 1.0e3
+    p.kn_w = 1.0e3
+    p.mew = 0.3
+    p.mew_w = 0.3
+    p.des_en_input[0] = 0.9
+    p.des_en_wall_input[0] = 0.9
+    p.d_p0[0] = 1.0e-3
+    p.ro_s0[0] = 2500.0
+    p.max_radius = 0.5e-3
+    r = np.full(n, 0.5e-3)
+    pos = rsa_positions(seed, r, [0, 0, 0], [0.02, 0.08, 0.02])
+    parts = make_particles(pos, r, 2500.0)
+    return p, parts, None
+
+
+def lognormal_diameters(rng, n, median=1.0e-3, sigma=0.25, dmin=0.5e-3, dmax=2.0e-3):
+    d = median * np.exp(sigma * rng.standard_normal(n))
+    return np.clip(d, dmin, dmax)
+
+
+def frozen_gas(p: abi.Params, seed=7, u0=1.2, amp=0.05):
+    """Uniform upflow V_G = u0 with a seeded 5 % sinusoidal perturbation on
+    each component; P_FORCE = -dp/dy of a hydrostatic gas column."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = p.imax + 2, p.jmax + 2, p.kmax + 2
+    # cell index arrays in MFiX order: J fastest, then I, then K
+    k, i, j = np.meshgrid(np.arange(1, nk + 1), np.arange(1, ni + 1),
+                          np.arange(1, nj + 1), indexing="ij")
+    x = (i - 1.5) / p.imax
+    y = (j - 1.5) / p.jmax
+    z = (k - 1.5) / p.kmax
+    ph = rng.random(9) * 2 * np.pi
+    kx, ky, kz = 2 * np.pi * (1 + rng.integers(0, 3, 3))
+    def wave(a, b, c):
+        return np.sin(kx * x + ph[a]) * np.sin(ky * y + ph[b]) * np.sin(kz * z + ph[c])
+    u = amp * u0 * wave(0, 1, 2)
+    v = u0 * (1.0 + amp * wave(3, 4, 5))
+    w = amp * u0 * wave(6, 7, 8)
+    pf = np.zeros((ni * nj * nk, 3))
+    pf[:, 1] = p.ro_g0 * 9.81
+    return Gas(np.ascontiguousarray(u.ravel()), np.ascontiguousarray(v.ravel()),
+               np.ascontiguousarray(w.ravel()), np.ascontiguousarray(pf))
+
+
+def fluidised_bed(n=1_000_000, seed=2, phi=0.30, precision=abi.FP64,
+                  lx=0.128, ly=0.256, lz=0.128, cells=(64, 128, 64),
+                  dg=(26, 52, 26)):
+    """FP64 CFD-DEM with a frozen gas field (1M polydisperse Hertz spheres at
+    full size).  Smaller n keeps the same box and shrinks the bed height."""
+    rng = np.random.default_rng(seed)
+    d = lognormal_diameters(rng, n)
+    order = np.argsort(-d, kind="stable")   # RSA places large spheres first
+    d = d[order]
+    r = 0.5 * d
+    vol = (np.pi / 6.0) * np.sum(d ** 3)
+    h = min(ly * 0.9, vol / (phi * lx * lz))
+    pos = rsa_positions(seed, r, [0, 0, 0], [lx, h, lz])
+    perm = rng.permutation(n)               # global ids in random spatial order
+    pos = pos[:, perm]
+    r = r[perm]
+    p = abi.default_params()
+    p.coll_model = abi.HERTZ
+    p.integrator = abi.EULER
+    p.precision = precision
+    p.nphases = 1
+    p.max_pip = n
+    p.coupled = 1
+    p.drag_type = abi.DRAG_GIDASPOW
+    p.interp_scheme = abi.INTERP_GARG_2012
+    p.xlength, p.ylength, p.zlength = lx, ly, lz
+    p.imax, p.jmax, p.kmax = cells
+    p.desgridsearch_imax, p.desgridsearch_jmax, p.desgridsearch_kmax = dg
+    p.dtsolid = 1.0e-5
+    p.grav[:] = [0.0, -9.81, 0.0]
+    p.mew = 0.3
+    p.mew_w = 0.3
+    p.des_en_input[0] = 0.9
+    p.des_et_input[0] = 0.9
+    p.des_en_wall_input[0] = 0.9
+    p.des_et_wall_input[0] = 0.9
+    p.e_young[0] = 1.0e7
+    p.v_poisson[0] = 0.3
+    p.ew_young = 1.0e7
+    p.vw_poisson = 0.3
+    p.d_p0[0] = 1.0e-3
+    p.ro_s0[0] = 2500.0
+    p.max_radius = float(r.max())
+    p.ro_g0 = 1.2
+    p.mu_g0 = 1.8e-5
+    parts = make_particles(pos, r, 2500.0)
+    gas = frozen_gas(p)
+    return p, parts, gas
+
+
+def dem01(kn=1.0e4, en=0.9, integrator=abi.EULER):
+    """tests/dem/DEM01: one particle (d = 0.2 m, rho = 2600) released at
+    y = 0.5 in a 1 m box with default walls; KN = KN_W, EN = EN_WALL."""
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = integrator
+    p.nphases = 1
+    p.max_pip = 1
+    p.xlength = p.ylength = p.zlength = 1.0
+    p.imax = p.jmax = p.kmax = 1
+    p.desgridsearch_imax = p.desgridsearch_jmax = p.desgridsearch_kmax = 1
+    p.neighbor_search_n = 99
+    p.kn = kn
+    p.kn_w = kn
+    p.mew = 0.0
+    p.mew_w = 0.0
+    p.des_en_input[0] = en
+    p.des_en_wall_input[0] = en
+    p.d_p0[0] = 0.20
+    p.ro_s0[0] = 2600.0
+    p.max_radius = 0.1
+    parts = make_particles(np.array([[0.6], [0.5], [0.6]]), 0.1, 2600.0)
+    return p, parts, None

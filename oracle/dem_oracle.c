@@ -1,0 +1,1733 @@
+/*
+ * dem_oracle.c -- CPU restatement of MFiX 25.2.1's DEM particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Nothing in the product library
+ * links or calls this file.
+ *
+ * Each function follows the loop order and FP64 expression order of the
+ * reference routine it cites (paths relative to the reference's model/),
+ * so that a serial build is bit-reproducible and can serve as the parity
+ * oracle for the HIP path.  Fortran `a*b*c` is evaluated left to right,
+ * `x**2` / `x**3` as repeated products, DOT_PRODUCT as ((a1*b1+a2*b2)+a3*b3);
+ * the file is compiled with -ffp-contract=off so no FMA is formed.
+ *
+ * Parity pins (see tests/): DEM01 golden trajectories (Euler and
+ * Adams-Bashforth, LSD wall) from the reference's own AUTOTEST files;
+ * brute-force NSQUARE pair sets; closed-form and invariant checks for the
+ * contact laws, drag correlations and deposition (see tests/test_oracle*.py).
+ */
+#include "dem_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI_D 3.141592653589793 /* 4.D0*ATAN(1.D0), constant_mod.f:168 */
+#define SMALL_NUMBER 1.0e-15   /* param1_mod.f:26 */
+#define LARGE_NUMBER 1.0e32    /* param1_mod.f:25 */
+#define UNDEF MFIXDEM_UNDEFINED
+#define NORMAL_PARTICLE 1
+#define NONEXISTENT 0
+#define COLLISION_ARRAY_MAX 8  /* discretelement_mod.f:637 */
+#define MAXF 16                /* facets this restatement supports */
+#define MP MFIXDEM_MAX_PHASES
+
+struct orc_sys {
+  mfixdem_params p;
+  mfixdem_derived d;
+  double kt, kt_w;
+  /* fluid grid (MFiX serial: cells 1..IMAX+2, faces XE(0:IMAX+2)) */
+  int ni, nj, nk, ncell;
+  double dx, dy, dz;
+  double *xe, *yn, *zt;
+  double *vol, *vol_surr;
+  unsigned char* fluid;
+  double x_max, y_max, z_max;
+  /* DES grid */
+  int dg_isize, dg_jsize, dg_ksize;
+  int dg_ni, dg_nj, dg_nk, dg_ncell;
+  int dg_istart1, dg_iend1, dg_jstart1, dg_jend1, dg_kstart1, dg_kend1;
+  int dg_istart2, dg_iend2, dg_jstart2, dg_jend2, dg_kstart2, dg_kend2;
+  double dg_xstart, dg_ystart, dg_zstart, dg_dxinv, dg_dyinv, dg_dzinv;
+  int* dg_cnt;   /* per DES cell */
+  int* dg_off;   /* CSR offsets, dg_ncell+1 */
+  int* dg_list;  /* particle indices (0-based), ascending within a cell */
+  /* facets */
+  int nf;
+  double vert[MAXF][3][3]; /* VERTEX(v, c, nf) -> vert[nf][v][c] */
+  double norm[MAXF][3];
+  unsigned char ignore_edge[MAXF][4];
+  int* fc_count;           /* facets_at_dg count */
+  int (*fc_id)[MAXF];
+  int (*fc_dir)[MAXF];
+  double (*fc_min)[MAXF];
+  double (*fc_max)[MAXF];
+  /* particles */
+  int64_t max_pip, ld;
+  int64_t pip;
+  double *pos, *vel, *omega;          /* [c*ld + p] */
+  double *radius, *ro_sol, *pvol, *pmass, *omoi;
+  int* phase;                         /* 0-based */
+  signed char* state;
+  double *fc, *tow, *fc_last, *tow_last;
+  double *pos_old, *vel_old, *omega_old, *acc_old, *rot_acc_old;
+  int first_pass;
+  int* dg_pijk;                       /* 1-based DES cell */
+  int* pijk;                          /* [c*ld + p], c = 0..3: I,J,K,IJK (1-based) */
+  /* neighbour CSR (1-based positions, reference layout) */
+  int64_t cap;
+  int *neighbor_index, *neighbor_index_old;
+  int *neighbors, *neighbors_old;
+  double *pft, *pft_old;              /* [3*c + d] */
+  /* wall collision slots (calc_collision_wall_mod.f:651-792) */
+  int* wall_fid;                      /* [slot*ld + p] */
+  double* wall_pft;                   /* [(slot*3 + d)*ld + p] */
+  /* gas */
+  int gas_set;
+  double *u_g, *v_g, *w_g, *p_force, *ep_g, *rop_s, *rops_node;
+  int* pinc;
+  int* pic_off;
+  int* pic_list;
+  mfixdem_status st;
+  int inited;
+};
+
+/* ---------------------------------------------------------------- helpers */
+static inline double dot3(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+/* CROSS, discretelement_mod.f:906-922 */
+static inline void cross3(const double x[3], const double y[3], double c[3]) {
+  c[0] = x[1] * y[2] - x[2] * y[1];
+  c[1] = x[2] * y[0] - x[0] * y[2];
+  c[2] = x[0] * y[1] - x[1] * y[0];
+}
+static inline int fluid_ijk(const orc_sys* s, int i, int j, int k) {
+  /* FUNIJK (functions.inc:37-42), serial: J fastest; 1-based result */
+  return j + (i - 1) * s->nj + (k - 1) * s->nj * s->ni;
+}
+static inline int dg_funijk(const orc_sys* s, int i, int j, int k) {
+  /* dg_funijk (desgrid_mod.f:184-188) with istart2=jstart2=kstart2=1 */
+  return j + (i - 1) * s->dg_nj + (k - 1) * s->dg_nj * s->dg_ni;
+}
+static inline void dg_ijk_of(const orc_sys* s, int ijk, int* i, int* j, int* k) {
+  int z = ijk - 1;
+  *j = z % s->dg_nj + 1;
+  z /= s->dg_nj;
+  *i = z % s->dg_ni + 1;
+  *k = z / s->dg_ni + 1;
+}
+/* floor of a double, saturated to the int range (NaN -> very negative) */
+static inline int sat_floor(double v) {
+  if (!(v > -1.0e9)) return -1000000000;
+  if (v > 1.0e9) return 1000000000;
+  return (int)floor(v);
+}
+static inline int iofpos(const orc_sys* s, double x) {
+  return sat_floor((x - s->dg_xstart) * s->dg_dxinv) + s->dg_istart1;
+}
+static inline int jofpos(const orc_sys* s, double y) {
+  return sat_floor((y - s->dg_ystart) * s->dg_dyinv) + s->dg_jstart1;
+}
+static inline int kofpos(const orc_sys* s, double z) {
+  return sat_floor((z - s->dg_zstart) * s->dg_dzinv) + s->dg_kstart1;
+}
+static inline int imin_i(int a, int b) { return a < b ? a : b; }
+static inline int imax_i(int a, int b) { return a > b ? a : b; }
+
+int orc_omp_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------ parameter derivation */
+/* CHECK_SOLIDS_DEM_COLL_LSD, check_data/check_solids_dem.f:519-782 and
+   REF_MASS / REF_RADIUS at :1162-1163. */
+static void derive_lsd(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  int n = p->nphases, lc = 0;
+  double tcoll = UNDEF;
+  double kt_fac = (p->kt_fac == UNDEF) ? 2.0 / 7.0 : p->kt_fac;
+  double kt_w_fac = (p->kt_w_fac == UNDEF) ? 2.0 / 7.0 : p->kt_w_fac;
+  double etat_fac = (p->des_etat_fac == UNDEF) ? 0.5 : p->des_etat_fac;
+  double etat_w_fac = (p->des_etat_w_fac == UNDEF) ? 0.5 : p->des_etat_w_fac;
+  s->kt = kt_fac * p->kn;
+  s->kt_w = kt_w_fac * p->kn_w;
+  for (int m = 0; m < n; m++) {
+    double dm = p->d_p0[m];
+    double mass_m = (PI_D / 6.0) * (dm * dm * dm) * p->ro_s0[m];
+    for (int l = m; l < n; l++) {
+      double en = p->des_en_input[lc++];
+      double dl = p->d_p0[l];
+      double mass_l = (PI_D / 6.0) * (dl * dl * dl) * p->ro_s0[l];
+      double mass_eff = mass_m * mass_l / (mass_m + mass_l);
+      double etan, tc;
+      if (en != 0.0) {
+        double lg = log(en);
+        etan = 2.0 * sqrt(p->kn * mass_eff) * fabs(lg);
+        etan = etan / sqrt(PI_D * PI_D + (lg * lg));
+        double q = etan / mass_eff;
+        tc = PI_D / sqrt(p->kn / mass_eff - (q * q) / 4.0);
+      } else {
+        etan = 2.0 * sqrt(p->kn * mass_eff);
+        tc = PI_D / sqrt(p->kn / mass_eff);
+      }
+      if (tc < tcoll) tcoll = tc;
+      etan = etan / sqrt(mass_eff);
+      s->d.des_etan[m][l] = etan;
+      s->d.des_etat[m][l] = etat_fac * etan;
+      s->d.des_etan[l][m] = s->d.des_etan[m][l];
+      s->d.des_etat[l][m] = s->d.des_etat[m][l];
+    }
+    double en = p->des_en_wall_input[m];
+    double mass_eff = mass_m, etan;
+    if (en != 0.0) {
+      double lg = log(en);
+      etan = 2.0 * sqrt(p->kn_w * mass_eff) * fabs(lg);
+      etan = etan / sqrt(PI_D * PI_D + (lg * lg));
+    } else {
+      etan = 2.0 * sqrt(p->kn_w * mass_eff);
+    }
+    etan = etan / sqrt(mass_eff);
+    s->d.des_etan_wall[m] = etan;
+    s->d.des_etat_wall[m] = etat_w_fac * etan;
+  }
+  s->d.tcoll = tcoll;
+}
+
+/* CHECK_SOLIDS_DEM_COLL_HERTZ, check_data/check_solids_dem.f:798-1122 */
+static void derive_hertz(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  int n = p->nphases, lc = 0;
+  double tcoll = UNDEF;
+  double g_mod[MP];
+  double g_mod_wall = 0.5 * p->ew_young / (1.0 + p->vw_poisson);
+  for (int m = 0; m < n; m++) g_mod[m] = 0.5 * p->e_young[m] / (1.0 + p->v_poisson[m]);
+  for (int m = 0; m < n; m++) {
+    double dm = p->d_p0[m];
+    double mass_m = (PI_D / 6.0) * (dm * dm * dm) * p->ro_s0[m];
+    double rm = 0.5 * dm;
+    for (int l = m; l < n; l++) {
+      double en = p->des_en_input[lc];
+      double et = p->des_et_input[lc];
+      lc++;
+      double dl = p->d_p0[l];
+      double mass_l = (PI_D / 6.0) * (dl * dl * dl) * p->ro_s0[l];
+      double rl = 0.5 * dl;
+      double mass_eff = (mass_m * mass_l) / (mass_m + mass_l);
+      double red_mass_eff = (2.0 / 7.0) * mass_eff;
+      double r_eff = rm * rl / (rm + rl);
+      double vm = p->v_poisson[m], vl = p->v_poisson[l];
+      double e_eff = p->e_young[m] * p->e_young[l] /
+                     (p->e_young[m] * (1.0 - vl * vl) + p->e_young[l] * (1.0 - vm * vm));
+      double g_eff = g_mod[m] * g_mod[l] /
+                     (g_mod[m] * (2.0 - vl) + g_mod[l] * (2.0 - vm));
+      double hkn = (4.0 / 3.0) * sqrt(r_eff) * e_eff;
+      double hkt = 8.0 * sqrt(r_eff) * g_eff;
+      double etan, etat, tc;
+      if (en != 0.0) {
+        double lg = log(en);
+        etan = sqrt(5.0) * sqrt(hkn * mass_eff) * fabs(lg);
+        etan = etan / sqrt(PI_D * PI_D + (lg * lg));
+      } else {
+        etan = sqrt(5.0) * sqrt(hkn * mass_eff);
+      }
+      if (et != 0.0) {
+        double lg = log(et);
+        etat = sqrt(5.0) * sqrt(hkt * red_mass_eff) * fabs(lg);
+        etat = etat / sqrt(PI_D * PI_D + (lg * lg));
+      } else {
+        etat = sqrt(5.0) * sqrt(hkt * red_mass_eff);
+      }
+      if (en != 0.0) {
+        double q = etan / mass_eff;
+        tc = PI_D / sqrt(hkn / mass_eff - (q * q) / 5.0);
+      } else {
+        tc = PI_D / sqrt(hkn / mass_eff);
+      }
+      if (tc < tcoll) tcoll = tc;
+      etan = etan / sqrt(hkn * mass_eff);
+      etat = etat / sqrt(hkt * mass_eff);
+      hkn = hkn / sqrt(r_eff);
+      hkt = hkt / sqrt(r_eff);
+      s->d.hert_kn[m][l] = s->d.hert_kn[l][m] = hkn;
+      s->d.hert_kt[m][l] = s->d.hert_kt[l][m] = hkt;
+      s->d.des_etan[m][l] = s->d.des_etan[l][m] = etan;
+      s->d.des_etat[m][l] = s->d.des_etat[l][m] = etat;
+    }
+    /* wall */
+    double en = p->des_en_wall_input[m], et = p->des_et_wall_input[m];
+    double mass_eff = mass_m;
+    double red_mass_eff = (2.0 / 7.0) * mass_eff;
+    double r_eff = rm;
+    double vm = p->v_poisson[m], vw = p->vw_poisson;
+    double e_eff = p->e_young[m] * p->ew_young /
+                   (p->e_young[m] * (1.0 - vw * vw) + p->ew_young * (1.0 - vm * vm));
+    double g_eff = g_mod[m] * g_mod_wall /
+                   (g_mod[m] * (2.0 - vw) + g_mod_wall * (2.0 - vm));
+    double hkwn = (4.0 / 3.0) * sqrt(r_eff) * e_eff;
+    double hkwt = 8.0 * sqrt(r_eff) * g_eff;
+    double etan, etat;
+    if (en != 0.0) {
+      double lg = log(en);
+      etan = sqrt(5.0) * sqrt(hkwn * mass_eff) * fabs(lg);
+      etan = etan / sqrt(PI_D * PI_D + (lg * lg));
+    } else {
+      etan = sqrt(5.0) * sqrt(hkwn * mass_eff);
+    }
+    if (et != 0.0) {
+      double lg = log(et);
+      etat = sqrt(5.0) * sqrt(hkwt * red_mass_eff) * fabs(lg);
+      etat = etat / sqrt(PI_D * PI_D + (lg * lg));
+    } else {
+      etat = sqrt(5.0) * sqrt(hkwt * red_mass_eff);
+    }
+    etan = etan / sqrt(hkwn * mass_eff);
+    etat = etat / sqrt(hkwt * mass_eff);
+    hkwn = hkwn / sqrt(r_eff);
+    hkwt = hkwt / sqrt(r_eff);
+    s->d.hert_kwn[m] = hkwn;
+    s->d.hert_kwt[m] = hkwt;
+    s->d.des_etan_wall[m] = etan;
+    s->d.des_etat_wall[m] = etat;
+  }
+  s->d.tcoll = tcoll;
+}
+
+/* -------------------------------------------------------- geometry set-up */
+/* Fluid faces: CHECK_AXIS uniform DA = ALENGTH/DBLE(NA) (check_axis.f:152-156),
+   SET_GEOMETRY_DES cumulative faces (set_geometry_des.f:72-87); VOL = DX*DY*DZ
+   (set_geometry1.f:125 with X=1); VOL_SURR (make_arrays_des.f:78-101). */
+static void setup_fluid_grid(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  s->ni = p->imax + 2;
+  s->nj = p->jmax + 2;
+  s->nk = p->kmax + 2;
+  s->ncell = s->ni * s->nj * s->nk;
+  s->dx = p->xlength / (double)p->imax;
+  s->dy = p->ylength / (double)p->jmax;
+  s->dz = p->zlength / (double)p->kmax;
+  s->x_max = p->x_min + p->xlength;
+  s->y_max = p->y_min + p->ylength;
+  s->z_max = p->z_min + p->zlength;
+  s->xe = calloc(s->ni + 1, sizeof(double));
+  s->yn = calloc(s->nj + 1, sizeof(double));
+  s->zt = calloc(s->nk + 1, sizeof(double));
+  s->xe[0] = p->x_min - s->dx;
+  for (int i = 1; i <= s->ni; i++) s->xe[i] = s->xe[i - 1] + s->dx;
+  s->yn[0] = p->y_min - s->dy;
+  for (int j = 1; j <= s->nj; j++) s->yn[j] = s->yn[j - 1] + s->dy;
+  s->zt[0] = p->z_min - s->dz;
+  for (int k = 1; k <= s->nk; k++) s->zt[k] = s->zt[k - 1] + s->dz;
+  s->vol = calloc(s->ncell, sizeof(double));
+  s->vol_surr = calloc(s->ncell, sizeof(double));
+  s->fluid = calloc(s->ncell, 1);
+  for (int k = 1; k <= s->nk; k++)
+    for (int i = 1; i <= s->ni; i++)
+      for (int j = 1; j <= s->nj; j++) {
+        int ijk = fluid_ijk(s, i, j, k) - 1;
+        s->vol[ijk] = s->dx * s->dy * (1.0 * s->dz);
+        s->fluid[ijk] = (i >= 2 && i <= p->imax + 1 && j >= 2 && j <= p->jmax + 1 &&
+                         k >= 2 && k <= p->kmax + 1);
+      }
+  for (int k = 1; k <= p->kmax + 1; k++)
+    for (int j = 1; j <= p->jmax + 1; j++)
+      for (int i = 1; i <= p->imax + 1; i++) {
+        int ijk = fluid_ijk(s, i, j, k) - 1;
+        for (int kk = k; kk <= k + 1; kk++)
+          for (int jj = j; jj <= j + 1; jj++)
+            for (int ii = i; ii <= i + 1; ii++) {
+              int ijk2 = fluid_ijk(s, ii, jj, kk) - 1;
+              if (s->fluid[ijk2]) s->vol_surr[ijk] = s->vol_surr[ijk] + s->vol[ijk2];
+            }
+      }
+}
+
+/* DES grid: default size check_geometry.f:134-170; desgrid_init
+   desgrid_mod.f:572-574 (sizes) and :833-847 (xstart, dxinv). */
+static void setup_des_grid(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  double max_diam = 2.0 * p->max_radius;
+  double width = 3.0 * max_diam;
+  int gi = p->desgridsearch_imax, gj = p->desgridsearch_jmax, gk = p->desgridsearch_kmax;
+  double wx = width < s->dx ? width : s->dx;
+  double wy = width < s->dy ? width : s->dy;
+  double wz = width < s->dz ? width : s->dz;
+  if (gi <= 0) { gi = (int)(p->xlength / wx); if (gi < 1) gi = 1; }
+  if (gj <= 0) { gj = (int)(p->ylength / wy); if (gj < 1) gj = 1; }
+  if (gk <= 0) { gk = (int)(p->zlength / wz); if (gk < 1) gk = 1; }
+  double ltx = p->xlength / gi, lty = p->ylength / gj, ltz = p->zlength / gk;
+  /* NINT((xe(iend1)-xe(istart1-1))/ltempdx), iend1 = IMAX+1, istart1-1 = 1 */
+  double ex = s->xe[p->imax + 1] - s->xe[1];
+  double ey = s->yn[p->jmax + 1] - s->yn[1];
+  double ez = s->zt[p->kmax + 1] - s->zt[1];
+  s->dg_isize = (int)lround(ex / ltx);
+  s->dg_jsize = (int)lround(ey / lty);
+  s->dg_ksize = (int)lround(ez / ltz);
+  s->dg_istart1 = 2; s->dg_iend1 = s->dg_isize + 1;
+  s->dg_jstart1 = 2; s->dg_jend1 = s->dg_jsize + 1;
+  s->dg_kstart1 = 2; s->dg_kend1 = s->dg_ksize + 1;
+  s->dg_istart2 = 1; s->dg_iend2 = s->dg_isize + 2;
+  s->dg_jstart2 = 1; s->dg_jend2 = s->dg_jsize + 2;
+  s->dg_kstart2 = 1; s->dg_kend2 = s->dg_ksize + 2;
+  s->dg_ni = s->dg_isize + 2;
+  s->dg_nj = s->dg_jsize + 2;
+  s->dg_nk = s->dg_ksize + 2;
+  s->dg_ncell = s->dg_ni * s->dg_nj * s->dg_nk;
+  s->dg_xstart = s->xe[1];
+  s->dg_ystart = s->yn[1];
+  s->dg_zstart = s->zt[1];
+  s->dg_dxinv = (double)(s->dg_iend1 - s->dg_istart1 + 1) / (s->xe[p->imax + 1] - s->dg_xstart);
+  s->dg_dyinv = (double)(s->dg_jend1 - s->dg_jstart1 + 1) / (s->yn[p->jmax + 1] - s->dg_ystart);
+  s->dg_dzinv = (double)(s->dg_kend1 - s->dg_kstart1 + 1) / (s->zt[p->kmax + 1] - s->dg_zstart);
+  s->dg_cnt = calloc(s->dg_ncell, sizeof(int));
+  s->dg_off = calloc(s->dg_ncell + 1, sizeof(int));
+  s->d.dg_imax = s->dg_isize;
+  s->d.dg_jmax = s->dg_jsize;
+  s->d.dg_kmax = s->dg_ksize;
+  s->d.dg_xstart = s->dg_xstart;
+  s->d.dg_ystart = s->dg_ystart;
+  s->d.dg_zstart = s->dg_zstart;
+  s->d.dg_dxinv = s->dg_dxinv;
+  s->d.dg_dyinv = s->dg_dyinv;
+  s->d.dg_dzinv = s->dg_dzinv;
+}
+
+/* TRI_BOX_OVERLAP (stl_functions_des_mod.f:280-536): separating-axis
+   triangle/box test of Akenine-Moller, restated with the reference's
+   axis order and comparisons. */
+static int tri_box_overlap(const double c[3], const double h[3], const double vt[3][3]) {
+  double v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+  for (int i = 0; i < 3; i++) {
+    v0[i] = vt[0][i] - c[i];
+    v1[i] = vt[1][i] - c[i];
+    v2[i] = vt[2][i] - c[i];
+  }
+  for (int i = 0; i < 3; i++) {
+    e0[i] = v1[i] - v0[i];
+    e1[i] = v2[i] - v1[i];
+    e2[i] = v0[i] - v2[i];
+  }
+#define AXTEST(pa, pb, ra) do { double lo_ = (pa) < (pb) ? (pa) : (pb); \
+    double hi_ = (pa) < (pb) ? (pb) : (pa); if (lo_ > (ra) || hi_ < -(ra)) return 0; } while (0)
+  {
+    double a = e0[2], b = e0[1], fa = fabs(e0[2]), fb = fabs(e0[1]);
+    double p0 = a * v0[1] - b * v0[2], p2 = a * v2[1] - b * v2[2];
+    AXTEST(p0, p2, fa * h[1] + fb * h[2]);
+    a = e0[2]; b = e0[0]; fa = fabs(e0[2]); fb = fabs(e0[0]);
+    p0 = -a * v0[0] + b * v0[2]; p2 = -a * v2[0] + b * v2[2];
+    AXTEST(p0, p2, fa * h[0] + fb * h[2]);
+    a = e0[1]; b = e0[0]; fa = fabs(e0[1]); fb = fabs(e0[0]);
+    double p1 = a * v1[0] - b * v1[1]; p2 = a * v2[0] - b * v2[1];
+    { double lo = p2 < p1 ? p2 : p1, hi = p2 < p1 ? p1 : p2, r = fa * h[0] + fb * h[1];
+      if (lo > r || hi < -r) return 0; }
+  }
+  {
+    double a = e1[2], b = e1[1], fa = fabs(e1[2]), fb = fabs(e1[1]);
+    double p0 = a * v0[1] - b * v0[2], p2 = a * v2[1] - b * v2[2];
+    AXTEST(p0, p2, fa * h[1] + fb * h[2]);
+    a = e1[2]; b = e1[0]; fa = fabs(e1[2]); fb = fabs(e1[0]);
+    p0 = -a * v0[0] + b * v0[2]; p2 = -a * v2[0] + b * v2[2];
+    AXTEST(p0, p2, fa * h[0] + fb * h[2]);
+    a = e1[1]; b = e1[0]; fa = fabs(e1[1]); fb = fabs(e1[0]);
+    p0 = a * v0[0] - b * v0[1]; double p1 = a * v1[0] - b * v1[1];
+    AXTEST(p0, p1, fa * h[0] + fb * h[1]);
+  }
+  {
+    double a = e2[2], b = e2[1], fa = fabs(e2[2]), fb = fabs(e2[1]);
+    double p0 = a * v0[1] - b * v0[2], p1 = a * v1[1] - b * v1[2];
+    AXTEST(p0, p1, fa * h[1] + fb * h[2]);
+    a = e2[2]; b = e2[0]; fa = fabs(e2[2]); fb = fabs(e2[0]);
+    p0 = -a * v0[0] + b * v0[2]; p1 = -a * v1[0] + b * v1[2];
+    AXTEST(p0, p1, fa * h[0] + fb * h[2]);
+    a = e2[1]; b = e2[0]; fa = fabs(e2[1]); fb = fabs(e2[0]);
+    p1 = a * v1[0] - b * v1[1]; double p2 = a * v2[0] - b * v2[1];
+    { double lo = p2 < p1 ? p2 : p1, hi = p2 < p1 ? p1 : p2, r = fa * h[0] + fb * h[1];
+      if (lo > r || hi < -r) return 0; }
+  }
+#undef AXTEST
+  for (int i = 0; i < 3; i++) {
+    double mn = v0[i], mx = v0[i];
+    if (v1[i] < mn) mn = v1[i];
+    if (v2[i] < mn) mn = v2[i];
+    if (v1[i] > mx) mx = v1[i];
+    if (v2[i] > mx) mx = v2[i];
+    if (mn > h[i] || mx < -h[i]) return 0;
+  }
+  double nrm[3];
+  nrm[0] = e0[1] * e1[2] - e0[2] * e1[1];
+  nrm[1] = e0[2] * e1[0] - e0[0] * e1[2];
+  nrm[2] = e0[0] * e1[1] - e0[1] * e1[0];
+  double vmin[3], vmax[3];
+  for (int q = 0; q < 3; q++) {
+    double v = v0[q];
+    if (nrm[q] > 0.0) {
+      vmin[q] = -h[q] - v;
+      vmax[q] = h[q] - v;
+    } else {
+      vmin[q] = h[q] - v;
+      vmax[q] = -h[q] - v;
+    }
+  }
+  if (dot3(nrm, vmin) > 0.0) return 0;
+  if (dot3(nrm, vmax) >= 0.0) return 1;
+  return 0;
+}
+
+/* CONVERT_DEFAULT_WALLS_TO_STL (stl_preproc_des_mod.f:973-1047): one large
+   triangle per box face; BIN_FACETS_TO_DG (:135-203) with ADD_FACET_FOR_DES
+   (:210-260) and ADD_FACET (:268-345); FLAG_FACETS_WHERE_EDGE_COLLISION_IS_
+   IGNORED (:362-480). */
+static void setup_facets(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  double xn = p->x_min, xx = s->x_max, yn = p->y_min, yx = s->y_max, zn = p->z_min, zx = s->z_max;
+  double XL = p->xlength, YL = p->ylength, ZL = p->zlength;
+  int n = 0;
+#define SETF(a0, a1, a2, b0, b1, b2, c0, c1, c2, n0, n1, n2) do { \
+    s->vert[n][0][0] = a0; s->vert[n][0][1] = a1; s->vert[n][0][2] = a2; \
+    s->vert[n][1][0] = b0; s->vert[n][1][1] = b1; s->vert[n][1][2] = b2; \
+    s->vert[n][2][0] = c0; s->vert[n][2][1] = c1; s->vert[n][2][2] = c2; \
+    s->norm[n][0] = n0; s->norm[n][1] = n1; s->norm[n][2] = n2; n++; } while (0)
+  SETF(xn, yn, zn, xn, yx + YL, zn, xn, yn, zx + ZL, 1.0, 0.0, 0.0);   /* west   */
+  SETF(xx, yn, zn, xx, yx + YL, zn, xx, yn, zx + ZL, -1.0, 0.0, 0.0);  /* east   */
+  SETF(xn, yn, zn, xx + XL, yn, zn, xn, yn, zx + ZL, 0.0, 1.0, 0.0);   /* south  */
+  SETF(xn, yx, zn, xx + XL, yx, zn, xn, yx, zx + ZL, 0.0, -1.0, 0.0);  /* north  */
+  SETF(xn, yn, zn, xx + XL, yn, zn, xn, yx + YL, zn, 0.0, 0.0, 1.0);   /* bottom */
+  SETF(xn, yn, zx, xx + XL, yn, zx, xn, yx + YL, zx, 0.0, 0.0, -1.0);  /* top    */
+#undef SETF
+  s->nf = n;
+  s->d.n_facets = n;
+  s->fc_count = calloc(s->dg_ncell, sizeof(int));
+  s->fc_id = calloc(s->dg_ncell, sizeof(*s->fc_id));
+  s->fc_dir = calloc(s->dg_ncell, sizeof(*s->fc_dir));
+  s->fc_min = calloc(s->dg_ncell, sizeof(*s->fc_min));
+  s->fc_max = calloc(s->dg_ncell, sizeof(*s->fc_max));
+  const double tol_stl = 1.0e-6; /* cartesian_grid_init_namelist.f:748 */
+  double buffer = 1.1 * p->max_radius;
+  double ldx = 1.0 / s->dg_dxinv, ldy = 1.0 / s->dg_dyinv, ldz = 1.0 / s->dg_dzinv;
+  for (int nn = 0; nn < n; nn++) {
+    double X1 = s->vert[nn][0][0], X2 = X1, Y1 = s->vert[nn][0][1], Y2 = Y1,
+           Z1 = s->vert[nn][0][2], Z2 = Z1;
+    for (int v = 1; v < 3; v++) {
+      if (s->vert[nn][v][0] < X1) X1 = s->vert[nn][v][0];
+      if (s->vert[nn][v][0] > X2) X2 = s->vert[nn][v][0];
+      if (s->vert[nn][v][1] < Y1) Y1 = s->vert[nn][v][1];
+      if (s->vert[nn][v][1] > Y2) Y2 = s->vert[nn][v][1];
+      if (s->vert[nn][v][2] < Z1) Z1 = s->vert[nn][v][2];
+      if (s->vert[nn][v][2] > Z2) Z2 = s->vert[nn][v][2];
+    }
+    int I1 = s->dg_iend2, I2 = s->dg_istart2;
+    if (X2 >= p->x_min - tol_stl && X1 <= s->x_max + tol_stl) {
+      I1 = imax_i(iofpos(s, X1) - 1, s->dg_istart2);
+      I2 = imin_i(iofpos(s, X2) + 1, s->dg_iend2);
+    }
+    int J1 = s->dg_jend2, J2 = s->dg_jstart2;
+    if (Y2 >= p->y_min - tol_stl && Y1 <= s->y_max + tol_stl) {
+      J1 = imax_i(jofpos(s, Y1) - 1, s->dg_jstart2);
+      J2 = imin_i(jofpos(s, Y2) + 1, s->dg_jend2);
+    }
+    int K1 = s->dg_kend2, K2 = s->dg_kstart2;
+    if (Z2 >= p->z_min - tol_stl && Z1 <= s->z_max + tol_stl) {
+      K1 = imax_i(kofpos(s, Z1) - 1, s->dg_kstart2);
+      K2 = imin_i(kofpos(s, Z2) + 1, s->dg_kend2);
+    }
+    for (int kk = K1; kk <= K2; kk++)
+      for (int jj = J1; jj <= J2; jj++)
+        for (int ii = I1; ii <= I2; ii++) {
+          int ijk = dg_funijk(s, ii, jj, kk) - 1;
+          double c[3], h[3];
+          c[0] = s->dg_xstart + ((double)(ii - s->dg_istart1) + 0.5) * ldx;
+          h[0] = 0.5 * ldx + buffer;
+          c[1] = s->dg_ystart + ((double)(jj - s->dg_jstart1) + 0.5) * ldy;
+          h[1] = 0.5 * ldy + buffer;
+          c[2] = s->dg_zstart + ((double)(kk - s->dg_kstart1) + 0.5) * ldz;
+          h[2] = 0.5 * ldz + buffer;
+          if (!tri_box_overlap(c, h, s->vert[nn])) continue;
+          int dup = 0;
+          for (int q = 0; q < s->fc_count[ijk]; q++)
+            if (s->fc_id[ijk][q] == nn) dup = 1;
+          if (dup) continue;
+          int q = s->fc_count[ijk]++;
+          s->fc_id[ijk][q] = nn;
+          double smallest = 3.4028234663852886e38; /* HUGE(0.0) */
+          for (int ax = 0; ax < 3; ax++) {
+            double mn = s->vert[nn][0][ax], mx = mn;
+            for (int v = 1; v < 3; v++) {
+              if (s->vert[nn][v][ax] < mn) mn = s->vert[nn][v][ax];
+              if (s->vert[nn][v][ax] > mx) mx = s->vert[nn][v][ax];
+            }
+            if (fabs(mx - mn) < smallest) {
+              s->fc_dir[ijk][q] = ax;
+              s->fc_min[ijk][q] = mn;
+              s->fc_max[ijk][q] = mx;
+              smallest = fabs(mx - mn);
+            }
+          }
+        }
+  }
+  /* edge-collision flags */
+  memset(s->ignore_edge, 0, sizeof(s->ignore_edge));
+  const double tol_edge = 1.0e-9;
+  double cos_nb = cos(5.0 / 180.0 * PI_D) + 1.0e-6;
+  for (int ijk = 0; ijk < s->dg_ncell; ijk++) {
+    int cnt = s->fc_count[ijk];
+    for (int f1 = 0; f1 < cnt; f1++) {
+      int fid = s->fc_id[ijk][f1];
+      double ec[3][3];
+      for (int c = 0; c < 3; c++) {
+        ec[0][c] = 0.5 * (s->vert[fid][0][c] + s->vert[fid][1][c]);
+        ec[1][c] = 0.5 * (s->vert[fid][0][c] + s->vert[fid][2][c]);
+        ec[2][c] = 0.5 * (s->vert[fid][1][c] + s->vert[fid][2][c]);
+      }
+      for (int e = 0; e < 3; e++)
+        for (int f2 = 0; f2 < cnt; f2++) {
+          if (f1 == f2) continue;
+          int nb = s->fc_id[ijk][f2];
+          double nec[3][3];
+          for (int c = 0; c < 3; c++) {
+            nec[0][c] = 0.5 * (s->vert[nb][0][c] + s->vert[nb][1][c]);
+            nec[1][c] = 0.5 * (s->vert[nb][0][c] + s->vert[nb][2][c]);
+            nec[2][c] = 0.5 * (s->vert[nb][1][c] + s->vert[nb][2][c]);
+          }
+          for (int ne = 0; ne < 3; ne++) {
+            double dd[3] = {nec[ne][0] - ec[e][0], nec[ne][1] - ec[e][1], nec[ne][2] - ec[e][2]};
+            if (dot3(dd, dd) < tol_edge) {
+              if (dot3(s->norm[fid], s->norm[nb]) > cos_nb) {
+                s->ignore_edge[fid][e + 1] = 1;
+                s->ignore_edge[nb][ne + 1] = 1;
+              }
+            }
+          }
+        }
+    }
+  }
+}
+
+/* --------------------------------------------------------------- create */
+orc_sys* orc_create(const mfixdem_params* p) {
+  if (!p || p->nphases < 1 || p->nphases > MP || p->max_pip < 1) return NULL;
+  if (p->imax < 1 || p->jmax < 1 || p->kmax < 1) return NULL;
+  orc_sys* s = calloc(1, sizeof(orc_sys));
+  s->p = *p;
+  if (p->coll_model == MFIXDEM_HERTZ) derive_hertz(s);
+  else derive_lsd(s);
+  s->d.kt = s->kt;
+  s->d.kt_w = s->kt_w;
+  s->d.dtsolid = (p->dtsolid == UNDEF) ? s->d.tcoll / p->dtsolid_factor : p->dtsolid;
+  setup_fluid_grid(s);
+  setup_des_grid(s);
+  setup_facets(s);
+  int64_t ld = p->max_pip;
+  s->max_pip = ld;
+  s->ld = ld;
+#define A(ptr, cnt, T) ptr = (T*)calloc((size_t)(cnt), sizeof(T))
+  A(s->pos, 3 * ld, double); A(s->vel, 3 * ld, double); A(s->omega, 3 * ld, double);
+  A(s->radius, ld, double); A(s->ro_sol, ld, double); A(s->pvol, ld, double);
+  A(s->pmass, ld, double); A(s->omoi, ld, double);
+  A(s->phase, ld, int); A(s->state, ld, signed char);
+  A(s->fc, 3 * ld, double); A(s->tow, 3 * ld, double);
+  A(s->fc_last, 3 * ld, double); A(s->tow_last, 3 * ld, double);
+  A(s->pos_old, 3 * ld, double); A(s->vel_old, 3 * ld, double); A(s->omega_old, 3 * ld, double);
+  A(s->acc_old, 3 * ld, double); A(s->rot_acc_old, 3 * ld, double);
+  A(s->dg_pijk, ld, int); A(s->pijk, 4 * ld, int);
+  A(s->dg_list, ld, int);
+  A(s->neighbor_index, ld, int); A(s->neighbor_index_old, ld, int);
+  s->cap = 4 * ld + 16;
+  A(s->neighbors, s->cap, int); A(s->neighbors_old, s->cap, int);
+  A(s->pft, 3 * s->cap, double); A(s->pft_old, 3 * s->cap, double);
+  A(s->wall_fid, COLLISION_ARRAY_MAX * ld, int);
+  A(s->wall_pft, 3 * COLLISION_ARRAY_MAX * ld, double);
+  A(s->u_g, s->ncell, double); A(s->v_g, s->ncell, double); A(s->w_g, s->ncell, double);
+  A(s->p_force, 3 * s->ncell, double); A(s->ep_g, s->ncell, double);
+  A(s->rop_s, (size_t)s->ncell * p->nphases, double);
+  A(s->rops_node, (size_t)s->ncell * p->nphases, double);
+  A(s->pinc, s->ncell, int); A(s->pic_off, s->ncell + 1, int); A(s->pic_list, ld, int);
+#undef A
+  for (int64_t q = 0; q < COLLISION_ARRAY_MAX * ld; q++) s->wall_fid[q] = -1;
+  for (int c = 0; c < s->ncell; c++) s->ep_g[c] = 1.0;
+  s->first_pass = 1;
+  return s;
+}
+
+void orc_destroy(orc_sys* s) {
+  if (!s) return;
+  void* ptrs[] = {s->xe, s->yn, s->zt, s->vol, s->vol_surr, s->fluid, s->dg_cnt, s->dg_off,
+                  s->dg_list, s->fc_count, s->fc_id, s->fc_dir, s->fc_min, s->fc_max,
+                  s->pos, s->vel, s->omega, s->radius, s->ro_sol, s->pvol, s->pmass,
+                  s->omoi, s->phase, s->state, s->fc, s->tow, s->fc_last, s->tow_last,
+                  s->pos_old, s->vel_old, s->omega_old, s->acc_old, s->rot_acc_old,
+                  s->dg_pijk, s->pijk, s->neighbor_index, s->neighbor_index_old,
+                  s->neighbors, s->neighbors_old, s->pft, s->pft_old, s->wall_fid,
+                  s->wall_pft, s->u_g, s->v_g, s->w_g, s->p_force, s->ep_g, s->rop_s,
+                  s->rops_node, s->pinc, s->pic_off, s->pic_list};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  free(s);
+}
+
+int orc_get_derived(const orc_sys* s, mfixdem_derived* out) {
+  if (!s || !out) return MFIXDEM_ERR_ARG;
+  *out = s->d;
+  return 0;
+}
+double orc_dtsolid(const orc_sys* s) { return s->d.dtsolid; }
+
+/* Per-particle PVOL, PMASS, OMOI: make_arrays_des.f:288-291. */
+int orc_set_particles(orc_sys* s, int64_t n, const double* pos, const double* vel,
+                      const double* omega, const double* radius, const double* ro_sol,
+                      const int32_t* phase, int64_t ld) {
+  if (!s || n < 0 || n > s->max_pip || ld < n) return MFIXDEM_ERR_ARG;
+  int64_t L = s->ld;
+  for (int64_t q = 0; q < n; q++) {
+    for (int c = 0; c < 3; c++) {
+      s->pos[c * L + q] = pos[c * ld + q];
+      s->vel[c * L + q] = vel ? vel[c * ld + q] : 0.0;
+      s->omega[c * L + q] = omega ? omega[c * ld + q] : 0.0;
+    }
+    double r = radius[q];
+    s->radius[q] = r;
+    s->ro_sol[q] = ro_sol[q];
+    s->phase[q] = phase ? phase[q] - 1 : 0;
+    if (s->phase[q] < 0 || s->phase[q] >= s->p.nphases) return MFIXDEM_ERR_ARG;
+    s->pvol[q] = (4.0 / 3.0) * PI_D * (r * r * r);
+    s->pmass[q] = s->pvol[q] * s->ro_sol[q];
+    s->omoi[q] = 2.5 / (s->pmass[q] * (r * r));
+    s->state[q] = NORMAL_PARTICLE;
+  }
+  for (int64_t q = n; q < s->max_pip; q++) s->state[q] = NONEXISTENT;
+  s->pip = n;
+  s->first_pass = 1;
+  s->inited = 0;
+  return 0;
+}
+
+int orc_get_particles(const orc_sys* s, double* pos, double* vel, double* omega,
+                      int8_t* state, int64_t ld) {
+  int64_t L = s->ld, n = s->max_pip;
+  if (ld < n) n = ld;
+  for (int64_t q = 0; q < n; q++) {
+    for (int c = 0; c < 3; c++) {
+      if (pos) pos[c * ld + q] = s->pos[c * L + q];
+      if (vel) vel[c * ld + q] = s->vel[c * L + q];
+      if (omega) omega[c * ld + q] = s->omega[c * L + q];
+    }
+    if (state) state[q] = s->state[q];
+  }
+  return 0;
+}
+
+int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g, const double* w_g,
+                const double* p_force) {
+  for (int c = 0; c < s->ncell; c++) {
+    s->u_g[c] = u_g[c];
+    s->v_g[c] = v_g[c];
+    s->w_g[c] = w_g[c];
+    for (int d = 0; d < 3; d++) s->p_force[3 * c + d] = p_force ? p_force[3 * c + d] : 0.0;
+  }
+  s->gas_set = 1;
+  return 0;
+}
+
+int orc_get_gas(const orc_sys* s, double* ep_g, double* rop_s) {
+  for (int c = 0; c < s->ncell; c++) {
+    if (ep_g) ep_g[c] = s->ep_g[c];
+    if (rop_s)
+      for (int m = 0; m < s->p.nphases; m++) rop_s[(size_t)m * s->ncell + c] = s->rop_s[(size_t)m * s->ncell + c];
+  }
+  return 0;
+}
+
+/* --------------------------------------------------------- DESGRID_PIC */
+/* desgrid_mod.f:877-995, serial fill order (ascending particle index). */
+int orc_desgrid_pic(orc_sys* s) {
+  int64_t L = s->ld;
+  memset(s->dg_cnt, 0, sizeof(int) * s->dg_ncell);
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] == NONEXISTENT) continue;
+    int li = imin_i(s->dg_iend2, imax_i(s->dg_istart2, iofpos(s, s->pos[q])));
+    int lj = imin_i(s->dg_jend2, imax_i(s->dg_jstart2, jofpos(s, s->pos[L + q])));
+    int lk = imin_i(s->dg_kend2, imax_i(s->dg_kstart2, kofpos(s, s->pos[2 * L + q])));
+    s->dg_pijk[q] = dg_funijk(s, li, lj, lk);
+    s->dg_cnt[s->dg_pijk[q] - 1]++;
+  }
+  s->dg_off[0] = 0;
+  for (int c = 0; c < s->dg_ncell; c++) s->dg_off[c + 1] = s->dg_off[c] + s->dg_cnt[c];
+  int* cur = malloc(sizeof(int) * s->dg_ncell);
+  memcpy(cur, s->dg_off, sizeof(int) * s->dg_ncell);
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] == NONEXISTENT) continue;
+    s->dg_list[cur[s->dg_pijk[q] - 1]++] = (int)q;
+  }
+  free(cur);
+  return 0;
+}
+
+static void delete_particle(orc_sys* s, int64_t q) {
+  /* DELETE_PARTICLE, mass_outflow_dem.f:201-280 */
+  int64_t L = s->ld;
+  for (int c = 0; c < 3; c++) {
+    s->pos[c * L + q] = 0.0;
+    s->vel[c * L + q] = 0.0;
+    s->omega[c * L + q] = 0.0;
+    s->pos_old[c * L + q] = 0.0;
+    s->vel_old[c * L + q] = 0.0;
+    s->omega_old[c * L + q] = 0.0;
+    s->fc[c * L + q] = 0.0;
+    s->tow[c * L + q] = 0.0;
+  }
+  s->radius[q] = 0.0;
+  s->pmass[q] = 3.4028234663852886e38;
+  s->pvol[q] = 3.4028234663852886e38;
+  s->ro_sol[q] = 0.0;
+  s->omoi[q] = 0.0;
+  for (int k = 0; k < COLLISION_ARRAY_MAX; k++) s->wall_fid[k * L + q] = -1;
+  s->state[q] = NONEXISTENT;
+  s->pip--;
+}
+
+/* ------------------------------------------------- DESGRID_NEIGH_BUILD */
+static void add_pair(orc_sys* s, int64_t ii, int jj) {
+  /* add_pair / NEIGHBOR_GROW, des_allocate_mod.f:555-668 */
+  int64_t pos1 = s->neighbor_index[ii];
+  if (pos1 > s->cap) {
+    int64_t ncap = s->cap;
+    while (ncap < pos1) ncap *= 2;
+    s->neighbors = realloc(s->neighbors, sizeof(int) * ncap);
+    s->neighbors_old = realloc(s->neighbors_old, sizeof(int) * ncap);
+    s->pft = realloc(s->pft, sizeof(double) * 3 * ncap);
+    s->pft_old = realloc(s->pft_old, sizeof(double) * 3 * ncap);
+    for (int64_t c = s->cap; c < ncap; c++) {
+      s->neighbors[c] = 0;
+      s->neighbors_old[c] = 0;
+      for (int d = 0; d < 3; d++) s->pft[3 * c + d] = s->pft_old[3 * c + d] = 0.0;
+    }
+    s->cap = ncap;
+  }
+  s->neighbors[pos1 - 1] = jj;
+  s->neighbor_index[ii] = (int)(pos1 + 1);
+}
+
+/* candidate pairs of one row particle, in the reference's stencil order
+   (desgrid_mod.f:1106-1214); returns the number written into out[] */
+static int row_candidates(orc_sys* s, int64_t q, int* out, int* rogue) {
+  int64_t L = s->ld;
+  int lijk = s->dg_pijk[q];
+  int lic, ljc, lkc;
+  dg_ijk_of(s, lijk, &lic, &ljc, &lkc);
+  if (lic < s->dg_istart1 || lic > s->dg_iend1 || ljc < s->dg_jstart1 ||
+      ljc > s->dg_jend1 || lkc < s->dg_kstart1 || lkc > s->dg_kend1) {
+    *rogue = 1;
+    return 0;
+  }
+  int il = 1, iu = 1, jl = 1, ju = 1, kl = 1, ku = 1;
+  double px = s->pos[q], py = s->pos[L + q], pz = s->pos[2 * L + q];
+  double frlm = s->p.factor_rlm, mr = s->p.max_radius;
+  int coarse_x = 1.0 > 2.0 * mr * frlm * s->dg_dxinv;
+  int coarse_y = 1.0 > 2.0 * mr * frlm * s->dg_dyinv;
+  int coarse_z = 1.0 > 2.0 * mr * frlm * s->dg_dzinv;
+  if (coarse_x) {
+    double t = (px - s->dg_xstart) * s->dg_dxinv;
+    if (t - floor(t) >= 0.5) il = 0; else iu = 0;
+  }
+  if (coarse_y) {
+    double t = (py - s->dg_ystart) * s->dg_dyinv;
+    if (t - floor(t) >= 0.5) jl = 0; else ju = 0;
+  }
+  if (coarse_z) {
+    double t = (pz - s->dg_zstart) * s->dg_dzinv;
+    if (t - floor(t) >= 0.5) kl = 0; else ku = 0;
+  }
+  int n = 0;
+  for (int lk = lkc - kl; lk <= lkc + ku; lk++)
+    for (int lj = ljc - jl; lj <= ljc + ju; lj++)
+      for (int li = lic - il; li <= lic + iu; li++) {
+        int c = dg_funijk(s, li, lj, lk) - 1;
+        for (int t = s->dg_off[c]; t < s->dg_off[c + 1]; t++) {
+          int nb = s->dg_list[t];
+          if (nb == q) continue;
+          if (nb < q && s->state[nb] == NORMAL_PARTICLE) continue;
+          if (s->state[nb] == NONEXISTENT) continue;
+          double sr = frlm * (s->radius[q] + s->radius[nb]);
+          double d0 = px - s->pos[nb], d1 = py - s->pos[L + nb], d2 = pz - s->pos[2 * L + nb];
+          double ds = d0 * d0 + d1 * d1 + d2 * d2;
+          if (ds > sr * sr) continue;
+          out[n++] = nb + 1;
+        }
+      }
+  return n;
+}
+
+static int desgrid_neigh_build(orc_sys* s) {
+  int* buf = malloc(sizeof(int) * (s->max_pip + 1));
+  int do_exit = 0;
+  /* NOTE: the reference never clears do_exit inside the loop
+     (desgrid_mod.f:1096-1104): after the first rogue particle every later
+     row is left empty.  Restated as is. */
+#ifdef _OPENMP
+  int nth = omp_get_max_threads();
+  if (nth > 1) {
+    /* row search in parallel chunks, merged in ascending particle order as
+       the reference's ordered merge (desgrid_mod.f:1219-1238) does */
+    int64_t np = s->max_pip;
+    int* cnt = calloc(np, sizeof(int));
+    int* rog = calloc(np, sizeof(int));
+    int** rows = calloc(np, sizeof(int*));
+#pragma omp parallel
+    {
+      int* lb = malloc(sizeof(int) * (s->max_pip + 1));
+#pragma omp for schedule(static)
+      for (int64_t q = 0; q < np; q++) {
+        if (s->state[q] != NORMAL_PARTICLE) continue;
+        int rg = 0;
+        int n = row_candidates(s, q, lb, &rg);
+        rog[q] = rg;
+        cnt[q] = n;
+        if (n) {
+          rows[q] = malloc(sizeof(int) * n);
+          memcpy(rows[q], lb, sizeof(int) * n);
+        }
+      }
+      free(lb);
+    }
+    for (int64_t q = 0; q < np; q++) {
+      if (s->neighbor_index[q] == 0) s->neighbor_index[q] = (q == 0) ? 1 : s->neighbor_index[q - 1];
+      if (s->state[q] != NORMAL_PARTICLE) { free(rows[q]); continue; }
+      if (rog[q]) {
+        do_exit = 1;
+        s->st.rogue_deleted++;
+        if (s->p.remove_rogue) delete_particle(s, q);
+      }
+      if (!do_exit)
+        for (int t = 0; t < cnt[q]; t++) add_pair(s, q, rows[q][t]);
+      free(rows[q]);
+    }
+    free(cnt); free(rog); free(rows); free(buf);
+    return (do_exit && !s->p.remove_rogue) ? MFIXDEM_ERR_ROGUE : 0;
+  }
+#endif
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->neighbor_index[q] == 0) s->neighbor_index[q] = (q == 0) ? 1 : s->neighbor_index[q - 1];
+    if (s->state[q] != NORMAL_PARTICLE) continue;
+    int rg = 0;
+    int n = row_candidates(s, q, buf, &rg);
+    if (rg) {
+      do_exit = 1;
+      s->st.rogue_deleted++;
+      if (s->p.remove_rogue) delete_particle(s, q);
+    }
+    if (do_exit) continue;
+    for (int t = 0; t < n; t++) add_pair(s, q, buf[t]);
+  }
+  free(buf);
+  return (do_exit && !s->p.remove_rogue) ? MFIXDEM_ERR_ROGUE : 0;
+}
+
+/* NEIGHBOUR, neighbour.f:21-126 (DES_NEIGHBOR_SEARCH = 4). */
+int orc_neighbour(orc_sys* s) {
+  int64_t np = s->max_pip;
+  memcpy(s->neighbor_index_old, s->neighbor_index, sizeof(int) * np);
+  memcpy(s->neighbors_old, s->neighbors, sizeof(int) * s->cap);
+  memcpy(s->pft_old, s->pft, sizeof(double) * 3 * s->cap);
+  memset(s->neighbor_index, 0, sizeof(int) * np);
+  int rc = desgrid_neigh_build(s);
+  /* the arrays may have grown; old copies were sized before */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (int64_t ll = 0; ll < np; ll++) {
+    int cs = (ll > 0) ? s->neighbor_index[ll - 1] : 1;
+    int ce = s->neighbor_index[ll];
+    int cso = (ll > 0) ? s->neighbor_index_old[ll - 1] : 1;
+    int ceo = s->neighbor_index_old[ll];
+    for (int cc = cs; cc < ce; cc++) {
+      int found = 0;
+      for (int co = cso; co < ceo; co++) {
+        if (s->neighbors[cc - 1] == s->neighbors_old[co - 1]) {
+          for (int d = 0; d < 3; d++) s->pft[3 * (cc - 1) + d] = s->pft_old[3 * (co - 1) + d];
+          found = 1;
+          break;
+        }
+      }
+      if (!found)
+        for (int d = 0; d < 3; d++) s->pft[3 * (cc - 1) + d] = 0.0;
+    }
+  }
+  s->st.rebuilds++;
+  return rc;
+}
+
+int64_t orc_nsquare_pairs(const orc_sys* s, int32_t* pairs, int64_t capacity) {
+  /* NSQUARE criterion (nsquare.f:59-60,131-133) without periodic images */
+  int64_t L = s->ld, n = 0;
+  double frlm = s->p.factor_rlm;
+  for (int64_t i = 0; i < s->max_pip; i++) {
+    if (s->state[i] != NORMAL_PARTICLE) continue;
+    for (int64_t j = i + 1; j < s->max_pip; j++) {
+      if (s->state[j] == NONEXISTENT) continue;
+      double sr = frlm * (s->radius[i] + s->radius[j]);
+      double d0 = s->pos[i] - s->pos[j], d1 = s->pos[L + i] - s->pos[L + j],
+             d2 = s->pos[2 * L + i] - s->pos[2 * L + j];
+      double ds = d0 * d0 + d1 * d1 + d2 * d2;
+      if (ds > sr * sr) continue;
+      if (n < capacity) {
+        pairs[2 * n] = (int32_t)(i + 1);
+        pairs[2 * n + 1] = (int32_t)(j + 1);
+      }
+      n++;
+    }
+  }
+  return n;
+}
+
+/* ----------------------------------------------------- wall contact */
+/* ClosestPtPointTriangle, stl_functions_des_mod.f:25-160 */
+static void closest_pt_tri(const orc_sys* s, int nf, const double pp[3], double cp[3],
+                           int* pe, int* edge) {
+  const double(*v)[3] = s->vert[nf];
+  double ab[3], ac[3], ap[3], bp[3], cpv[3];
+  *pe = 0;
+  *edge = 0;
+  for (int i = 0; i < 3; i++) {
+    ab[i] = v[1][i] - v[0][i];
+    ac[i] = v[2][i] - v[0][i];
+    ap[i] = pp[i] - v[0][i];
+  }
+  double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 < 0.0 && d2 < 0.0) {
+    for (int i = 0; i < 3; i++) cp[i] = v[0][i];
+    *pe = 1; *edge = 1;
+    if (s->ignore_edge[nf][1]) cp[0] = UNDEF;
+    return;
+  }
+  for (int i = 0; i < 3; i++) bp[i] = pp[i] - v[1][i];
+  double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 > 0.0 && d4 < d3) {
+    for (int i = 0; i < 3; i++) cp[i] = v[1][i];
+    *pe = 1; *edge = 1;
+    if (s->ignore_edge[nf][1]) cp[0] = UNDEF;
+    return;
+  }
+  double vc = d1 * d4 - d3 * d2;
+  if (vc < 0.0 && d1 > 0.0 && d3 < 0.0) {
+    double vv = d1 / (d1 - d3);
+    for (int i = 0; i < 3; i++) cp[i] = v[0][i] + vv * ab[i];
+    *pe = 1; *edge = 1;
+    if (s->ignore_edge[nf][1]) cp[0] = UNDEF;
+    return;
+  }
+  for (int i = 0; i < 3; i++) cpv[i] = pp[i] - v[2][i];
+  double d5 = dot3(ab, cpv), d6 = dot3(ac, cpv);
+  if (d6 > 0.0 && d5 < d6) {
+    for (int i = 0; i < 3; i++) cp[i] = v[2][i];
+    *pe = 1; *edge = 2;
+    if (s->ignore_edge[nf][2]) cp[0] = UNDEF;
+    return;
+  }
+  double vb = d5 * d2 - d1 * d6;
+  if (vb < 0.0 && d2 > 0.0 && d6 < 0.0) {
+    double w = d2 / (d2 - d6);
+    for (int i = 0; i < 3; i++) cp[i] = v[0][i] + w * ac[i];
+    *pe = 1; *edge = 2;
+    if (s->ignore_edge[nf][2]) cp[0] = UNDEF;
+    return;
+  }
+  double va = d3 * d6 - d5 * d4;
+  if (va < 0.0 && (d4 - d3) > 0.0 && (d5 - d6) > 0.0) {
+    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int i = 0; i < 3; i++) cp[i] = v[1][i] + w * (v[2][i] - v[1][i]);
+    *pe = 1; *edge = 3;
+    if (s->ignore_edge[nf][3]) cp[0] = UNDEF;
+    return;
+  }
+  double denom = 1.0 / (va + vb + vc);
+  double vv = vb * denom, w = vc * denom;
+  for (int i = 0; i < 3; i++) cp[i] = v[0][i] + ab[i] * vv + ac[i] * w;
+}
+
+static void remove_collision(orc_sys* s, int64_t q, int nf) {
+  int64_t L = s->ld;
+  for (int cc = 0; cc < COLLISION_ARRAY_MAX; cc++)
+    if (s->wall_fid[cc * L + q] == nf) {
+      s->wall_fid[cc * L + q] = -1;
+      return;
+    }
+}
+
+static void get_collision(orc_sys* s, int64_t q, int nf, double out[3]) {
+  int64_t L = s->ld;
+  int free_index = -1;
+  for (int cc = 0; cc < COLLISION_ARRAY_MAX; cc++) {
+    if (nf == s->wall_fid[cc * L + q]) {
+      for (int d = 0; d < 3; d++) out[d] = s->wall_pft[(cc * 3 + d) * L + q];
+      return;
+    } else if (s->wall_fid[cc * L + q] == -1) {
+      free_index = cc;
+    }
+  }
+  if (free_index == -1) {
+    int cell = s->dg_pijk[q] - 1;
+    for (int cc = 0; cc < COLLISION_ARRAY_MAX && free_index == -1; cc++) {
+      int listed = 0;
+      for (int lc = 0; lc < s->fc_count[cell]; lc++)
+        if (s->wall_fid[cc * L + q] == s->fc_id[cell][lc]) listed = 1;
+      if (!listed) free_index = cc;
+    }
+  }
+  if (free_index == -1) free_index = COLLISION_ARRAY_MAX - 1; /* facets <= 8: unreachable */
+  s->wall_fid[free_index * L + q] = nf;
+  for (int d = 0; d < 3; d++) {
+    s->wall_pft[(free_index * 3 + d) * L + q] = 0.0;
+    out[d] = 0.0;
+  }
+}
+
+static void update_collision(orc_sys* s, int64_t q, int nf, const double pft[3]) {
+  int64_t L = s->ld;
+  for (int cc = 0; cc < COLLISION_ARRAY_MAX; cc++)
+    if (nf == s->wall_fid[cc * L + q]) {
+      for (int d = 0; d < 3; d++) s->wall_pft[(cc * 3 + d) * L + q] = pft[d];
+      return;
+    }
+}
+
+/* CALC_DEM_FORCE_WITH_WALL_STL, calc_collision_wall_mod.f:54-794 (sphere,
+   no cohesion, no rolling friction, static walls) */
+static void wall_force_particle(orc_sys* s, int64_t q) {
+  int64_t L = s->ld;
+  if (s->state[q] != NORMAL_PARTICLE) return;
+  int cell = s->dg_pijk[q] - 1;
+  if (s->fc_count[cell] < 1) {
+    for (int cc = 0; cc < COLLISION_ARRAY_MAX; cc++) {
+      s->wall_fid[cc * L + q] = -1;
+      for (int d = 0; d < 3; d++) s->wall_pft[(cc * 3 + d) * L + q] = 0.0;
+    }
+    return;
+  }
+  double r = s->radius[q];
+  double radsq = r * r;
+  double pos[3] = {s->pos[q], s->pos[L + q], s->pos[2 * L + q]};
+  double pmax[3], pmin[3];
+  for (int c = 0; c < 3; c++) {
+    pmax[c] = pos[c] + r;
+    pmin[c] = pos[c] - r;
+  }
+  int ph = s->phase[q];
+  for (int cnt = 0; cnt < s->fc_count[cell]; cnt++) {
+    int axis = s->fc_dir[cell][cnt];
+    int nf = s->fc_id[cell][cnt];
+    if (s->fc_min[cell][cnt] > pmax[axis]) { remove_collision(s, q, nf); continue; }
+    if (s->fc_max[cell][cnt] < pmin[axis]) { remove_collision(s, q, nf); continue; }
+    double vv[3] = {s->vert[nf][0][0] - pos[0], s->vert[nf][0][1] - pos[1], s->vert[nf][0][2] - pos[2]};
+    double line_t = dot3(vv, s->norm[nf]);
+    if (line_t <= -1.0001 * r) { remove_collision(s, q, nf); continue; }
+    double cp[3];
+    int pe, edge;
+    closest_pt_tri(s, nf, pos, cp, &pe, &edge);
+    double dist[3] = {cp[0] - pos[0], cp[1] - pos[1], cp[2] - pos[2]};
+    double distsq = dot3(dist, dist);
+    if (distsq >= radsq - SMALL_NUMBER) { remove_collision(s, q, nf); continue; }
+    if (pe && s->ignore_edge[nf][edge]) continue;
+    double distmod = sqrt(distsq);
+    double normal[3] = {dist[0], dist[1], dist[2]};
+    if (distsq > 0.0) {
+      double sq = sqrt(distsq);
+      for (int c = 0; c < 3; c++) normal[c] = dist[c] / sq;
+    }
+    double overlap_n = r - distmod;
+    double om[3] = {s->omega[q], s->omega[L + q], s->omega[2 * L + q]};
+    double cr[3], vrot[3];
+    cross3(om, normal, cr);
+    for (int c = 0; c < 3; c++) vrot[c] = distmod * cr[c];
+    /* CFRELVEL_WALL (:806-847), static wall */
+    double vrel[3];
+    for (int c = 0; c < 3; c++) {
+      vrel[c] = s->vel[c * L + q] + vrot[c];
+      vrel[c] = vrel[c] - (0.0 + 0.0);
+    }
+    double vrn = dot3(vrel, normal);
+    double vrt[3];
+    for (int c = 0; c < 3; c++) vrt[c] = vrel[c] - vrn * normal[c];
+    double kn, kt, etan, etat;
+    if (s->p.coll_model == MFIXDEM_HERTZ) {
+      double sqo = 0.0;
+      if (overlap_n > 0.0) sqo = sqrt(overlap_n);
+      kn = s->d.hert_kwn[ph] * sqo * sqrt(r);
+      kt = s->d.hert_kwt[ph] * sqo * sqrt(r);
+      etan = s->d.des_etan_wall[ph] * sqrt(sqo) * sqrt(s->d.hert_kwn[ph] * sqrt(r) * s->pmass[q]);
+      etat = s->d.des_etat_wall[ph] * sqrt(sqo) * sqrt(s->d.hert_kwt[ph] * sqrt(r) * s->pmass[q]);
+    } else {
+      kn = s->p.kn_w;
+      kt = s->kt_w;
+      etan = s->d.des_etan_wall[ph] * sqrt(s->pmass[q]);
+      etat = s->d.des_etat_wall[ph] * sqrt(s->pmass[q]);
+    }
+    double fnorm[3], ot[3], ftan[3], hist[3];
+    for (int c = 0; c < 3; c++) fnorm[c] = -(kn * overlap_n * normal[c] + etan * vrn * normal[c]);
+    get_collision(s, q, nf, hist);
+    for (int c = 0; c < 3; c++) ot[c] = s->d.dtsolid * vrt[c] + hist[c];
+    double mag = sqrt(dot3(ot, ot));
+    if (mag > 0.0) {
+      for (int c = 0; c < 3; c++) ftan[c] = -kt * ot[c] - etat * vrt[c];
+      double ftmd = sqrt(dot3(ftan, ftan));
+      double fnmd = s->p.mew_w * sqrt(dot3(fnorm, fnorm));
+      if (ftmd > fnmd) {
+        double tg[3];
+        for (int c = 0; c < 3; c++) tg[c] = ot[c] / mag;
+        for (int c = 0; c < 3; c++) ftan[c] = -fnmd * tg[c];
+        for (int c = 0; c < 3; c++) ot[c] = (fnmd / kt) * tg[c];
+      }
+    } else {
+      ftan[0] = ftan[1] = ftan[2] = 0.0;
+    }
+    update_collision(s, q, nf, ot);
+    for (int c = 0; c < 3; c++) s->fc[c * L + q] = s->fc[c * L + q] + fnorm[c] + ftan[c];
+    double tw[3];
+    cross3(normal, ftan, tw);
+    for (int c = 0; c < 3; c++) s->tow[c * L + q] = s->tow[c * L + q] + distmod * tw[c];
+  }
+}
+
+/* ----------------------------------------------------- CALC_FORCE_DEM */
+#ifdef _OPENMP
+#define OMP_ATOMIC _Pragma("omp atomic")
+#else
+#define OMP_ATOMIC
+#endif
+
+/* calc_force_dem.f:18-544 (sphere path) with CFRELVEL (cfrelvel.f:21-81) */
+int orc_calc_force_dem(orc_sys* s) {
+  int64_t L = s->ld;
+  int err = 0;
+  int64_t contacts = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
+  for (int64_t q = 0; q < s->max_pip; q++) wall_force_particle(s, q);
+  double dt = s->d.dtsolid;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : contacts) reduction(| : err)
+#endif
+  for (int64_t ll = 0; ll < s->max_pip; ll++) {
+    if (s->state[ll] == NONEXISTENT) continue;
+    double pos[3] = {s->pos[ll], s->pos[L + ll], s->pos[2 * L + ll]};
+    double rad = s->radius[ll];
+    int cs = (ll > 0) ? s->neighbor_index[ll - 1] : 1;
+    int ce = s->neighbor_index[ll];
+    for (int cc = cs; cc < ce; cc++) {
+      int64_t i = s->neighbors[cc - 1] - 1;
+      if (s->state[i] == NONEXISTENT) continue;
+      double r_lm = rad + s->radius[i];
+      double dist[3] = {s->pos[i] - pos[0], s->pos[L + i] - pos[1], s->pos[2 * L + i] - pos[2]};
+      double dist_mag = dot3(dist, dist);
+      double* pft = &s->pft[3 * (cc - 1)];
+      double rs = r_lm - SMALL_NUMBER;
+      if (dist_mag > rs * rs) {
+        pft[0] = pft[1] = pft[2] = 0.0;
+        continue;
+      }
+      if (dist_mag == 0.0) {
+        err = 1;
+        continue;
+      }
+      contacts++;
+      dist_mag = sqrt(dist_mag);
+      double normal[3];
+      for (int c = 0; c < 3; c++) normal[c] = dist[c] / dist_mag;
+      double overlap_n = r_lm - dist_mag;
+      /* CFRELVEL */
+      double vrel[3];
+      for (int c = 0; c < 3; c++) vrel[c] = s->vel[c * L + ll] - s->vel[c * L + i];
+      double rL = s->radius[ll], rI = s->radius[i];
+      double dcl = (dist_mag * dist_mag + rL * rL - rI * rI) / (2.0 * dist_mag);
+      double dci = dist_mag - dcl;
+      double osum[3], vrot[3];
+      for (int c = 0; c < 3; c++) osum[c] = s->omega[c * L + ll] * dcl + s->omega[c * L + i] * dci;
+      cross3(osum, normal, vrot);
+      for (int c = 0; c < 3; c++) vrel[c] = vrel[c] + vrot[c];
+      double vrn = dot3(vrel, normal);
+      double vslip[3];
+      for (int c = 0; c < 3; c++) vslip[c] = vrel[c] - vrn * normal[c];
+      int phl = s->phase[ll], phi = s->phase[i];
+      double rad_eff = (rad * s->radius[i]) / (rad + s->radius[i]);
+      double m_eff = (s->pmass[ll] * s->pmass[i]) / (s->pmass[ll] + s->pmass[i]);
+      double kn, kt, etan, etat;
+      if (s->p.coll_model == MFIXDEM_HERTZ) {
+        double sqo = sqrt(overlap_n);
+        kn = s->d.hert_kn[phl][phi] * sqo * sqrt(rad_eff);
+        kt = s->d.hert_kt[phl][phi] * sqo * sqrt(rad_eff);
+        sqo = sqrt(sqo);
+        etan = s->d.des_etan[phl][phi] * sqo * sqrt(s->d.hert_kn[phl][phi] * sqrt(rad_eff) * m_eff);
+        etat = s->d.des_etat[phl][phi] * sqo * sqrt(s->d.hert_kt[phl][phi] * sqrt(rad_eff) * m_eff);
+      } else {
+        kn = s->p.kn;
+        kt = s->kt;
+        etan = s->d.des_etan[phl][phi] * sqrt(m_eff);
+        etat = s->d.des_etat[phl][phi] * sqrt(m_eff);
+      }
+      double fn[3], ot[3], ft[3];
+      for (int c = 0; c < 3; c++) fn[c] = -(kn * overlap_n * normal[c] + etan * vrn * normal[c]);
+      for (int c = 0; c < 3; c++) ot[c] = dt * vslip[c] + pft[c];
+      double mag = sqrt(dot3(ot, ot));
+      if (mag > 0.0) {
+        for (int c = 0; c < 3; c++) ft[c] = -kt * ot[c] - etat * vslip[c];
+        double ftmd = sqrt(dot3(ft, ft));
+        double fnmd = s->p.mew * sqrt(dot3(fn, fn));
+        if (ftmd > fnmd) {
+          double tg[3];
+          for (int c = 0; c < 3; c++) tg[c] = ot[c] / mag;
+          for (int c = 0; c < 3; c++) ft[c] = -fnmd * tg[c];
+          for (int c = 0; c < 3; c++) ot[c] = (fnmd / kt) * tg[c];
+        }
+      } else {
+        ft[0] = ft[1] = ft[2] = 0.0;
+      }
+      for (int c = 0; c < 3; c++) pft[c] = ot[c];
+      double dist_cl = dist_mag / 2.0 + (s->radius[ll] * s->radius[ll] - s->radius[i] * s->radius[i]) / (2.0 * dist_mag);
+      double dist_ci = dist_mag - dist_cl;
+      double tf[3];
+      cross3(normal, ft, tf);
+      double fct[3];
+      for (int c = 0; c < 3; c++) fct[c] = 0.0 + fn[c] + ft[c];
+      for (int c = 0; c < 3; c++) {
+        OMP_ATOMIC
+        s->fc[c * L + ll] += fct[c];
+      }
+      for (int c = 0; c < 3; c++) {
+        OMP_ATOMIC
+        s->fc[c * L + i] -= fct[c];
+      }
+      for (int c = 0; c < 3; c++) {
+        double t1 = dist_cl * tf[c];
+        OMP_ATOMIC
+        s->tow[c * L + ll] += t1;
+      }
+      for (int c = 0; c < 3; c++) {
+        double t2 = dist_ci * tf[c];
+        OMP_ATOMIC
+        s->tow[c * L + i] += t2;
+      }
+    }
+  }
+  s->st.n_contacts = contacts;
+  if (err) {
+    s->st.zero_distance++;
+    return MFIXDEM_ERR_ZERO_DISTANCE;
+  }
+  return 0;
+}
+
+/* ----------------------------------------------------------- drag */
+double orc_drag_fgp(int32_t drag_type, double epg, double mu, double rog, double vrel,
+                    double dpm, double pvol) {
+  double ropg = rog * epg;
+  double dga;
+  if (drag_type == MFIXDEM_DRAG_WEN_YU) {
+    /* DRAG_WEN_YU, drag_gs.f:703-758 */
+    double re = (mu > 0.0) ? dpm * vrel * ropg / mu : LARGE_NUMBER;
+    if (re == 0.0) {
+      dga = 0.0;
+    } else {
+      double cd;
+      if (re <= 1000.0) cd = (24.0 / (re + SMALL_NUMBER)) * (1.0 + 0.15 * pow(re, 0.687));
+      else cd = 0.44;
+      dga = 0.75 * cd * vrel * ropg * pow(epg, -2.65) / dpm;
+    }
+  } else {
+    /* DRAG_GIDASPOW, drag_gs.f:536-598; C_DS_SN drag_mod.f:48-56 */
+    double re = (mu > 0.0) ? dpm * vrel * ropg / mu : LARGE_NUMBER;
+    if (re == 0.0) {
+      dga = 0.0;
+    } else if (epg <= 0.8) {
+      dga = 150.0 * (1.0 - epg) * mu / (epg * (dpm * dpm)) + 1.75 * rog * vrel / dpm;
+    } else {
+      double cd;
+      if (re <= 1000.0) cd = 24.0 * (1.0 + 0.15 * pow(re, 0.687)) / (re + SMALL_NUMBER);
+      else cd = 0.44;
+      dga = 0.75 * cd * vrel * ropg * pow(epg, -2.65) / dpm;
+    }
+  }
+  /* DES_DRAG_GP Model A, UR_F_GS = 1 (drag_gp_des.f:249-257) */
+  return dga * pvol;
+}
+
+/* node velocities of a fluid cell's 8 corners, DRAG_GS_DES0 :140-180 */
+static void cell_nodes(const orc_sys* s, int i, int j, int k, double gst[2][2][2][3],
+                       double vst[2][2][2][3]) {
+  const double af = 0.25;
+  int iw = i - 1, js = j - 1, kb = k - 1;
+  for (int kk = 0; kk < 2; kk++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int ii = 0; ii < 2; ii++) {
+        int I = iw + ii, J = js + jj, K = kb + kk;
+        gst[kk][jj][ii][0] = s->xe[I];
+        gst[kk][jj][ii][1] = s->yn[J];
+        gst[kk][jj][ii][2] = s->zt[K];
+        if (!vst) continue;
+        int cur = fluid_ijk(s, I, J, K) - 1, ipjk = fluid_ijk(s, I + 1, J, K) - 1;
+        int ijpk = fluid_ijk(s, I, J + 1, K) - 1, ipjpk = fluid_ijk(s, I + 1, J + 1, K) - 1;
+        int ijpkp = fluid_ijk(s, I, J + 1, K + 1) - 1, ipjkp = fluid_ijk(s, I + 1, J, K + 1) - 1;
+        int ijkp = fluid_ijk(s, I, J, K + 1) - 1;
+        double vx = af * (s->u_g[cur] + s->u_g[ijpk]);
+        double vy = af * (s->v_g[cur] + s->v_g[ipjk]);
+        vx = vx + af * (s->u_g[ijkp] + s->u_g[ijpkp]);
+        vy = vy + af * (s->v_g[ijkp] + s->v_g[ipjkp]);
+        double vz = af * (s->w_g[cur] + s->w_g[ijpk] + s->w_g[ipjk] + s->w_g[ipjpk]);
+        vst[kk][jj][ii][0] = vx;
+        vst[kk][jj][ii][1] = vy;
+        vst[kk][jj][ii][2] = vz;
+      }
+}
+
+static void trilinear_weights(const double gst[2][2][2][3], const double pos[3], double w[2][2][2]) {
+  /* DRAG_INTERPOLATION / DRAG_WEIGHTFACTOR, drag_gs_des0.f:507-566,
+     comp_mean_fields0.f:470-530 */
+  double dxx = gst[0][0][1][0] - gst[0][0][0][0];
+  double dyy = gst[0][1][0][1] - gst[0][0][0][1];
+  double dzz = gst[1][0][0][2] - gst[0][0][0][2];
+  double z0 = pos[0] - gst[0][0][0][0], z1 = pos[1] - gst[0][0][0][1];
+  z0 = z0 / dxx;
+  z1 = z1 / dyy;
+  double z2 = pos[2] - gst[0][0][0][2];
+  z2 = z2 / dzz;
+  double xv[2] = {1.0 - z0, z0}, yv[2] = {1.0 - z1, z1}, zv[2] = {1.0 - z2, z2};
+  for (int kk = 0; kk < 2; kk++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int ii = 0; ii < 2; ii++) w[kk][jj][ii] = xv[ii] * yv[jj] * zv[kk];
+}
+
+/* CALC_DRAG_DES -> DRAG_GS_DES0 (GARG_2012), drag_gs_des0.f:22-219 */
+int orc_calc_drag_des(orc_sys* s) {
+  if (!s->p.coupled) return 0;
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk] || s->pinc[ijk] == 0) continue;
+    int z = ijk;
+    int j = z % s->nj + 1;
+    z /= s->nj;
+    int i = z % s->ni + 1;
+    int k = z / s->ni + 1;
+    double gst[2][2][2][3], vst[2][2][2][3];
+    cell_nodes(s, i, j, k, gst, vst);
+    for (int t = s->pic_off[ijk]; t < s->pic_off[ijk + 1]; t++) {
+      int np = s->pic_list[t];
+      if (s->state[np] == NONEXISTENT) continue;
+      double pos[3] = {s->pos[np], s->pos[L + np], s->pos[2 * L + np]};
+      double w[2][2][2];
+      trilinear_weights(gst, pos, w);
+      double velfp[3] = {0.0, 0.0, 0.0};
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++)
+            for (int c = 0; c < 3; c++) velfp[c] = velfp[c] + vst[kk][jj][ii][c] * w[kk][jj][ii];
+      double vel[3] = {s->vel[np], s->vel[L + np], s->vel[2 * L + np]};
+      double vslp[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+      double vrel = sqrt(dot3(vslp, vslp));
+      double fgp = orc_drag_fgp(p->drag_type, s->ep_g[ijk], p->mu_g0, p->ro_g0, vrel,
+                                2.0 * s->radius[np], s->pvol[np]);
+      for (int c = 0; c < 3; c++) {
+        double dfc = fgp * (velfp[c] - vel[c]);
+        s->fc[c * L + np] = s->fc[c * L + np] + dfc;
+      }
+      if (!p->model_b)
+        for (int c = 0; c < 3; c++)
+          s->fc[c * L + np] = s->fc[c * L + np] + s->p_force[3 * ijk + c] * s->pvol[np];
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------- integrator */
+/* CFUPDATEOLD, cfupdateold.f:15-46 */
+static void cfupdateold(orc_sys* s) {
+  int64_t L = s->ld;
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] == NONEXISTENT) continue;
+    for (int c = 0; c < 3; c++) {
+      s->pos_old[c * L + q] = s->pos[c * L + q];
+      s->vel_old[c * L + q] = s->vel[c * L + q];
+      s->omega_old[c * L + q] = s->omega[c * L + q];
+    }
+  }
+}
+
+/* CFNEWVALUES, cfnewvalues.f:641-864 (EULER :700-738, ADAMS_BASHFORTH
+   :673-682 and :742-811) */
+int orc_cfnewvalues(orc_sys* s) {
+  int64_t L = s->ld;
+  double dt = s->d.dtsolid;
+  const double* g = s->p.grav;
+  if (s->p.integrator == MFIXDEM_ADAMS_BASHFORTH) {
+    if (s->first_pass) {
+      for (int64_t q = 0; q < s->max_pip; q++) {
+        if (s->state[q] == NONEXISTENT) continue;
+        for (int c = 0; c < 3; c++) {
+          s->acc_old[c * L + q] = s->fc[c * L + q] / s->pmass[q] + g[c];
+          s->rot_acc_old[c * L + q] = s->tow[c * L + q];
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t q = 0; q < s->max_pip; q++) {
+      for (int c = 0; c < 3; c++) {
+        int64_t k = c * L + q;
+        if (s->state[q] == NORMAL_PARTICLE) {
+          s->fc[k] = s->fc[k] / s->pmass[q] + g[c];
+          s->vel[k] = s->vel_old[k] + 0.5 * (3.0 * s->fc[k] - s->acc_old[k]) * dt;
+        }
+        s->acc_old[k] = s->fc[k];
+        if (s->state[q] < 101) s->pos[k] = s->pos_old[k] + 0.5 * (s->vel_old[k] + s->vel[k]) * dt;
+        s->fc[k] = 0.0;
+      }
+      for (int c = 0; c < 3; c++) {
+        int64_t k = c * L + q;
+        if (s->state[q] == NORMAL_PARTICLE) {
+          s->omega[k] = s->omega_old[k] + 0.5 * (3.0 * s->tow[k] * s->omoi[q] - s->rot_acc_old[k]) * dt;
+          s->rot_acc_old[k] = s->tow[k] * s->omoi[q];
+        }
+        s->tow[k] = 0.0;
+      }
+    }
+  } else {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t q = 0; q < s->max_pip; q++) {
+      for (int c = 0; c < 3; c++) {
+        int64_t k = c * L + q;
+        if (s->state[q] == NORMAL_PARTICLE) s->vel[k] = s->vel[k] + dt * (s->fc[k] / s->pmass[q] + g[c]);
+        if (s->state[q] < 101) s->pos[k] = s->pos[k] + s->vel[k] * dt;
+        s->fc[k] = 0.0;
+      }
+      for (int c = 0; c < 3; c++) {
+        int64_t k = c * L + q;
+        if (s->state[q] == NORMAL_PARTICLE) s->omega[k] = s->omega[k] + s->tow[k] * s->omoi[q] * dt;
+        s->tow[k] = 0.0;
+      }
+    }
+  }
+  s->first_pass = 0;
+  return 0;
+}
+
+/* ------------------------------------------------- PARTICLES_IN_CELL */
+/* particles_in_cell.f:20-234 with PIC_SEARCH :347-370; the incremental
+   +/-1 search lands on the same unique face interval as the scan. */
+static int locate(const double* faces, int nmax2, double x) {
+  for (int idx = 1; idx <= nmax2; idx++)
+    if (x >= faces[idx - 1] && x < faces[idx]) return idx;
+  return nmax2 + 1;
+}
+
+int orc_particles_in_cell(orc_sys* s) {
+  int64_t L = s->ld;
+  memset(s->pinc, 0, sizeof(int) * s->ncell);
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] == NONEXISTENT) continue;
+    int i = locate(s->xe, s->ni, s->pos[q]);
+    int j = locate(s->yn, s->nj, s->pos[L + q]);
+    int k = locate(s->zt, s->nk, s->pos[2 * L + q]);
+    if (i <= s->ni && j <= s->nj && k <= s->nk) {
+      int ijk = fluid_ijk(s, i, j, k);
+      s->pijk[q] = i;
+      s->pijk[L + q] = j;
+      s->pijk[2 * L + q] = k;
+      s->pijk[3 * L + q] = ijk;
+      s->pinc[ijk - 1]++;
+    } else {
+      delete_particle(s, q);
+      s->st.out_of_domain++;
+    }
+  }
+  s->pic_off[0] = 0;
+  for (int c = 0; c < s->ncell; c++) s->pic_off[c + 1] = s->pic_off[c] + s->pinc[c];
+  int* cur = malloc(sizeof(int) * s->ncell);
+  memcpy(cur, s->pic_off, sizeof(int) * s->ncell);
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] == NONEXISTENT) continue;
+    int ijk = s->pijk[3 * L + q] - 1;
+    s->pic_list[cur[ijk]++] = (int)q;
+  }
+  free(cur);
+  return 0;
+}
+
+/* --------------------------------------------------- COMP_MEAN_FIELDS */
+/* COMP_MEAN_FIELDS0 (GARG), comp_mean_fields0.f:6-466, then CALC_EPG_DES
+   calc_epg_des.f:19-205 */
+int orc_comp_mean_fields(orc_sys* s) {
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+  int nph = p->nphases;
+  size_t C = (size_t)s->ncell;
+  memset(s->rop_s, 0, sizeof(double) * C * nph);
+  memset(s->rops_node, 0, sizeof(double) * C * nph);
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk] || s->pinc[ijk] == 0) continue;
+    int z = ijk;
+    int j = z % s->nj + 1;
+    z /= s->nj;
+    int i = z % s->ni + 1;
+    int k = z / s->ni + 1;
+    double gst[2][2][2][3];
+    cell_nodes(s, i, j, k, gst, NULL);
+    for (int t = s->pic_off[ijk]; t < s->pic_off[ijk + 1]; t++) {
+      int np = s->pic_list[t];
+      double pos[3] = {s->pos[np], s->pos[L + np], s->pos[2 * L + np]};
+      double w[2][2][2];
+      trilinear_weights(gst, pos, w);
+      int m = s->phase[np];
+      double temp2 = s->pmass[np];
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++) {
+            int cur = fluid_ijk(s, i - 1 + ii, j - 1 + jj, k - 1 + kk) - 1;
+            double temp1 = w[kk][jj][ii] * temp2;
+            s->rops_node[m * C + cur] = s->rops_node[m * C + cur] + temp1;
+          }
+    }
+  }
+  for (int k = 1; k <= p->kmax + 1; k++)
+    for (int j = 1; j <= p->jmax + 1; j++)
+      for (int i = 1; i <= p->imax + 1; i++) {
+        int ijk = fluid_ijk(s, i, j, k) - 1;
+        if (s->vol_surr[ijk] == 0.0) continue;
+        for (int m = 0; m < nph; m++) {
+          double dens = s->rops_node[m * C + ijk] / s->vol_surr[ijk];
+          for (int kk = k; kk <= k + 1; kk++)
+            for (int jj = j; jj <= j + 1; jj++)
+              for (int ii = i; ii <= i + 1; ii++) {
+                int ijk2 = fluid_ijk(s, ii, jj, kk) - 1;
+                s->rop_s[m * C + ijk2] = s->rop_s[m * C + ijk2] + dens * s->vol[ijk2];
+              }
+        }
+      }
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk]) continue;
+    for (int m = 0; m < nph; m++)
+      if (s->rop_s[m * C + ijk] > 0.0) s->rop_s[m * C + ijk] = s->rop_s[m * C + ijk] / s->vol[ijk];
+  }
+  double packed = 1.0;
+  if (p->des_epg_clip != UNDEF) packed = 1.0 - p->des_epg_clip;
+  int64_t ier = 0;
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk]) continue;
+    double sum_eps = 0.0;
+    for (int m = 0; m < nph; m++) sum_eps = sum_eps + s->rop_s[m * C + ijk] / p->ro_s0[m];
+    s->ep_g[ijk] = 1.0 - (packed < sum_eps ? packed : sum_eps);
+    if (s->ep_g[ijk] <= 0.0 || s->ep_g[ijk] > 1.0) ier++;
+  }
+  s->st.epg_errors = ier;
+  return ier ? MFIXDEM_ERR_EPG : 0;
+}
+
+/* ------------------------------------------------------- driver */
+int orc_desgrid_and_neighbour(orc_sys* s) {
+  orc_desgrid_pic(s);
+  return orc_neighbour(s);
+}
+
+/* MAKE_ARRAYS_DES, make_arrays_des.f:343-432 */
+int orc_init(orc_sys* s) {
+  int rc = 0;
+  orc_desgrid_pic(s);
+  if (s->p.coupled) {
+    if (!s->gas_set) return MFIXDEM_ERR_NOT_READY;
+    orc_particles_in_cell(s);
+  }
+  memset(s->neighbor_index, 0, sizeof(int) * s->max_pip);
+  rc = orc_neighbour(s);
+  if (rc) return rc;
+  if (s->p.coupled) rc = orc_comp_mean_fields(s);
+  s->inited = 1;
+  s->st.steps = 0;
+  return rc;
+}
+
+/* DES_TIME_STEP(NN), des_time_march.f:293-460 */
+int orc_time_step(orc_sys* s, int32_t nn) {
+  if (!s->inited) return MFIXDEM_ERR_NOT_READY;
+  int64_t L = s->ld;
+  int rc = orc_calc_force_dem(s);
+  if (rc) return rc;
+  if (s->p.coupled) orc_calc_drag_des(s);
+  if (s->p.integrator == MFIXDEM_ADAMS_BASHFORTH) cfupdateold(s);
+  memcpy(s->fc_last, s->fc, sizeof(double) * 3 * L);
+  memcpy(s->tow_last, s->tow, sizeof(double) * 3 * L);
+  orc_cfnewvalues(s);
+  int do_nsearch = (nn == 1) || (nn % s->p.neighbor_search_n == 0);
+  if (do_nsearch) {
+    orc_desgrid_pic(s);
+    rc = orc_neighbour(s);
+    if (rc) return rc;
+  }
+  if (s->p.coupled) {
+    orc_particles_in_cell(s);
+    rc = orc_comp_mean_fields(s);
+  }
+  s->st.steps++;
+  return rc;
+}
+
+int orc_run(orc_sys* s, int32_t nn_first, int32_t nsteps) {
+  for (int32_t t = 0; t < nsteps; t++) {
+    int rc = orc_time_step(s, nn_first + t);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld) {
+  int64_t L = s->ld, n = s->max_pip < ld ? s->max_pip : ld;
+  for (int64_t q = 0; q < n; q++)
+    for (int c = 0; c < 3; c++) {
+      if (fc) fc[c * ld + q] = s->fc_last[c * L + q];
+      if (tow) tow[c * ld + q] = s->tow_last[c * L + q];
+    }
+  return 0;
+}
+
+int orc_get_cells(const orc_sys* s, int32_t* dg_pijk, int32_t* pijk, int64_t ld) {
+  int64_t L = s->ld, n = s->max_pip < ld ? s->max_pip : ld;
+  for (int64_t q = 0; q < n; q++) {
+    if (dg_pijk) dg_pijk[q] = s->dg_pijk[q];
+    if (pijk)
+      for (int c = 0; c < 4; c++) pijk[c * ld + q] = s->pijk[c * L + q];
+  }
+  return 0;
+}
+
+int orc_get_pairs(const orc_sys* s, int32_t* neighbor_index, int32_t* neighbors, double* pft,
+                  int64_t capacity, int64_t* npairs) {
+  int64_t np = s->max_pip;
+  int64_t total = s->neighbor_index[np - 1] - 1;
+  if (total < 0) total = 0;
+  if (npairs) *npairs = total;
+  if (capacity == 0) return 0;
+  if (capacity < total) return MFIXDEM_ERR_CAPACITY;
+  if (neighbor_index) memcpy(neighbor_index, s->neighbor_index, sizeof(int) * np);
+  if (neighbors) memcpy(neighbors, s->neighbors, sizeof(int) * total);
+  if (pft) memcpy(pft, s->pft, sizeof(double) * 3 * total);
+  return 0;
+}
+
+int orc_get_status(const orc_sys* s, mfixdem_status* out) {
+  *out = s->st;
+  int64_t n = 0;
+  for (int64_t q = 0; q < s->max_pip; q++) n += (s->state[q] != NONEXISTENT);
+  out->n_particles = n;
+  int64_t tot = s->neighbor_index[s->max_pip - 1] - 1;
+  out->n_pairs = tot > 0 ? tot : 0;
+  return 0;
+}

@@ -1,0 +1,75 @@
+/*
+ * dem_oracle.h -- CPU restatement of MFiX's DEM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
+ * against; it is never linked into, loaded by, or called from the product
+ * library (mfix-cfd-dem_amd/).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it.
+ *
+ * Built twice from the same source: serially (liboracle.so, the parity
+ * oracle, bit-reproducible) and with OpenMP (liboracle_omp.so, the timed CPU
+ * baseline; force sums use `omp atomic` exactly like calc_force_dem.f:371-412,
+ * so its results are thread-order dependent).
+ */
+#ifndef DEM_ORACLE_H
+#define DEM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mfixdem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sys orc_sys;
+
+orc_sys* orc_create(const mfixdem_params* p);
+void orc_destroy(orc_sys* s);
+int orc_get_derived(const orc_sys* s, mfixdem_derived* out);
+
+int orc_set_particles(orc_sys* s, int64_t n, const double* pos, const double* vel,
+                      const double* omega, const double* radius,
+                      const double* ro_sol, const int32_t* phase, int64_t ld);
+int orc_get_particles(const orc_sys* s, double* pos, double* vel, double* omega,
+                      int8_t* state, int64_t ld);
+int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g,
+                const double* w_g, const double* p_force);
+int orc_get_gas(const orc_sys* s, double* ep_g, double* rop_s);
+
+/* MAKE_ARRAYS_DES tail and DES_TIME_STEP(NN). */
+int orc_init(orc_sys* s);
+int orc_time_step(orc_sys* s, int32_t nn);
+int orc_run(orc_sys* s, int32_t nn_first, int32_t nsteps);
+
+/* individual reference routines */
+int orc_desgrid_pic(orc_sys* s);
+int orc_neighbour(orc_sys* s);
+int orc_calc_force_dem(orc_sys* s);
+int orc_calc_drag_des(orc_sys* s);
+int orc_cfnewvalues(orc_sys* s);
+int orc_particles_in_cell(orc_sys* s);
+int orc_comp_mean_fields(orc_sys* s);
+
+/* inspection */
+int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);
+int orc_get_cells(const orc_sys* s, int32_t* dg_pijk, int32_t* pijk, int64_t ld);
+int orc_get_pairs(const orc_sys* s, int32_t* neighbor_index, int32_t* neighbors,
+                  double* pft, int64_t capacity, int64_t* npairs);
+int orc_get_status(const orc_sys* s, mfixdem_status* out);
+double orc_dtsolid(const orc_sys* s);
+
+/* NSQUARE (nsquare.f:15-142): brute-force pair set, same criterion; returns
+   pairs as (i,j) 1-based with i<j. */
+int64_t orc_nsquare_pairs(const orc_sys* s, int32_t* pairs, int64_t capacity);
+
+/* Gas-solids drag coefficient of one particle (DES_DRAG_GP, drag_gp_des.f:29
+   with DRAG_GIDASPOW / DRAG_WEN_YU, drag_gs.f:536,703): returns F_GP. */
+double orc_drag_fgp(int32_t drag_type, double epg, double mu, double rog,
+                    double vrel, double dpm, double pvol);
+
+int orc_omp_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

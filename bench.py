@@ -1,0 +1,193 @@
+"""Benchmark: particle-steps/s of the MFiX DEM hot path on one MI355X.
+
+Workload (default): FP64 CFD-DEM with a frozen gas field -- 1,000,000
+polydisperse spheres (log-normal d, median 1 mm, sigma 0.25, clipped to
+[0.5, 2] mm), Hertz contact (E = 1e7 Pa, nu = 0.3), e = 0.9, mu = 0.3,
+gas grid 64 x 128 x 64 with 1.2 m/s upflow plus a seeded 5 % sinusoidal
+perturbation, Gidaspow drag, GARG_2012 trilinear deposition, dt = 1e-5 s.
+A "step" is one DES_TIME_STEP over all particles (walls, pairs, drag,
+integration, fluid-cell binning, deposition, EP_G; a neighbour-list rebuild
+every 25 steps).  Particle state is resident in HBM before timing starts.
+
+Prints ONE JSON line.  `value` = FP64 particle-steps/s; the mixed-precision
+run of the same workload is reported beside it.  `roofline` covers the
+dominant kernel (the fused particle step) timed with HIP events;
+`cpu_baseline` times the CPU restatement (OpenMP) on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mfix-cfd-dem_amd"))
+
+import numpy as np  # noqa: E402
+
+from mfixdem import abi, workloads  # noqa: E402
+from mfixdem.system import DemSystem  # noqa: E402
+
+METRIC = "particle-steps/s at 1M spheres FP64 vs mixed; p50/p99 sweep-request latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_step_kernel(n, pairs):
+    """Bytes one fused step launch must move: per particle read pos, vel,
+    omega, radius and write pos, vel, omega (152 B); per half-list pair the
+    neighbour index (4 B) and the tangential history read + write (48 B)."""
+    return 152.0 * n + 52.0 * pairs
+
+
+def make_system(p, parts, gas):
+    s = DemSystem(p)
+    s.set_particles(parts)
+    s.set_gas(gas)
+    s.init()
+    return s
+
+
+def time_steps(s, nn0, steps):
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(nn0, steps)
+    s.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(s, nn0, steps):
+    """Per-kernel-group HIP event times over `steps` eager steps."""
+    s.set_debug(abi.DEBUG_TIME_KERNELS)
+    s.lib.mfixdem_reset_kernel_times(s.h)
+    s.run(nn0, steps)
+    arr = (C.c_double * abi.TIME_GROUPS)()
+    calls = (C.c_int64 * abi.TIME_GROUPS)()
+    s.lib.mfixdem_get_kernel_times(s.h, arr, calls, abi.TIME_GROUPS)
+    s.set_debug(0)
+    return [arr[i] / max(calls[i], 1) for i in range(abi.TIME_GROUPS)], list(calls)
+
+
+def cpu_baseline(p, parts, gas, seconds=15.0):
+    """Times the OpenMP CPU restatement (oracle/liboracle_omp.so) on the same
+    workload for as many whole steps as fit in about `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_py import OracleSystem
+
+    o = OracleSystem(p, omp=True)
+    o.set_particles(parts)
+    o.set_gas(gas)
+    o.init()
+    o.step(1)  # rebuild step, untimed
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(2 + steps)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or steps >= 200:
+            break
+    threads = o.lib.orc_omp_threads()
+    o.close()
+    return {"value": p.max_pip * steps / el, "unit": "particle-steps/s", "cores": int(threads),
+            "kind": "port",
+            "sample": f"{steps} DES_TIME_STEP calls (no rebuild) of the same {p.max_pip}-sphere "
+                      f"workload, OpenMP CPU restatement, {threads} threads, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one DEM system per GPU on a single GPU; "
+              f"--gpus {args.gpus} is not supported (multi-GPU sharding is design-only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    p, parts, gas = workloads.fluidised_bed(n=args.n)
+    log(f"workload built in {time.perf_counter() - t0:.1f} s: n={parts.n}")
+
+    # FP64
+    s = make_system(p, parts, gas)
+    s.run(1, args.warmup)
+    el = time_steps(s, 1 + args.warmup, args.steps)
+    st = s.status()
+    pairs = st.n_pairs
+    value = parts.n * args.steps / el
+    log(f"fp64: {args.steps} steps in {el:.3f} s -> {value:.4e} particle-steps/s, "
+        f"pairs/particle {pairs / parts.n:.2f}")
+    kt, calls = kernel_times(s, 1 + args.warmup + args.steps, 50)
+    t_step_ms = kt[abi.TIME_STEP]
+    bytes_step = algorithmic_bytes_step_kernel(parts.n, pairs)
+    achieved = bytes_step / (t_step_ms * 1e-3) / 1e9
+    log(f"fused step kernel {t_step_ms * 1e3:.1f} us, rebuild {kt[abi.TIME_REBUILD] * 1e3:.1f} us, "
+        f"gas {kt[abi.TIME_GAS] * 1e3:.1f} us")
+    s.close()
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_fluidised_1M.json")
+    if os.path.exists(tpath) and args.n == 1_000_000:
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    mixed = None
+    if not args.no_mixed:
+        pm = abi.Params.from_buffer_copy(p)
+        pm.precision = abi.MIXED
+        m = make_system(pm, parts, gas)
+        m.run(1, args.warmup)
+        elm = time_steps(m, 1 + args.warmup, args.steps)
+        ktm, _ = kernel_times(m, 1 + args.warmup + args.steps, 50)
+        mixed = {"value": parts.n * args.steps / elm, "ms_per_step": 1e3 * elm / args.steps,
+                 "step_kernel_us": ktm[abi.TIME_STEP] * 1e3}
+        log(f"mixed: {mixed['value']:.4e} particle-steps/s")
+        m.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p, parts, gas, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded RSA packing, seeded frozen gas field)",
+        "config": {"workload": "fluidised bed, FP64 CFD-DEM with frozen gas (Hertz, "
+                               "Gidaspow, GARG_2012)",
+                   "n_particles": parts.n, "gas_grid": [p.imax, p.jmax, p.kmax],
+                   "des_grid": [p.desgridsearch_imax, p.desgridsearch_jmax,
+                                p.desgridsearch_kmax],
+                   "dt": p.dtsolid, "neighbor_search_n": p.neighbor_search_n,
+                   "pairs_per_particle": pairs / parts.n},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_step (fused walls+pairs+drag+integrate+locate)",
+                     "kernel_us": t_step_ms * 1e3, "algorithmic_bytes": bytes_step},
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "kernel_groups_us": {"step": kt[0] * 1e3, "rebuild": kt[1] * 1e3, "gas": kt[2] * 1e3},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

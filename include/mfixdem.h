@@ -66,7 +66,13 @@ enum { MFIXDEM_DRAG_GIDASPOW = 0, MFIXDEM_DRAG_WEN_YU = 1 }; /* DRAG_TYPE       
 enum { MFIXDEM_INTERP_GARG_2012 = 0 };                       /* DES_INTERP_SCHEME */
 
 /* debug flags for mfixdem_set_debug */
-enum { MFIXDEM_DEBUG_STORE_FORCES = 1 };
+enum {
+  MFIXDEM_DEBUG_STORE_FORCES = 1, /* keep FC/TOW of the last step for mfixdem_get_forces */
+  MFIXDEM_DEBUG_TIME_KERNELS = 2  /* eager launches, HIP-event time per kernel group */
+};
+/* kernel groups reported by mfixdem_get_kernel_times */
+enum { MFIXDEM_TIME_STEP = 0, MFIXDEM_TIME_REBUILD = 1, MFIXDEM_TIME_GAS = 2,
+       MFIXDEM_TIME_GROUPS = 3 };
 
 /*
  * Run parameters.  Names follow the MFiX namelist keywords they stand for
@@ -176,7 +182,8 @@ int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
    calc_pg_grad.f:20), MFiX cell layout described above. */
 int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
                         const double* w_g, const double* p_force);
-/* EP_G and ROP_S (solids phase summed) after the last deposition. */
+/* EP_G (ncell) and ROP_S (nphases x ncell, phase-major) after the last
+   deposition (comp_mean_fields0.f, calc_epg_des.f). */
 int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s);
 
 /* --- the time step -------------------------------------------------------- */
@@ -205,7 +212,8 @@ int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
 /* --- inspection for parity and restart ------------------------------------ */
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags);
 /* FC and TOW as they stood just before the integrator of the last step
-   (requires MFIXDEM_DEBUG_STORE_FORCES, or the unfused entry points). */
+   (in global-id order; kept while MFIXDEM_DEBUG_STORE_FORCES is set, by
+   the fused step and by mfixdem_cfnewvalues). */
 int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t ld);
 /* DG_PIJK (1-based DES cell) and PIJK(:,1:4) (fluid I,J,K,IJK; 1-based). */
 int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk,
@@ -218,6 +226,16 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index,
                           int32_t* neighbors, double* pft, int64_t capacity,
                           int64_t* npairs);
 int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
+
+/* --- measurement (no counterpart in the reference; its only throughput
+   figure is the NITs/SEC log line, des_time_march.f:481-488) ------------- */
+/* The handle's HIP stream (hipStream_t), for callers that time or order
+   work against it. */
+int32_t mfixdem_get_stream(mfixdem_handle* h, void** stream);
+/* Accumulated milliseconds and call counts per kernel group (needs
+   MFIXDEM_DEBUG_TIME_KERNELS); n = number of groups requested. */
+int32_t mfixdem_get_kernel_times(mfixdem_handle* h, double* ms, int64_t* calls, int32_t n);
+int32_t mfixdem_reset_kernel_times(mfixdem_handle* h);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,8 @@ struct StepPtrs {
   const int *nbr_start, *nbr_split, *nbr, *nbr_pid;
   const double* pft_in;
   double* pft_out;
-  double *fc, *tow;
+  double *fc, *tow;             // FC/TOW in slot order (unfused path)
+  double *fc_id, *tow_id;       // FC/TOW in global-id order (inspection)
   double *acc_old, *rot_acc_old;
   int* fcell;
   const double *xe, *yn, *zt;
@@ -253,9 +254,10 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     load3(a.fc, S, i, fc);
     load3(a.tow, S, i, tow);
   }
-  if (MODE == kModeFused && P.store_forces) {
-    store3(a.fc, S, i, fc);
-    store3(a.tow, S, i, tow);
+  if (P.store_forces) {
+    const int g = a.gid[i];
+    store3(a.fc_id, S, g, fc);
+    store3(a.tow_id, S, g, tow);
   }
 
   // ---- CFNEWVALUES (cfnewvalues.f:700-811)

@@ -38,8 +38,8 @@ using namespace mfixdem;
     if (r_ != MFIXDEM_OK) return r_; \
   } while (0)
 
-enum { kTimeStep = 0, kTimeRebuild, kTimeGas, kTimeSlots };
-enum { MFIXDEM_DEBUG_TIME_KERNELS_ = 2 };
+enum { kTimeStep = MFIXDEM_TIME_STEP, kTimeRebuild = MFIXDEM_TIME_REBUILD,
+       kTimeGas = MFIXDEM_TIME_GAS, kTimeSlots = MFIXDEM_TIME_GROUPS };
 
 struct mfixdem_handle {
   Setup S;
@@ -55,7 +55,7 @@ struct mfixdem_handle {
   int *gid[2]{}, *fcell[2]{}, *dgc[2]{};
   uint8_t *state[2]{}, *phase[2]{}, *wmask[2]{};
   double* wpft[2]{};
-  double *fc = nullptr, *tow = nullptr;
+  double *fc = nullptr, *tow = nullptr, *fc_id = nullptr, *tow_id = nullptr;
   // rebuild
   uint64_t* keys[2]{};
   int* vals[2]{};
@@ -143,7 +143,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.nbr_start = h->nbr_start[l]; a.nbr_split = h->nbr_split[l];
   a.nbr = h->nbr[l]; a.nbr_pid = h->nbr_pid[l];
   a.pft_in = h->pft[p]; a.pft_out = h->pft[1 - p];
-  a.fc = h->fc; a.tow = h->tow;
+  a.fc = h->fc; a.tow = h->tow; a.fc_id = h->fc_id; a.tow_id = h->tow_id;
   a.acc_old = h->acc[t]; a.rot_acc_old = h->racc[t];
   a.fcell = h->fcell[t];
   a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
@@ -160,7 +160,7 @@ void invalidate_graphs(mfixdem_handle* h) {
     }
 }
 
-bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS_) != 0; }
+bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS) != 0; }
 void t_begin(mfixdem_handle* h) {
   if (timing(h)) (void)hipEventRecord(h->ev[0], h->stream);
 }
@@ -630,6 +630,7 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
       return fail(rc);
   }
   if ((rc = dalloc(&h->fc, 3 * S)) || (rc = dalloc(&h->tow, 3 * S)) ||
+      (rc = dalloc(&h->fc_id, 3 * S)) || (rc = dalloc(&h->tow_id, 3 * S)) ||
       (rc = dalloc(&h->fstart, h->n_fine)) || (rc = dalloc(&h->fend, h->n_fine)) ||
       (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
       (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
@@ -670,7 +671,7 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
     dfree(h->keys[b]); dfree(h->vals[b]); dfree(h->nbr_start[b]); dfree(h->nbr_split[b]);
     dfree(h->pair_base[b]); dfree(h->nbr[b]); dfree(h->nbr_pid[b]); dfree(h->pft[b]);
   }
-  dfree(h->fc); dfree(h->tow); dfree(h->sort_tmp); dfree(h->fstart); dfree(h->fend);
+  dfree(h->fc); dfree(h->tow); dfree(h->fc_id); dfree(h->tow_id); dfree(h->sort_tmp); dfree(h->fstart); dfree(h->fend);
   dfree(h->own_cnt); dfree(h->own_tmp); dfree(h->own_key); dfree(h->in_cnt); dfree(h->in_cur);
   dfree(h->tot_cnt); dfree(h->scan_bsum); dfree(h->flags); dfree(h->xe); dfree(h->yn);
   dfree(h->zt); dfree(h->vnode); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
@@ -739,6 +740,8 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
     HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   }
   HIPOK(hipMemset(h->flags, 0, sizeof(int) * kFlagCount));
+  HIPOK(hipMemset(h->fc_id, 0, sizeof(double) * 3 * S));
+  HIPOK(hipMemset(h->tow_id, 0, sizeof(double) * 3 * S));
   h->st = mfixdem_status{};
   h->n_pairs = 0;
   h->inited = false;
@@ -898,8 +901,16 @@ int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {
 
 int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t ld) {
   if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
-  if (fc) CHECKRC(scatter_d3(h, h->fc, fc, ld));
-  if (tow) CHECKRC(scatter_d3(h, h->tow, tow, ld));
+  const int64_t n = h->n, S = h->stride;
+  std::vector<double> tmp;
+  if (fc) {
+    CHECKRC(download(h, tmp, h->fc_id, 3 * S));
+    for (int c = 0; c < 3; c++) std::memcpy(fc + c * ld, tmp.data() + c * S, sizeof(double) * n);
+  }
+  if (tow) {
+    CHECKRC(download(h, tmp, h->tow_id, 3 * S));
+    for (int c = 0; c < 3; c++) std::memcpy(tow + c * ld, tmp.data() + c * S, sizeof(double) * n);
+  }
   return MFIXDEM_OK;
 }
 

@@ -23,6 +23,8 @@ FP64, MIXED = 0, 1
 DRAG_GIDASPOW, DRAG_WEN_YU = 0, 1
 INTERP_GARG_2012 = 0
 DEBUG_STORE_FORCES = 1
+DEBUG_TIME_KERNELS = 2
+TIME_STEP, TIME_REBUILD, TIME_GAS, TIME_GROUPS = 0, 1, 2, 3
 
 ERRORS = {
     0: "OK", 1: "ARG", 2: "HIP", 3: "OOM", 4: "ROGUE", 5: "EPG",
@@ -180,6 +182,9 @@ def _declare(lib):
         "mfixdem_get_cells": ([_P, _IP, _IP, C.c_int64], _I),
         "mfixdem_get_pairs": ([_P, _IP, _IP, _DP, C.c_int64, C.POINTER(C.c_int64)], _I),
         "mfixdem_get_status": ([_P, C.POINTER(Status)], _I),
+        "mfixdem_get_stream": ([_P, C.POINTER(_P)], _I),
+        "mfixdem_get_kernel_times": ([_P, _DP, C.POINTER(C.c_int64), _I], _I),
+        "mfixdem_reset_kernel_times": ([_P], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

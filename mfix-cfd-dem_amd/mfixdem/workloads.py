@@ -93,15 +93,15 @@ def rsa_positions(seed, radii, lo, hi, max_tries=1_000_000):
     return pos
 
 
-def settling_bed(n=20_000, seed=1, steps=2000):
-    """Plain FP64 DEM settling bed (the reference's CPU-runnable case)."""
-    rng = np.random.default_rng(seed)
+def settling_bed(n=20_000, seed=1, box=(0.02, 0.08, 0.02)):
+    """Plain FP64 DEM settling bed (the reference's CPU-runnable case):
+    n spheres of d = 1 mm placed by RSA in the box (X, Y-up, Z)."""
     p = abi.default_params()
     p.coll_model = abi.LSD
     p.integrator = abi.EULER
     p.nphases = 1
     p.max_pip = n
-    p.xlength, p.ylength, p.zlength = 0.02, 0.08, 0.02
+    p.xlength, p.ylength, p.zlength = box
     p.imax = p.jmax = p.kmax = 1
     p.dtsolid = 1.0e-5
     p.grav[:] = [0.0, -9.81, 0.0]
@@ -115,7 +115,7 @@ def settling_bed(n=20_000, seed=1, steps=2000):
     p.ro_s0[0] = 2500.0
     p.max_radius = 0.5e-3
     r = np.full(n, 0.5e-3)
-    pos = rsa_positions(seed, r, [0, 0, 0], [0.02, 0.08, 0.02])
+    pos = rsa_positions(seed, r, [0, 0, 0], list(box))
     parts = make_particles(pos, r, 2500.0)
     return p, parts, None
 
@@ -197,6 +197,20 @@ def fluidised_bed(n=1_000_000, seed=2, phi=0.30, precision=abi.FP64,
     parts = make_particles(pos, r, 2500.0)
     gas = frozen_gas(p)
     return p, parts, gas
+
+
+def jostled(p: abi.Params, parts: Particles, seed=5, shrink=0.98, v0=0.3, w0=200.0):
+    """Test variant of a packing: the spheres are grown by 1/shrink after
+    placement (small initial overlaps with neighbours and walls) and given
+    seeded random translational and angular velocities, so contacts,
+    friction and wall history are exercised from the first step."""
+    rng = np.random.default_rng(seed)
+    n = parts.n
+    radius = parts.radius / shrink
+    vel = v0 * rng.uniform(-1.0, 1.0, (3, n))
+    om = w0 * rng.uniform(-1.0, 1.0, (3, n))
+    p.max_radius = float(radius.max())
+    return p, make_particles(parts.pos, radius, parts.ro_sol, vel, om, parts.phase)
 
 
 def dem01(kn=1.0e4, en=0.9, integrator=abi.EULER):

@@ -1,0 +1,254 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle on
+the same seeded inputs.
+
+Bar: DES cells, fluid cells, neighbour pair sets (and their row order) are
+bit-exact.  FP64 trajectories are bit-exact wherever no libm pow enters
+(LSD/Hertz contact, walls, integrators; the library is built with
+-ffp-contract=off and the oracle keeps the reference's expression order).
+With gas coupling the Gidaspow/Wen-Yu laws call pow (Re^0.687, eps^-2.65),
+whose ocml and glibc results may differ in the last ulp: those cases are
+held to the tolerances written in each test.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _paths  # noqa: F401
+from mfixdem import abi, workloads
+from mfixdem.system import DemSystem, pair_table
+from oracle_py import OracleSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def both(p, parts, gas=None, debug=0):
+    g, o = DemSystem(p), OracleSystem(p)
+    for s in (g, o):
+        s.set_particles(parts)
+        if gas is not None:
+            s.set_gas(gas)
+        s.init()
+    if debug:
+        g.set_debug(debug)
+    return g, o
+
+
+def settling(n=2000, seed=3, jostle=True, integrator=abi.EULER):
+    p, parts, _ = workloads.settling_bed(n=n, seed=seed, box=(0.01, 0.035, 0.01))
+    p.integrator = integrator
+    if jostle:
+        p, parts = workloads.jostled(p, parts, seed=seed + 1)
+    return p, parts, None
+
+
+def fluidised(n=4000, seed=2, coupled=True, precision=abi.FP64, jostle=True):
+    p, parts, gas = workloads.fluidised_bed(n=n, seed=seed, lx=0.02, ly=0.04, lz=0.02,
+                                            cells=(10, 20, 10), dg=(0, 0, 0),
+                                            precision=precision)
+    if jostle:
+        p, parts = workloads.jostled(p, parts, seed=seed + 1, v0=0.2)
+    p.coupled = 1 if coupled else 0
+    return p, parts, (gas if coupled else None)
+
+
+def assert_same_pairs(g, o):
+    gi, gn, gp = g.get_pairs()
+    oi, on, op = o.get_pairs()
+    assert np.array_equal(gi, oi), "NEIGHBOR_INDEX differs"
+    assert np.array_equal(gn, on), "NEIGHBORS differs (set or row order)"
+    return gp, op
+
+
+def test_derived_parameters_bit_exact():
+    for p, _, _ in (settling(), fluidised()):
+        g, o = DemSystem(p), OracleSystem(p)
+        dg, do = g.derived(), o.derived()
+        for name, _t in abi.Derived._fields_:
+            a, b = getattr(dg, name), getattr(do, name)
+            if hasattr(a, "__len__"):
+                assert np.array_equal(np.ctypeslib.as_array(a), np.ctypeslib.as_array(b)), name
+            else:
+                assert a == b, name
+        g.close()
+
+
+@pytest.mark.parametrize("case", ["settling", "fluidised"])
+def test_init_cells_and_pairs_bit_exact(case):
+    p, parts, gas = settling() if case == "settling" else fluidised()
+    g, o = both(p, parts, gas)
+    gd, gpijk = g.get_cells()
+    od, opijk = o.get_cells()
+    assert np.array_equal(gd, od), "DG_PIJK differs"
+    if gas is not None:
+        assert np.array_equal(gpijk, opijk), "PIJK differs"
+    gp, op = assert_same_pairs(g, o)
+    assert not gp.any() and not op.any()
+    g.close()
+
+
+@pytest.mark.parametrize("integrator", [abi.EULER, abi.ADAMS_BASHFORTH], ids=["euler", "ab2"])
+def test_settling_trajectory_bit_exact(integrator):
+    """LSD, walls, friction: 60 steps (rebuilds at 1, 25, 50), FP64."""
+    p, parts, _ = settling(integrator=integrator)
+    g, o = both(p, parts, debug=abi.DEBUG_STORE_FORCES)
+    for nn in range(1, 61):
+        g.step(nn)
+        o.step(nn)
+        if nn in (1, 2, 24, 25, 26, 60):
+            gf, gt = g.get_forces()
+            of, ot = o.get_forces()
+            assert np.array_equal(gf, of), f"FC differs at step {nn}"
+            assert np.array_equal(gt, ot), f"TOW differs at step {nn}"
+    for a, b in zip(g.get_particles(), o.get_particles()):
+        assert np.array_equal(a, b)
+    gp, op = assert_same_pairs(g, o)
+    assert np.array_equal(gp, op), "PFT history differs"
+    assert o.status().n_contacts > 50
+    g.close()
+
+
+def test_hertz_polydisperse_walls_bit_exact():
+    """Hertz, polydisperse 0.5-2 mm, walls, no gas: 60 steps bit-exact."""
+    p, parts, _ = fluidised(coupled=False)
+    g, o = both(p, parts)
+    g.run(1, 60)
+    o.run(1, 60)
+    for a, b in zip(g.get_particles(), o.get_particles()):
+        assert np.array_equal(a, b)
+    gp, op = assert_same_pairs(g, o)
+    assert np.array_equal(gp, op)
+    g.close()
+
+
+def test_coupled_gas_deposition_and_drag():
+    """GARG_2012 deposition + Gidaspow drag.  Deposition (cells, weights,
+    summation order) is bit-exact at init; the trajectory after 60 steps
+    differs only through pow ulps: |dx| <= 1e-12 m, relative EP_G <= 1e-12."""
+    p, parts, gas = fluidised()
+    g, o = both(p, parts, gas)
+    ge, gr = g.get_gas()
+    oe, orr = o.get_gas()
+    assert np.array_equal(ge, oe) and np.array_equal(gr, orr), "init deposition differs"
+    g.run(1, 60)
+    o.run(1, 60)
+    gp, gv, gw, gs = g.get_particles()
+    op, ov, ow, os_ = o.get_particles()
+    assert np.array_equal(gs, os_)
+    assert np.max(np.abs(gp - op)) <= 1e-12
+    assert np.max(np.abs(gv - ov)) <= 1e-9 * max(1.0, np.max(np.abs(ov)))
+    ge, gr = g.get_gas()
+    oe, orr = o.get_gas()
+    assert np.max(np.abs(ge - oe)) <= 1e-12
+    assert np.max(np.abs(gr - orr)) <= 1e-12 * np.max(np.abs(orr))
+    gd, gpijk = g.get_cells()
+    od, opijk = o.get_cells()
+    assert np.array_equal(gpijk, opijk)
+    g.close()
+
+
+def test_unfused_entry_points_match_fused():
+    """calc_force_dem / calc_drag_des / cfnewvalues / desgrid_pic /
+    neighbour / particles_in_cell / comp_mean_fields called one by one give
+    the same state as the fused step."""
+    p, parts, gas = fluidised(n=2000)
+    a, _ = both(p, parts, gas)
+    b = DemSystem(p)
+    b.set_particles(parts)
+    b.set_gas(gas)
+    b.init()
+    for nn in range(1, 31):
+        a.step(nn)
+        b.calc_force_dem()
+        b.calc_drag_des()
+        b.cfnewvalues()
+        if nn == 1 or nn % p.neighbor_search_n == 0:
+            b.desgrid_pic()
+            b.neighbour()
+        b.particles_in_cell()
+        b.comp_mean_fields()
+    for x, y in zip(a.get_particles(), b.get_particles()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.get_gas()[0], b.get_gas()[0])
+    a.close()
+    b.close()
+
+
+def test_graph_replay_matches_eager_steps():
+    """mfixdem_run (hipGraph replay between rebuilds) == step-by-step."""
+    p, parts, gas = fluidised(n=2000)
+    a, _ = both(p, parts, gas)
+    b = DemSystem(p)
+    b.set_particles(parts)
+    b.set_gas(gas)
+    b.init()
+    b.set_debug(abi.DEBUG_STORE_FORCES)  # forces the eager path
+    a.run(1, 55)
+    for nn in range(1, 56):
+        b.step(nn)
+    for x, y in zip(a.get_particles(), b.get_particles()):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("k", [0, 5, 19, 24], ids=lambda k: f"run{k}")
+def test_dem01_on_gpu_matches_reference_output(k):
+    import json
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "dem01_post.json")))
+    run = gold["runs"][k]
+    integ = abi.EULER if run["scheme"] == "EULER" else abi.ADAMS_BASHFORTH
+    p, parts, _ = workloads.dem01(kn=run["kn"], en=run["en"], integrator=integ)
+    g = DemSystem(p)
+    g.set_particles(parts)
+    g.init()
+    dt = g.derived().dtsolid
+    rows = np.array(run["rows"])
+    steps = np.rint(rows[:, 0] / dt).astype(int)
+    ys = {0: 0.5}
+    for nn in range(1, steps.max() + 1):
+        g.step(nn)
+        if nn in set(steps.tolist()):
+            ys[nn] = g.get_particles()[0][1, 0]
+    y = np.array([ys[s] for s in steps])
+    assert np.max(np.abs(y - rows[:, 1])) <= 1.0e-8
+    g.close()
+
+
+def test_fortran_driver_through_c_abi():
+    """The Fortran ISO_C_BINDING shim drives DEM01 to the same trajectory."""
+    exe = os.path.join(_paths.PKG, "lib", "dem01_driver")
+    out = subprocess.run([exe, "10000", "0.9", "0", "300"], capture_output=True, text=True,
+                         timeout=120, check=True).stdout.split("\n")
+    fy = np.array([float(l.split()[2]) for l in out if l.strip()])
+    p, parts, _ = workloads.dem01(kn=1e4, en=0.9)
+    o = OracleSystem(p)
+    o.set_particles(parts)
+    o.init()
+    ys = []
+    for nn in range(1, 301):
+        o.step(nn)
+        ys.append(o.get_particles()[0][1, 0])
+    assert np.array_equal(fy, np.array(ys))
+
+
+def test_mixed_precision_force_error():
+    """FP32 contact and drag arithmetic: max relative force error against
+    the FP64 path on the same state <= 1e-4 (error / max |F|)."""
+    p64, parts, gas = fluidised(n=4000)
+    pmx, _, _ = fluidised(n=4000, precision=abi.MIXED)
+    a, _ = both(p64, parts, gas, debug=abi.DEBUG_STORE_FORCES)
+    b = DemSystem(pmx)
+    b.set_particles(parts)
+    b.set_gas(gas)
+    b.init()
+    b.set_debug(abi.DEBUG_STORE_FORCES)
+    a.step(1)
+    b.step(1)
+    fa, ta = a.get_forces()
+    fb, tb = b.get_forces()
+    scale = np.max(np.abs(fa))
+    assert np.max(np.abs(fa - fb)) / scale <= 1e-4
+    a.close()
+    b.close()

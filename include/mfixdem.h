@@ -68,7 +68,10 @@ enum { MFIXDEM_INTERP_GARG_2012 = 0 };                       /* DES_INTERP_SCHEM
 /* debug flags for mfixdem_set_debug */
 enum {
   MFIXDEM_DEBUG_STORE_FORCES = 1, /* keep FC/TOW of the last step for mfixdem_get_forces */
-  MFIXDEM_DEBUG_TIME_KERNELS = 2  /* eager launches, HIP-event time per kernel group */
+  MFIXDEM_DEBUG_TIME_KERNELS = 2, /* eager launches, HIP-event time per kernel group */
+  MFIXDEM_DEBUG_REFERENCE_DEPOSITION = 4 /* eps_s deposition summed in the reference's
+                                     serial order (PIC lists, node gather) instead
+                                     of order-independent fixed-point sums */
 };
 /* kernel groups reported by mfixdem_get_kernel_times */
 enum { MFIXDEM_TIME_STEP = 0, MFIXDEM_TIME_REBUILD = 1, MFIXDEM_TIME_GAS = 2,

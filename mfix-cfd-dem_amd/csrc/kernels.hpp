@@ -41,6 +41,7 @@ struct StepPtrs {
   const double *xe, *yn, *zt;
   const double *vnode, *p_force, *ep_g;
   const uint8_t* fluid;
+  unsigned long long* rops_fix;  // fixed-point node mass sums being accumulated
   int* flags;
 };
 
@@ -107,6 +108,43 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
     for (int d = 0; d < 3; d++) fc[d] = fc[d] + (double)(fgp * vs[d]);
   }
   for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
+}
+
+// COMP_MEAN_FIELDS0 particle -> node deposition (comp_mean_fields0.f:167-210)
+// as order-independent fixed-point sums: each corner weight * PMASS (the
+// reference's TEMP1, same FP64 product) is scaled by an exact power of two,
+// rounded to an integer and added with a 64-bit integer atomic, so the node
+// totals do not depend on the order particles arrive in.
+__device__ __forceinline__ void deposit_fixed(const DevParams& P, const double* xe,
+                                              const double* yn, const double* zt, int c,
+                                              const double pos[3], double m, int ph,
+                                              unsigned long long* fix) {
+  const int cj = c % P.nj + 1;
+  const int t = c / P.nj;
+  const int ci = t % P.ni + 1;
+  const int ck = t / P.ni + 1;
+  const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
+  const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
+  double zx = pos[0] - x0, zy = pos[1] - y0;
+  zx = zx / dxx;
+  zy = zy / dyy;
+  double zz = pos[2] - z0;
+  zz = zz / dzz;
+  const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
+  unsigned long long* f = fix + (size_t)ph * P.ncell;
+  const double sc = P.dep_scale[ph];
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const double w = xv[ii] * yv[jj] * zv[kk];
+        const double v = w * m;
+        const unsigned long long q = (unsigned long long)__double2ll_rn(v * sc);
+        atomicAdd(&f[base + jj + ii * P.nj + kk * P.nj * P.ni], q);
+      }
 }
 
 // One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
@@ -301,8 +339,29 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
     const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
     const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
-    a.fcell[i] = (I && J && K) ? (J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni : -1;
+    const int c = (I && J && K) ? (J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni : -1;
+    a.fcell[i] = c;
+    // ---- COMP_MEAN_FIELDS deposition of the new position, fused
+    if (P.deposit && c >= 0 && a.fluid[c])
+      deposit_fixed(P, a.xe, a.yn, a.zt, c, pos, mi, phi, a.rops_fix);
   }
+}
+
+// deposition of the current positions (set-up and the unfused entry point)
+__global__ __launch_bounds__(kBS) void k_deposit_atomic(DevParams P, const double* pos,
+                                                        const double* pmass,
+                                                        const uint8_t* phase,
+                                                        const uint8_t* state, const int* fcell,
+                                                        const uint8_t* fluid, const double* xe,
+                                                        const double* yn, const double* zt,
+                                                        unsigned long long* fix) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= P.n || state[i] == 0) return;
+  const int c = fcell[i];
+  if (c < 0 || !fluid[c]) return;
+  const int S = P.stride;
+  const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+  deposit_fixed(P, xe, yn, zt, c, x, pmass[i], phase[i], fix);
 }
 
 // ---------------------------------------------------------------- scans
@@ -848,15 +907,28 @@ __global__ __launch_bounds__(kBS) void k_deposit_nodes(int imax, int jmax, int k
 
 // node -> cell ROP_S (comp_mean_fields0.f:298-345), ROP_S/VOL (:347-370) and
 // CALC_EPG_DES (calc_epg_des.f:95-150).  Nodes visited K, J, I ascending.
+// FIXED: node sums come from the fixed-point accumulators (converted exactly
+// by the inverse power-of-two scale after one rounding to FP64), and the
+// other accumulator buffer is cleared for the next step.
+struct FieldPtrs {
+  const double* rops_node;               // FP64 node sums (reference-order path)
+  const unsigned long long* fix;         // fixed-point node sums
+  unsigned long long* fix_clear;         // buffer to zero for the next step
+  const double *vol, *vol_surr, *ro_s;
+  const uint8_t* fluid;
+  double *rop_s, *ep_g;
+  int* flags;
+};
+template <bool FIXED>
 __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kmax, int ni, int nj,
-                                                     int nk, int nph, const double* rops_node,
-                                                     const double* vol, const double* vol_surr,
-                                                     const uint8_t* fluid, const double* ro_s,
-                                                     double packed_eps, double* rop_s,
-                                                     double* ep_g, int* flags) {
+                                                     int nk, int nph, double packed_eps,
+                                                     double inv_scale0, double inv_scale1,
+                                                     double inv_scale2, double inv_scale3,
+                                                     FieldPtrs f) {
   const int c = blockIdx.x * kBS + threadIdx.x;
   const int ncell = ni * nj * nk;
   if (c >= ncell) return;
+  const double inv[kMaxPh] = {inv_scale0, inv_scale1, inv_scale2, inv_scale3};
   const int cj = c % nj + 1;
   const int ci = (c / nj) % ni + 1;
   const int ck = c / (nj * ni) + 1;
@@ -868,26 +940,55 @@ __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kma
       for (int I = ci - 1; I <= ci; I++) {
         if (I < 1 || I > imax + 1) continue;
         const int node = (J - 1) + (I - 1) * nj + (K - 1) * nj * ni;
-        const double vs = vol_surr[node];
+        const double vs = f.vol_surr[node];
         if (vs == 0.0) continue;
         for (int m = 0; m < nph; m++) {
-          const double dens = rops_node[(size_t)m * ncell + node] / vs;
-          acc[m] = acc[m] + dens * vol[c];
+          double nodev;
+          if (FIXED) {
+            const unsigned long long q = f.fix[(size_t)m * ncell + node];
+            if (q >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
+            nodev = (double)q * inv[m];
+          } else {
+            nodev = f.rops_node[(size_t)m * ncell + node];
+          }
+          const double dens = nodev / vs;
+          acc[m] = acc[m] + dens * f.vol[c];
         }
       }
     }
   }
-  if (fluid[c])
+  if (FIXED)
+    for (int m = 0; m < nph; m++) f.fix_clear[(size_t)m * ncell + c] = 0ull;
+  if (f.fluid[c])
     for (int m = 0; m < nph; m++)
-      if (acc[m] > 0.0) acc[m] = acc[m] / vol[c];
-  for (int m = 0; m < nph; m++) rop_s[(size_t)m * ncell + c] = acc[m];
-  if (fluid[c]) {
+      if (acc[m] > 0.0) acc[m] = acc[m] / f.vol[c];
+  for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = acc[m];
+  if (f.fluid[c]) {
     double sum_eps = 0.0;
-    for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / ro_s[m];
+    for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / f.ro_s[m];
     const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);
-    ep_g[c] = e;
-    if (e <= 0.0 || e > 1.0) atomicAdd(&flags[kFlagEpg], 1);
+    f.ep_g[c] = e;
+    if (e <= 0.0 || e > 1.0) atomicAdd(&f.flags[kFlagEpg], 1);
   }
+}
+
+// PARTICLES_IN_CELL deletion of particles outside the fluid grid
+// (particles_in_cell.f:175-184), for particles the fused step flagged
+__global__ __launch_bounds__(kBS) void k_delete_outside(int n, int S, const int* fcell,
+                                                        uint8_t* state, uint8_t* wmask,
+                                                        double* pos, double* vel, double* omega,
+                                                        double* radius, int* flags) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n || state[i] == 0 || fcell[i] >= 0) return;
+  state[i] = 0;
+  wmask[i] = 0;
+  for (int d = 0; d < 3; d++) {
+    pos[d * S + i] = 0.0;
+    vel[d * S + i] = 0.0;
+    omega[d * S + i] = 0.0;
+  }
+  radius[i] = 0.0;
+  atomicAdd(&flags[kFlagOutOfDomain], 1);
 }
 
 // ------------------------------------------------------------- exports

@@ -81,14 +81,17 @@ struct mfixdem_handle {
          *vol_surr = nullptr, *ro_s = nullptr;
   uint8_t* fluid = nullptr;
   int *pic_cnt = nullptr, *pic_off = nullptr, *pic_cur = nullptr, *pic = nullptr;
+  unsigned long long* rops_fix[2]{};  // fixed-point node sums, double-buffered
+  int fb = 0;
+  double dep_inv[MFIXDEM_MAX_PHASES]{};
   uint16_t* cell_facets = nullptr;
   double* stage = nullptr;  // export staging, 3*n doubles
   // status
   mfixdem_status st{};
   bool inited = false, gas_set = false, first_pass = true, binned = false;
   int debug = 0;
-  // graphs: [cur parity]
-  hipGraphExec_t graph[2]{nullptr, nullptr};
+  // graphs: [cur parity][deposition buffer]
+  hipGraphExec_t graph[2][2]{};
   // kernel timing
   hipEvent_t ev[2]{nullptr, nullptr};
   double t_ms[kTimeSlots]{};
@@ -148,16 +151,22 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.fcell = h->fcell[t];
   a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
   a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
+  a.rops_fix = h->rops_fix[h->fb];
   a.flags = h->flags;
   return a;
 }
 
 void invalidate_graphs(mfixdem_handle* h) {
-  for (auto& g : h->graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto& row : h->graph)
+    for (auto& g : row)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+}
+
+bool ref_deposition(const mfixdem_handle* h) {
+  return (h->debug & MFIXDEM_DEBUG_REFERENCE_DEPOSITION) != 0;
 }
 
 bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS) != 0; }
@@ -179,6 +188,7 @@ int32_t launch_step(mfixdem_handle* h) {
   DevParams P = h->P;
   P.first_pass = h->first_pass ? 1 : 0;
   P.store_forces = (h->debug & MFIXDEM_DEBUG_STORE_FORCES) ? 1 : 0;
+  P.deposit = h->S.p.coupled && !ref_deposition(h);
   StepPtrs a = step_ptrs(h);
   if (h->S.p.precision == MFIXDEM_MIXED)
     hipLaunchKernelGGL((k_step<true, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
@@ -354,26 +364,73 @@ int32_t do_pic_lists(mfixdem_handle* h) {
   return MFIXDEM_OK;
 }
 
-// COMP_MEAN_FIELDS (GARG) + CALC_EPG_DES
-int32_t do_mean_fields(mfixdem_handle* h) {
+FieldPtrs field_ptrs(mfixdem_handle* h) {
+  FieldPtrs f{};
+  f.rops_node = h->rops_node;
+  f.fix = h->rops_fix[h->fb];
+  f.fix_clear = h->rops_fix[1 - h->fb];
+  f.vol = h->vol; f.vol_surr = h->vol_surr; f.ro_s = h->ro_s; f.fluid = h->fluid;
+  f.rop_s = h->rop_s; f.ep_g = h->ep_g; f.flags = h->flags;
+  return f;
+}
+
+// node sums -> ROP_S, EP_G (comp_mean_fields0.f:298-370, calc_epg_des.f)
+int32_t do_cell_fields(mfixdem_handle* h, bool fixed) {
   const mfixdem_params& p = h->S.p;
-  const int nnode = (p.imax + 1) * (p.jmax + 1) * (p.kmax + 1);
-  const int t = h->tw, c = h->cur;
-  hipLaunchKernelGGL(k_deposit_nodes, dim3(nblk(nnode)), dim3(kBS), 0, h->stream, p.imax, p.jmax,
-                     p.kmax, h->S.ni, h->S.nj, p.nphases, h->stride, h->xe, h->yn, h->zt,
-                     h->fluid, h->pic_off, h->pic, h->pos[c], h->pmass[t], h->phase[t],
-                     h->rops_node, h->S.ncell);
   const double packed = (p.des_epg_clip != kUndef) ? 1.0 - p.des_epg_clip : 1.0;
-  hipLaunchKernelGGL(k_cell_fields, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream, p.imax,
-                     p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, h->rops_node, h->vol,
-                     h->vol_surr, h->fluid, h->ro_s, packed, h->rop_s, h->ep_g, h->flags);
+  FieldPtrs f = field_ptrs(h);
+  const double* iv = h->dep_inv;
+  if (fixed) {
+    hipLaunchKernelGGL(k_cell_fields<true>, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
+                       p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, packed,
+                       iv[0], iv[1], iv[2], iv[3], f);
+    h->fb = 1 - h->fb;
+  } else {
+    hipLaunchKernelGGL(k_cell_fields<false>, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
+                       p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, packed,
+                       iv[0], iv[1], iv[2], iv[3], f);
+  }
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
 
-int32_t gas_tail(mfixdem_handle* h) {
+int32_t delete_outside(mfixdem_handle* h) {
+  const int t = h->tw, c = h->cur;
+  hipLaunchKernelGGL(k_delete_outside, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->n, h->stride,
+                     h->fcell[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c],
+                     h->radius[t], h->flags);
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+// reference-order deposition: PIC lists sorted by particle id, node gather
+int32_t ref_mean_fields(mfixdem_handle* h) {
+  const mfixdem_params& p = h->S.p;
   CHECKRC(do_pic_lists(h));
-  return do_mean_fields(h);
+  const int nnode = (p.imax + 1) * (p.jmax + 1) * (p.kmax + 1);
+  hipLaunchKernelGGL(k_deposit_nodes, dim3(nblk(nnode)), dim3(kBS), 0, h->stream, p.imax, p.jmax,
+                     p.kmax, h->S.ni, h->S.nj, p.nphases, h->stride, h->xe, h->yn, h->zt,
+                     h->fluid, h->pic_off, h->pic, h->pos[h->cur], h->pmass[h->tw],
+                     h->phase[h->tw], h->rops_node, h->S.ncell);
+  HIPOK(hipGetLastError());
+  return do_cell_fields(h, false);
+}
+
+// deposition of the current positions into the fixed-point accumulators
+int32_t deposit_current(mfixdem_handle* h) {
+  DevParams P = h->P;
+  hipLaunchKernelGGL(k_deposit_atomic, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
+                     h->pos[h->cur], h->pmass[h->tw], h->phase[h->tw], h->state[h->tw],
+                     h->fcell[h->tw], h->fluid, h->xe, h->yn, h->zt, h->rops_fix[h->fb]);
+  HIPOK(hipGetLastError());
+  return do_cell_fields(h, true);
+}
+
+// the gas-side tail of a step after the fused kernel (and any rebuild)
+int32_t gas_tail(mfixdem_handle* h) {
+  if (ref_deposition(h)) return ref_mean_fields(h);  // deletion inside do_pic_lists
+  CHECKRC(delete_outside(h));
+  return do_cell_fields(h, true);
 }
 
 // the part of a step that is identical between rebuilds (graph-capturable)
@@ -390,15 +447,16 @@ int32_t one_step(mfixdem_handle* h, int32_t nn) {
   const bool use_graph = !rebuild && !h->first_pass && !timing(h) &&
                          !(h->debug & MFIXDEM_DEBUG_STORE_FORCES);
   if (use_graph) {
-    hipGraphExec_t& g = h->graph[h->cur];
+    hipGraphExec_t& g = h->graph[h->cur][h->fb];
     if (!g) {
-      const int c0 = h->cur, p0 = h->pcur;
+      const int c0 = h->cur, p0 = h->pcur, f0 = h->fb;
       HIPOK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       int32_t rc = plain_step_body(h);
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(h->stream, &graph);
       h->cur = c0;
       h->pcur = p0;
+      h->fb = f0;
       if (rc != MFIXDEM_OK) return rc;
       HIPOK(e);
       HIPOK(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0));
@@ -407,6 +465,7 @@ int32_t one_step(mfixdem_handle* h, int32_t nn) {
     HIPOK(hipGraphLaunch(g, h->stream));
     h->cur = 1 - h->cur;
     h->pcur = 1 - h->pcur;
+    if (h->S.p.coupled) h->fb = 1 - h->fb;
   } else {
     t_begin(h);
     CHECKRC(launch_step<kModeFused>(h));
@@ -498,7 +557,7 @@ void fill_devparams(mfixdem_handle* h) {
     P.hkwn[a] = S.d.hert_kwn[a];
     P.hkwt[a] = S.d.hert_kwt[a];
   }
-  P.ni = S.ni; P.nj = S.nj; P.nk = S.nk;
+  P.ni = S.ni; P.nj = S.nj; P.nk = S.nk; P.ncell = S.ncell;
   P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
   for (size_t f = 0; f < S.facets.size(); f++) {
     FacetDev& F = P.facets[f];
@@ -645,8 +704,13 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
       (rc = dalloc(&h->fluid, nc)) || (rc = dalloc(&h->pic_cnt, nc)) ||
       (rc = dalloc(&h->pic_off, nc + 1)) || (rc = dalloc(&h->pic_cur, nc)) ||
       (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, h->S.dg_ncell)) ||
-      (rc = dalloc(&h->stage, 3 * n)))
+      (rc = dalloc(&h->stage, 3 * n)) ||
+      (rc = dalloc(&h->rops_fix[0], nc * p->nphases)) ||
+      (rc = dalloc(&h->rops_fix[1], nc * p->nphases)))
     return fail(rc);
+  for (int b = 0; b < 2; b++)
+    if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * p->nphases) != hipSuccess)
+      return fail(MFIXDEM_ERR_HIP);
   if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
                                          h->vals[0], h->vals[1], (int)n, 0, 30 + h->G.gid_bits,
                                          h->stream) != hipSuccess)
@@ -677,7 +741,7 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
   dfree(h->zt); dfree(h->vnode); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
   dfree(h->rops_node); dfree(h->vol); dfree(h->vol_surr); dfree(h->ro_s); dfree(h->fluid);
   dfree(h->pic_cnt); dfree(h->pic_off); dfree(h->pic_cur); dfree(h->pic);
-  dfree(h->cell_facets); dfree(h->stage);
+  dfree(h->cell_facets); dfree(h->stage); dfree(h->rops_fix[0]); dfree(h->rops_fix[1]);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -718,8 +782,23 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
     PH[q] = (uint8_t)ph;
     ST[q] = 1;
   }
+  // Fixed-point scale per phase: the mass a node can collect is bounded by
+  // the solids in its 8 cells, <= 8 * VOL * rho_max; allow twice that and
+  // keep the int64 sum below 2^62.
+  {
+    double rho_max[MFIXDEM_MAX_PHASES] = {0, 0, 0, 0};
+    for (int64_t q = 0; q < n; q++) rho_max[PH[q]] = std::max(rho_max[PH[q]], ro_sol[q]);
+    const double vcell = h->S.dx * h->S.dy * h->S.dz;
+    for (int m = 0; m < MFIXDEM_MAX_PHASES; m++) {
+      const double bound = 16.0 * vcell * std::max(rho_max[m], 1.0);
+      const int e = (int)std::floor(std::log2(std::ldexp(1.0, 62) / bound));
+      h->P.dep_scale[m] = std::ldexp(1.0, e);
+      h->dep_inv[m] = std::ldexp(1.0, -e);
+    }
+  }
   invalidate_graphs(h);
   h->cur = h->pcur = h->tw = h->lt = 0;
+  h->fb = 0;
   HIPOK(hipMemcpy(h->pos[0], P3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vel[0], V3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->omega[0], W3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
@@ -825,7 +904,14 @@ int32_t mfixdem_init(mfixdem_handle* h) {
   }
   CHECKRC(do_bin(h));
   CHECKRC(do_neighbour(h));
-  if (h->S.p.coupled) CHECKRC(gas_tail(h));
+  if (h->S.p.coupled) {
+    if (ref_deposition(h)) {
+      CHECKRC(ref_mean_fields(h));
+    } else {
+      CHECKRC(delete_outside(h));
+      CHECKRC(deposit_current(h));
+    }
+  }
   HIPOK(hipStreamSynchronize(h->stream));
   h->inited = true;
   h->first_pass = true;
@@ -885,11 +971,22 @@ int32_t mfixdem_particles_in_cell(mfixdem_handle* h) {
                      h->S.nj, h->S.nk, h->xe, h->yn, h->zt, h->pos[h->cur], h->state[h->tw],
                      h->fcell[h->tw]);
   HIPOK(hipGetLastError());
-  return do_pic_lists(h);
+  if (ref_deposition(h)) return do_pic_lists(h);
+  return delete_outside(h);
 }
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h) {
   if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
-  return do_mean_fields(h);
+  if (ref_deposition(h)) {
+    const mfixdem_params& p = h->S.p;
+    const int nnode = (p.imax + 1) * (p.jmax + 1) * (p.kmax + 1);
+    hipLaunchKernelGGL(k_deposit_nodes, dim3(nblk(nnode)), dim3(kBS), 0, h->stream, p.imax,
+                       p.jmax, p.kmax, h->S.ni, h->S.nj, p.nphases, h->stride, h->xe, h->yn,
+                       h->zt, h->fluid, h->pic_off, h->pic, h->pos[h->cur], h->pmass[h->tw],
+                       h->phase[h->tw], h->rops_node, h->S.ncell);
+    HIPOK(hipGetLastError());
+    return do_cell_fields(h, false);
+  }
+  return deposit_current(h);
 }
 
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {

@@ -24,6 +24,7 @@ DRAG_GIDASPOW, DRAG_WEN_YU = 0, 1
 INTERP_GARG_2012 = 0
 DEBUG_STORE_FORCES = 1
 DEBUG_TIME_KERNELS = 2
+DEBUG_REFERENCE_DEPOSITION = 4
 TIME_STEP, TIME_REBUILD, TIME_GAS, TIME_GROUPS = 0, 1, 2, 3
 
 ERRORS = {

@@ -43,7 +43,6 @@ struct DevParams {
   double etan_w[kMaxPh], etat_w[kMaxPh], hkwn[kMaxPh], hkwt[kMaxPh];
   // fluid grid
   int ni, nj, nk, ncell;
-  int deposit;      // fused fixed-point eps_s deposition on
   double ro_g0, mu_g0;
   double dep_scale[kMaxPh];  // fixed-point scale of the node mass sums (power of 2)
   FacetDev facets[kMaxF];

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstdint>
 
 #include "device_physics.hpp"
@@ -41,7 +42,6 @@ struct StepPtrs {
   const double *xe, *yn, *zt;
   const double *vnode, *p_force, *ep_g;
   const uint8_t* fluid;
-  unsigned long long* rops_fix;  // fixed-point node mass sums being accumulated
   int* flags;
 };
 
@@ -108,43 +108,6 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
     for (int d = 0; d < 3; d++) fc[d] = fc[d] + (double)(fgp * vs[d]);
   }
   for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
-}
-
-// COMP_MEAN_FIELDS0 particle -> node deposition (comp_mean_fields0.f:167-210)
-// as order-independent fixed-point sums: each corner weight * PMASS (the
-// reference's TEMP1, same FP64 product) is scaled by an exact power of two,
-// rounded to an integer and added with a 64-bit integer atomic, so the node
-// totals do not depend on the order particles arrive in.
-__device__ __forceinline__ void deposit_fixed(const DevParams& P, const double* xe,
-                                              const double* yn, const double* zt, int c,
-                                              const double pos[3], double m, int ph,
-                                              unsigned long long* fix) {
-  const int cj = c % P.nj + 1;
-  const int t = c / P.nj;
-  const int ci = t % P.ni + 1;
-  const int ck = t / P.ni + 1;
-  const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
-  const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
-  double zx = pos[0] - x0, zy = pos[1] - y0;
-  zx = zx / dxx;
-  zy = zy / dyy;
-  double zz = pos[2] - z0;
-  zz = zz / dzz;
-  const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
-  const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
-  unsigned long long* f = fix + (size_t)ph * P.ncell;
-  const double sc = P.dep_scale[ph];
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-    for (int jj = 0; jj < 2; jj++)
-#pragma unroll
-      for (int ii = 0; ii < 2; ii++) {
-        const double w = xv[ii] * yv[jj] * zv[kk];
-        const double v = w * m;
-        const unsigned long long q = (unsigned long long)__double2ll_rn(v * sc);
-        atomicAdd(&f[base + jj + ii * P.nj + kk * P.nj * P.ni], q);
-      }
 }
 
 // One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
@@ -341,27 +304,92 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
     const int c = (I && J && K) ? (J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni : -1;
     a.fcell[i] = c;
-    // ---- COMP_MEAN_FIELDS deposition of the new position, fused
-    if (P.deposit && c >= 0 && a.fluid[c])
-      deposit_fixed(P, a.xe, a.yn, a.zt, c, pos, mi, phi, a.rops_fix);
   }
 }
 
-// deposition of the current positions (set-up and the unfused entry point)
-__global__ __launch_bounds__(kBS) void k_deposit_atomic(DevParams P, const double* pos,
-                                                        const double* pmass,
-                                                        const uint8_t* phase,
-                                                        const uint8_t* state, const int* fcell,
-                                                        const uint8_t* fluid, const double* xe,
-                                                        const double* yn, const double* zt,
-                                                        unsigned long long* fix) {
+// COMP_MEAN_FIELDS0 deposition of the current positions, tiled: the
+// workgroup's 256 particles (consecutive in the Morton order, so spatially
+// compact) add their corner contributions into an 8x8x8-node int64 tile in
+// LDS anchored at the workgroup's smallest node; each touched tile node then
+// costs one global atomic.  Contributions outside the tile go straight to
+// global memory.  Integer sums: the result does not depend on the order.
+constexpr int kTile = 8;
+__global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double* pos,
+                                                       const double* pmass, const uint8_t* phase,
+                                                       const uint8_t* state, const int* fcell,
+                                                       const uint8_t* fluid, const double* xe,
+                                                       const double* yn, const double* zt,
+                                                       unsigned long long* fix) {
+  __shared__ unsigned long long tile[kTile * kTile * kTile];
+  __shared__ int lo[3];
   const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= P.n || state[i] == 0) return;
-  const int c = fcell[i];
-  if (c < 0 || !fluid[c]) return;
   const int S = P.stride;
-  const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
-  deposit_fixed(P, xe, yn, zt, c, x, pmass[i], phase[i], fix);
+  int c = -1, ci = 0, cj = 0, ck = 0, ph = 0;
+  if (i < P.n && state[i] != 0) {
+    c = fcell[i];
+    if (c >= 0 && !fluid[c]) c = -1;
+  }
+  if (c >= 0) {
+    cj = c % P.nj + 1;
+    const int t = c / P.nj;
+    ci = t % P.ni + 1;
+    ck = t / P.ni + 1;
+    ph = phase[i];
+  }
+  for (int q = threadIdx.x; q < kTile * kTile * kTile; q += kBS) tile[q] = 0ull;
+  if (threadIdx.x == 0) lo[0] = lo[1] = lo[2] = INT_MAX;
+  __syncthreads();
+  if (c >= 0) {
+    atomicMin(&lo[0], ci - 1);
+    atomicMin(&lo[1], cj - 1);
+    atomicMin(&lo[2], ck - 1);
+  }
+  __syncthreads();
+  const int bi = lo[0], bj = lo[1], bk = lo[2];
+  const bool single_phase = (P.nph == 1);
+  if (c >= 0) {
+    const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+    const double m = pmass[i];
+    const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
+    const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
+    double zx = x[0] - x0, zy = x[1] - y0;
+    zx = zx / dxx;
+    zy = zy / dyy;
+    double zz = x[2] - z0;
+    zz = zz / dzz;
+    const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+    const double sc = P.dep_scale[ph];
+    const bool in_tile = single_phase && (ci - 1 - bi) + 1 < kTile && (cj - 1 - bj) + 1 < kTile &&
+                         (ck - 1 - bk) + 1 < kTile;
+    unsigned long long* f = fix + (size_t)ph * P.ncell;
+    const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+        for (int ii = 0; ii < 2; ii++) {
+          const double w = xv[ii] * yv[jj] * zv[kk];
+          const double v = w * m;
+          const unsigned long long q = (unsigned long long)__double2ll_rn(v * sc);
+          if (in_tile) {
+            const int ti = ci - 1 + ii - bi, tj = cj - 1 + jj - bj, tk = ck - 1 + kk - bk;
+            atomicAdd(&tile[(tk * kTile + tj) * kTile + ti], q);
+          } else {
+            atomicAdd(&f[base + jj + ii * P.nj + kk * P.nj * P.ni], q);
+          }
+        }
+  }
+  __syncthreads();
+  if (single_phase && bi != INT_MAX) {
+    for (int q = threadIdx.x; q < kTile * kTile * kTile; q += kBS) {
+      const unsigned long long v = tile[q];
+      if (v == 0ull) continue;
+      const int ti = q % kTile, tj = (q / kTile) % kTile, tk = q / (kTile * kTile);
+      const int I = bi + ti, J = bj + tj, K = bk + tk;  // node (I,J,K), 1-based
+      atomicAdd(&fix[(J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni], v);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- scans
@@ -679,6 +707,142 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, Build
   }
   if (cnt > b.maxrow) atomicMax(&b.flags[kFlagOwnOverflow], cnt);
   b.own_cnt[i] = cnt;
+}
+
+// Own rows, one wavefront per search-grid cell: the lanes take the
+// candidates of the 27 surrounding cells 64 at a time, the home particles
+// of the cell are walked in a uniform loop; the lanes that pass the
+// reference's criteria are compacted with a ballot into the home row, kept
+// in registers (lane k holds entry k, at most 64 per row) and finally put in
+// the reference's order by a rank sort.  A row longer than 64 raises the
+// overflow flag (the host then reruns the thread-per-particle kernel).
+__device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const uint64_t lowmask = (pos + step >= 64) ? ~0ull : ((1ull << (pos + step)) - 1ull);
+    if (__popcll(m & lowmask) <= k) pos += step;
+  }
+  return pos;
+}
+
+__global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPtrs b, int ncells) {
+  const int lane = threadIdx.x & 63;
+  const int c = (int)((blockIdx.x * (unsigned)kBS + threadIdx.x) >> 6);
+  if (c >= ncells) return;                       // whole wavefront leaves together
+  const int h0 = b.fstart[c], h1 = b.fend[c];
+  if (h1 <= h0) return;
+  const int fx = c % G.fn[0], fy = (c / G.fn[0]) % G.fn[1], fz = c / (G.fn[0] * G.fn[1]);
+  // lane k < 27 holds neighbour cell k (x fastest): its first slot and the
+  // inclusive prefix of the candidate counts
+  int ns = 0, nc = 0;
+  if (lane < 27) {
+    const int x = fx + lane % 3 - 1, y = fy + (lane / 3) % 3 - 1, z = fz + lane / 9 - 1;
+    if (x >= 0 && x < G.fn[0] && y >= 0 && y < G.fn[1] && z >= 0 && z < G.fn[2]) {
+      const int cc = x + G.fn[0] * (y + G.fn[1] * z);
+      ns = b.fstart[cc];
+      nc = b.fend[cc] - ns;
+    }
+  }
+  int pre = nc;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    const int y = __shfl_up(pre, o, 64);
+    if (lane >= o) pre += y;
+  }
+  const int total = __shfl(pre, 26, 64);
+  const int g_r = b.flags[kFlagRogueGid];
+  for (int h = h0; h < h1; h++) {
+    if (b.state[h] != 1) {
+      if (lane == 0) b.own_cnt[h] = 0;
+      continue;
+    }
+    const int gh = b.gid[h];
+    int ci, cj, ck;
+    dg_decode(G, b.dgc[h], ci, cj, ck);
+    if (dg_ghost(G, ci, cj, ck) || gh >= g_r) {
+      if (lane == 0) b.own_cnt[h] = 0;
+      continue;
+    }
+    const double x[3] = {b.pos[h], b.pos[S + h], b.pos[2 * S + h]};
+    const double r = b.radius[h];
+    int lo[3], hi[3];
+    stencil_off(G, x, lo, hi);
+    int cnt = 0;
+    int row_j = -1;
+    uint64_t row_key = ~0ull;
+    for (int base = 0; base < total; base += 64) {
+      const int t = base + lane;
+      // candidate t lives in the first neighbour cell k with prefix > t
+      // (all lanes take part in the shuffles; t is clamped for the idle ones)
+      const int tq = t < total ? t : total - 1;
+      int k = 0;
+#pragma unroll
+      for (int step = 16; step > 0; step >>= 1) {
+        const int q = k + step - 1 < 26 ? k + step - 1 : 26;
+        const int pq = __shfl(pre, q, 64);
+        if (k + step <= 26 && pq <= tq) k += step;
+      }
+      const int before = __shfl(pre - nc, k, 64);
+      const int cs = __shfl(ns, k, 64);
+      bool pass = false;
+      int j = -1;
+      uint64_t key = 0;
+      if (t < total) {
+        j = cs + (t - before);
+        if (j != h && b.state[j] != 0) {
+          const int gj = b.gid[j];
+          if (gj > gh) {
+            const double sr = G.factor_rlm * (r + b.radius[j]);
+            const double d0 = x[0] - b.pos[j], d1 = x[1] - b.pos[S + j], d2 = x[2] - b.pos[2 * S + j];
+            const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+            if (!(ds > sr * sr)) {
+              int li, lj, lk;
+              dg_decode(G, b.dgc[j], li, lj, lk);
+              if (li >= ci - lo[0] && li <= ci + hi[0] && lj >= cj - lo[1] && lj <= cj + hi[1] &&
+                  lk >= ck - lo[2] && lk <= ck + hi[2]) {
+                pass = true;
+                key = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
+              }
+            }
+          }
+        }
+      }
+      const uint64_t mask = __ballot(pass);
+      const int np = __popcll(mask);
+      if (np) {
+        // lane d in [cnt, cnt+np) pulls the (d-cnt)-th passing entry
+        const int want = lane - cnt;
+        const int src = (want >= 0 && want < np) ? kth_set_bit(mask, want) : lane;
+        const int jj = __shfl(j, src, 64);
+        const uint64_t kk = __shfl(key, src, 64);
+        if (want >= 0 && want < np) {
+          row_j = jj;
+          row_key = kk;
+        }
+        cnt += np;
+      }
+    }
+    if (cnt > 64) {
+      if (lane == 0) atomicMax(&b.flags[kFlagOwnOverflow], 1 << 20);
+      continue;
+    }
+    if (cnt > b.maxrow) {
+      if (lane == 0) atomicMax(&b.flags[kFlagOwnOverflow], cnt);
+      continue;
+    }
+    // rank sort by key (keys are distinct: the id is part of the key)
+    int rank = 0;
+    for (int m = 0; m < cnt; m++) {
+      const uint64_t km = __shfl(row_key, m, 64);
+      rank += (km < row_key) ? 1 : 0;
+    }
+    if (lane < cnt) {
+      b.own_tmp[(size_t)h * b.maxrow + rank] = row_j;
+      b.own_key[(size_t)h * b.maxrow + rank] = row_key;
+    }
+    if (lane == 0) b.own_cnt[h] = cnt;
+  }
 }
 
 __global__ __launch_bounds__(kBS) void k_count_in(int n, const int* own_cnt, const int* own_tmp,

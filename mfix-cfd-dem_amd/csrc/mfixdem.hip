@@ -151,7 +151,6 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.fcell = h->fcell[t];
   a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
   a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
-  a.rops_fix = h->rops_fix[h->fb];
   a.flags = h->flags;
   return a;
 }
@@ -188,7 +187,6 @@ int32_t launch_step(mfixdem_handle* h) {
   DevParams P = h->P;
   P.first_pass = h->first_pass ? 1 : 0;
   P.store_forces = (h->debug & MFIXDEM_DEBUG_STORE_FORCES) ? 1 : 0;
-  P.deposit = h->S.p.coupled && !ref_deposition(h);
   StepPtrs a = step_ptrs(h);
   if (h->S.p.precision == MFIXDEM_MIXED)
     hipLaunchKernelGGL((k_step<true, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
@@ -274,22 +272,36 @@ int32_t do_neighbour(mfixdem_handle* h) {
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
   b.own_cnt = h->own_cnt; b.flags = h->flags;
   int fl[kFlagCount];
+  bool wave_rows = true;
   for (;;) {
     b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow;
-    hipLaunchKernelGGL(k_own_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b);
+    if (wave_rows) {
+      // particles outside every cell range (state 0) get no row
+      CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
+      const int64_t threads = (int64_t)h->n_fine * 64;
+      hipLaunchKernelGGL(k_own_rows_wave, dim3(nblk(threads)), dim3(kBS), 0, h->stream, S, h->G,
+                         b, h->n_fine);
+    } else {
+      hipLaunchKernelGGL(k_own_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b);
+    }
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
     HIPOK(hipStreamSynchronize(h->stream));
     if (fl[kFlagOwnOverflow] <= h->maxrow) break;
+    int z = 0;
+    HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, &z, sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+    if (fl[kFlagOwnOverflow] >= (1 << 20)) {
+      // a row longer than the wavefront kernel holds: thread-per-particle rows
+      wave_rows = false;
+      continue;
+    }
     const int mr = fl[kFlagOwnOverflow] + 16;
     dfree(h->own_tmp);
     dfree(h->own_key);
     CHECKRC(dalloc(&h->own_tmp, (int64_t)n * mr));
     CHECKRC(dalloc(&h->own_key, (int64_t)n * mr));
     h->maxrow = mr;
-    int z = 0;
-    HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, &z, sizeof(int), hipMemcpyHostToDevice,
-                         h->stream));
   }
   if (fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
   const int nl = 1 - h->lt;
@@ -419,18 +431,19 @@ int32_t ref_mean_fields(mfixdem_handle* h) {
 // deposition of the current positions into the fixed-point accumulators
 int32_t deposit_current(mfixdem_handle* h) {
   DevParams P = h->P;
-  hipLaunchKernelGGL(k_deposit_atomic, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
+  hipLaunchKernelGGL(k_deposit_tiled, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
                      h->pos[h->cur], h->pmass[h->tw], h->phase[h->tw], h->state[h->tw],
                      h->fcell[h->tw], h->fluid, h->xe, h->yn, h->zt, h->rops_fix[h->fb]);
   HIPOK(hipGetLastError());
   return do_cell_fields(h, true);
 }
 
-// the gas-side tail of a step after the fused kernel (and any rebuild)
+// the gas-side tail of a step after the fused kernel (and any rebuild):
+// PARTICLES_IN_CELL deletions, deposition, ROP_S / EP_G
 int32_t gas_tail(mfixdem_handle* h) {
   if (ref_deposition(h)) return ref_mean_fields(h);  // deletion inside do_pic_lists
   CHECKRC(delete_outside(h));
-  return do_cell_fields(h, true);
+  return deposit_current(h);
 }
 
 // the part of a step that is identical between rebuilds (graph-capturable)
@@ -908,8 +921,7 @@ int32_t mfixdem_init(mfixdem_handle* h) {
     if (ref_deposition(h)) {
       CHECKRC(ref_mean_fields(h));
     } else {
-      CHECKRC(delete_outside(h));
-      CHECKRC(deposit_current(h));
+      CHECKRC(gas_tail(h));
     }
   }
   HIPOK(hipStreamSynchronize(h->stream));

@@ -1,8 +1,8 @@
 !----------------------------------------------------------------------!
 ! mfixdem_c: ISO_C_BINDING interface to the gfx950 DEM library        !
 ! (include/mfixdem.h).  MFiX-side UDF overrides of the DEM routines    !
-! (see udf/) USE this module and call these entry points instead of   !
-! running the CPU loops.  Types and names mirror the C header.         !
+! USE this module and call these entry points instead of running the   !
+! CPU loops.  Types and names mirror the C header.                     !
 !----------------------------------------------------------------------!
 module mfixdem_c
   use, intrinsic :: iso_c_binding

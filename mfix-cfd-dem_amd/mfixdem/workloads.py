@@ -236,3 +236,61 @@ def dem01(kn=1.0e4, en=0.9, integrator=abi.EULER):
     p.max_radius = 0.1
     parts = make_particles(np.array([[0.6], [0.5], [0.6]]), 0.1, 2600.0)
     return p, parts, None
+
+
+def dem03(en=1.0, integrator=abi.ADAMS_BASHFORTH):
+    """tests/dem/DEM03: two stacked, compressed spheres (d = 1 mm; rho 2e4
+    and 1e4, two phases) between the south and north walls of a 1 x 0.0018 x
+    1 m box; KN = KN_W = 1e3, KT_FAC = KT_W_FAC = 1, DES_ETAT_FAC = 1,
+    EN = EN_WALL for every pair, no friction."""
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = integrator
+    p.nphases = 2
+    p.max_pip = 2
+    p.xlength, p.ylength, p.zlength = 1.0, 0.0018, 1.0
+    p.imax = p.jmax = p.kmax = 1
+    p.desgridsearch_imax = p.desgridsearch_jmax = p.desgridsearch_kmax = 1
+    p.neighbor_search_n = 99
+    p.kn = p.kn_w = 1.0e3
+    p.kt_fac = p.kt_w_fac = 1.0
+    p.des_etat_fac = p.des_etat_w_fac = 1.0
+    p.mew = p.mew_w = 0.0
+    for i in range(3):
+        p.des_en_input[i] = en
+    for i in range(2):
+        p.des_en_wall_input[i] = en
+    p.d_p0[0] = p.d_p0[1] = 1.0e-3
+    p.ro_s0[0], p.ro_s0[1] = 2.0e4, 1.0e4
+    p.max_radius = 5.0e-4
+    pos = np.array([[0.5, 0.5], [0.00045, 0.00135], [0.5, 0.5]])
+    parts = make_particles(pos, 5.0e-4, np.array([2.0e4, 1.0e4]),
+                           phase=np.array([1, 2], dtype=np.int32))
+    return p, parts, None
+
+
+def dem04(mew=0.2, integrator=abi.ADAMS_BASHFORTH):
+    """tests/dem/DEM04: a sphere (d = 1 mm, rho = 1e4) resting on the south
+    wall of a 1 m box and launched along +x at 0.1 m/s; KN = KN_W = 1e4,
+    KT_FAC = KT_W_FAC = 1, EN = 1, DES_ETAT_FAC = 1, MEW = MEW_W = mew."""
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = integrator
+    p.nphases = 1
+    p.max_pip = 1
+    p.xlength = p.ylength = p.zlength = 1.0
+    p.imax = p.jmax = p.kmax = 1
+    p.desgridsearch_imax = p.desgridsearch_jmax = p.desgridsearch_kmax = 1
+    p.neighbor_search_n = 99
+    p.kn = p.kn_w = 1.0e4
+    p.kt_fac = p.kt_w_fac = 1.0
+    p.des_etat_fac = p.des_etat_w_fac = 1.0
+    p.mew = p.mew_w = mew
+    p.des_en_input[0] = 1.0
+    p.des_en_wall_input[0] = 1.0
+    p.d_p0[0] = 1.0e-3
+    p.ro_s0[0] = 1.0e4
+    p.max_radius = 5.0e-4
+    parts = make_particles(np.array([[0.001], [0.0005], [0.5]]), 5.0e-4, 1.0e4,
+                           vel=np.array([[0.1], [0.0], [0.0]]))
+    return p, parts, None

@@ -4,6 +4,10 @@ Reads the AUTOTEST files the reference ships for its DEM tests (data, not
 code) and keeps, per run, the solver columns the tests compare:
   DEM01  tests/dem/DEM01/AUTOTEST/POST_POS.dat, POST_VEL.dat
          (scheme, KN, EN) -> rows of (time, y_mfix, v_mfix)
+  DEM03  tests/dem/DEM03/AUTOTEST/POST_POS1.dat, POST_POS2.dat
+         (scheme, KN, EN) -> rows of (time, y1_mfix, y2_mfix)
+  DEM04  tests/dem/DEM04/AUTOTEST/POST_TIME.dat, POST_TVEL.dat, POST_AVEL.dat
+         MEW -> (t_end*MEW*g/u0, v/u0, omega*r/u0) at the end of slip
 Run from the repository root when /root/reference is mounted:
   python tests/golden/make_dem_fixtures.py
 """
@@ -15,7 +19,7 @@ REF = "/root/reference/tests/dem"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def parse_blocks(path):
+def parse_blocks(path, ncol=6, col=3):
     blocks, cur = [], None
     for line in open(path):
         m = re.search(r"Time Stepping Scheme:\s*(\S+)", line)
@@ -32,16 +36,63 @@ def parse_blocks(path):
             cur["en"] = float(m.group(1))
             continue
         parts = line.split()
-        if cur is not None and len(parts) == 6:
+        if cur is not None and len(parts) == ncol:
             try:
                 vals = [float(x) for x in parts]
             except ValueError:
                 continue
-            cur["rows"].append([vals[0], vals[3]])  # time, MFIX column
+            cur["rows"].append([vals[0], vals[col]])  # time, MFIX column
     return blocks
 
 
+def parse_table(path, ncol=4):
+    """Rows of `ncol` numbers in a 4-column AUTOTEST table (key, analytic,
+    MFIX, %err), in file order."""
+    rows = []
+    for line in open(path):
+        parts = line.split()
+        if len(parts) != ncol:
+            continue
+        try:
+            rows.append([float(x) for x in parts])
+        except ValueError:
+            continue
+    return rows
+
+
+def dem03():
+    b1 = parse_blocks(os.path.join(REF, "DEM03", "AUTOTEST", "POST_POS1.dat"), ncol=4, col=2)
+    b2 = parse_blocks(os.path.join(REF, "DEM03", "AUTOTEST", "POST_POS2.dat"), ncol=4, col=2)
+    assert len(b1) == len(b2)
+    out = []
+    for a, b in zip(b1, b2):
+        assert a["en"] == b["en"] and len(a["rows"]) == len(b["rows"])
+        rows = [[p[0], p[1], q[1]] for p, q in zip(a["rows"], b["rows"])]
+        out.append({"scheme": a["scheme"], "kn": a["kn"], "en": a["en"], "rows": rows})
+    with open(os.path.join(HERE, "dem03_post.json"), "w") as f:
+        json.dump({"source": "tests/dem/DEM03/AUTOTEST/POST_POS1.dat, POST_POS2.dat",
+                   "columns": ["time", "y1_mfix", "y2_mfix"], "runs": out}, f,
+                  separators=(",", ":"))
+    print("DEM03:", len(out), "runs,", sum(len(b["rows"]) for b in out), "rows")
+
+
+def dem04():
+    d = os.path.join(REF, "DEM04", "AUTOTEST")
+    t = parse_table(os.path.join(d, "POST_TIME.dat"))
+    v = parse_table(os.path.join(d, "POST_TVEL.dat"))
+    w = parse_table(os.path.join(d, "POST_AVEL.dat"))
+    assert len(t) == len(v) == len(w)
+    runs = [{"mew": a[0], "time_nd": a[2], "tvel_nd": b[2], "avel_nd": c[2]}
+            for a, b, c in zip(t, v, w)]
+    with open(os.path.join(HERE, "dem04_post.json"), "w") as f:
+        json.dump({"source": "tests/dem/DEM04/AUTOTEST/POST_TIME.dat, POST_TVEL.dat, "
+                             "POST_AVEL.dat (MFIX column)", "runs": runs}, f, indent=0)
+    print("DEM04:", len(runs), "runs")
+
+
 def main():
+    dem03()
+    dem04()
     pos = parse_blocks(os.path.join(REF, "DEM01", "AUTOTEST", "POST_POS.dat"))
     vel = parse_blocks(os.path.join(REF, "DEM01", "AUTOTEST", "POST_VEL.dat"))
     assert len(pos) == len(vel)

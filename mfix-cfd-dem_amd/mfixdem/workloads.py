@@ -294,3 +294,38 @@ def dem04(mew=0.2, integrator=abi.ADAMS_BASHFORTH):
     parts = make_particles(np.array([[0.001], [0.0005], [0.5]]), 5.0e-4, 1.0e4,
                            vel=np.array([[0.1], [0.0], [0.0]]))
     return p, parts, None
+
+
+def dem05(rows, model="LSD", integrator=abi.ADAMS_BASHFORTH):
+    """tests/dem/DEM05: oblique impacts, no gravity.  `rows` are the 93
+    lines of its particle_input.dat (x y z r rho u v w); rho 4000 is phase
+    1, 5000 phase 2.  MEW = MEW_W = 0.092, EN = 1 everywhere.  LSD: KN = KN_W
+    = 1.72e7, KT_FAC = KT_W_FAC = 1.48/1.72.  HERTZ: E = 380e9 / 70e9 (wall
+    70e9), nu = 0.23 / 0.25 (wall 0.25), ET = 1."""
+    a = np.asarray(rows, dtype=np.float64)
+    p = abi.default_params()
+    p.coll_model = abi.HERTZ if model == "HERTZ" else abi.LSD
+    p.integrator = integrator
+    p.nphases = 2
+    p.max_pip = a.shape[0]
+    p.xlength, p.ylength, p.zlength = 0.5, 0.1, 0.5
+    p.imax, p.jmax, p.kmax = 5, 1, 5
+    p.grav[:] = [0.0, 0.0, 0.0]
+    p.mew = p.mew_w = 0.092
+    for i in range(3):
+        p.des_en_input[i] = 1.0
+        p.des_et_input[i] = 1.0
+    for i in range(2):
+        p.des_en_wall_input[i] = 1.0
+        p.des_et_wall_input[i] = 1.0
+    p.kn = p.kn_w = 1.72e7
+    p.kt_fac = p.kt_w_fac = 1.48 / 1.72
+    p.e_young[0], p.e_young[1] = 380.0e9, 70.0e9
+    p.v_poisson[0], p.v_poisson[1] = 0.23, 0.25
+    p.ew_young, p.vw_poisson = 70.0e9, 0.25
+    p.d_p0[0] = p.d_p0[1] = 5.0e-3
+    p.ro_s0[0], p.ro_s0[1] = 4000.0, 5000.0
+    p.max_radius = float(a[:, 3].max())
+    phase = np.where(a[:, 4] > 4500.0, 2, 1).astype(np.int32)
+    parts = make_particles(a[:, 0:3].T, a[:, 3], a[:, 4], vel=a[:, 5:8].T, phase=phase)
+    return p, parts, None

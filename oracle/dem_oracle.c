@@ -734,6 +734,24 @@ int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g, const double* 
   return 0;
 }
 
+/* Overwrite position, velocity and angular velocity of particles
+   q0 .. q0+count-1 (0-based) between steps, the way a reference user hook
+   (USR2_DES, des_time_march.f:453) edits DES_POS_NEW / DES_VEL_NEW /
+   OMEGA_NEW.  Arrays are [c*count + k]; any may be NULL to leave that field
+   alone. */
+int orc_user_set_particles(orc_sys* s, int64_t q0, int64_t count, const double* pos,
+                           const double* vel, const double* omega) {
+  if (q0 < 0 || count < 0 || q0 + count > s->pip) return MFIXDEM_ERR_ARG;
+  const int64_t L = s->ld;
+  for (int64_t k = 0; k < count; k++)
+    for (int c = 0; c < 3; c++) {
+      if (pos) s->pos[c * L + q0 + k] = pos[c * count + k];
+      if (vel) s->vel[c * L + q0 + k] = vel[c * count + k];
+      if (omega) s->omega[c * L + q0 + k] = omega[c * count + k];
+    }
+  return MFIXDEM_OK;
+}
+
 int orc_get_gas(const orc_sys* s, double* ep_g, double* rop_s) {
   for (int c = 0; c < s->ncell; c++) {
     if (ep_g) ep_g[c] = s->ep_g[c];

@@ -8,6 +8,9 @@ code) and keeps, per run, the solver columns the tests compare:
          (scheme, KN, EN) -> rows of (time, y1_mfix, y2_mfix)
   DEM04  tests/dem/DEM04/AUTOTEST/POST_TIME.dat, POST_TVEL.dat, POST_AVEL.dat
          MEW -> (t_end*MEW*g/u0, v/u0, omega*r/u0) at the end of slip
+  DEM05  tests/dem/DEM05/particle_input.dat (the 93 particles) and
+         AUTOTEST/POST_ALPHA.dat, POST_COEFF.dat, POST_OMEGA.dat
+         (LSD, HERTZ) -> rows of (impact angle, part-part, part-wall)
 Run from the repository root when /root/reference is mounted:
   python tests/golden/make_dem_fixtures.py
 """
@@ -90,8 +93,47 @@ def dem04():
     print("DEM04:", len(runs), "runs")
 
 
+def dem05_table(path):
+    """Blocks of (angle, part-part, part-wall) rows, one per collision
+    model, skipping the experimental rows at the top of the file."""
+    blocks, cur = [], None
+    for line in open(path):
+        if "Collision model:" in line:
+            cur = {"model": "HERTZ" if "HERTZ" in line.upper() else "LSD", "rows": []}
+            blocks.append(cur)
+            continue
+        parts = line.split()
+        if cur is None or len(parts) != 3:
+            continue
+        try:
+            cur["rows"].append([float(x) for x in parts])
+        except ValueError:
+            continue
+    return blocks
+
+
+def dem05():
+    d = os.path.join(REF, "DEM05")
+    parts = [[float(x) for x in line.split()] for line in open(os.path.join(d, "particle_input.dat"))
+             if line.strip()]
+    tabs = {k: dem05_table(os.path.join(d, "AUTOTEST", f"POST_{k.upper()}.dat"))
+            for k in ("alpha", "coeff", "omega")}
+    runs = []
+    for i, blk in enumerate(tabs["alpha"]):
+        runs.append({"model": blk["model"], "alpha": blk["rows"],
+                     "coeff": tabs["coeff"][i]["rows"], "omega": tabs["omega"][i]["rows"]})
+        assert tabs["coeff"][i]["model"] == blk["model"] == tabs["omega"][i]["model"]
+    with open(os.path.join(HERE, "dem05_post.json"), "w") as f:
+        json.dump({"source": "tests/dem/DEM05/particle_input.dat (x y z r rho u v w); "
+                             "AUTOTEST/POST_ALPHA.dat, POST_COEFF.dat, POST_OMEGA.dat",
+                   "columns": ["impact_angle", "part_part", "part_wall"],
+                   "particles": parts, "runs": runs}, f, separators=(",", ":"))
+    print("DEM05:", len(parts), "particles,", [(r["model"], len(r["alpha"])) for r in runs])
+
+
 def main():
     dem03()
+    dem05()
     dem04()
     pos = parse_blocks(os.path.join(REF, "DEM01", "AUTOTEST", "POST_POS.dat"))
     vel = parse_blocks(os.path.join(REF, "DEM01", "AUTOTEST", "POST_VEL.dat"))

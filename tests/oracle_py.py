@@ -56,6 +56,7 @@ def load_oracle(omp: bool = False):
         "orc_drag_fgp": ([I, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                           C.c_double], C.c_double),
         "orc_omp_threads": ([], C.c_int),
+        "orc_user_set_particles": ([P, L, L, D, D, D], C.c_int),
     }
     for n, (a, r) in sig.items():
         f = getattr(lib, n)
@@ -171,6 +172,16 @@ class OracleSystem:
         s = abi.Status()
         self._chk(self.lib.orc_get_status(self.h, C.byref(s)), "status")
         return s
+
+    def user_set_particles(self, q0, pos=None, vel=None, omega=None):
+        """Edit particles q0.. (arrays of shape (3, count)) between steps,
+        like a reference USR2_DES hook."""
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+               for a in (pos, vel, omega)]
+        count = next(a.shape[1] for a in arr if a is not None)
+        self._chk(self.lib.orc_user_set_particles(self.h, q0, count,
+                                                  *(abi.dptr(a) for a in arr)),
+                  "user_set_particles")
 
     def nsquare_pairs(self):
         cap = 8 * self.n + 16

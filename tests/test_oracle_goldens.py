@@ -98,3 +98,78 @@ def test_dem04_slip_to_roll(k):
     # 1.5 * overlap_mean / R relative
     assert -1.5e-5 <= d_avel <= 0.0
     o.close()
+
+
+DEM05 = json.load(open(os.path.join(HERE, "dem05_post.json")))
+
+
+@pytest.mark.parametrize("model", ["LSD", "HERTZ"])
+def test_dem05_oblique_wall_impacts(model):
+    """DEM05 (tests/dem/DEM05): 31 spheres hit the south wall at impact
+    angles 0-65 deg with friction 0.092 -- sticking/sliding transitions of
+    the wall's tangential history, LSD and Hertz.  Compared (part-wall
+    column, printed with 4 decimals): rebound angle, tangential restitution
+    and spin after TSTOP = 2.5 ms, computed as tests/dem/DEM05/usr3_des.f
+    does.  The other 62 particles (the particle-particle half of the test,
+    whose targets the reference re-pins every step from user code) never
+    touch these 31."""
+    run = next(r for r in DEM05["runs"] if r["model"] == model)
+    p, parts, _ = workloads.dem05(DEM05["particles"], model=model)
+    o = _system(p, parts)
+    dt = o.dtsolid()
+    # pure-DEM step count: CEILING(real((TSTOP-TIME)/DTSOLID)), des_time_march.f:130
+    nsteps = int(np.ceil(np.float32(0.0025 / dt)))
+    o.run(1, nsteps)
+    _, vel, om, _ = o.get_particles()
+    v0 = parts.vel
+    vt0 = np.sqrt(v0[0, :31] ** 2 + v0[2, :31] ** 2)
+    ang0 = np.abs(np.arctan(vt0 / v0[1, :31])) * 180.0 / np.pi
+    vt = np.sqrt(vel[0, :31] ** 2 + vel[2, :31] ** 2)
+    alpha = np.arctan(vt / vel[1, :31]) * 180.0 / np.pi
+    spin = np.sqrt(om[0, :31] ** 2 + om[2, :31] ** 2)
+    ga, gc, gw = (np.array(run[k]) for k in ("alpha", "coeff", "omega"))
+    assert np.allclose(ang0, ga[:, 0], atol=5.1e-5)
+    tol = 5.1e-5  # half a unit of the 4th printed decimal
+    assert np.max(np.abs(alpha - ga[:, 2])) <= tol
+    assert np.max(np.abs(spin - gw[:, 2])) <= tol
+    assert np.max(np.abs(vt[1:] / vt0[1:] - gc[:, 2])) <= tol
+    o.close()
+
+
+@pytest.mark.parametrize("model", ["LSD", "HERTZ"])
+def test_dem05_oblique_particle_impacts(model):
+    """DEM05 particle-particle half: spheres 32-62 (rho 4000) hit spheres
+    63-93 (rho 5000), which the reference's user hook puts back every step
+    at y = 0.0475 under their partner's x, z with zero velocity and spin
+    (tests/dem/DEM05/usr2_des.f).  The same hook is applied to the oracle
+    between steps.  Pins particle-particle LSD and Hertz contact with
+    tangential history, the Coulomb cap and the torque split between two
+    phases.  Compared (part-part column, 4 printed decimals): rebound angle,
+    tangential restitution and spin."""
+    run = next(r for r in DEM05["runs"] if r["model"] == model)
+    p, parts, _ = workloads.dem05(DEM05["particles"], model=model)
+    o = _system(p, parts)
+    dt = o.dtsolid()
+    nsteps = int(np.ceil(np.float32(0.0025 / dt)))
+    zero = np.zeros((3, 31))
+    tgt = np.zeros((3, 31))
+    tgt[1, :] = 0.0475
+    for nn in range(1, nsteps + 1):
+        o.step(nn)
+        pos = o.get_particles()[0]
+        tgt[0, :] = pos[0, 31:62]
+        tgt[2, :] = pos[2, 31:62]
+        o.user_set_particles(62, pos=tgt, vel=zero, omega=zero)
+    _, vel, om, _ = o.get_particles()
+    sl = slice(31, 62)
+    v0 = parts.vel
+    vt0 = np.sqrt(v0[0, sl] ** 2 + v0[2, sl] ** 2)
+    vt = np.sqrt(vel[0, sl] ** 2 + vel[2, sl] ** 2)
+    alpha = np.arctan(vt / vel[1, sl]) * 180.0 / np.pi
+    spin = np.sqrt(om[0, sl] ** 2 + om[2, sl] ** 2)
+    ga, gc, gw = (np.array(run[k]) for k in ("alpha", "coeff", "omega"))
+    tol = 5.1e-5  # half a unit of the 4th printed decimal
+    assert np.max(np.abs(alpha - ga[:, 1])) <= tol
+    assert np.max(np.abs(spin - gw[:, 1])) <= tol
+    assert np.max(np.abs(vt[1:] / vt0[1:] - gc[:, 1])) <= tol
+    o.close()

@@ -309,24 +309,34 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
 
 // COMP_MEAN_FIELDS0 deposition of the current positions, tiled: the
 // workgroup's 256 particles (consecutive in the Morton order, so spatially
-// compact) add their corner contributions into an 8x8x8-node int64 tile in
-// LDS anchored at the workgroup's smallest node; each touched tile node then
-// costs one global atomic.  Contributions outside the tile go straight to
-// global memory.  Integer sums: the result does not depend on the order.
-constexpr int kTile = 8;
+// compact) add their corner contributions into an int64 tile in LDS that
+// covers the bounding box of the nodes they touch (up to kTileCap nodes);
+// each touched tile node then costs one global atomic.  A workgroup whose
+// box is larger, or a multi-phase system, adds straight to global memory.
+// Integer sums: the result does not depend on the order.
+constexpr int kTileCap = 2048;
 __global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double* pos,
                                                        const double* pmass, const uint8_t* phase,
                                                        const uint8_t* state, const int* fcell,
                                                        const uint8_t* fluid, const double* xe,
                                                        const double* yn, const double* zt,
                                                        unsigned long long* fix) {
-  __shared__ unsigned long long tile[kTile * kTile * kTile];
-  __shared__ int lo[3];
+  __shared__ unsigned long long tile[kTileCap];
+  __shared__ int box[6];
   const int i = blockIdx.x * kBS + threadIdx.x;
   const int S = P.stride;
   int c = -1, ci = 0, cj = 0, ck = 0, ph = 0;
-  if (i < P.n && state[i] != 0) {
+  // independent loads first: only the fluid-flag lookup depends on another
+  double x[3] = {0.0, 0.0, 0.0}, m = 0.0;
+  if (i < P.n) {
+    const uint8_t st = state[i];
     c = fcell[i];
+    x[0] = pos[i];
+    x[1] = pos[S + i];
+    x[2] = pos[2 * S + i];
+    m = pmass[i];
+    ph = phase[i];
+    if (st == 0) c = -1;
     if (c >= 0 && !fluid[c]) c = -1;
   }
   if (c >= 0) {
@@ -334,22 +344,29 @@ __global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double
     const int t = c / P.nj;
     ci = t % P.ni + 1;
     ck = t / P.ni + 1;
-    ph = phase[i];
   }
-  for (int q = threadIdx.x; q < kTile * kTile * kTile; q += kBS) tile[q] = 0ull;
-  if (threadIdx.x == 0) lo[0] = lo[1] = lo[2] = INT_MAX;
+  if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
+  else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
   __syncthreads();
   if (c >= 0) {
-    atomicMin(&lo[0], ci - 1);
-    atomicMin(&lo[1], cj - 1);
-    atomicMin(&lo[2], ck - 1);
+    atomicMin(&box[0], ci);
+    atomicMin(&box[1], cj);
+    atomicMin(&box[2], ck);
+    atomicMax(&box[3], ci);
+    atomicMax(&box[4], cj);
+    atomicMax(&box[5], ck);
   }
   __syncthreads();
-  const int bi = lo[0], bj = lo[1], bk = lo[2];
-  const bool single_phase = (P.nph == 1);
+  // tile nodes: I in [bi-1, bi + ni_t - 2] etc. (cell ci touches nodes ci-1, ci)
+  const int bi = box[0], bj = box[1], bk = box[2];
+  const int ti_n = box[3] - bi + 2, tj_n = box[4] - bj + 2, tk_n = box[5] - bk + 2;
+  const bool any = bi != INT_MAX;
+  const int vol = any ? ti_n * tj_n * tk_n : 0;
+  const bool tiled = any && P.nph == 1 && vol <= kTileCap;
+  if (tiled)
+    for (int q = threadIdx.x; q < vol; q += kBS) tile[q] = 0ull;
+  __syncthreads();
   if (c >= 0) {
-    const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
-    const double m = pmass[i];
     const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
     const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
     double zx = x[0] - x0, zy = x[1] - y0;
@@ -359,10 +376,9 @@ __global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double
     zz = zz / dzz;
     const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
     const double sc = P.dep_scale[ph];
-    const bool in_tile = single_phase && (ci - 1 - bi) + 1 < kTile && (cj - 1 - bj) + 1 < kTile &&
-                         (ck - 1 - bk) + 1 < kTile;
     unsigned long long* f = fix + (size_t)ph * P.ncell;
     const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
+    const int tbase = (ci - bi) + ti_n * ((cj - bj) + tj_n * (ck - bk));
 #pragma unroll
     for (int kk = 0; kk < 2; kk++)
 #pragma unroll
@@ -372,23 +388,20 @@ __global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double
           const double w = xv[ii] * yv[jj] * zv[kk];
           const double v = w * m;
           const unsigned long long q = (unsigned long long)__double2ll_rn(v * sc);
-          if (in_tile) {
-            const int ti = ci - 1 + ii - bi, tj = cj - 1 + jj - bj, tk = ck - 1 + kk - bk;
-            atomicAdd(&tile[(tk * kTile + tj) * kTile + ti], q);
-          } else {
+          if (tiled)
+            atomicAdd(&tile[tbase + ii + ti_n * (jj + tj_n * kk)], q);
+          else
             atomicAdd(&f[base + jj + ii * P.nj + kk * P.nj * P.ni], q);
-          }
         }
   }
+  if (!tiled) return;
   __syncthreads();
-  if (single_phase && bi != INT_MAX) {
-    for (int q = threadIdx.x; q < kTile * kTile * kTile; q += kBS) {
-      const unsigned long long v = tile[q];
-      if (v == 0ull) continue;
-      const int ti = q % kTile, tj = (q / kTile) % kTile, tk = q / (kTile * kTile);
-      const int I = bi + ti, J = bj + tj, K = bk + tk;  // node (I,J,K), 1-based
-      atomicAdd(&fix[(J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni], v);
-    }
+  for (int q = threadIdx.x; q < vol; q += kBS) {
+    const unsigned long long v = tile[q];
+    if (v == 0ull) continue;
+    const int ti = q % ti_n, tj = (q / ti_n) % tj_n, tk = q / (ti_n * tj_n);
+    const int I = bi - 1 + ti, J = bj - 1 + tj, K = bk - 1 + tk;  // node (I,J,K), 1-based
+    atomicAdd(&fix[(J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni], v);
   }
 }
 
@@ -726,7 +739,13 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
   return pos;
 }
 
+constexpr int kRowCap = 192;  // candidates staged in LDS per wavefront
+
 __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPtrs b, int ncells) {
+  __shared__ double s_x[kBS / 64][kRowCap], s_y[kBS / 64][kRowCap], s_z[kBS / 64][kRowCap],
+      s_r[kBS / 64][kRowCap];
+  __shared__ int s_j[kBS / 64][kRowCap], s_g[kBS / 64][kRowCap], s_d[kBS / 64][kRowCap];
+  const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int c = (int)((blockIdx.x * (unsigned)kBS + threadIdx.x) >> 6);
   if (c >= ncells) return;                       // whole wavefront leaves together
@@ -751,6 +770,42 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
     if (lane >= o) pre += y;
   }
   const int total = __shfl(pre, 26, 64);
+  // slot of candidate t: first neighbour cell k with prefix > t (every lane
+  // takes part in the shuffles)
+  auto slot_of = [&](int t) {
+    const int tq = t < total ? t : total - 1;
+    int k = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1) {
+      const int q = k + step - 1 < 26 ? k + step - 1 : 26;
+      const int pq = __shfl(pre, q, 64);
+      if (k + step <= 26 && pq <= tq) k += step;
+    }
+    const int before = __shfl(pre - nc, k, 64);
+    const int cs = __shfl(ns, k, 64);
+    return cs + (tq - before);
+  };
+  // stage the first kRowCap candidates; a nonexistent one gets id INT_MIN
+  // so that it never passes the id test
+  const int staged = total < kRowCap ? total : kRowCap;
+  for (int base = 0; base < staged; base += 64) {
+    const int t = base + lane;
+    const int j = slot_of(t);
+    if (t < staged) {
+      const uint8_t st = b.state[j];
+      const int g = b.gid[j];
+      s_x[wv][t] = b.pos[j];
+      s_y[wv][t] = b.pos[S + j];
+      s_z[wv][t] = b.pos[2 * S + j];
+      s_r[wv][t] = b.radius[j];
+      s_d[wv][t] = b.dgc[j];
+      s_j[wv][t] = j;
+      s_g[wv][t] = st != 0 ? g : INT_MIN;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int g_r = b.flags[kFlagRogueGid];
   for (int h = h0; h < h1; h++) {
     if (b.state[h] != 1) {
@@ -773,38 +828,43 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
     uint64_t row_key = ~0ull;
     for (int base = 0; base < total; base += 64) {
       const int t = base + lane;
-      // candidate t lives in the first neighbour cell k with prefix > t
-      // (all lanes take part in the shuffles; t is clamped for the idle ones)
-      const int tq = t < total ? t : total - 1;
-      int k = 0;
-#pragma unroll
-      for (int step = 16; step > 0; step >>= 1) {
-        const int q = k + step - 1 < 26 ? k + step - 1 : 26;
-        const int pq = __shfl(pre, q, 64);
-        if (k + step <= 26 && pq <= tq) k += step;
+      int j = -1, gj = INT_MIN, dj = 0;
+      double xj = 0.0, yj = 0.0, zj = 0.0, rj = 0.0;
+      if (base < kRowCap) {
+        if (t < staged) {
+          j = s_j[wv][t];
+          gj = s_g[wv][t];
+          xj = s_x[wv][t];
+          yj = s_y[wv][t];
+          zj = s_z[wv][t];
+          rj = s_r[wv][t];
+          dj = s_d[wv][t];
+        }
+      } else {
+        const int js = slot_of(t);
+        if (t < total) {
+          j = js;
+          gj = b.state[j] != 0 ? b.gid[j] : INT_MIN;
+          xj = b.pos[j];
+          yj = b.pos[S + j];
+          zj = b.pos[2 * S + j];
+          rj = b.radius[j];
+          dj = b.dgc[j];
+        }
       }
-      const int before = __shfl(pre - nc, k, 64);
-      const int cs = __shfl(ns, k, 64);
       bool pass = false;
-      int j = -1;
       uint64_t key = 0;
-      if (t < total) {
-        j = cs + (t - before);
-        if (j != h && b.state[j] != 0) {
-          const int gj = b.gid[j];
-          if (gj > gh) {
-            const double sr = G.factor_rlm * (r + b.radius[j]);
-            const double d0 = x[0] - b.pos[j], d1 = x[1] - b.pos[S + j], d2 = x[2] - b.pos[2 * S + j];
-            const double ds = d0 * d0 + d1 * d1 + d2 * d2;
-            if (!(ds > sr * sr)) {
-              int li, lj, lk;
-              dg_decode(G, b.dgc[j], li, lj, lk);
-              if (li >= ci - lo[0] && li <= ci + hi[0] && lj >= cj - lo[1] && lj <= cj + hi[1] &&
-                  lk >= ck - lo[2] && lk <= ck + hi[2]) {
-                pass = true;
-                key = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
-              }
-            }
+      if (j != h && gj > gh) {
+        const double sr = G.factor_rlm * (r + rj);
+        const double d0 = x[0] - xj, d1 = x[1] - yj, d2 = x[2] - zj;
+        const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!(ds > sr * sr)) {
+          int li, lj, lk;
+          dg_decode(G, dj, li, lj, lk);
+          if (li >= ci - lo[0] && li <= ci + hi[0] && lj >= cj - lo[1] && lj <= cj + hi[1] &&
+              lk >= ck - lo[2] && lk <= ck + hi[2]) {
+            pass = true;
+            key = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
           }
         }
       }
@@ -1083,6 +1143,11 @@ struct FieldPtrs {
   double *rop_s, *ep_g;
   int* flags;
 };
+// Node sums -> ROP_S, EP_G per fluid cell (comp_mean_fields0.f:339-432,
+// calc_epg_des.f:133-145).  All corner loads are issued before any is
+// used (the kernel is one short pass over the grid, so it is bound by load
+// latency, not bandwidth); the sum runs over the corners in the reference's
+// order (K, J, I) and skips nodes with VOL_SURR = 0 exactly as it does.
 template <bool FIXED>
 __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kmax, int ni, int nj,
                                                      int nk, int nph, double packed_eps,
@@ -1096,38 +1161,50 @@ __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kma
   const int cj = c % nj + 1;
   const int ci = (c / nj) % ni + 1;
   const int ck = c / (nj * ni) + 1;
+  int node[8];
+  double vs[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int I = ci - 1 + (q & 1), J = cj - 1 + ((q >> 1) & 1), K = ck - 1 + (q >> 2);
+    const bool ok = I >= 1 && I <= imax + 1 && J >= 1 && J <= jmax + 1 && K >= 1 && K <= kmax + 1;
+    node[q] = ok ? (J - 1) + (I - 1) * nj + (K - 1) * nj * ni : -1;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) vs[q] = node[q] >= 0 ? f.vol_surr[node[q]] : 0.0;
+  const double vc = f.vol[c];
+  const uint8_t fl = f.fluid[c];
   double acc[kMaxPh] = {0.0, 0.0, 0.0, 0.0};
-  for (int K = ck - 1; K <= ck; K++) {
-    if (K < 1 || K > kmax + 1) continue;
-    for (int J = cj - 1; J <= cj; J++) {
-      if (J < 1 || J > jmax + 1) continue;
-      for (int I = ci - 1; I <= ci; I++) {
-        if (I < 1 || I > imax + 1) continue;
-        const int node = (J - 1) + (I - 1) * nj + (K - 1) * nj * ni;
-        const double vs = f.vol_surr[node];
-        if (vs == 0.0) continue;
-        for (int m = 0; m < nph; m++) {
-          double nodev;
-          if (FIXED) {
-            const unsigned long long q = f.fix[(size_t)m * ncell + node];
-            if (q >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
-            nodev = (double)q * inv[m];
-          } else {
-            nodev = f.rops_node[(size_t)m * ncell + node];
-          }
-          const double dens = nodev / vs;
-          acc[m] = acc[m] + dens * f.vol[c];
-        }
+  for (int m = 0; m < nph; m++) {
+    double nodev[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      nodev[q] = 0.0;
+      if (node[q] < 0) continue;
+      if (FIXED) {
+        const unsigned long long u = f.fix[(size_t)m * ncell + node[q]];
+        if (u >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
+        nodev[q] = (double)u * inv[m];
+      } else {
+        nodev[q] = f.rops_node[(size_t)m * ncell + node[q]];
       }
     }
+    // corner order q = I fastest, then J, then K: the reference's loop order
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      if (node[q] < 0 || vs[q] == 0.0) continue;
+      const double dens = nodev[q] / vs[q];
+      a = a + dens * vc;
+    }
+    acc[m] = a;
   }
   if (FIXED)
     for (int m = 0; m < nph; m++) f.fix_clear[(size_t)m * ncell + c] = 0ull;
-  if (f.fluid[c])
+  if (fl)
     for (int m = 0; m < nph; m++)
-      if (acc[m] > 0.0) acc[m] = acc[m] / f.vol[c];
+      if (acc[m] > 0.0) acc[m] = acc[m] / vc;
   for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = acc[m];
-  if (f.fluid[c]) {
+  if (fl) {
     double sum_eps = 0.0;
     for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / f.ro_s[m];
     const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);

@@ -72,6 +72,41 @@ def kernel_times(s, nn0, steps):
     return [arr[i] / max(calls[i], 1) for i in range(abi.TIME_GROUPS)], list(calls)
 
 
+def kinetic_energy(parts, vel, om):
+    """Translational + rotational kinetic energy of the particle set (J)."""
+    r = parts.radius
+    m = (4.0 / 3.0) * np.pi * r ** 3 * parts.ro_sol
+    inertia = 0.4 * m * r * r
+    return float(0.5 * np.sum(m * np.sum(vel * vel, axis=0)) +
+                 0.5 * np.sum(inertia * np.sum(om * om, axis=0)))
+
+
+def bed_height(parts, pos, frac=0.95):
+    """Height (y) below which `frac` of the solid volume lies."""
+    vol = (4.0 / 3.0) * np.pi * parts.radius ** 3
+    order = np.argsort(pos[1])
+    c = np.cumsum(vol[order])
+    return float(pos[1][order][np.searchsorted(c, frac * c[-1])])
+
+
+def force_error(p, parts, gas):
+    """Max |F_mixed - F_fp64| / max |F_fp64| over all particles for the
+    first step from the same state (FC as the integrator sees it)."""
+    out = []
+    for prec in (abi.FP64, abi.MIXED):
+        q = abi.Params.from_buffer_copy(p)
+        q.precision = prec
+        s = DemSystem(q)
+        s.set_particles(parts)
+        s.set_gas(gas)
+        s.set_debug(abi.DEBUG_STORE_FORCES)
+        s.init()
+        s.step(1)
+        out.append(s.get_forces()[0])
+        s.close()
+    return float(np.max(np.abs(out[1] - out[0])) / np.max(np.abs(out[0])))
+
+
 def cpu_baseline(p, parts, gas, seconds=15.0):
     """Times the OpenMP CPU restatement (oracle/liboracle_omp.so) on the same
     workload for as many whole steps as fit in about `seconds`."""
@@ -133,6 +168,8 @@ def main():
     achieved = bytes_step / (t_step_ms * 1e-3) / 1e9
     log(f"fused step kernel {t_step_ms * 1e3:.1f} us, rebuild {kt[abi.TIME_REBUILD] * 1e3:.1f} us, "
         f"gas {kt[abi.TIME_GAS] * 1e3:.1f} us")
+    nsteps_total = args.warmup + args.steps + 50
+    pos64, vel64, om64, _ = s.get_particles()
     s.close()
 
     traffic = None
@@ -148,10 +185,22 @@ def main():
         m.run(1, args.warmup)
         elm = time_steps(m, 1 + args.warmup, args.steps)
         ktm, _ = kernel_times(m, 1 + args.warmup + args.steps, 50)
-        mixed = {"value": parts.n * args.steps / elm, "ms_per_step": 1e3 * elm / args.steps,
-                 "step_kernel_us": ktm[abi.TIME_STEP] * 1e3}
-        log(f"mixed: {mixed['value']:.4e} particle-steps/s")
+        posm, velm, omm, _ = m.get_particles()
         m.close()
+        ke64, kem = kinetic_energy(parts, vel64, om64), kinetic_energy(parts, velm, omm)
+        ke0 = kinetic_energy(parts, parts.vel, parts.omega)
+        h64, hm = bed_height(parts, pos64), bed_height(parts, posm)
+        mixed = {"value": parts.n * args.steps / elm, "ms_per_step": 1e3 * elm / args.steps,
+                 "step_kernel_us": ktm[abi.TIME_STEP] * 1e3,
+                 "steps_compared": nsteps_total,
+                 "max_rel_force_error": force_error(p, parts, gas),
+                 "ke_fp64": ke64, "ke_mixed": kem, "ke_initial": ke0,
+                 "ke_rel_diff": abs(kem - ke64) / ke64,
+                 "bed_height_95_fp64_m": h64, "bed_height_95_mixed_m": hm,
+                 "bed_height_rel_diff": abs(hm - h64) / h64}
+        log(f"mixed: {mixed['value']:.4e} particle-steps/s, force err "
+            f"{mixed['max_rel_force_error']:.2e}, KE diff {mixed['ke_rel_diff']:.2e}, "
+            f"bed height {h64:.5f} / {hm:.5f} m")
 
     cpu = None
     if not args.no_cpu:

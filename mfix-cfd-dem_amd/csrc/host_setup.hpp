@@ -451,6 +451,8 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
   setup_grids(s);
   if ((int64_t)s.dg_ni * s.dg_nj * s.dg_nk > (int64_t)1 << 30) return MFIXDEM_ERR_ARG;
   if (s.dg_ni > 1023 || s.dg_nj > 1023 || s.dg_nk > 1023) return MFIXDEM_ERR_UNSUPPORTED;
+  // fluid cells are packed in 10 bits per axis on the device
+  if (s.ni > 1023 || s.nj > 1023 || s.nk > 1023) return MFIXDEM_ERR_UNSUPPORTED;
   setup_facets(s);
   if ((int)s.facets.size() > kMaxFacets) return MFIXDEM_ERR_UNSUPPORTED;
   return MFIXDEM_OK;

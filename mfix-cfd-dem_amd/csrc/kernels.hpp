@@ -61,17 +61,41 @@ __device__ __forceinline__ void store3(double* a, int S, int i, const double v[3
   a[2 * S + i] = v[2];
 }
 
+// Fluid cell of a particle, kept packed as I | J << 10 | K << 20 (1-based,
+// ghost layer included; negative = outside the grid) so that kernels get
+// I, J, K with shifts instead of divisions by the grid size.
+__device__ __forceinline__ int pack_ijk(int I, int J, int K) { return I | (J << 10) | (K << 20); }
+__device__ __forceinline__ void unpack_ijk(int c, int& I, int& J, int& K) {
+  I = c & 1023;
+  J = (c >> 10) & 1023;
+  K = c >> 20;
+}
+// linear IJK (FUNIJK, J fastest), 0-based
+__device__ __forceinline__ int lin_ijk(int I, int J, int K, int ni, int nj) {
+  return (J - 1) + (I - 1) * nj + (K - 1) * nj * ni;
+}
+
+// 10-bit value spread to every third bit (Morton codes)
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+
 // GARG_2012 drag on one particle (DRAG_GS_DES0 :140-212 with
 // DRAG_INTERPOLATION :507-566); adds D_FORCE and P_FORCE*PVOL to fc.
 template <bool MIXED>
 __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, int i,
                                          const double pos[3], const double vel[3], double fc[3]) {
-  const int c = a.fcell[i];
-  if (c < 0 || !a.fluid[c]) return;
-  const int cj = c % P.nj + 1;
-  const int t = c / P.nj;
-  const int ci = t % P.ni + 1;
-  const int ck = t / P.ni + 1;
+  const int pc = a.fcell[i];
+  if (pc < 0) return;
+  int ci, cj, ck;
+  unpack_ijk(pc, ci, cj, ck);
+  const int c = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  if (!a.fluid[c]) return;
   const double x0 = a.xe[ci - 1], y0 = a.yn[cj - 1], z0 = a.zt[ck - 1];
   const double dxx = a.xe[ci] - x0, dyy = a.yn[cj] - y0, dzz = a.zt[ck] - z0;
   double zx = pos[0] - x0, zy = pos[1] - y0;
@@ -302,71 +326,116 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
     const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
     const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
-    const int c = (I && J && K) ? (J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni : -1;
-    a.fcell[i] = c;
+    a.fcell[i] = (I && J && K) ? pack_ijk(I, J, K) : -1;
   }
 }
 
-// COMP_MEAN_FIELDS0 deposition of the current positions, tiled: the
-// workgroup's 256 particles (consecutive in the Morton order, so spatially
-// compact) add their corner contributions into an int64 tile in LDS that
-// covers the bounding box of the nodes they touch (up to kTileCap nodes);
-// each touched tile node then costs one global atomic.  A workgroup whose
-// box is larger, or a multi-phase system, adds straight to global memory.
-// Integer sums: the result does not depend on the order.
-constexpr int kTileCap = 2048;
-__global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double* pos,
-                                                       const double* pmass, const uint8_t* phase,
-                                                       const uint8_t* state, const int* fcell,
-                                                       const uint8_t* fluid, const double* xe,
-                                                       const double* yn, const double* zt,
-                                                       unsigned long long* fix) {
-  __shared__ unsigned long long tile[kTileCap];
-  __shared__ int box[6];
+// Sort key of the deposition order: Morton code of the packed fluid cell
+// (I, J, K < 1024); particles without a fluid cell go last.
+__global__ __launch_bounds__(kBS) void k_dep_keys(int n, const uint8_t* state, const int* fcell,
+                                                  uint64_t* keys, int* vals) {
   const int i = blockIdx.x * kBS + threadIdx.x;
-  const int S = P.stride;
-  int c = -1, ci = 0, cj = 0, ck = 0, ph = 0;
-  // independent loads first: only the fluid-flag lookup depends on another
-  double x[3] = {0.0, 0.0, 0.0}, m = 0.0;
-  if (i < P.n) {
-    const uint8_t st = state[i];
-    c = fcell[i];
-    x[0] = pos[i];
-    x[1] = pos[S + i];
-    x[2] = pos[2 * S + i];
-    m = pmass[i];
-    ph = phase[i];
-    if (st == 0) c = -1;
-    if (c >= 0 && !fluid[c]) c = -1;
+  if (i >= n) return;
+  vals[i] = i;
+  const int pc = fcell[i];
+  if (state[i] == 0 || pc < 0) {
+    keys[i] = (1ull << 30) - 1;
+    return;
   }
-  if (c >= 0) {
-    cj = c % P.nj + 1;
-    const int t = c / P.nj;
-    ci = t % P.ni + 1;
-    ck = t / P.ni + 1;
+  int I, J, K;
+  unpack_ijk(pc, I, J, K);
+  keys[i] = (uint64_t)(spread10(I) | (spread10(J) << 1) | (spread10(K) << 2));
+}
+
+// COMP_MEAN_FIELDS0 deposition (comp_mean_fields0.f:186-210) of the current
+// positions.  The particles are visited in the deposition order built at
+// the last rebuild (Morton order of their fluid cells then), kDepPPT
+// consecutive ones per thread: a run of particles in the same fluid cell
+// is summed in registers, each run adds its 8 corner values into an int64
+// tile in LDS covering the bounding box of the workgroup's nodes (up to
+// kDepCap nodes), and each touched tile node then costs one global atomic.
+// A workgroup whose box is larger, or a multi-phase system, adds straight
+// to global memory.  Particles that changed fluid cell since the rebuild
+// only widen the box.  Weights are rounded to fixed point per particle and
+// summed as integers, so the result does not depend on the order.
+constexpr int kDepPPT = 4;
+constexpr int kDepCap = 4096;
+__global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* list,
+                                                      const double* pos, const double* pmass,
+                                                      const uint8_t* phase, const uint8_t* state,
+                                                      const int* fcell, const uint8_t* fluid,
+                                                      const double* xe, const double* yn,
+                                                      const double* zt, unsigned long long* fix) {
+  __shared__ unsigned long long tile[kDepCap];
+  __shared__ int box[6];
+  const int S = P.stride;
+  const int k0 = (blockIdx.x * kBS + threadIdx.x) * kDepPPT;
+  int pi[kDepPPT], pcell[kDepPPT];
+  int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
+#pragma unroll
+  for (int u = 0; u < kDepPPT; u++) {
+    const int k = k0 + u;
+    pi[u] = k < P.n ? list[k] : -1;
+    pcell[u] = -1;
+  }
+#pragma unroll
+  for (int u = 0; u < kDepPPT; u++) {
+    const int i = pi[u];
+    if (i < 0) continue;
+    const int pc = fcell[i];
+    if (state[i] == 0 || pc < 0) continue;
+    int I, J, K;
+    unpack_ijk(pc, I, J, K);
+    if (!fluid[lin_ijk(I, J, K, P.ni, P.nj)]) continue;
+    pcell[u] = pc;
+    lo0 = min(lo0, I); lo1 = min(lo1, J); lo2 = min(lo2, K);
+    hi0 = max(hi0, I); hi1 = max(hi1, J); hi2 = max(hi2, K);
   }
   if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
   else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
   __syncthreads();
-  if (c >= 0) {
-    atomicMin(&box[0], ci);
-    atomicMin(&box[1], cj);
-    atomicMin(&box[2], ck);
-    atomicMax(&box[3], ci);
-    atomicMax(&box[4], cj);
-    atomicMax(&box[5], ck);
+  if (lo0 != INT_MAX) {
+    atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+    atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
   }
   __syncthreads();
-  // tile nodes: I in [bi-1, bi + ni_t - 2] etc. (cell ci touches nodes ci-1, ci)
+  // tile nodes I in [bi-1, box_hi]: a cell I touches nodes I-1 and I
   const int bi = box[0], bj = box[1], bk = box[2];
   const int ti_n = box[3] - bi + 2, tj_n = box[4] - bj + 2, tk_n = box[5] - bk + 2;
   const bool any = bi != INT_MAX;
   const int vol = any ? ti_n * tj_n * tk_n : 0;
-  const bool tiled = any && P.nph == 1 && vol <= kTileCap;
+  const bool tiled = any && P.nph == 1 && vol <= kDepCap;
   if (tiled)
     for (int q = threadIdx.x; q < vol; q += kBS) tile[q] = 0ull;
   __syncthreads();
-  if (c >= 0) {
+  unsigned long long acc[8];
+  int run_cell = -1, run_ph = 0;
+  auto flush_run = [&]() {
+    if (run_cell < 0) return;
+    int ci, cj, ck;
+    unpack_ijk(run_cell, ci, cj, ck);
+    if (tiled) {
+      const int tb = (ci - bi) + ti_n * ((cj - bj) + tj_n * (ck - bk));
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&tile[tb + (q & 1) + ti_n * (((q >> 1) & 1) + tj_n * (q >> 2))], acc[q]);
+    } else {
+      unsigned long long* f = fix + (size_t)run_ph * P.ncell;
+      const int base = lin_ijk(ci - 1, cj - 1, ck - 1, P.ni, P.nj);
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&f[base + ((q >> 1) & 1) + (q & 1) * P.nj + (q >> 2) * P.nj * P.ni], acc[q]);
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < kDepPPT; u++) {
+    if (pcell[u] < 0) continue;
+    const int i = pi[u];
+    const int ph = phase[i];
+    const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+    const double m = pmass[i];
+    int ci, cj, ck;
+    unpack_ijk(pcell[u], ci, cj, ck);
     const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
     const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
     double zx = x[0] - x0, zy = x[1] - y0;
@@ -376,32 +445,45 @@ __global__ __launch_bounds__(kBS) void k_deposit_tiled(DevParams P, const double
     zz = zz / dzz;
     const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
     const double sc = P.dep_scale[ph];
-    unsigned long long* f = fix + (size_t)ph * P.ncell;
-    const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
-    const int tbase = (ci - bi) + ti_n * ((cj - bj) + tj_n * (ck - bk));
+    const bool same = pcell[u] == run_cell && ph == run_ph;
+    if (!same) {
+      flush_run();
+      run_cell = pcell[u];
+      run_ph = ph;
+    }
 #pragma unroll
-    for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-      for (int jj = 0; jj < 2; jj++)
-#pragma unroll
-        for (int ii = 0; ii < 2; ii++) {
-          const double w = xv[ii] * yv[jj] * zv[kk];
-          const double v = w * m;
-          const unsigned long long q = (unsigned long long)__double2ll_rn(v * sc);
-          if (tiled)
-            atomicAdd(&tile[tbase + ii + ti_n * (jj + tj_n * kk)], q);
-          else
-            atomicAdd(&f[base + jj + ii * P.nj + kk * P.nj * P.ni], q);
-        }
+    for (int q = 0; q < 8; q++) {
+      const double w = xv[q & 1] * yv[(q >> 1) & 1] * zv[q >> 2];
+      const double v = w * m;
+      const unsigned long long qv = (unsigned long long)__double2ll_rn(v * sc);
+      acc[q] = same ? acc[q] + qv : qv;
+    }
   }
+  flush_run();
   if (!tiled) return;
   __syncthreads();
+  // walk this thread's tile entries q = tid, tid + kBS, ... keeping (ti, tj,
+  // tk) by carries: one division at the start, none in the loop
+  int ti = threadIdx.x % ti_n, tj = threadIdx.x / ti_n, tk = tj / tj_n;
+  tj -= tk * tj_n;
+  const int step_i = kBS % ti_n, step_r = kBS / ti_n;
   for (int q = threadIdx.x; q < vol; q += kBS) {
     const unsigned long long v = tile[q];
-    if (v == 0ull) continue;
-    const int ti = q % ti_n, tj = (q / ti_n) % tj_n, tk = q / (ti_n * tj_n);
-    const int I = bi - 1 + ti, J = bj - 1 + tj, K = bk - 1 + tk;  // node (I,J,K), 1-based
-    atomicAdd(&fix[(J - 1) + (I - 1) * P.nj + (K - 1) * P.nj * P.ni], v);
+    if (v != 0ull) {
+      const int I = bi - 1 + ti, J = bj - 1 + tj, K = bk - 1 + tk;  // node (I,J,K), 1-based
+      atomicAdd(&fix[lin_ijk(I, J, K, P.ni, P.nj)], v);
+    }
+    ti += step_i;
+    int rows = step_r;
+    if (ti >= ti_n) {
+      ti -= ti_n;
+      rows++;
+    }
+    tj += rows;
+    while (tj >= tj_n) {
+      tj -= tj_n;
+      tk++;
+    }
   }
 }
 
@@ -503,14 +585,6 @@ __device__ __forceinline__ int floor_sat_d(double v) {
   if (!(v > -1.0e9)) return -1000000000;
   if (v > 1.0e9) return 1000000000;
   return (int)floor(v);
-}
-__device__ __forceinline__ uint32_t spread10(uint32_t v) {
-  v &= 0x3ff;
-  v = (v | (v << 16)) & 0x030000FF;
-  v = (v | (v << 8)) & 0x0300F00F;
-  v = (v | (v << 4)) & 0x030C30C3;
-  v = (v | (v << 2)) & 0x09249249;
-  return v;
 }
 __device__ __forceinline__ void dg_decode(const GridDev& G, int c, int& i, int& j, int& k) {
   j = c % G.dg_nj + 1;
@@ -722,23 +796,15 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, Build
   b.own_cnt[i] = cnt;
 }
 
-// Own rows, one wavefront per search-grid cell: the lanes take the
-// candidates of the 27 surrounding cells 64 at a time, the home particles
-// of the cell are walked in a uniform loop; the lanes that pass the
-// reference's criteria are compacted with a ballot into the home row, kept
-// in registers (lane k holds entry k, at most 64 per row) and finally put in
-// the reference's order by a rank sort.  A row longer than 64 raises the
-// overflow flag (the host then reruns the thread-per-particle kernel).
-__device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
-  int pos = 0;
-#pragma unroll
-  for (int step = 32; step > 0; step >>= 1) {
-    const uint64_t lowmask = (pos + step >= 64) ? ~0ull : ((1ull << (pos + step)) - 1ull);
-    if (__popcll(m & lowmask) <= k) pos += step;
-  }
-  return pos;
-}
-
+// Own rows, one wavefront per search-grid cell.  The candidates of the 27
+// surrounding cells are staged in LDS once per cell (DES cells already
+// decoded); the home particles of the cell are walked in a uniform loop and
+// the lanes take the candidates 64 at a time.  The lanes that pass the
+// reference's criteria are appended to the home row in lane order with a
+// ballot, the row is kept in registers (lane k holds entry k, at most 64
+// per row) and finally put in the reference's order by a rank sort.  A row
+// longer than 64 raises the overflow flag (the host then reruns the
+// thread-per-particle kernel).
 constexpr int kRowCap = 192;  // candidates staged in LDS per wavefront
 
 __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPtrs b, int ncells) {
@@ -798,7 +864,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
       s_y[wv][t] = b.pos[S + j];
       s_z[wv][t] = b.pos[2 * S + j];
       s_r[wv][t] = b.radius[j];
-      s_d[wv][t] = b.dgc[j];
+      int li, lj, lk;
+      dg_decode(G, b.dgc[j], li, lj, lk);
+      s_d[wv][t] = (lk * 1024 + lj) * 1024 + li;
       s_j[wv][t] = j;
       s_g[wv][t] = st != 0 ? g : INT_MIN;
     }
@@ -849,7 +917,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
           yj = b.pos[S + j];
           zj = b.pos[2 * S + j];
           rj = b.radius[j];
-          dj = b.dgc[j];
+          int li, lj, lk;
+          dg_decode(G, b.dgc[j], li, lj, lk);
+          dj = (lk * 1024 + lj) * 1024 + li;
         }
       }
       bool pass = false;
@@ -859,28 +929,27 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
         const double d0 = x[0] - xj, d1 = x[1] - yj, d2 = x[2] - zj;
         const double ds = d0 * d0 + d1 * d1 + d2 * d2;
         if (!(ds > sr * sr)) {
-          int li, lj, lk;
-          dg_decode(G, dj, li, lj, lk);
+          const int li = dj & 1023, lj = (dj >> 10) & 1023, lk = dj >> 20;
           if (li >= ci - lo[0] && li <= ci + hi[0] && lj >= cj - lo[1] && lj <= cj + hi[1] &&
               lk >= ck - lo[2] && lk <= ck + hi[2]) {
             pass = true;
-            key = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
+            key = ((uint64_t)(uint32_t)dj << 32) | (uint32_t)gj;
           }
         }
       }
-      const uint64_t mask = __ballot(pass);
-      const int np = __popcll(mask);
-      if (np) {
-        // lane d in [cnt, cnt+np) pulls the (d-cnt)-th passing entry
-        const int want = lane - cnt;
-        const int src = (want >= 0 && want < np) ? kth_set_bit(mask, want) : lane;
-        const int jj = __shfl(j, src, 64);
-        const uint64_t kk = __shfl(key, src, 64);
-        if (want >= 0 && want < np) {
+      // append the passing lanes in lane order: lane cnt+k takes the k-th
+      uint64_t mask = __ballot(pass);
+      while (mask) {
+        const int src = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int jj = __builtin_amdgcn_readlane(j, src);
+        const uint32_t klo = __builtin_amdgcn_readlane((int)(uint32_t)key, src);
+        const uint32_t khi = __builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), src);
+        if (lane == cnt) {
           row_j = jj;
-          row_key = kk;
+          row_key = ((uint64_t)khi << 32) | klo;
         }
-        cnt += np;
+        cnt++;
       }
     }
     if (cnt > 64) {
@@ -894,7 +963,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
     // rank sort by key (keys are distinct: the id is part of the key)
     int rank = 0;
     for (int m = 0; m < cnt; m++) {
-      const uint64_t km = __shfl(row_key, m, 64);
+      const uint32_t klo = __builtin_amdgcn_readlane((int)(uint32_t)row_key, m);
+      const uint32_t khi = __builtin_amdgcn_readlane((int)(uint32_t)(row_key >> 32), m);
+      const uint64_t km = ((uint64_t)khi << 32) | klo;
       rank += (km < row_key) ? 1 : 0;
     }
     if (lane < cnt) {
@@ -1029,18 +1100,18 @@ __global__ __launch_bounds__(kBS) void k_locate(int n, int S, int ni, int nj, in
   const int I = locate_cell(xe, ni, 1.0 / (xe[1] - xe[0]), pos[i]);
   const int J = locate_cell(yn, nj, 1.0 / (yn[1] - yn[0]), pos[S + i]);
   const int K = locate_cell(zt, nk, 1.0 / (zt[1] - zt[0]), pos[2 * S + i]);
-  fcell[i] = (I && J && K) ? (J - 1) + (I - 1) * nj + (K - 1) * nj * ni : -1;
+  fcell[i] = (I && J && K) ? pack_ijk(I, J, K) : -1;
 }
 
 // PIC counts; deletes particles outside the fluid grid (particles_in_cell.f:175-184)
-__global__ __launch_bounds__(kBS) void k_pic_count(int n, int S, const int* fcell,
+__global__ __launch_bounds__(kBS) void k_pic_count(int n, int S, int ni, int nj, const int* fcell,
                                                    uint8_t* state, uint8_t* wmask, double* pos,
                                                    double* vel, double* omega, double* radius,
                                                    int* cnt, int* flags) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n || state[i] == 0) return;
-  const int c = fcell[i];
-  if (c < 0) {
+  const int pc = fcell[i];
+  if (pc < 0) {
     state[i] = 0;
     wmask[i] = 0;
     for (int d = 0; d < 3; d++) {
@@ -1052,14 +1123,19 @@ __global__ __launch_bounds__(kBS) void k_pic_count(int n, int S, const int* fcel
     atomicAdd(&flags[kFlagOutOfDomain], 1);
     return;
   }
-  atomicAdd(&cnt[c], 1);
+  int I, J, K;
+  unpack_ijk(pc, I, J, K);
+  atomicAdd(&cnt[lin_ijk(I, J, K, ni, nj)], 1);
 }
 
-__global__ __launch_bounds__(kBS) void k_pic_scatter(int n, const int* fcell, const uint8_t* state,
-                                                     const int* off, int* cur, int* pic) {
+__global__ __launch_bounds__(kBS) void k_pic_scatter(int n, int ni, int nj, const int* fcell,
+                                                     const uint8_t* state, const int* off,
+                                                     int* cur, int* pic) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n || state[i] == 0) return;
-  const int c = fcell[i];
+  int I, J, K;
+  unpack_ijk(fcell[i], I, J, K);
+  const int c = lin_ijk(I, J, K, ni, nj);
   const int s = atomicAdd(&cur[c], 1);
   pic[off[c] + s] = i;
 }

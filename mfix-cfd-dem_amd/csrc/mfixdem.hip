@@ -82,6 +82,7 @@ struct mfixdem_handle {
   uint8_t* fluid = nullptr;
   int *pic_cnt = nullptr, *pic_off = nullptr, *pic_cur = nullptr, *pic = nullptr;
   unsigned long long* rops_fix[2]{};  // fixed-point node sums, double-buffered
+  int* dep_list = nullptr;            // deposition order (slots by fluid-cell Morton code)
   int fb = 0;
   double dep_inv[MFIXDEM_MAX_PHASES]{};
   uint16_t* cell_facets = nullptr;
@@ -258,6 +259,16 @@ int32_t do_bin(mfixdem_handle* h) {
   hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G, h->keys[1],
                      h->state[h->tw], h->fstart, h->fend);
   HIPOK(hipGetLastError());
+  if (h->S.p.coupled) {
+    // deposition order for the steps up to the next rebuild (the key buffers
+    // are free again: the cell ranges above were their last reader)
+    hipLaunchKernelGGL(k_dep_keys, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->state[h->tw],
+                       h->fcell[h->tw], h->keys[0], h->vals[0]);
+    HIPOK(hipGetLastError());
+    tb = h->sort_tmp_bytes;
+    HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
+                                             h->dep_list, n, 0, 30, h->stream));
+  }
   h->binned = true;
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -362,13 +373,15 @@ int32_t do_neighbour(mfixdem_handle* h) {
 int32_t do_pic_lists(mfixdem_handle* h) {
   const int n = h->n, S = h->stride, t = h->tw, c = h->cur, nc = h->S.ncell;
   CHECKRC(fill_i32(h->stream, h->pic_cnt, nc, 0));
-  hipLaunchKernelGGL(k_pic_count, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->fcell[t],
+  hipLaunchKernelGGL(k_pic_count, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->S.ni, h->S.nj,
+                     h->fcell[t],
                      h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c], h->radius[t],
                      h->pic_cnt, h->flags);
   HIPOK(hipGetLastError());
   CHECKRC(scan_i32(h, h->pic_cnt, h->pic_off, nc));
   CHECKRC(fill_i32(h->stream, h->pic_cur, nc, 0));
-  hipLaunchKernelGGL(k_pic_scatter, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->fcell[t],
+  hipLaunchKernelGGL(k_pic_scatter, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->S.ni, h->S.nj,
+                     h->fcell[t],
                      h->state[t], h->pic_off, h->pic_cur, h->pic);
   hipLaunchKernelGGL(k_pic_sort, dim3(nblk(nc)), dim3(kBS), 0, h->stream, nc, h->pic_off,
                      h->gid[t], h->pic);
@@ -431,9 +444,10 @@ int32_t ref_mean_fields(mfixdem_handle* h) {
 // deposition of the current positions into the fixed-point accumulators
 int32_t deposit_current(mfixdem_handle* h) {
   DevParams P = h->P;
-  hipLaunchKernelGGL(k_deposit_tiled, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
-                     h->pos[h->cur], h->pmass[h->tw], h->phase[h->tw], h->state[h->tw],
-                     h->fcell[h->tw], h->fluid, h->xe, h->yn, h->zt, h->rops_fix[h->fb]);
+  hipLaunchKernelGGL(k_deposit_list, dim3(nblk(((int64_t)h->n + kDepPPT - 1) / kDepPPT)),
+                     dim3(kBS), 0, h->stream, P, h->dep_list, h->pos[h->cur], h->pmass[h->tw],
+                     h->phase[h->tw], h->state[h->tw], h->fcell[h->tw], h->fluid, h->xe, h->yn,
+                     h->zt, h->rops_fix[h->fb]);
   HIPOK(hipGetLastError());
   return do_cell_fields(h, true);
 }
@@ -719,15 +733,19 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
       (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, h->S.dg_ncell)) ||
       (rc = dalloc(&h->stage, 3 * n)) ||
       (rc = dalloc(&h->rops_fix[0], nc * p->nphases)) ||
-      (rc = dalloc(&h->rops_fix[1], nc * p->nphases)))
+      (rc = dalloc(&h->rops_fix[1], nc * p->nphases)) || (rc = dalloc(&h->dep_list, n)))
     return fail(rc);
   for (int b = 0; b < 2; b++)
     if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * p->nphases) != hipSuccess)
       return fail(MFIXDEM_ERR_HIP);
+  size_t dep_tmp = 0;
   if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
                                          h->vals[0], h->vals[1], (int)n, 0, 30 + h->G.gid_bits,
-                                         h->stream) != hipSuccess)
+                                         h->stream) != hipSuccess ||
+      hipcub::DeviceRadixSort::SortPairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0],
+                                         h->dep_list, (int)n, 0, 30, h->stream) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
+  h->sort_tmp_bytes = std::max(h->sort_tmp_bytes, dep_tmp);
   if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
     return fail(MFIXDEM_ERR_OOM);
   if ((rc = upload_setup(h))) return fail(rc);
@@ -748,7 +766,7 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
     dfree(h->keys[b]); dfree(h->vals[b]); dfree(h->nbr_start[b]); dfree(h->nbr_split[b]);
     dfree(h->pair_base[b]); dfree(h->nbr[b]); dfree(h->nbr_pid[b]); dfree(h->pft[b]);
   }
-  dfree(h->fc); dfree(h->tow); dfree(h->fc_id); dfree(h->tow_id); dfree(h->sort_tmp); dfree(h->fstart); dfree(h->fend);
+  dfree(h->fc); dfree(h->tow); dfree(h->fc_id); dfree(h->tow_id); dfree(h->sort_tmp); dfree(h->dep_list); dfree(h->fstart); dfree(h->fend);
   dfree(h->own_cnt); dfree(h->own_tmp); dfree(h->own_key); dfree(h->in_cnt); dfree(h->in_cur);
   dfree(h->tot_cnt); dfree(h->scan_bsum); dfree(h->flags); dfree(h->xe); dfree(h->yn);
   dfree(h->zt); dfree(h->vnode); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
@@ -1035,12 +1053,13 @@ int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk, in
     const int q = g[i];
     if (dg_pijk) dg_pijk[q] = d[i] + 1;
     if (pijk) {
-      const int c = f[i];
+      const int c = f[i];  // packed I | J << 10 | K << 20
       if (c >= 0) {
-        pijk[q] = (c / nj) % ni + 1;
-        pijk[ld + q] = c % nj + 1;
-        pijk[2 * ld + q] = c / (nj * ni) + 1;
-        pijk[3 * ld + q] = c + 1;
+        const int I = c & 1023, J = (c >> 10) & 1023, K = c >> 20;
+        pijk[q] = I;
+        pijk[ld + q] = J;
+        pijk[2 * ld + q] = K;
+        pijk[3 * ld + q] = J + (I - 1) * nj + (K - 1) * nj * ni;
       } else {
         pijk[q] = pijk[ld + q] = pijk[2 * ld + q] = pijk[3 * ld + q] = 0;
       }

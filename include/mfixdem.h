@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MFIXDEM_ABI_VERSION 1
+#define MFIXDEM_ABI_VERSION 2
 #define MFIXDEM_MAX_PHASES 4
 #define MFIXDEM_MAX_PAIRS_IN (MFIXDEM_MAX_PHASES * (MFIXDEM_MAX_PHASES + 1) / 2)
 /* MFiX's UNDEFINED sentinel (model/param1_mod.f:20). */
@@ -63,7 +63,11 @@ enum { MFIXDEM_LSD = 0, MFIXDEM_HERTZ = 1 };                 /* DES_COLL_MODEL  
 enum { MFIXDEM_EULER = 0, MFIXDEM_ADAMS_BASHFORTH = 1 };     /* DES_INTG_METHOD  */
 enum { MFIXDEM_FP64 = 0, MFIXDEM_MIXED = 1 };                /* arithmetic mode  */
 enum { MFIXDEM_DRAG_GIDASPOW = 0, MFIXDEM_DRAG_WEN_YU = 1 }; /* DRAG_TYPE        */
-enum { MFIXDEM_INTERP_GARG_2012 = 0 };                       /* DES_INTERP_SCHEME */
+/* DES_INTERP_SCHEME (check_solids_common_discrete.f:752-775) */
+enum { MFIXDEM_INTERP_GARG_2012 = 0, MFIXDEM_INTERP_NONE = 1, MFIXDEM_INTERP_SQUARE_DPVM = 2,
+       MFIXDEM_INTERP_GAUSS_DPVM = 3, MFIXDEM_INTERP_LINEAR_HAT = 4 };
+/* DES_NEIGHBOR_SEARCH (des_init_namelist.f:1149): 1 = NSQUARE, 4 = grid */
+enum { MFIXDEM_NSEARCH_NSQUARE = 1, MFIXDEM_NSEARCH_GRID = 4 };
 
 /* debug flags for mfixdem_set_debug */
 enum {
@@ -95,7 +99,7 @@ typedef struct mfixdem_params {
   int32_t neighbor_search_n;  /* NEIGHBOR_SEARCH_N (25)                     */
   int32_t coupled;            /* DES_CONTINUUM_COUPLED with a frozen gas    */
   int32_t drag_type;          /* MFIXDEM_DRAG_*                             */
-  int32_t interp_scheme;      /* MFIXDEM_INTERP_GARG_2012                   */
+  int32_t interp_scheme;      /* MFIXDEM_INTERP_* (GARG_2012)               */
   int32_t model_b;            /* MODEL_B (0 = Model A)                      */
   int32_t max_pip;            /* particle capacity                          */
   int32_t imax, jmax, kmax;   /* fluid grid cells (IMAX, JMAX, KMAX)        */
@@ -103,7 +107,8 @@ typedef struct mfixdem_params {
   int32_t desgridsearch_jmax;
   int32_t desgridsearch_kmax;
   int32_t remove_rogue;       /* REMOVE_ROGUE_PARTICLES (1)                 */
-  int32_t reserved0, reserved1;
+  int32_t des_interp_on;      /* DES_INTERP_ON: interpolate drag fields (0)  */
+  int32_t des_interp_mean_fields; /* DES_INTERP_MEAN_FIELDS (0; GARG forces 1) */
   double x_min, y_min, z_min;        /* X_MIN ... (0)                       */
   double xlength, ylength, zlength;  /* XLENGTH ...                         */
   double dtsolid;             /* DTSOLID; UNDEFINED = TCOLL/DTSOLID_FACTOR  */
@@ -124,6 +129,9 @@ typedef struct mfixdem_params {
   double max_radius;          /* MAX_RADIUS of the particle set             */
   double ro_g0, mu_g0;        /* gas density and viscosity (frozen field)   */
   double des_epg_clip;        /* DES_EPG_CLIP (UNDEFINED)                   */
+  double des_interp_width;    /* DES_INTERP_WIDTH (UNDEFINED; DPVM needs it) */
+  int32_t des_neighbor_search;/* MFIXDEM_NSEARCH_* (4)                      */
+  int32_t reserved2;
 } mfixdem_params;
 
 /* Quantities derived once at set-up (check_solids_dem.f:519-1122,
@@ -210,6 +218,7 @@ int32_t mfixdem_cfnewvalues(mfixdem_handle* h);      /* cfnewvalues.f:641    */
 int32_t mfixdem_desgrid_pic(mfixdem_handle* h);      /* desgrid_mod.f:877    */
 int32_t mfixdem_neighbour(mfixdem_handle* h);        /* neighbour.f:21       */
 int32_t mfixdem_particles_in_cell(mfixdem_handle* h);/* particles_in_cell.f:20 */
+int32_t mfixdem_calc_interp_weights(mfixdem_handle* h); /* calc_interp_weights.f:22 */
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
 
 /* --- inspection for parity and restart ------------------------------------ */

@@ -44,7 +44,11 @@ struct DevParams {
   // fluid grid
   int ni, nj, nk, ncell;
   double ro_g0, mu_g0;
-  double dep_scale[kMaxPh];  // fixed-point scale of the node mass sums (power of 2)
+  double dep_scale[kMaxPh];  // fixed-point scale of the deposition sums (power of 2)
+  // filter schemes (DES_INTERP_SCHEME other than GARG_2012)
+  int interp, interp_on;     // MFIXDEM_INTERP_*, DES_INTERP_ON
+  double fw, fw3, oofv;      // FILTER_WIDTH_INTERP, x3, OoFILTER_VOL (DPVM)
+  double dxc[3];             // uniform fluid cell size per axis
   FacetDev facets[kMaxF];
 };
 

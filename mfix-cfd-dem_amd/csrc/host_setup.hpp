@@ -439,8 +439,26 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
   if (p.integrator != MFIXDEM_EULER && p.integrator != MFIXDEM_ADAMS_BASHFORTH)
     return MFIXDEM_ERR_ARG;
   if (p.precision != MFIXDEM_FP64 && p.precision != MFIXDEM_MIXED) return MFIXDEM_ERR_ARG;
-  if (p.coupled && (p.interp_scheme != MFIXDEM_INTERP_GARG_2012 || p.model_b))
-    return MFIXDEM_ERR_UNSUPPORTED;
+  if (p.coupled && p.model_b) return MFIXDEM_ERR_UNSUPPORTED;
+  if (p.des_neighbor_search != MFIXDEM_NSEARCH_GRID &&
+      p.des_neighbor_search != MFIXDEM_NSEARCH_NSQUARE)
+    return MFIXDEM_ERR_ARG;
+  // DES_INTERP_SCHEME checks (check_solids_common_discrete.f:752-860)
+  switch (p.interp_scheme) {
+    case MFIXDEM_INTERP_GARG_2012:
+    case MFIXDEM_INTERP_LINEAR_HAT:
+      if (p.des_interp_width != kUndef) return MFIXDEM_ERR_ARG;
+      break;
+    case MFIXDEM_INTERP_NONE:
+      if (p.des_interp_on || p.des_interp_mean_fields || p.des_interp_width != kUndef)
+        return MFIXDEM_ERR_ARG;
+      break;
+    case MFIXDEM_INTERP_SQUARE_DPVM:
+    case MFIXDEM_INTERP_GAUSS_DPVM:
+      if (p.des_interp_width == kUndef) return MFIXDEM_ERR_ARG;
+      break;
+    default: return MFIXDEM_ERR_ARG;
+  }
   if (p.drag_type != MFIXDEM_DRAG_GIDASPOW && p.drag_type != MFIXDEM_DRAG_WEN_YU)
     return MFIXDEM_ERR_ARG;
   s = Setup{};
@@ -453,6 +471,15 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
   if (s.dg_ni > 1023 || s.dg_nj > 1023 || s.dg_nk > 1023) return MFIXDEM_ERR_UNSUPPORTED;
   // fluid cells are packed in 10 bits per axis on the device
   if (s.ni > 1023 || s.nj > 1023 || s.nk > 1023) return MFIXDEM_ERR_UNSUPPORTED;
+  if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
+      p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
+    // set_filter_des.f:70-87: half-width within a fluid cell and a DES cell
+    const double dmin = std::min(std::min(s.dx, s.dy), s.dz);
+    const double dgmin = std::min(std::min(1.0 / s.d.dg_dxinv, 1.0 / s.d.dg_dyinv),
+                                  1.0 / s.d.dg_dzinv);
+    if (0.5 * p.des_interp_width > dmin || 0.5 * p.des_interp_width > dgmin)
+      return MFIXDEM_ERR_ARG;
+  }
   setup_facets(s);
   if ((int)s.facets.size() > kMaxFacets) return MFIXDEM_ERR_UNSUPPORTED;
   return MFIXDEM_OK;

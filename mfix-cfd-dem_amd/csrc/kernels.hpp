@@ -15,6 +15,7 @@
 #include <climits>
 #include <cstdint>
 
+#include "../../include/mfixdem.h"
 #include "device_physics.hpp"
 
 namespace mfixdem {
@@ -41,6 +42,7 @@ struct StepPtrs {
   int* fcell;
   const double *xe, *yn, *zt;
   const double *vnode, *p_force, *ep_g;
+  const double* ugc;            // cell-centre gas velocity [3*ijk + d] (filter schemes)
   const uint8_t* fluid;
   int* flags;
 };
@@ -85,11 +87,157 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   return v;
 }
 
+// ---------------------------------------------------- filter schemes (DES1)
+__device__ __forceinline__ bool filter_scheme(int interp) {
+  return interp == MFIXDEM_INTERP_SQUARE_DPVM || interp == MFIXDEM_INTERP_GAUSS_DPVM ||
+         interp == MFIXDEM_INTERP_LINEAR_HAT;
+}
+// CALC_FILTER_WEIGHTS (calc_interp_weights.f:141-156): mass of the cubic
+// kernel of half-width FILTER_WIDTH_INTERP beyond a face
+__device__ __forceinline__ double dpvm_w(const DevParams& P, double pos1, double pos2) {
+  const double overlap = pos1 - pos2;
+  if (overlap < P.fw) {
+    const double height = P.fw - overlap;
+    return height * height * (P.fw3 - height) * P.oofv;
+  }
+  return 0.0;
+}
+// IS_ON_MYPE_PLUS2LAYERS (functions.inc:169-181, serial) and FLUID_AT
+__device__ __forceinline__ bool on_grid(const DevParams& P, int I, int J, int K) {
+  return I >= 1 && I <= P.ni && J >= 1 && J <= P.nj && K >= 1 && K <= P.nk;
+}
+__device__ __forceinline__ bool fluid_at(const DevParams& P, const uint8_t* fluid, int I, int J,
+                                         int K) {
+  return on_grid(P, I, J, K) && fluid[lin_ijk(I, J, K, P.ni, P.nj)];
+}
+// one axis of CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:365-460)
+__device__ __forceinline__ void lhat_axis(double xpos, double face_lo, double d, bool fm, bool fp,
+                                          double sw[3]) {
+  const double xc = face_lo + 0.5 * d;
+  const double oo = 1.0 / d;  // oDX_E = ONE/(HALF*(DX+DX)) on a uniform grid
+  sw[0] = 0.0;
+  sw[1] = 1.0;
+  sw[2] = 0.0;
+  if (xpos < xc) {
+    if (fm) {
+      sw[0] = (xc - xpos) * oo;
+      sw[1] = 1.0 - sw[0];
+    }
+  } else if (fp) {
+    sw[2] = (xpos - xc) * oo;
+    sw[1] = 1.0 - sw[2];
+  }
+}
+// The 27 (FILTER_CELL, FILTER_WEIGHT) pairs of a particle in fluid cell
+// (ci, cj, ck), in the reference's order: SQUARE/GAUSS_DPVM
+// (CALC_INTERP_WEIGHTS1 :47-139, K then I then J fastest; a non-fluid
+// target gives its weight to the own cell) and LINEAR_HAT
+// (CALC_INTERP_WEIGHTS2 :351-528, K then J then I fastest; a non-fluid
+// target gets weight 0).  fn(cell, weight) is called for each, cell 0-based.
+template <class Fn>
+__device__ __forceinline__ void for_filter(const DevParams& P, const double* xe, const double* yn,
+                                           const double* zt, const uint8_t* fluid, int ci, int cj,
+                                           int ck, const double pos[3], Fn fn) {
+  const int own = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  if (P.interp == MFIXDEM_INTERP_LINEAR_HAT) {
+    double sx[3], sy[3], sz[3];
+    lhat_axis(pos[0], xe[ci - 1], P.dxc[0], fluid_at(P, fluid, ci - 1, cj, ck),
+              fluid_at(P, fluid, ci + 1, cj, ck), sx);
+    lhat_axis(pos[1], yn[cj - 1], P.dxc[1], fluid_at(P, fluid, ci, cj - 1, ck),
+              fluid_at(P, fluid, ci, cj + 1, ck), sy);
+    lhat_axis(pos[2], zt[ck - 1], P.dxc[2], fluid_at(P, fluid, ci, cj, ck - 1),
+              fluid_at(P, fluid, ci, cj, ck + 1), sz);
+#pragma unroll 1
+    for (int kc = 0; kc < 3; kc++)
+#pragma unroll 1
+      for (int jc = 0; jc < 3; jc++)
+#pragma unroll 1
+        for (int ic = 0; ic < 3; ic++) {
+          const int I = ci - 1 + ic, J = cj - 1 + jc, K = ck - 1 + kc;
+          const double w = sx[ic] * sy[jc] * sz[kc];
+          if (fluid_at(P, fluid, I, J, K)) fn(lin_ijk(I, J, K, P.ni, P.nj), w);
+          else fn(own, 0.0);
+        }
+  } else {
+    double wi[3], wj[3], wk[3];
+    wi[0] = dpvm_w(P, pos[0], xe[ci - 1]);
+    wi[2] = dpvm_w(P, xe[ci], pos[0]);
+    wi[1] = 1.0 - wi[0] - wi[2];
+    wj[0] = dpvm_w(P, pos[1], yn[cj - 1]);
+    wj[2] = dpvm_w(P, yn[cj], pos[1]);
+    wj[1] = 1.0 - wj[0] - wj[2];
+    wk[0] = dpvm_w(P, pos[2], zt[ck - 1]);
+    wk[2] = dpvm_w(P, zt[ck], pos[2]);
+    wk[1] = 1.0 - wk[0] - wk[2];
+#pragma unroll 1
+    for (int kc = 0; kc < 3; kc++)
+#pragma unroll 1
+      for (int ic = 0; ic < 3; ic++)
+#pragma unroll 1
+        for (int jc = 0; jc < 3; jc++) {
+          const int I = ci - 1 + ic, J = cj - 1 + jc, K = ck - 1 + kc;
+          const double w = wi[ic] * wj[jc] * wk[kc];
+          if (!on_grid(P, I, J, K)) fn(own, 0.0);
+          else if (fluid[lin_ijk(I, J, K, P.ni, P.nj)]) fn(lin_ijk(I, J, K, P.ni, P.nj), w);
+          else fn(own, w);
+        }
+  }
+}
+
+// DRAG_GS_DES1 on one particle (drag_gs_des1.f:50-160): gas velocity at
+// the cell centres, voidage and P_FORCE interpolated with the filter weights
+// when DES_INTERP_ON (else the particle's cell), voidage below 1e-6 falls
+// back to the cell value, no drag outside fluid cells.
+template <bool MIXED>
+__device__ __forceinline__ void add_drag_des1(const DevParams& P, const StepPtrs& a, int i,
+                                              const double pos[3], const double vel[3],
+                                              double fc[3]) {
+  const int pc = a.fcell[i];
+  if (pc < 0) return;
+  int ci, cj, ck;
+  unpack_ijk(pc, ci, cj, ck);
+  const int c0 = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
+  if (P.interp_on && filter_scheme(P.interp)) {
+    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+      lepg = lepg + a.ep_g[c] * w;
+      for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
+      for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + a.p_force[3 * c + d] * w;
+    });
+  } else {
+    lepg = a.ep_g[c0];
+    for (int d = 0; d < 3; d++) velfp[d] = a.ugc[3 * c0 + d];
+    for (int d = 0; d < 3; d++) lpf[d] = a.p_force[3 * c0 + d];
+  }
+  if (lepg < 1.0e-6) lepg = a.ep_g[c0];
+  if (!a.fluid[c0]) return;
+  const double pvol = a.pvol[i];
+  if (!MIXED) {
+    const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+    const double vrel = sqrt(dot3(vs, vs));
+    const double fgp = drag_fgp_fp64(P.drag_type, lepg, P.mu_g0, P.ro_g0, vrel,
+                                     2.0 * a.radius[i], pvol);
+    for (int d = 0; d < 3; d++) fc[d] = fc[d] + fgp * (velfp[d] - vel[d]);
+  } else {
+    const float vs[3] = {(float)(velfp[0] - vel[0]), (float)(velfp[1] - vel[1]),
+                         (float)(velfp[2] - vel[2])};
+    const float vrel = fsqrt(dot3f(vs, vs));
+    const float fgp = drag_fgp_fp32(P.drag_type, (float)lepg, (float)P.mu_g0, (float)P.ro_g0,
+                                    vrel, (float)(2.0 * a.radius[i]), (float)pvol);
+    for (int d = 0; d < 3; d++) fc[d] = fc[d] + (double)(fgp * vs[d]);
+  }
+  for (int d = 0; d < 3; d++) fc[d] = fc[d] + lpf[d] * pvol;
+}
+
 // GARG_2012 drag on one particle (DRAG_GS_DES0 :140-212 with
 // DRAG_INTERPOLATION :507-566); adds D_FORCE and P_FORCE*PVOL to fc.
-template <bool MIXED>
+template <bool MIXED, bool FILTER>
 __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, int i,
                                          const double pos[3], const double vel[3], double fc[3]) {
+  if (FILTER) {  // any DES_INTERP_SCHEME other than GARG_2012
+    add_drag_des1<MIXED>(P, a, i, pos, vel, fc);
+    return;
+  }
   const int pc = a.fcell[i];
   if (pc < 0) return;
   int ci, cj, ck;
@@ -135,8 +283,10 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
 }
 
 // One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
-// unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES).
-template <bool MIXED, int MODE>
+// unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES);
+// FILTER selects the DRAG_GS_DES1 drag (any interpolation scheme other than
+// GARG_2012), so that each kernel carries only one drag path.
+template <bool MIXED, int MODE, bool FILTER>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= P.n) return;
@@ -270,11 +420,11 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
   }
   if (MODE == kModeDrag) {
     load3(a.fc, S, i, fc);
-    if (P.coupled) add_drag<MIXED>(P, a, i, pos, vel, fc);
+    if (P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
     store3(a.fc, S, i, fc);
     return;
   }
-  if (MODE == kModeFused && P.coupled) add_drag<MIXED>(P, a, i, pos, vel, fc);
+  if (MODE == kModeFused && P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
   if (MODE == kModeIntegrate) {
     load3(a.fc, S, i, fc);
     load3(a.tow, S, i, tow);
@@ -484,6 +634,80 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
       tj -= tj_n;
       tk++;
     }
+  }
+}
+
+// COMP_MEAN_FIELDS1 deposition (comp_mean_fields1.f:42-64): PVOL times the
+// filter weight into the 27 filter cells (or PVOL into the own cell for the
+// NONE scheme), fixed-point int64 sums per cell.  Visits particles in the
+// deposition order; the workgroup's cells are summed in an LDS tile over
+// the bounding box of its targets (up to kDepCap cells), else directly.
+__global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* list,
+                                                        const double* pos, const double* pvol,
+                                                        const uint8_t* phase, const uint8_t* state,
+                                                        const int* fcell, const double* xe,
+                                                        const double* yn, const double* zt,
+                                                        const uint8_t* fluid,
+                                                        unsigned long long* fix) {
+  __shared__ unsigned long long tile[kDepCap];
+  __shared__ int box[6];
+  const int S = P.stride;
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  int i = -1, pc = -1, ci = 0, cj = 0, ck = 0;
+  if (k < P.n) {
+    i = list[k];
+    pc = fcell[i];
+    if (state[i] == 0 || pc < 0) pc = -1;
+    else unpack_ijk(pc, ci, cj, ck);
+  }
+  if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
+  else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
+  __syncthreads();
+  if (pc >= 0) {
+    atomicMin(&box[0], ci); atomicMin(&box[1], cj); atomicMin(&box[2], ck);
+    atomicMax(&box[3], ci); atomicMax(&box[4], cj); atomicMax(&box[5], ck);
+  }
+  __syncthreads();
+  // target cells lie in [min-1, max+1] per axis
+  const int bi = box[0] - 1, bj = box[1] - 1, bk = box[2] - 1;
+  const int ti_n = box[3] - box[0] + 3, tj_n = box[4] - box[1] + 3, tk_n = box[5] - box[2] + 3;
+  const bool any = box[0] != INT_MAX;
+  const int vol = any ? ti_n * tj_n * tk_n : 0;
+  const bool tiled = any && P.nph == 1 && vol <= kDepCap;
+  if (tiled)
+    for (int q = threadIdx.x; q < vol; q += kBS) tile[q] = 0ull;
+  __syncthreads();
+  if (pc >= 0) {
+    const int ph = phase[i];
+    const double vol_wt = pvol[i];
+    const double sc = P.dep_scale[ph];
+    unsigned long long* f = fix + (size_t)ph * P.ncell;
+    const int nj = P.nj, ninj = P.nj * P.ni;
+    auto add = [&](int c, double w) {
+      const double v = vol_wt * w;
+      const unsigned long long qv = (unsigned long long)__double2ll_rn(v * sc);
+      if (qv == 0ull) return;
+      if (tiled) {
+        const int K = c / ninj, r = c - K * ninj, I = r / nj, J = r - I * nj;  // 0-based
+        atomicAdd(&tile[(I + 1 - bi) + ti_n * ((J + 1 - bj) + tj_n * (K + 1 - bk))], qv);
+      } else {
+        atomicAdd(&f[c], qv);
+      }
+    };
+    if (filter_scheme(P.interp)) {
+      const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+      for_filter(P, xe, yn, zt, fluid, ci, cj, ck, x, add);
+    } else {
+      add(lin_ijk(ci, cj, ck, P.ni, P.nj), 1.0);
+    }
+  }
+  if (!tiled) return;
+  __syncthreads();
+  for (int q = threadIdx.x; q < vol; q += kBS) {
+    const unsigned long long v = tile[q];
+    if (v == 0ull) continue;
+    const int ti = q % ti_n, tj = (q / ti_n) % tj_n, tk = q / (ti_n * tj_n);
+    atomicAdd(&fix[lin_ijk(bi + ti, bj + tj, bk + tk, P.ni, P.nj)], v);
   }
 }
 
@@ -1279,6 +1503,35 @@ __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kma
   if (fl)
     for (int m = 0; m < nph; m++)
       if (acc[m] > 0.0) acc[m] = acc[m] / vc;
+  for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = acc[m];
+  if (fl) {
+    double sum_eps = 0.0;
+    for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / f.ro_s[m];
+    const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);
+    f.ep_g[c] = e;
+    if (e <= 0.0 || e > 1.0) atomicAdd(&f.flags[kFlagEpg], 1);
+  }
+}
+
+// SOLVOLINC -> ROP_S = RO_S*SOLVOLINC/VOL in fluid cells, 0 elsewhere
+// (comp_mean_fields1.f:66-80), then EP_G (calc_epg_des.f:133-145); clears
+// the other accumulator buffer for the next step.
+__global__ __launch_bounds__(kBS) void k_cell_fields_des1(int ncell, int nph, double packed_eps,
+                                                          double inv_scale0, double inv_scale1,
+                                                          double inv_scale2, double inv_scale3,
+                                                          FieldPtrs f) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  if (c >= ncell) return;
+  const double inv[kMaxPh] = {inv_scale0, inv_scale1, inv_scale2, inv_scale3};
+  const uint8_t fl = f.fluid[c];
+  const double vc = f.vol[c];
+  double acc[kMaxPh] = {0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < nph; m++) {
+    const unsigned long long u = f.fix[(size_t)m * ncell + c];
+    f.fix_clear[(size_t)m * ncell + c] = 0ull;
+    if (u >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
+    if (fl) acc[m] = f.ro_s[m] * ((double)u * inv[m]) / vc;
+  }
   for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = acc[m];
   if (fl) {
     double sum_eps = 0.0;

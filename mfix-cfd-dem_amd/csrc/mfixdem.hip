@@ -76,6 +76,7 @@ struct mfixdem_handle {
   int* scan_bsum = nullptr;
   int* flags = nullptr;
   // gas
+  double* ugc = nullptr;  // cell-centre gas velocity (filter schemes)
   double *xe = nullptr, *yn = nullptr, *zt = nullptr, *vnode = nullptr, *p_force = nullptr,
          *ep_g = nullptr, *rop_s = nullptr, *rops_node = nullptr, *vol = nullptr,
          *vol_surr = nullptr, *ro_s = nullptr;
@@ -152,6 +153,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.fcell = h->fcell[t];
   a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
   a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
+  a.ugc = h->ugc;
   a.flags = h->flags;
   return a;
 }
@@ -165,8 +167,10 @@ void invalidate_graphs(mfixdem_handle* h) {
       }
 }
 
+// the reference-order deposition exists for the GARG_2012 node sums
 bool ref_deposition(const mfixdem_handle* h) {
-  return (h->debug & MFIXDEM_DEBUG_REFERENCE_DEPOSITION) != 0;
+  return (h->debug & MFIXDEM_DEBUG_REFERENCE_DEPOSITION) != 0 &&
+         h->S.p.interp_scheme == MFIXDEM_INTERP_GARG_2012;
 }
 
 bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS) != 0; }
@@ -189,10 +193,15 @@ int32_t launch_step(mfixdem_handle* h) {
   P.first_pass = h->first_pass ? 1 : 0;
   P.store_forces = (h->debug & MFIXDEM_DEBUG_STORE_FORCES) ? 1 : 0;
   StepPtrs a = step_ptrs(h);
-  if (h->S.p.precision == MFIXDEM_MIXED)
-    hipLaunchKernelGGL((k_step<true, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
-  else
-    hipLaunchKernelGGL((k_step<false, MODE>), dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P, a);
+  const dim3 grid(nblk(h->n)), blk(kBS);
+  const bool mixed = h->S.p.precision == MFIXDEM_MIXED;
+  if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
+    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, true>), grid, blk, 0, h->stream, P, a);
+    else hipLaunchKernelGGL((k_step<false, MODE, true>), grid, blk, 0, h->stream, P, a);
+  } else {
+    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, false>), grid, blk, 0, h->stream, P, a);
+    else hipLaunchKernelGGL((k_step<false, MODE, false>), grid, blk, 0, h->stream, P, a);
+  }
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
@@ -405,6 +414,13 @@ int32_t do_cell_fields(mfixdem_handle* h, bool fixed) {
   const double packed = (p.des_epg_clip != kUndef) ? 1.0 - p.des_epg_clip : 1.0;
   FieldPtrs f = field_ptrs(h);
   const double* iv = h->dep_inv;
+  if (p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
+    hipLaunchKernelGGL(k_cell_fields_des1, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
+                       h->S.ncell, p.nphases, packed, iv[0], iv[1], iv[2], iv[3], f);
+    HIPOK(hipGetLastError());
+    h->fb = 1 - h->fb;
+    return MFIXDEM_OK;
+  }
   if (fixed) {
     hipLaunchKernelGGL(k_cell_fields<true>, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
                        p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, packed,
@@ -444,6 +460,14 @@ int32_t ref_mean_fields(mfixdem_handle* h) {
 // deposition of the current positions into the fixed-point accumulators
 int32_t deposit_current(mfixdem_handle* h) {
   DevParams P = h->P;
+  if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
+    hipLaunchKernelGGL(k_deposit_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
+                       h->dep_list, h->pos[h->cur], h->pvol[h->tw], h->phase[h->tw],
+                       h->state[h->tw], h->fcell[h->tw], h->xe, h->yn, h->zt, h->fluid,
+                       h->rops_fix[h->fb]);
+    HIPOK(hipGetLastError());
+    return do_cell_fields(h, true);
+  }
   hipLaunchKernelGGL(k_deposit_list, dim3(nblk(((int64_t)h->n + kDepPPT - 1) / kDepPPT)),
                      dim3(kBS), 0, h->stream, P, h->dep_list, h->pos[h->cur], h->pmass[h->tw],
                      h->phase[h->tw], h->state[h->tw], h->fcell[h->tw], h->fluid, h->xe, h->yn,
@@ -586,6 +610,16 @@ void fill_devparams(mfixdem_handle* h) {
   }
   P.ni = S.ni; P.nj = S.nj; P.nk = S.nk; P.ncell = S.ncell;
   P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
+  P.interp = p.interp_scheme;
+  P.interp_on = p.des_interp_on;
+  P.dxc[0] = S.dx; P.dxc[1] = S.dy; P.dxc[2] = S.dz;
+  if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
+      p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
+    // set_filter_des.f:85,99-100
+    P.fw = 0.500 * p.des_interp_width;
+    P.oofv = 0.25 / (P.fw * P.fw * P.fw);
+    P.fw3 = P.fw * 3;
+  }
   for (size_t f = 0; f < S.facets.size(); f++) {
     FacetDev& F = P.facets[f];
     std::memcpy(F.v, S.facets[f].v, sizeof(F.v));
@@ -654,6 +688,8 @@ int32_t mfixdem_params_default(mfixdem_params* p) {
   }
   p->ew_young = p->vw_poisson = kUndef;
   p->des_epg_clip = kUndef;
+  p->des_interp_width = kUndef;
+  p->des_neighbor_search = MFIXDEM_NSEARCH_GRID;
   p->ro_g0 = 1.2;
   p->mu_g0 = 1.8e-5;
   return MFIXDEM_OK;
@@ -725,6 +761,7 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
       (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
       (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
       (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
+      (rc = dalloc(&h->ugc, 3 * nc)) ||
       (rc = dalloc(&h->ep_g, nc)) || (rc = dalloc(&h->rop_s, nc * p->nphases)) ||
       (rc = dalloc(&h->rops_node, nc * p->nphases)) || (rc = dalloc(&h->vol, nc)) ||
       (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
@@ -769,7 +806,7 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
   dfree(h->fc); dfree(h->tow); dfree(h->fc_id); dfree(h->tow_id); dfree(h->sort_tmp); dfree(h->dep_list); dfree(h->fstart); dfree(h->fend);
   dfree(h->own_cnt); dfree(h->own_tmp); dfree(h->own_key); dfree(h->in_cnt); dfree(h->in_cur);
   dfree(h->tot_cnt); dfree(h->scan_bsum); dfree(h->flags); dfree(h->xe); dfree(h->yn);
-  dfree(h->zt); dfree(h->vnode); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
+  dfree(h->zt); dfree(h->vnode); dfree(h->ugc); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
   dfree(h->rops_node); dfree(h->vol); dfree(h->vol_surr); dfree(h->ro_s); dfree(h->fluid);
   dfree(h->pic_cnt); dfree(h->pic_off); dfree(h->pic_cur); dfree(h->pic);
   dfree(h->cell_facets); dfree(h->stage); dfree(h->rops_fix[0]); dfree(h->rops_fix[1]);
@@ -820,8 +857,11 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
     double rho_max[MFIXDEM_MAX_PHASES] = {0, 0, 0, 0};
     for (int64_t q = 0; q < n; q++) rho_max[PH[q]] = std::max(rho_max[PH[q]], ro_sol[q]);
     const double vcell = h->S.dx * h->S.dy * h->S.dz;
+    // the filter schemes sum solids volume into cells: bounded by the
+    // volume of the 27 cells a weight can come from
+    const bool vols = h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012;
     for (int m = 0; m < MFIXDEM_MAX_PHASES; m++) {
-      const double bound = 16.0 * vcell * std::max(rho_max[m], 1.0);
+      const double bound = vols ? 32.0 * vcell : 16.0 * vcell * std::max(rho_max[m], 1.0);
       const int e = (int)std::floor(std::log2(std::ldexp(1.0, 62) / bound));
       h->P.dep_scale[m] = std::ldexp(1.0, e);
       h->dep_inv[m] = std::ldexp(1.0, -e);
@@ -903,6 +943,22 @@ int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
         vn[3 * cur + 2] = vz;
       }
   HIPOK(hipMemcpy(h->vnode, vn.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
+  // cell-centre gas velocity of DRAG_GS_DES1 (CALC_CELL_CENTER_GAS_VEL,
+  // drag_gs_des1.f:313-370) on a uniform grid: FX = 1/2 in AVG_X_E
+  // (fun_avg.inc:38-44), HALF*(m + p) in AVG_Y_N / AVG_Z_T; 0 outside fluid
+  std::vector<double> gc((size_t)3 * nc, 0.0);
+  for (int K = 1; K <= S.nk; K++)
+    for (int I = 1; I <= S.ni; I++)
+      for (int J = 1; J <= S.nj; J++) {
+        const int c = fluid_ijk(S, I, J, K);
+        if (!S.fluid[c]) continue;
+        const int imjk = fluid_ijk(S, I - 1, J, K), ijmk = fluid_ijk(S, I, J - 1, K);
+        const int ijkm = fluid_ijk(S, I, J, K - 1);
+        gc[3 * c] = 0.5 * u_g[imjk] + (1.0 - 0.5) * u_g[c];
+        gc[3 * c + 1] = 0.5 * (v_g[ijmk] + v_g[c]);
+        gc[3 * c + 2] = 0.5 * (w_g[ijkm] + w_g[c]);
+      }
+  HIPOK(hipMemcpy(h->ugc, gc.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
   std::vector<double> pf((size_t)3 * nc, 0.0);
   if (p_force) std::memcpy(pf.data(), p_force, sizeof(double) * 3 * nc);
   HIPOK(hipMemcpy(h->p_force, pf.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
@@ -1004,6 +1060,14 @@ int32_t mfixdem_particles_in_cell(mfixdem_handle* h) {
   if (ref_deposition(h)) return do_pic_lists(h);
   return delete_outside(h);
 }
+// CALC_INTERP_WEIGHTS (calc_interp_weights.f:22-33): the filter weights are
+// evaluated inside the drag and deposition kernels from the current
+// positions, so there is nothing to store here.
+int32_t mfixdem_calc_interp_weights(mfixdem_handle* h) {
+  if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  return MFIXDEM_OK;
+}
+
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h) {
   if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
   if (ref_deposition(h)) {

@@ -8,7 +8,7 @@ module mfixdem_c
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: MFIXDEM_ABI_VERSION = 1
+  integer(c_int32_t), parameter :: MFIXDEM_ABI_VERSION = 2
   integer, parameter :: MFIXDEM_MAX_PHASES = 4
   integer, parameter :: MFIXDEM_MAX_PAIRS_IN = 10
   real(c_double), parameter :: MFIXDEM_UNDEFINED = 9.87654321d31
@@ -18,6 +18,10 @@ module mfixdem_c
   integer(c_int32_t), parameter :: MFIXDEM_EULER = 0, MFIXDEM_ADAMS_BASHFORTH = 1
   integer(c_int32_t), parameter :: MFIXDEM_FP64 = 0, MFIXDEM_MIXED = 1
   integer(c_int32_t), parameter :: MFIXDEM_DRAG_GIDASPOW = 0, MFIXDEM_DRAG_WEN_YU = 1
+  integer(c_int32_t), parameter :: MFIXDEM_INTERP_GARG_2012 = 0, MFIXDEM_INTERP_NONE = 1
+  integer(c_int32_t), parameter :: MFIXDEM_INTERP_SQUARE_DPVM = 2, MFIXDEM_INTERP_GAUSS_DPVM = 3
+  integer(c_int32_t), parameter :: MFIXDEM_INTERP_LINEAR_HAT = 4
+  integer(c_int32_t), parameter :: MFIXDEM_NSEARCH_NSQUARE = 1, MFIXDEM_NSEARCH_GRID = 4
   integer(c_int32_t), parameter :: MFIXDEM_DEBUG_STORE_FORCES = 1
 
   type, bind(C) :: mfixdem_params
@@ -25,7 +29,7 @@ module mfixdem_c
      integer(c_int32_t) :: nphases, neighbor_search_n, coupled, drag_type
      integer(c_int32_t) :: interp_scheme, model_b, max_pip, imax, jmax, kmax
      integer(c_int32_t) :: desgridsearch_imax, desgridsearch_jmax, desgridsearch_kmax
-     integer(c_int32_t) :: remove_rogue, reserved0, reserved1
+     integer(c_int32_t) :: remove_rogue, des_interp_on, des_interp_mean_fields
      real(c_double) :: x_min, y_min, z_min, xlength, ylength, zlength
      real(c_double) :: dtsolid, dtsolid_factor
      real(c_double) :: grav(3)
@@ -37,6 +41,8 @@ module mfixdem_c
      real(c_double) :: ew_young, vw_poisson
      real(c_double) :: d_p0(MFIXDEM_MAX_PHASES), ro_s0(MFIXDEM_MAX_PHASES)
      real(c_double) :: max_radius, ro_g0, mu_g0, des_epg_clip
+     real(c_double) :: des_interp_width
+     integer(c_int32_t) :: des_neighbor_search, reserved2
   end type mfixdem_params
 
   type, bind(C) :: mfixdem_status
@@ -124,6 +130,11 @@ module mfixdem_c
        type(c_ptr), value :: h
      end function
      integer(c_int32_t) function mfixdem_particles_in_cell(h) bind(C, name="mfixdem_particles_in_cell")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function
+     integer(c_int32_t) function mfixdem_calc_interp_weights(h) &
+          bind(C, name="mfixdem_calc_interp_weights")
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function

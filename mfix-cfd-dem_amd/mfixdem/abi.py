@@ -15,13 +15,14 @@ import numpy as np
 MAX_PHASES = 4
 MAX_PAIRS_IN = MAX_PHASES * (MAX_PHASES + 1) // 2
 UNDEFINED = 9.87654321e31
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 LSD, HERTZ = 0, 1
 EULER, ADAMS_BASHFORTH = 0, 1
 FP64, MIXED = 0, 1
 DRAG_GIDASPOW, DRAG_WEN_YU = 0, 1
-INTERP_GARG_2012 = 0
+INTERP_GARG_2012, INTERP_NONE, INTERP_SQUARE_DPVM, INTERP_GAUSS_DPVM, INTERP_LINEAR_HAT = 0, 1, 2, 3, 4
+NSEARCH_NSQUARE, NSEARCH_GRID = 1, 4
 DEBUG_STORE_FORCES = 1
 DEBUG_TIME_KERNELS = 2
 DEBUG_REFERENCE_DEPOSITION = 4
@@ -44,7 +45,7 @@ class Params(C.Structure):
         ("model_b", _I), ("max_pip", _I), ("imax", _I), ("jmax", _I),
         ("kmax", _I), ("desgridsearch_imax", _I), ("desgridsearch_jmax", _I),
         ("desgridsearch_kmax", _I), ("remove_rogue", _I),
-        ("reserved0", _I), ("reserved1", _I),
+        ("des_interp_on", _I), ("des_interp_mean_fields", _I),
         ("x_min", _D), ("y_min", _D), ("z_min", _D),
         ("xlength", _D), ("ylength", _D), ("zlength", _D),
         ("dtsolid", _D), ("dtsolid_factor", _D), ("grav", _D * 3),
@@ -57,6 +58,7 @@ class Params(C.Structure):
         ("ew_young", _D), ("vw_poisson", _D),
         ("d_p0", _D * MAX_PHASES), ("ro_s0", _D * MAX_PHASES),
         ("max_radius", _D), ("ro_g0", _D), ("mu_g0", _D), ("des_epg_clip", _D),
+        ("des_interp_width", _D), ("des_neighbor_search", _I), ("reserved2", _I),
     ]
 
 
@@ -118,6 +120,8 @@ def default_params() -> Params:
     p.ew_young = UNDEFINED
     p.vw_poisson = UNDEFINED
     p.des_epg_clip = UNDEFINED
+    p.des_interp_width = UNDEFINED
+    p.des_neighbor_search = NSEARCH_GRID
     p.ro_g0 = 1.2
     p.mu_g0 = 1.8e-5
     return p
@@ -177,6 +181,7 @@ def _declare(lib):
         "mfixdem_desgrid_pic": ([_P], _I),
         "mfixdem_neighbour": ([_P], _I),
         "mfixdem_particles_in_cell": ([_P], _I),
+        "mfixdem_calc_interp_weights": ([_P], _I),
         "mfixdem_comp_mean_fields": ([_P], _I),
         "mfixdem_set_debug": ([_P, _I], _I),
         "mfixdem_get_forces": ([_P, _DP, _DP, C.c_int64], _I),

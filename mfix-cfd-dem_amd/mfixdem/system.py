@@ -94,6 +94,9 @@ class DemSystem:
     def particles_in_cell(self):
         abi.check(self.lib.mfixdem_particles_in_cell(self.h), "particles_in_cell")
 
+    def calc_interp_weights(self):
+        abi.check(self.lib.mfixdem_calc_interp_weights(self.h), "calc_interp_weights")
+
     def comp_mean_fields(self):
         abi.check(self.lib.mfixdem_comp_mean_fields(self.h), "comp_mean_fields")
 

@@ -35,6 +35,7 @@
 #define COLLISION_ARRAY_MAX 8  /* discretelement_mod.f:637 */
 #define MAXF 16                /* facets this restatement supports */
 #define MP MFIXDEM_MAX_PHASES
+#define FILTER_N 27            /* FILTER_SIZE = merge(27, 9, DO_K), 3-D */
 
 struct orc_sys {
   mfixdem_params p;
@@ -89,6 +90,12 @@ struct orc_sys {
   /* gas */
   int gas_set;
   double *u_g, *v_g, *w_g, *p_force, *ep_g, *rop_s, *rops_node;
+  /* filter schemes (DES1 path) */
+  double fw, fw3, oofv;               /* FILTER_WIDTH_INTERP, x3, OoFILTER_VOL */
+  double* ugc;                        /* cell-centre gas velocity [3*ijk + d] */
+  int* filter_cell;                   /* FILTER_CELL(27, MAX_PIP) -> [lc*ld + p], 0-based */
+  double* filter_weight;              /* FILTER_WEIGHT(27, MAX_PIP) */
+  double* solvolinc;                  /* SOLVOLINC(ijk, m) -> [m*ncell + ijk] */
   int* pinc;
   int* pic_off;
   int* pic_list;
@@ -611,9 +618,32 @@ static void setup_facets(orc_sys* s) {
 }
 
 /* --------------------------------------------------------------- create */
+/* DES_INTERP_SCHEME checks of check_solids_common_discrete.f:752-860 and
+   the width check of set_filter_des.f:70-87 (returns 0 when acceptable). */
+static int check_interp(const orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  const double w = p->des_interp_width;
+  switch (p->interp_scheme) {
+    case MFIXDEM_INTERP_GARG_2012: return w != UNDEF;
+    case MFIXDEM_INTERP_NONE: return p->des_interp_on || p->des_interp_mean_fields || w != UNDEF;
+    case MFIXDEM_INTERP_LINEAR_HAT: return w != UNDEF;
+    case MFIXDEM_INTERP_SQUARE_DPVM:
+    case MFIXDEM_INTERP_GAUSS_DPVM: {
+      if (w == UNDEF) return 1;
+      double dmin = fmin(fmin(s->dx, s->dy), s->dz);
+      double dgmin = fmin(fmin(1.0 / s->dg_dxinv, 1.0 / s->dg_dyinv), 1.0 / s->dg_dzinv);
+      return 0.5 * w > dmin || 0.5 * w > dgmin;
+    }
+    default: return 1;
+  }
+}
+
 orc_sys* orc_create(const mfixdem_params* p) {
   if (!p || p->nphases < 1 || p->nphases > MP || p->max_pip < 1) return NULL;
   if (p->imax < 1 || p->jmax < 1 || p->kmax < 1) return NULL;
+  if (p->des_neighbor_search != MFIXDEM_NSEARCH_GRID &&
+      p->des_neighbor_search != MFIXDEM_NSEARCH_NSQUARE)
+    return NULL;
   orc_sys* s = calloc(1, sizeof(orc_sys));
   s->p = *p;
   if (p->coll_model == MFIXDEM_HERTZ) derive_hertz(s);
@@ -624,6 +654,17 @@ orc_sys* orc_create(const mfixdem_params* p) {
   setup_fluid_grid(s);
   setup_des_grid(s);
   setup_facets(s);
+  if (check_interp(s)) {
+    orc_destroy(s);
+    return NULL;
+  }
+  if (p->interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
+      p->interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
+    /* set_filter_des.f:85,99-100 */
+    s->fw = 0.500 * p->des_interp_width;
+    s->oofv = 0.25 / (s->fw * s->fw * s->fw);
+    s->fw3 = s->fw * 3;
+  }
   int64_t ld = p->max_pip;
   s->max_pip = ld;
   s->ld = ld;
@@ -649,6 +690,9 @@ orc_sys* orc_create(const mfixdem_params* p) {
   A(s->rop_s, (size_t)s->ncell * p->nphases, double);
   A(s->rops_node, (size_t)s->ncell * p->nphases, double);
   A(s->pinc, s->ncell, int); A(s->pic_off, s->ncell + 1, int); A(s->pic_list, ld, int);
+  A(s->ugc, 3 * s->ncell, double); A(s->filter_cell, FILTER_N * ld, int);
+  A(s->filter_weight, FILTER_N * ld, double);
+  A(s->solvolinc, (size_t)s->ncell * p->nphases, double);
 #undef A
   for (int64_t q = 0; q < COLLISION_ARRAY_MAX * ld; q++) s->wall_fid[q] = -1;
   for (int c = 0; c < s->ncell; c++) s->ep_g[c] = 1.0;
@@ -666,7 +710,8 @@ void orc_destroy(orc_sys* s) {
                   s->dg_pijk, s->pijk, s->neighbor_index, s->neighbor_index_old,
                   s->neighbors, s->neighbors_old, s->pft, s->pft_old, s->wall_fid,
                   s->wall_pft, s->u_g, s->v_g, s->w_g, s->p_force, s->ep_g, s->rop_s,
-                  s->rops_node, s->pinc, s->pic_off, s->pic_list};
+                  s->rops_node, s->pinc, s->pic_off, s->pic_list, s->ugc, s->filter_cell,
+                  s->filter_weight, s->solvolinc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(s);
 }
@@ -722,6 +767,175 @@ int orc_get_particles(const orc_sys* s, double* pos, double* vel, double* omega,
   return 0;
 }
 
+/* ------------------------------------------------ filter schemes (DES1) */
+/* CALC_CELL_CENTER_GAS_VEL (drag_gs_des1.f:313-370) on a uniform grid
+   without cut cells: FX = 1/2, so AVG_X_E = FX*U(IMJK) + FX_BAR*U(IJK)
+   (fun_avg.inc:38-44); AVG_Y_N, AVG_Z_T = HALF*(m + p).  Non-fluid cells: 0. */
+static void cell_center_gas_vel(orc_sys* s) {
+  for (int k = 1; k <= s->nk; k++)
+    for (int i = 1; i <= s->ni; i++)
+      for (int j = 1; j <= s->nj; j++) {
+        int ijk = fluid_ijk(s, i, j, k) - 1;
+        double* g = s->ugc + 3 * ijk;
+        if (!s->fluid[ijk]) {
+          g[0] = g[1] = g[2] = 0.0;
+          continue;
+        }
+        int imjk = fluid_ijk(s, i - 1, j, k) - 1;
+        int ijmk = fluid_ijk(s, i, j - 1, k) - 1;
+        int ijkm = fluid_ijk(s, i, j, k - 1) - 1;
+        g[0] = 0.5 * s->u_g[imjk] + (1.0 - 0.5) * s->u_g[ijk];
+        g[1] = 0.5 * (s->v_g[ijmk] + s->v_g[ijk]);
+        g[2] = 0.5 * (s->w_g[ijkm] + s->w_g[ijk]);
+      }
+}
+
+/* CALC_FILTER_WEIGHTS (calc_interp_weights.f:141-156): the mass of the
+   cubic (Epanechnikov) kernel of half-width FILTER_WIDTH_INTERP beyond a face. */
+static double dpvm_weight(const orc_sys* s, double pos1, double pos2) {
+  double overlap = pos1 - pos2;
+  if (overlap < s->fw) {
+    double height = s->fw - overlap;
+    return height * height * (s->fw3 - height) * s->oofv;
+  }
+  return 0.0;
+}
+
+static int on_mype_plus2(const orc_sys* s, int i, int j, int k) {
+  /* IS_ON_MYPE_PLUS2LAYERS (functions.inc:169-181), serial: 1..IMAX+2 */
+  return i >= 1 && i <= s->ni && j >= 1 && j <= s->nj && k >= 1 && k <= s->nk;
+}
+
+/* CALC_INTERP_WEIGHTS1 (calc_interp_weights.f:47-139): SQUARE_DPVM and
+   GAUSS_DPVM; loop order KC, IC, JC (J fastest). */
+static void interp_weights1(orc_sys* s, int64_t q) {
+  int64_t L = s->ld;
+  int i = s->pijk[q], j = s->pijk[L + q], k = s->pijk[2 * L + q];
+  int ijk = s->pijk[3 * L + q] - 1;
+  double x = s->pos[q], y = s->pos[L + q], z = s->pos[2 * L + q];
+  double wi[3], wj[3], wk[3];
+  wi[0] = dpvm_weight(s, x, s->xe[i - 1]);
+  wi[2] = dpvm_weight(s, s->xe[i], x);
+  wi[1] = 1.0 - wi[0] - wi[2];
+  wj[0] = dpvm_weight(s, y, s->yn[j - 1]);
+  wj[2] = dpvm_weight(s, s->yn[j], y);
+  wj[1] = 1.0 - wj[0] - wj[2];
+  wk[0] = dpvm_weight(s, z, s->zt[k - 1]);
+  wk[2] = dpvm_weight(s, s->zt[k], z);
+  wk[1] = 1.0 - wk[0] - wk[2];
+  int idx = 0;
+  for (int kc = -1; kc <= 1; kc++)
+    for (int ic = -1; ic <= 1; ic++)
+      for (int jc = -1; jc <= 1; jc++, idx++) {
+        double w = wi[ic + 1] * wj[jc + 1] * wk[kc + 1];
+        int cell = ijk;
+        if (on_mype_plus2(s, i + ic, j + jc, k + kc)) {
+          int t = fluid_ijk(s, i + ic, j + jc, k + kc) - 1;
+          if (s->fluid[t]) cell = t;
+        } else {
+          w = 0.0;
+        }
+        s->filter_cell[idx * L + q] = cell;
+        s->filter_weight[idx * L + q] = w;
+      }
+}
+
+/* one axis of CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:365-460): hat
+   weights about the cell centre, own cell if the neighbour is not fluid */
+static void lhat_axis(double xpos, double face_lo, double d, int fluid_m, int fluid_p,
+                      double sw[3]) {
+  double xc = face_lo + 0.5 * d;
+  double oo = 1.0 / d; /* oDX_E = ONE/(HALF*(DX+DX)) on a uniform grid */
+  if (xpos < xc) {
+    if (fluid_m) {
+      sw[0] = (xc - xpos) * oo;
+      sw[1] = 1.0 - sw[0];
+      sw[2] = 0.0;
+    } else {
+      sw[0] = 0.0; sw[1] = 1.0; sw[2] = 0.0;
+    }
+  } else {
+    if (fluid_p) {
+      sw[0] = 0.0;
+      sw[2] = (xpos - xc) * oo;
+      sw[1] = 1.0 - sw[2];
+    } else {
+      sw[0] = 0.0; sw[1] = 1.0; sw[2] = 0.0;
+    }
+  }
+}
+
+static int fluid_at(const orc_sys* s, int i, int j, int k) {
+  if (!on_mype_plus2(s, i, j, k)) return 0;
+  return s->fluid[fluid_ijk(s, i, j, k) - 1];
+}
+
+/* CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:351-528): LINEAR_HAT; loop
+   order KC, JC, IC (I fastest), non-fluid target -> own cell, weight 0. */
+static void interp_weights2(orc_sys* s, int64_t q) {
+  int64_t L = s->ld;
+  int i = s->pijk[q], j = s->pijk[L + q], k = s->pijk[2 * L + q];
+  int ijk = s->pijk[3 * L + q] - 1;
+  double sx[3], sy[3], sz[3];
+  lhat_axis(s->pos[q], s->xe[i - 1], s->dx, fluid_at(s, i - 1, j, k), fluid_at(s, i + 1, j, k),
+            sx);
+  lhat_axis(s->pos[L + q], s->yn[j - 1], s->dy, fluid_at(s, i, j - 1, k),
+            fluid_at(s, i, j + 1, k), sy);
+  lhat_axis(s->pos[2 * L + q], s->zt[k - 1], s->dz, fluid_at(s, i, j, k - 1),
+            fluid_at(s, i, j, k + 1), sz);
+  int idx = 0;
+  for (int kc = 0; kc <= 2; kc++)
+    for (int jc = 0; jc <= 2; jc++)
+      for (int ic = 0; ic <= 2; ic++, idx++) {
+        double w = sx[ic] * sy[jc] * sz[kc];
+        int cell = ijk;
+        if (fluid_at(s, i - 1 + ic, j - 1 + jc, k - 1 + kc))
+          cell = fluid_ijk(s, i - 1 + ic, j - 1 + jc, k - 1 + kc) - 1;
+        else
+          w = 0.0;
+        s->filter_cell[idx * L + q] = cell;
+        s->filter_weight[idx * L + q] = w;
+      }
+}
+
+/* CALC_INTERP_WEIGHTS (calc_interp_weights.f:22-33) for normal particles */
+int orc_calc_interp_weights(orc_sys* s) {
+  int sch = s->p.interp_scheme;
+  if (sch != MFIXDEM_INTERP_SQUARE_DPVM && sch != MFIXDEM_INTERP_GAUSS_DPVM &&
+      sch != MFIXDEM_INTERP_LINEAR_HAT)
+    return 0;
+  for (int64_t q = 0; q < s->max_pip; q++) {
+    if (s->state[q] != NORMAL_PARTICLE) continue;
+    if (sch == MFIXDEM_INTERP_LINEAR_HAT) interp_weights2(s, q);
+    else interp_weights1(s, q);
+  }
+  return 0;
+}
+
+static int filter_size(const orc_sys* s) {
+  int sch = s->p.interp_scheme;
+  return (sch == MFIXDEM_INTERP_SQUARE_DPVM || sch == MFIXDEM_INTERP_GAUSS_DPVM ||
+          sch == MFIXDEM_INTERP_LINEAR_HAT) ? FILTER_N : 0;
+}
+
+/* FILTER_CELL (0-based) and FILTER_WEIGHT of particle q after the last
+   CALC_INTERP_WEIGHTS; returns the filter size (0 for GARG_2012 / NONE). */
+int orc_get_filter(const orc_sys* s, int64_t q, int32_t* cells, double* weights) {
+  int nf = filter_size(s);
+  if (q < 0 || q >= s->max_pip) return -1;
+  for (int lc = 0; lc < nf; lc++) {
+    cells[lc] = s->filter_cell[lc * s->ld + q];
+    weights[lc] = s->filter_weight[lc * s->ld + q];
+  }
+  return nf;
+}
+
+/* cell-centre gas velocity of fluid cell ijk (0-based), for tests */
+int orc_get_ugc(const orc_sys* s, double* ugc) {
+  memcpy(ugc, s->ugc, sizeof(double) * 3 * s->ncell);
+  return 0;
+}
+
 int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g, const double* w_g,
                 const double* p_force) {
   for (int c = 0; c < s->ncell; c++) {
@@ -730,6 +944,7 @@ int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g, const double* 
     s->w_g[c] = w_g[c];
     for (int d = 0; d < 3; d++) s->p_force[3 * c + d] = p_force ? p_force[3 * c + d] : 0.0;
   }
+  cell_center_gas_vel(s);
   s->gas_set = 1;
   return 0;
 }
@@ -1420,8 +1635,50 @@ static void trilinear_weights(const double gst[2][2][2][3], const double pos[3],
 }
 
 /* CALC_DRAG_DES -> DRAG_GS_DES0 (GARG_2012), drag_gs_des0.f:22-219 */
+/* DRAG_GS_DES1 (drag_gs_des1.f:50-160), Model A/B as the GARG path. */
+static void drag_des1(orc_sys* s) {
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+  int nf = filter_size(s);
+  for (int64_t np = 0; np < s->max_pip; np++) {
+    if (s->state[np] != NORMAL_PARTICLE) continue;
+    int ijk0 = s->pijk[3 * L + np] - 1;
+    double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
+    if (p->des_interp_on) {
+      for (int lc = 0; lc < nf; lc++) {
+        int ijk = s->filter_cell[lc * L + np];
+        double w = s->filter_weight[lc * L + np];
+        lepg = lepg + s->ep_g[ijk] * w;
+        for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + s->ugc[3 * ijk + d] * w;
+        for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + s->p_force[3 * ijk + d] * w;
+      }
+    } else {
+      lepg = s->ep_g[ijk0];
+      for (int d = 0; d < 3; d++) velfp[d] = s->ugc[3 * ijk0 + d];
+      for (int d = 0; d < 3; d++) lpf[d] = s->p_force[3 * ijk0 + d];
+    }
+    if (lepg < 1.0e-6) lepg = s->ep_g[ijk0];
+    if (!s->fluid[ijk0]) continue; /* DRAG_FC = 0: FC unchanged */
+    double vel[3] = {s->vel[np], s->vel[L + np], s->vel[2 * L + np]};
+    double vslp[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+    double vrel = sqrt(dot3(vslp, vslp));
+    double fgp = orc_drag_fgp(p->drag_type, lepg, p->mu_g0, p->ro_g0, vrel,
+                              2.0 * s->radius[np], s->pvol[np]);
+    for (int d = 0; d < 3; d++) {
+      double dfc = fgp * (velfp[d] - vel[d]);
+      s->fc[d * L + np] = s->fc[d * L + np] + dfc;
+    }
+    if (!p->model_b)
+      for (int d = 0; d < 3; d++) s->fc[d * L + np] = s->fc[d * L + np] + lpf[d] * s->pvol[np];
+  }
+}
+
 int orc_calc_drag_des(orc_sys* s) {
   if (!s->p.coupled) return 0;
+  if (s->p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
+    drag_des1(s);
+    return 0;
+  }
   int64_t L = s->ld;
   const mfixdem_params* p = &s->p;
 #ifdef _OPENMP
@@ -1584,7 +1841,58 @@ int orc_particles_in_cell(orc_sys* s) {
 /* --------------------------------------------------- COMP_MEAN_FIELDS */
 /* COMP_MEAN_FIELDS0 (GARG), comp_mean_fields0.f:6-466, then CALC_EPG_DES
    calc_epg_des.f:19-205 */
+/* CALC_EPG_DES (calc_epg_des.f:100-145) */
+static int calc_epg_des(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  int nph = p->nphases;
+  size_t C = (size_t)s->ncell;
+  double packed = 1.0;
+  if (p->des_epg_clip != UNDEF) packed = 1.0 - p->des_epg_clip;
+  int64_t ier = 0;
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk]) continue;
+    double sum_eps = 0.0;
+    for (int m = 0; m < nph; m++) sum_eps = sum_eps + s->rop_s[m * C + ijk] / p->ro_s0[m];
+    s->ep_g[ijk] = 1.0 - (packed < sum_eps ? packed : sum_eps);
+    if (s->ep_g[ijk] <= 0.0 || s->ep_g[ijk] > 1.0) ier++;
+  }
+  s->st.epg_errors = ier;
+  return ier ? MFIXDEM_ERR_EPG : 0;
+}
+
+/* COMP_MEAN_FIELDS1 (comp_mean_fields1.f:7-116): filter-weighted solids
+   volume per cell, ROP_S = RO_S*SOLVOLINC/VOL in fluid cells. */
+static int comp_mean_fields1(orc_sys* s) {
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+  int nph = p->nphases, nf = filter_size(s);
+  size_t C = (size_t)s->ncell;
+  memset(s->solvolinc, 0, sizeof(double) * C * nph);
+  memset(s->rop_s, 0, sizeof(double) * C * nph);
+  for (int64_t np = 0; np < s->max_pip; np++) {
+    if (s->state[np] != NORMAL_PARTICLE) continue;
+    double vol_wt = s->pvol[np];
+    int m = s->phase[np];
+    if (nf == 0) {
+      int ijk = s->pijk[3 * L + np] - 1;
+      s->solvolinc[m * C + ijk] = s->solvolinc[m * C + ijk] + vol_wt;
+    }
+    for (int lc = 0; lc < nf; lc++) {
+      int ijk = s->filter_cell[lc * L + np];
+      double vw = vol_wt * s->filter_weight[lc * L + np];
+      s->solvolinc[m * C + ijk] = s->solvolinc[m * C + ijk] + vw;
+    }
+  }
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk]) continue;
+    for (int m = 0; m < nph; m++)
+      s->rop_s[m * C + ijk] = p->ro_s0[m] * s->solvolinc[m * C + ijk] / s->vol[ijk];
+  }
+  return calc_epg_des(s);
+}
+
 int orc_comp_mean_fields(orc_sys* s) {
+  if (s->p.interp_scheme != MFIXDEM_INTERP_GARG_2012) return comp_mean_fields1(s);
   int64_t L = s->ld;
   const mfixdem_params* p = &s->p;
   int nph = p->nphases;
@@ -1636,18 +1944,7 @@ int orc_comp_mean_fields(orc_sys* s) {
     for (int m = 0; m < nph; m++)
       if (s->rop_s[m * C + ijk] > 0.0) s->rop_s[m * C + ijk] = s->rop_s[m * C + ijk] / s->vol[ijk];
   }
-  double packed = 1.0;
-  if (p->des_epg_clip != UNDEF) packed = 1.0 - p->des_epg_clip;
-  int64_t ier = 0;
-  for (int ijk = 0; ijk < s->ncell; ijk++) {
-    if (!s->fluid[ijk]) continue;
-    double sum_eps = 0.0;
-    for (int m = 0; m < nph; m++) sum_eps = sum_eps + s->rop_s[m * C + ijk] / p->ro_s0[m];
-    s->ep_g[ijk] = 1.0 - (packed < sum_eps ? packed : sum_eps);
-    if (s->ep_g[ijk] <= 0.0 || s->ep_g[ijk] > 1.0) ier++;
-  }
-  s->st.epg_errors = ier;
-  return ier ? MFIXDEM_ERR_EPG : 0;
+  return calc_epg_des(s);
 }
 
 /* ------------------------------------------------------- driver */
@@ -1667,7 +1964,10 @@ int orc_init(orc_sys* s) {
   memset(s->neighbor_index, 0, sizeof(int) * s->max_pip);
   rc = orc_neighbour(s);
   if (rc) return rc;
-  if (s->p.coupled) rc = orc_comp_mean_fields(s);
+  if (s->p.coupled) {
+    orc_calc_interp_weights(s);
+    rc = orc_comp_mean_fields(s);
+  }
   s->inited = 1;
   s->st.steps = 0;
   return rc;
@@ -1692,6 +1992,7 @@ int orc_time_step(orc_sys* s, int32_t nn) {
   }
   if (s->p.coupled) {
     orc_particles_in_cell(s);
+    orc_calc_interp_weights(s);
     rc = orc_comp_mean_fields(s);
   }
   s->st.steps++;

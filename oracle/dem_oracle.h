@@ -50,6 +50,9 @@ int orc_calc_force_dem(orc_sys* s);
 int orc_calc_drag_des(orc_sys* s);
 int orc_cfnewvalues(orc_sys* s);
 int orc_particles_in_cell(orc_sys* s);
+int orc_calc_interp_weights(orc_sys* s);
+int orc_get_filter(const orc_sys* s, int64_t q, int32_t* cells, double* weights);
+int orc_get_ugc(const orc_sys* s, double* ugc);
 int orc_comp_mean_fields(orc_sys* s);
 
 /* inspection */

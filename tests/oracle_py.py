@@ -57,6 +57,8 @@ def load_oracle(omp: bool = False):
                           C.c_double], C.c_double),
         "orc_omp_threads": ([], C.c_int),
         "orc_user_set_particles": ([P, L, L, D, D, D], C.c_int),
+        "orc_get_filter": ([P, L, IP, D], C.c_int),
+        "orc_get_ugc": ([P, D], C.c_int),
     }
     for n, (a, r) in sig.items():
         f = getattr(lib, n)
@@ -182,6 +184,18 @@ class OracleSystem:
         self._chk(self.lib.orc_user_set_particles(self.h, q0, count,
                                                   *(abi.dptr(a) for a in arr)),
                   "user_set_particles")
+
+    def filter(self, q):
+        """(cells 0-based, weights) of particle q after the last weights pass."""
+        cells = np.zeros(27, dtype=np.int32)
+        w = np.zeros(27)
+        nf = self.lib.orc_get_filter(self.h, q, abi.iptr(cells), abi.dptr(w))
+        return cells[:nf], w[:nf]
+
+    def ugc(self):
+        out = np.zeros(3 * self.ncell)
+        self.lib.orc_get_ugc(self.h, abi.dptr(out))
+        return out.reshape(self.ncell, 3)
 
     def nsquare_pairs(self):
         cap = 8 * self.n + 16

@@ -35,7 +35,8 @@ def test_params_layout_and_defaults_match_c():
     c = abi.Params()
     assert lib.mfixdem_params_default(C.byref(c)) == 0
     py = abi.default_params()
-    assert C.sizeof(abi.Params) == 20 * 4 + 8 * (8 + 3 + 9 + 2 * 10 + 2 * 4 + 2 * 4 + 2 + 2 * 4 + 4)
+    assert C.sizeof(abi.Params) == 20 * 4 + 8 * (8 + 3 + 9 + 2 * 10 + 2 * 4 + 2 * 4 + 2 + 2 * 4 + 4) \
+        + 8 + 2 * 4
     for name, _t in abi.Params._fields_:
         a, b = getattr(c, name), getattr(py, name)
         if hasattr(a, "__len__"):

@@ -351,3 +351,64 @@ def test_dem05_on_gpu_bit_exact_and_wall_impacts(model):
     assert np.max(np.abs(spin - np.array(run["omega"])[:, 2])) <= 5.1e-5
     assert np.max(np.abs(vt[1:] / vt0[1:] - np.array(run["coeff"])[:, 2])) <= 5.1e-5
     g.close()
+
+
+FILTER_CASES = [
+    (abi.INTERP_NONE, 0, None), (abi.INTERP_SQUARE_DPVM, 1, 1.6e-3),
+    (abi.INTERP_GAUSS_DPVM, 1, 1.6e-3), (abi.INTERP_LINEAR_HAT, 1, None),
+    (abi.INTERP_LINEAR_HAT, 0, None),
+]
+
+
+@pytest.mark.parametrize("scheme,interp_on,width", FILTER_CASES,
+                         ids=["none", "square_dpvm", "gauss_dpvm", "linear_hat",
+                              "linear_hat_cellvalues"])
+def test_filter_scheme_drag_and_deposition(scheme, interp_on, width):
+    """DES_INTERP_SCHEME filter path (CALC_INTERP_WEIGHTS, DRAG_GS_DES1,
+    COMP_MEAN_FIELDS1) against the oracle.  ROP_S / EP_G at set-up agree to
+    1e-14 (fixed-point vs serial sums); after 60 steps the trajectory differs
+    only through pow ulps and deposition rounding: |dx| <= 1e-12 m,
+    |dEP_G| <= 1e-12."""
+    p, parts, gas = fluidised()
+    p.interp_scheme = scheme
+    p.des_interp_on = interp_on
+    p.des_interp_width = abi.UNDEFINED if width is None else width
+    g, o = both(p, parts, gas)
+    ge, gr = g.get_gas()
+    oe, orr = o.get_gas()
+    assert np.max(np.abs(ge - oe)) <= 1e-14
+    assert np.max(np.abs(gr - orr)) <= 1e-14 * np.max(np.abs(orr))
+    g.run(1, 60)
+    o.run(1, 60)
+    gp, gv, gw, gs = g.get_particles()
+    op, ov, ow, os_ = o.get_particles()
+    assert np.array_equal(gs, os_)
+    assert np.max(np.abs(gp - op)) <= 1e-12
+    assert np.max(np.abs(gv - ov)) <= 1e-9 * max(1.0, np.max(np.abs(ov)))
+    ge, gr = g.get_gas()
+    oe, orr = o.get_gas()
+    assert np.max(np.abs(ge - oe)) <= 1e-12
+    assert np.max(np.abs(gr - orr)) <= 1e-12 * np.max(np.abs(orr))
+    g.close()
+
+
+def test_filter_scheme_mixed_precision_force_error():
+    """FP32 drag law on the LINEAR_HAT path: relative force error <= 1e-4."""
+    p64, parts, gas = fluidised(n=4000)
+    pmx, _, _ = fluidised(n=4000, precision=abi.MIXED)
+    for p in (p64, pmx):
+        p.interp_scheme = abi.INTERP_LINEAR_HAT
+        p.des_interp_on = 1
+    a, _ = both(p64, parts, gas, debug=abi.DEBUG_STORE_FORCES)
+    b = DemSystem(pmx)
+    b.set_particles(parts)
+    b.set_gas(gas)
+    b.init()
+    b.set_debug(abi.DEBUG_STORE_FORCES)
+    a.step(1)
+    b.step(1)
+    fa, _ = a.get_forces()
+    fb, _ = b.get_forces()
+    assert np.max(np.abs(fa - fb)) / np.max(np.abs(fa)) <= 1e-4
+    a.close()
+    b.close()

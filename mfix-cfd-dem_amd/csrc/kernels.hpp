@@ -1200,6 +1200,71 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
   }
 }
 
+// Own rows of NSQUARE (nsquare.f:15-142, no periodic walls): every existing
+// particle against every existing particle with a larger id, DIST < R_LM**2
+// (strict), rows in ascending id.  Brute force over LDS tiles of candidates.
+__global__ __launch_bounds__(kBS) void k_own_rows_nsquare(int n, int S, double factor_rlm,
+                                                          BuildPtrs b) {
+  __shared__ double s_x[kBS], s_y[kBS], s_z[kBS], s_r[kBS];
+  __shared__ int s_g[kBS];
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  const bool home = i < n && b.state[i] != 0;
+  double xi[3] = {0.0, 0.0, 0.0}, ri = 0.0;
+  int gi = INT_MAX;
+  if (home) {
+    xi[0] = b.pos[i];
+    xi[1] = b.pos[S + i];
+    xi[2] = b.pos[2 * S + i];
+    ri = b.radius[i];
+    gi = b.gid[i];
+  }
+  int cnt = 0;
+  int* row = b.own_tmp + (size_t)(home ? i : 0) * b.maxrow;
+  uint64_t* key = b.own_key + (size_t)(home ? i : 0) * b.maxrow;
+  for (int base = 0; base < n; base += kBS) {
+    const int j = base + threadIdx.x;
+    if (j < n) {
+      s_x[threadIdx.x] = b.pos[j];
+      s_y[threadIdx.x] = b.pos[S + j];
+      s_z[threadIdx.x] = b.pos[2 * S + j];
+      s_r[threadIdx.x] = b.radius[j];
+      s_g[threadIdx.x] = b.state[j] != 0 ? b.gid[j] : INT_MIN;
+    } else {
+      s_g[threadIdx.x] = INT_MIN;
+    }
+    __syncthreads();
+    if (home) {
+      const int m = min(kBS, n - base);
+      for (int t = 0; t < m; t++) {
+        const int gj = s_g[t];
+        if (gj <= gi) continue;
+        double r_lm = ri + s_r[t];
+        r_lm = factor_rlm * r_lm;
+        const double d0 = s_x[t] - xi[0], d1 = s_y[t] - xi[1], d2 = s_z[t] - xi[2];
+        const double dist = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!(dist < r_lm * r_lm)) continue;
+        if (cnt < b.maxrow) {
+          const uint64_t sk = (uint64_t)(uint32_t)gj;
+          int q = cnt;
+          while (q > 0 && key[q - 1] > sk) {
+            key[q] = key[q - 1];
+            row[q] = row[q - 1];
+            q--;
+          }
+          key[q] = sk;
+          row[q] = base + t;
+        }
+        cnt++;
+      }
+    }
+    __syncthreads();
+  }
+  if (i < n) {
+    if (cnt > b.maxrow) atomicMax(&b.flags[kFlagOwnOverflow], cnt);
+    b.own_cnt[i] = home ? cnt : 0;
+  }
+}
+
 __global__ __launch_bounds__(kBS) void k_count_in(int n, const int* own_cnt, const int* own_tmp,
                                                   int maxrow, int* in_cnt) {
   const int i = blockIdx.x * kBS + threadIdx.x;

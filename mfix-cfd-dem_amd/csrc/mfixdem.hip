@@ -293,9 +293,15 @@ int32_t do_neighbour(mfixdem_handle* h) {
   b.own_cnt = h->own_cnt; b.flags = h->flags;
   int fl[kFlagCount];
   bool wave_rows = true;
+  // neighbour.f:60-66: NSQUARE for DES_NEIGHBOR_SEARCH = 1 (no rogue handling
+  // there: that lives in DESGRID_NEIGH_BUILD)
+  const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
   for (;;) {
     b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow;
-    if (wave_rows) {
+    if (nsq) {
+      hipLaunchKernelGGL(k_own_rows_nsquare, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
+                         h->S.p.factor_rlm, b);
+    } else if (wave_rows) {
       // particles outside every cell range (state 0) get no row
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       const int64_t threads = (int64_t)h->n_fine * 64;
@@ -323,7 +329,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
     CHECKRC(dalloc(&h->own_key, (int64_t)n * mr));
     h->maxrow = mr;
   }
-  if (fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
+  if (!nsq && fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
   const int nl = 1 - h->lt;
   // own rows -> pair ids
   CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
@@ -365,7 +371,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   // the old history buffer becomes the next step's output
   HIPOK(hipStreamSynchronize(h->stream));
   CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
-  if (fl[kFlagRogueCount] > 0) {
+  if (!nsq && fl[kFlagRogueCount] > 0) {
     hipLaunchKernelGGL(k_delete_rogue, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
                        h->dgc[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c],
                        h->radius[t]);

@@ -1046,6 +1046,38 @@ static void add_pair(orc_sys* s, int64_t ii, int jj) {
   s->neighbor_index[ii] = (int)(pos1 + 1);
 }
 
+/* NSQUARE (nsquare.f:15-142) without periodic walls: every existing L
+   against every existing LL > L, DIST < R_LM**2 (strict), rows in
+   ascending LL.  Slots after the early exit (PC >= PIP) keep the 0 that
+   NEIGHBOUR wrote (neighbour.f:60), which the force loop reads as empty
+   rows; here they get the explicit empty-row index. */
+static int nsquare_build(orc_sys* s) {
+  int64_t L = s->ld;
+  const double frlm = s->p.factor_rlm;
+  int64_t pc = 1, l = 0;
+  for (; l < s->max_pip; l++) {
+    s->neighbor_index[l] = (l > 0) ? s->neighbor_index[l - 1] : 1;
+    if (pc >= s->pip) break;
+    if (s->state[l] == NONEXISTENT) continue;
+    int64_t pnpc = s->pip - pc;
+    for (int64_t ll = l + 1; ll < s->max_pip; ll++) {
+      if (pnpc <= 0) break;
+      if (s->state[ll] == NONEXISTENT) continue;
+      double r_lm = s->radius[l] + s->radius[ll];
+      r_lm = frlm * r_lm;
+      double d0 = s->pos[ll] - s->pos[l], d1 = s->pos[L + ll] - s->pos[L + l],
+             d2 = s->pos[2 * L + ll] - s->pos[2 * L + l];
+      double dist = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dist < r_lm * r_lm) add_pair(s, l, (int)(ll + 1));
+      pnpc--;
+    }
+    pc++;
+  }
+  for (int64_t q = (l < s->max_pip ? l + 1 : l); q < s->max_pip; q++)
+    s->neighbor_index[q] = s->neighbor_index[q - 1];
+  return 0;
+}
+
 /* candidate pairs of one row particle, in the reference's stencil order
    (desgrid_mod.f:1106-1214); returns the number written into out[] */
 static int row_candidates(orc_sys* s, int64_t q, int* out, int* rogue) {
@@ -1168,7 +1200,9 @@ int orc_neighbour(orc_sys* s) {
   memcpy(s->neighbors_old, s->neighbors, sizeof(int) * s->cap);
   memcpy(s->pft_old, s->pft, sizeof(double) * 3 * s->cap);
   memset(s->neighbor_index, 0, sizeof(int) * np);
-  int rc = desgrid_neigh_build(s);
+  /* neighbour.f:60-66: NSQUARE for DES_NEIGHBOR_SEARCH = 1, else the grid */
+  int rc = (s->p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE) ? nsquare_build(s)
+                                                                 : desgrid_neigh_build(s);
   /* the arrays may have grown; old copies were sized before */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)

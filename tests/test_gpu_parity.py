@@ -412,3 +412,26 @@ def test_filter_scheme_mixed_precision_force_error():
     assert np.max(np.abs(fa - fb)) / np.max(np.abs(fa)) <= 1e-4
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("integrator", [abi.EULER, abi.ADAMS_BASHFORTH], ids=["euler", "ab2"])
+def test_nsquare_search_bit_exact(integrator):
+    """DES_NEIGHBOR_SEARCH = 1 (NSQUARE): pair lists (ascending partner id),
+    FC/TOW and the 60-step trajectory equal the oracle's bit for bit."""
+    p, parts, _ = settling(integrator=integrator)
+    p.des_neighbor_search = abi.NSEARCH_NSQUARE
+    g, o = both(p, parts, debug=abi.DEBUG_STORE_FORCES)
+    gp, op = assert_same_pairs(g, o)
+    for nn in range(1, 61):
+        g.step(nn)
+        o.step(nn)
+        if nn in (1, 25, 26, 60):
+            gf, gt = g.get_forces()
+            of, ot = o.get_forces()
+            assert np.array_equal(gf, of), f"FC differs at step {nn}"
+            assert np.array_equal(gt, ot), f"TOW differs at step {nn}"
+    for a, b in zip(g.get_particles(), o.get_particles()):
+        assert np.array_equal(a, b)
+    gp, op = assert_same_pairs(g, o)
+    assert np.array_equal(gp, op)
+    g.close()

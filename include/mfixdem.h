@@ -193,6 +193,13 @@ int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
    calc_pg_grad.f:20), MFiX cell layout described above. */
 int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
                         const double* w_g, const double* p_force);
+/* P_FORCE = -grad P_G from a cell-centred gas pressure field (ncell), as
+   CALC_PG_GRAD does once per fluid step (calc_pg_grad.f:20-110,
+   calc_grad_des.f:21-133); replaces the P_FORCE given to mfixdem_set_gas.
+   Call after mfixdem_set_gas. */
+int32_t mfixdem_set_gas_pressure(mfixdem_handle* h, const double* p_g);
+/* P_FORCE(3, ncell) as the drag uses it. */
+int32_t mfixdem_get_p_force(mfixdem_handle* h, double* p_force);
 /* EP_G (ncell) and ROP_S (nphases x ncell, phase-major) after the last
    deposition (comp_mean_fields0.f, calc_epg_des.f). */
 int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s);

@@ -52,6 +52,29 @@ enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFla
 
 enum { kModeFused = 0, kModeForce = 1, kModeDrag = 2, kModeIntegrate = 3 };
 
+// Ensembles: many independent systems in one set of particle arrays.  The
+// slots of system s are a fixed range; each slot carries its system id, and
+// parameters, grids, search-grid cell offsets, DES-cell offsets and rogue
+// flags are per-system tables.  The BATCH kernel variants read them; with
+// BATCH = false the kernels take the single system's blocks by value.
+struct GridDev;
+struct BatchDev {
+  const uint16_t* sys;      // system of each slot
+  const DevParams* ptab;    // per-system DevParams
+  const GridDev* gtab;      // per-system grids
+  const int* cell_off;      // search cells of system s: [cell_off[s], cell_off[s+1])
+  const int* dg_off;        // first DES cell of system s in cell_facets
+  int* rogue;               // [2s]: smallest rogue id, [2s+1]: rogue count
+  int nsys;
+  int sys_shift;            // bit position of the system id in the sort key
+};
+template <bool BATCH>
+__device__ __forceinline__ const DevParams& sys_params(const DevParams& P0, const BatchDev& B,
+                                                       int i) {
+  if constexpr (BATCH) return B.ptab[B.sys[i]];
+  else return P0;
+}
+
 __device__ __forceinline__ void load3(const double* a, int S, int i, double v[3]) {
   v[0] = a[i];
   v[1] = a[S + i];
@@ -286,11 +309,12 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
 // unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES);
 // FILTER selects the DRAG_GS_DES1 drag (any interpolation scheme other than
 // GARG_2012), so that each kernel carries only one drag path.
-template <bool MIXED, int MODE, bool FILTER>
-__global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
+template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
   const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= P.n) return;
-  const int S = P.stride;
+  if (i >= P0.n) return;
+  const int S = P0.stride;
+  const DevParams& P = sys_params<BATCH>(P0, B, i);
   const int st = a.state[i];
   if (st == 0) {
     if (MODE == kModeFused || MODE == kModeIntegrate) {
@@ -317,7 +341,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
 
   if (MODE == kModeFused || MODE == kModeForce) {
     // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
-    const uint16_t fm = a.cell_facets[a.dgc[i]];
+    const uint16_t fm = a.cell_facets[(BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i]];
     uint8_t wm = a.wmask[i];
     const uint8_t wm0 = wm;
     if (fm == 0) {
@@ -406,7 +430,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
         for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = o.pft[c];
         for (int c = 0; c < 3; c++) fc[c] = fc[c] + o.fct[c];
         for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.tl[c];
-        if (P.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
+        if (P0.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
       } else {
         for (int c = 0; c < 3; c++) fc[c] = fc[c] - o.fct[c];
         for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.ti[c];
@@ -429,7 +453,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     load3(a.fc, S, i, fc);
     load3(a.tow, S, i, tow);
   }
-  if (P.store_forces) {
+  if (P0.store_forces) {
     const int g = a.gid[i];
     store3(a.fc_id, S, g, fc);
     store3(a.tow_id, S, g, tow);
@@ -446,7 +470,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P, StepPtrs a) {
     for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
   } else {
     double acc[3], racc[3];
-    if (P.first_pass) {
+    if (P0.first_pass) {
       for (int c = 0; c < 3; c++) acc[c] = fc[c] / mi + P.grav[c];
       for (int c = 0; c < 3; c++) racc[c] = tow[c];
     } else {
@@ -1605,6 +1629,46 @@ __global__ __launch_bounds__(kBS) void k_cell_fields_des1(int ncell, int nph, do
     f.ep_g[c] = e;
     if (e <= 0.0 || e > 1.0) atomicAdd(&f.flags[kFlagEpg], 1);
   }
+}
+
+// calc_del_phi (calc_grad_des.f:104-133): 3-point derivative, a neighbour
+// that is not a valid pressure cell replaced by extrapolation
+__device__ __forceinline__ double calc_del_phi(bool valid_lo, bool valid_hi, double phi,
+                                               double dx, double phi_lo, double dx_lo,
+                                               double phi_hi, double dx_hi) {
+  if (!valid_lo && !valid_hi) return 0.0;
+  if (!valid_lo) phi_lo = phi_hi + (phi - phi_hi) * (dx_hi + 2.0 * dx + dx_lo) / (dx + dx_hi);
+  if (!valid_hi) phi_hi = phi_lo + (phi - phi_lo) * (dx_hi + 2.0 * dx + dx_lo) / (dx + dx_lo);
+  const double x = 0.5 * (dx_lo + dx);
+  const double x_hi = 0.5 * (dx_hi + dx) + x;
+  return phi_lo * (x - x_hi) / (x_hi * x) + phi * (2.0 * x - x_hi) / (x * (x - x_hi)) +
+         phi_hi * (x) / (x_hi * (x_hi - x));
+}
+
+// CALC_PG_GRAD (calc_pg_grad.f:20-110) / CALC_GRAD_DES (calc_grad_des.f:21-102):
+// P_FORCE = -grad P_G in fluid cells, 0 elsewhere (box walls: the valid
+// pressure cells are the fluid cells; no cyclic pressure drop).
+__global__ __launch_bounds__(kBS) void k_pg_grad(int ni, int nj, int nk, double dx, double dy,
+                                                 double dz, const uint8_t* fluid,
+                                                 const double* p_g, double* p_force) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  if (c >= ni * nj * nk) return;
+  double* f = p_force + 3 * c;
+  if (!fluid[c]) {
+    f[0] = f[1] = f[2] = 0.0;
+    return;
+  }
+  // fluid cells are interior, so every neighbour index is on the grid
+  const int sj = 1, si = nj, sk = nj * ni;
+  const double g0 = calc_del_phi(fluid[c - si], fluid[c + si], p_g[c], dx, p_g[c - si], dx,
+                                 p_g[c + si], dx);
+  const double g1 = calc_del_phi(fluid[c - sj], fluid[c + sj], p_g[c], dy, p_g[c - sj], dy,
+                                 p_g[c + sj], dy);
+  const double g2 = calc_del_phi(fluid[c - sk], fluid[c + sk], p_g[c], dz, p_g[c - sk], dz,
+                                 p_g[c + sk], dz);
+  f[0] = -g0;
+  f[1] = -g1;
+  f[2] = -g2;
 }
 
 // PARTICLES_IN_CELL deletion of particles outside the fluid grid

@@ -19,6 +19,7 @@
 #include "device_physics.hpp"
 #include "host_setup.hpp"
 #include "kernels.hpp"
+#include "params.hpp"
 
 using namespace mfixdem;
 
@@ -45,6 +46,9 @@ struct mfixdem_handle {
   Setup S;
   DevParams P{};
   GridDev G{};
+  // ensembles (nsys > 1): per-system set-ups and device tables
+  int nsys = 1;
+  BatchDev B{};
   hipStream_t stream = nullptr;
   int n = 0, stride = 0;
   int cur = 0, pcur = 0, tw = 0, lt = 0;
@@ -195,12 +199,16 @@ int32_t launch_step(mfixdem_handle* h) {
   StepPtrs a = step_ptrs(h);
   const dim3 grid(nblk(h->n)), blk(kBS);
   const bool mixed = h->S.p.precision == MFIXDEM_MIXED;
-  if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
-    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, true>), grid, blk, 0, h->stream, P, a);
-    else hipLaunchKernelGGL((k_step<false, MODE, true>), grid, blk, 0, h->stream, P, a);
+  const BatchDev B = h->B;
+  if (h->nsys > 1) {  // ensembles: pure DEM, per-system parameter tables
+    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, false, true>), grid, blk, 0, h->stream, P, a, B);
+    else hipLaunchKernelGGL((k_step<false, MODE, false, true>), grid, blk, 0, h->stream, P, a, B);
+  } else if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
+    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, true>), grid, blk, 0, h->stream, P, a, B);
+    else hipLaunchKernelGGL((k_step<false, MODE, true>), grid, blk, 0, h->stream, P, a, B);
   } else {
-    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, false>), grid, blk, 0, h->stream, P, a);
-    else hipLaunchKernelGGL((k_step<false, MODE, false>), grid, blk, 0, h->stream, P, a);
+    if (mixed) hipLaunchKernelGGL((k_step<true, MODE, false>), grid, blk, 0, h->stream, P, a, B);
+    else hipLaunchKernelGGL((k_step<false, MODE, false>), grid, blk, 0, h->stream, P, a, B);
   }
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;
@@ -586,62 +594,7 @@ int32_t upload_setup(mfixdem_handle* h) {
   return MFIXDEM_OK;
 }
 
-void fill_devparams(mfixdem_handle* h) {
-  const Setup& S = h->S;
-  const mfixdem_params& p = S.p;
-  DevParams& P = h->P;
-  P.n = h->n;
-  P.stride = h->stride;
-  P.hertz = p.coll_model == MFIXDEM_HERTZ;
-  P.ab2 = p.integrator == MFIXDEM_ADAMS_BASHFORTH;
-  P.coupled = p.coupled;
-  P.drag_type = p.drag_type;
-  P.nph = p.nphases;
-  P.nf = (int)S.facets.size();
-  P.dt = S.d.dtsolid;
-  for (int c = 0; c < 3; c++) P.grav[c] = p.grav[c];
-  P.kn = p.kn; P.kt = S.d.kt; P.kn_w = p.kn_w; P.kt_w = S.d.kt_w;
-  P.mew = p.mew; P.mew_w = p.mew_w;
-  for (int a = 0; a < kMaxPh; a++) {
-    for (int b = 0; b < kMaxPh; b++) {
-      P.etan[a][b] = S.d.des_etan[a][b];
-      P.etat[a][b] = S.d.des_etat[a][b];
-      P.hkn[a][b] = S.d.hert_kn[a][b];
-      P.hkt[a][b] = S.d.hert_kt[a][b];
-    }
-    P.etan_w[a] = S.d.des_etan_wall[a];
-    P.etat_w[a] = S.d.des_etat_wall[a];
-    P.hkwn[a] = S.d.hert_kwn[a];
-    P.hkwt[a] = S.d.hert_kwt[a];
-  }
-  P.ni = S.ni; P.nj = S.nj; P.nk = S.nk; P.ncell = S.ncell;
-  P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
-  P.interp = p.interp_scheme;
-  P.interp_on = p.des_interp_on;
-  P.dxc[0] = S.dx; P.dxc[1] = S.dy; P.dxc[2] = S.dz;
-  if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
-      p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
-    // set_filter_des.f:85,99-100
-    P.fw = 0.500 * p.des_interp_width;
-    P.oofv = 0.25 / (P.fw * P.fw * P.fw);
-    P.fw3 = P.fw * 3;
-  }
-  for (size_t f = 0; f < S.facets.size(); f++) {
-    FacetDev& F = P.facets[f];
-    std::memcpy(F.v, S.facets[f].v, sizeof(F.v));
-    std::memcpy(F.n, S.facets[f].n, sizeof(F.n));
-    F.fmin = S.facets[f].fmin;
-    F.fmax = S.facets[f].fmax;
-    F.dir = S.facets[f].dir;
-    std::memcpy(F.ignore_edge, S.facets[f].ignore_edge, 4);
-  }
-  GridDev& G = h->G;
-  G.dg_ni = S.dg_ni; G.dg_nj = S.dg_nj; G.dg_nk = S.dg_nk;
-  for (int d = 0; d < 3; d++) G.coarse[d] = S.coarse[d];
-  G.dg_lo[0] = S.d.dg_xstart; G.dg_lo[1] = S.d.dg_ystart; G.dg_lo[2] = S.d.dg_zstart;
-  G.dg_inv[0] = S.d.dg_dxinv; G.dg_inv[1] = S.d.dg_dyinv; G.dg_inv[2] = S.d.dg_dzinv;
-  G.factor_rlm = p.factor_rlm;
-}
+void fill_devparams(mfixdem_handle* h) { fill_params(h->S, h->n, h->stride, h->P, h->G); }
 
 }  // namespace
 
@@ -714,24 +667,9 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
   h->n = p->max_pip;
   h->stride = (h->n + 63) / 64 * 64;
   fill_devparams(h);
-  // neighbour-search grid: cell >= the largest cut-off FACTOR_RLM*(2 r_max)
-  {
-    const double hcut = p->factor_rlm * (2.0 * p->max_radius) * (1.0 + 1e-9);
-    const double lo[3] = {p->x_min - hcut, p->y_min - hcut, p->z_min - hcut};
-    const double len[3] = {p->xlength, p->ylength, p->zlength};
-    for (int d = 0; d < 3; d++) {
-      h->G.f_lo[d] = lo[d];
-      h->G.fn[d] = std::max(1, (int)std::ceil((len[d] + 2.0 * hcut) / hcut));
-      if (h->G.fn[d] > 1022) {
-        delete h;
-        return MFIXDEM_ERR_UNSUPPORTED;
-      }
-    }
-    h->G.f_inv = 1.0 / hcut;
-    h->n_fine = h->G.fn[0] * h->G.fn[1] * h->G.fn[2];
-    int gb = 1;
-    while ((1ll << gb) < (int64_t)h->n) gb++;
-    h->G.gid_bits = gb;
+  if (search_grid(*p, h->n, h->G, h->n_fine) != MFIXDEM_OK) {
+    delete h;
+    return MFIXDEM_ERR_UNSUPPORTED;
   }
   const int64_t n = h->n, S = h->stride, nc = h->S.ncell;
   auto fail = [&](int32_t r) {
@@ -970,6 +908,34 @@ int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
   HIPOK(hipMemcpy(h->p_force, pf.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
   h->gas_set = true;
   invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_gas_pressure(mfixdem_handle* h, const double* p_g) {
+  if (!h || !p_g) return MFIXDEM_ERR_ARG;
+  const Setup& S = h->S;
+  const int nc = S.ncell;
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * nc) != hipSuccess) return MFIXDEM_ERR_OOM;
+  int32_t rc = MFIXDEM_OK;
+  if (hipMemcpy(d, p_g, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = MFIXDEM_ERR_HIP;
+  } else {
+    hipLaunchKernelGGL(k_pg_grad, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk,
+                       S.dx, S.dy, S.dz, h->fluid, d, h->p_force);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = MFIXDEM_ERR_HIP;
+  }
+  hipFree(d);
+  invalidate_graphs(h);
+  return rc;
+}
+
+int32_t mfixdem_get_p_force(mfixdem_handle* h, double* p_force) {
+  if (!h || !p_force) return MFIXDEM_ERR_ARG;
+  HIPOK(hipMemcpyAsync(p_force, h->p_force, sizeof(double) * 3 * h->S.ncell,
+                       hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
   return MFIXDEM_OK;
 }
 

@@ -1,0 +1,89 @@
+// params.hpp -- device parameter blocks (DevParams, GridDev) from a
+// validated Setup; shared by the single-system and the ensemble handles.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+namespace mfixdem {
+
+inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev& G) {
+  const mfixdem_params& p = S.p;
+  P = DevParams{};
+  P.n = n;
+  P.stride = stride;
+  P.hertz = p.coll_model == MFIXDEM_HERTZ;
+  P.ab2 = p.integrator == MFIXDEM_ADAMS_BASHFORTH;
+  P.coupled = p.coupled;
+  P.drag_type = p.drag_type;
+  P.nph = p.nphases;
+  P.nf = (int)S.facets.size();
+  P.dt = S.d.dtsolid;
+  for (int c = 0; c < 3; c++) P.grav[c] = p.grav[c];
+  P.kn = p.kn; P.kt = S.d.kt; P.kn_w = p.kn_w; P.kt_w = S.d.kt_w;
+  P.mew = p.mew; P.mew_w = p.mew_w;
+  for (int a = 0; a < kMaxPh; a++) {
+    for (int b = 0; b < kMaxPh; b++) {
+      P.etan[a][b] = S.d.des_etan[a][b];
+      P.etat[a][b] = S.d.des_etat[a][b];
+      P.hkn[a][b] = S.d.hert_kn[a][b];
+      P.hkt[a][b] = S.d.hert_kt[a][b];
+    }
+    P.etan_w[a] = S.d.des_etan_wall[a];
+    P.etat_w[a] = S.d.des_etat_wall[a];
+    P.hkwn[a] = S.d.hert_kwn[a];
+    P.hkwt[a] = S.d.hert_kwt[a];
+  }
+  P.ni = S.ni; P.nj = S.nj; P.nk = S.nk; P.ncell = S.ncell;
+  P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
+  P.interp = p.interp_scheme;
+  P.interp_on = p.des_interp_on;
+  P.dxc[0] = S.dx; P.dxc[1] = S.dy; P.dxc[2] = S.dz;
+  if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
+      p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
+    // set_filter_des.f:85,99-100
+    P.fw = 0.500 * p.des_interp_width;
+    P.oofv = 0.25 / (P.fw * P.fw * P.fw);
+    P.fw3 = P.fw * 3;
+  }
+  for (size_t f = 0; f < S.facets.size(); f++) {
+    FacetDev& F = P.facets[f];
+    std::memcpy(F.v, S.facets[f].v, sizeof(F.v));
+    std::memcpy(F.n, S.facets[f].n, sizeof(F.n));
+    F.fmin = S.facets[f].fmin;
+    F.fmax = S.facets[f].fmax;
+    F.dir = S.facets[f].dir;
+    std::memcpy(F.ignore_edge, S.facets[f].ignore_edge, 4);
+  }
+  G.dg_ni = S.dg_ni; G.dg_nj = S.dg_nj; G.dg_nk = S.dg_nk;
+  for (int d = 0; d < 3; d++) G.coarse[d] = S.coarse[d];
+  G.dg_lo[0] = S.d.dg_xstart; G.dg_lo[1] = S.d.dg_ystart; G.dg_lo[2] = S.d.dg_zstart;
+  G.dg_inv[0] = S.d.dg_dxinv; G.dg_inv[1] = S.d.dg_dyinv; G.dg_inv[2] = S.d.dg_dzinv;
+  G.factor_rlm = p.factor_rlm;
+}
+
+
+// Neighbour-search grid of one system: cubic cells >= the largest cut-off
+// FACTOR_RLM*(2 r_max), one cell of margin around the box; particle ids
+// take gid_bits bits of the sort key.
+inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& n_fine) {
+  const double hcut = p.factor_rlm * (2.0 * p.max_radius) * (1.0 + 1e-9);
+  const double lo[3] = {p.x_min - hcut, p.y_min - hcut, p.z_min - hcut};
+  const double len[3] = {p.xlength, p.ylength, p.zlength};
+  for (int d = 0; d < 3; d++) {
+    G.f_lo[d] = lo[d];
+    G.fn[d] = std::max(1, (int)std::ceil((len[d] + 2.0 * hcut) / hcut));
+    if (G.fn[d] > 1022) return MFIXDEM_ERR_UNSUPPORTED;
+  }
+  G.f_inv = 1.0 / hcut;
+  n_fine = G.fn[0] * G.fn[1] * G.fn[2];
+  int gb = 1;
+  while ((1ll << gb) < n) gb++;
+  G.gid_bits = gb;
+  return MFIXDEM_OK;
+}
+
+}  // namespace mfixdem

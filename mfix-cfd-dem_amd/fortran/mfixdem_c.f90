@@ -133,6 +133,12 @@ module mfixdem_c
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function
+     integer(c_int32_t) function mfixdem_set_gas_pressure(h, p_g) &
+          bind(C, name="mfixdem_set_gas_pressure")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: p_g(*)
+     end function
      integer(c_int32_t) function mfixdem_calc_interp_weights(h) &
           bind(C, name="mfixdem_calc_interp_weights")
        import :: c_int32_t, c_ptr

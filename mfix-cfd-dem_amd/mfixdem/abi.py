@@ -171,6 +171,8 @@ def _declare(lib):
         "mfixdem_get_particles": ([_P, _DP, _DP, _DP, _BP, C.c_int64], _I),
         "mfixdem_set_gas": ([_P, _DP, _DP, _DP, _DP], _I),
         "mfixdem_get_gas": ([_P, _DP, _DP], _I),
+        "mfixdem_set_gas_pressure": ([_P, _DP], _I),
+        "mfixdem_get_p_force": ([_P, _DP], _I),
         "mfixdem_init": ([_P], _I),
         "mfixdem_step": ([_P, _I], _I),
         "mfixdem_run": ([_P, _I, _I], _I),

@@ -94,6 +94,15 @@ class DemSystem:
     def particles_in_cell(self):
         abi.check(self.lib.mfixdem_particles_in_cell(self.h), "particles_in_cell")
 
+    def set_gas_pressure(self, p_g):
+        abi.check(self.lib.mfixdem_set_gas_pressure(self.h, abi.dptr(np.ascontiguousarray(p_g))),
+                  "set_gas_pressure")
+
+    def get_p_force(self):
+        out = np.zeros(3 * self.ncell)
+        abi.check(self.lib.mfixdem_get_p_force(self.h, abi.dptr(out)), "get_p_force")
+        return out.reshape(self.ncell, 3)
+
     def calc_interp_weights(self):
         abi.check(self.lib.mfixdem_calc_interp_weights(self.h), "calc_interp_weights")
 

@@ -936,6 +936,54 @@ int orc_get_ugc(const orc_sys* s, double* ugc) {
   return 0;
 }
 
+/* calc_del_phi (calc_grad_des.f:104-133): 3-point derivative with a wall
+   neighbour (not a valid pressure cell) replaced by extrapolation */
+static double calc_del_phi(int valid_lo, int valid_hi, double phi, double dx, double phi_lo,
+                           double dx_lo, double phi_hi, double dx_hi) {
+  if (!valid_lo && !valid_hi) return 0.0;
+  if (!valid_lo) phi_lo = phi_hi + (phi - phi_hi) * (dx_hi + 2.0 * dx + dx_lo) / (dx + dx_hi);
+  if (!valid_hi) phi_hi = phi_lo + (phi - phi_lo) * (dx_hi + 2.0 * dx + dx_lo) / (dx + dx_lo);
+  double x = 0.5 * (dx_lo + dx);
+  double x_hi = 0.5 * (dx_hi + dx) + x;
+  return phi_lo * (x - x_hi) / (x_hi * x) + phi * (2.0 * x - x_hi) / (x * (x - x_hi)) +
+         phi_hi * (x) / (x_hi * (x_hi - x));
+}
+
+/* CALC_PG_GRAD (calc_pg_grad.f:20-110) with CALC_GRAD_DES
+   (calc_grad_des.f:21-102): P_FORCE = -grad P_G in fluid cells, 0 elsewhere;
+   no cyclic pressure drop and no impermeable internal surfaces (box walls:
+   the valid pressure cells are the fluid cells). */
+int orc_set_gas_pressure(orc_sys* s, const double* p_g) {
+  for (int k = 1; k <= s->nk; k++)
+    for (int i = 1; i <= s->ni; i++)
+      for (int j = 1; j <= s->nj; j++) {
+        int ijk = fluid_ijk(s, i, j, k) - 1;
+        double* f = s->p_force + 3 * ijk;
+        if (!s->fluid[ijk]) {
+          f[0] = f[1] = f[2] = 0.0;
+          continue;
+        }
+        int lo = fluid_ijk(s, i - 1, j, k) - 1, hi = fluid_ijk(s, i + 1, j, k) - 1;
+        f[0] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dx, p_g[lo], s->dx,
+                            p_g[hi], s->dx);
+        lo = fluid_ijk(s, i, j - 1, k) - 1;
+        hi = fluid_ijk(s, i, j + 1, k) - 1;
+        f[1] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dy, p_g[lo], s->dy,
+                            p_g[hi], s->dy);
+        lo = fluid_ijk(s, i, j, k - 1) - 1;
+        hi = fluid_ijk(s, i, j, k + 1) - 1;
+        f[2] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dz, p_g[lo], s->dz,
+                            p_g[hi], s->dz);
+        for (int d = 0; d < 3; d++) f[d] = -f[d];
+      }
+  return 0;
+}
+
+int orc_get_p_force(const orc_sys* s, double* p_force) {
+  memcpy(p_force, s->p_force, sizeof(double) * 3 * s->ncell);
+  return 0;
+}
+
 int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g, const double* w_g,
                 const double* p_force) {
   for (int c = 0; c < s->ncell; c++) {

@@ -36,6 +36,8 @@ int orc_set_gas(orc_sys* s, const double* u_g, const double* v_g,
                 const double* w_g, const double* p_force);
 int orc_user_set_particles(orc_sys* s, int64_t q0, int64_t count, const double* pos,
                            const double* vel, const double* omega);
+int orc_set_gas_pressure(orc_sys* s, const double* p_g);
+int orc_get_p_force(const orc_sys* s, double* p_force);
 int orc_get_gas(const orc_sys* s, double* ep_g, double* rop_s);
 
 /* MAKE_ARRAYS_DES tail and DES_TIME_STEP(NN). */

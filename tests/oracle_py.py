@@ -59,6 +59,8 @@ def load_oracle(omp: bool = False):
         "orc_user_set_particles": ([P, L, L, D, D, D], C.c_int),
         "orc_get_filter": ([P, L, IP, D], C.c_int),
         "orc_get_ugc": ([P, D], C.c_int),
+        "orc_set_gas_pressure": ([P, D], C.c_int),
+        "orc_get_p_force": ([P, D], C.c_int),
     }
     for n, (a, r) in sig.items():
         f = getattr(lib, n)
@@ -191,6 +193,15 @@ class OracleSystem:
         w = np.zeros(27)
         nf = self.lib.orc_get_filter(self.h, q, abi.iptr(cells), abi.dptr(w))
         return cells[:nf], w[:nf]
+
+    def set_gas_pressure(self, p_g):
+        self._chk(self.lib.orc_set_gas_pressure(self.h, abi.dptr(np.ascontiguousarray(p_g))),
+                  "set_gas_pressure")
+
+    def get_p_force(self):
+        out = np.zeros(3 * self.ncell)
+        self.lib.orc_get_p_force(self.h, abi.dptr(out))
+        return out.reshape(self.ncell, 3)
 
     def ugc(self):
         out = np.zeros(3 * self.ncell)

@@ -435,3 +435,24 @@ def test_nsquare_search_bit_exact(integrator):
     gp, op = assert_same_pairs(g, o)
     assert np.array_equal(gp, op)
     g.close()
+
+
+def test_pressure_gradient_and_coupled_run_bit_exact():
+    """mfixdem_set_gas_pressure (CALC_PG_GRAD) gives the oracle's P_FORCE bit
+    for bit; a coupled run driven by it matches as the GARG test does."""
+    p, parts, gas = fluidised()
+    g, o = DemSystem(p), OracleSystem(p)
+    nc = g.ncell
+    rng = np.random.default_rng(11)
+    p_g = 1.0e5 - 2500.0 * rng.uniform(0.0, 0.04, nc) + rng.normal(0.0, 5.0, nc)
+    for s in (g, o):
+        s.set_particles(parts)
+        s.set_gas(gas)
+        s.set_gas_pressure(p_g)
+        s.init()
+    assert np.array_equal(g.get_p_force(), o.get_p_force())
+    g.run(1, 30)
+    o.run(1, 30)
+    gp, op = g.get_particles()[0], o.get_particles()[0]
+    assert np.max(np.abs(gp - op)) <= 1e-12
+    g.close()

@@ -229,3 +229,38 @@ def test_drag_law_textbook(drag):
             got = lib.orc_drag_fgp(drag, eps, mu, rho, vrel, dp, pvol)
             want = textbook_fgp(drag, eps, mu, rho, vrel, dp, pvol)
             assert abs(got - want) <= 1e-13 * max(abs(want), 1e-300), (eps, vrel)
+
+
+def cell_centres(p):
+    ni, nj, nk = p.imax + 2, p.jmax + 2, p.kmax + 2
+    dx, dy, dz = p.xlength / p.imax, p.ylength / p.jmax, p.zlength / p.kmax
+    kk, ii, jj = np.meshgrid(np.arange(1, nk + 1), np.arange(1, ni + 1), np.arange(1, nj + 1),
+                             indexing="ij")
+    # J fastest, then I, then K: the reference's cell numbering
+    x = (ii - 1.5) * dx
+    y = (jj - 1.5) * dy
+    z = (kk - 1.5) * dz
+    fluid = (ii >= 2) & (ii <= p.imax + 1) & (jj >= 2) & (jj <= p.jmax + 1) & \
+        (kk >= 2) & (kk <= p.kmax + 1)
+    return x.ravel(), y.ravel(), z.ravel(), fluid.ravel(), (dx, dy, dz)
+
+
+def test_pressure_gradient_linear_and_quadratic_fields():
+    """CALC_PG_GRAD: the 3-point derivative (with linear extrapolation next
+    to walls) is exact for a linear pressure field everywhere and for a
+    quadratic one away from walls; P_FORCE = -grad p, 0 outside fluid."""
+    p, parts, gas, o = bed(abi.INTERP_GARG_2012, n=500)
+    x, y, z, fluid, (dx, dy, dz) = cell_centres(p)
+    a = np.array([30.0, -2500.0, 7.0])
+    o.set_gas_pressure(1.0e5 + a[0] * x + a[1] * y + a[2] * z)
+    pf = o.get_p_force()
+    assert np.allclose(pf[fluid], -a, rtol=1e-9, atol=1e-9)
+    assert not pf[~fluid].any()
+    o.set_gas_pressure(3.0e4 * y * y)
+    pf = o.get_p_force()
+    _, pijk = o.get_cells()
+    ny = p.jmax
+    jj = (np.arange(len(y)) % (ny + 2)) + 1
+    inner = fluid & (jj > 2) & (jj < ny + 1)
+    assert np.allclose(pf[inner, 1], -6.0e4 * y[inner], rtol=1e-9, atol=1e-6)
+    o.close()

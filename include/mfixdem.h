@@ -246,6 +246,28 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index,
                           int64_t* npairs);
 int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
 
+/* --- ensembles: many independent systems in one handle ------------------ */
+/* No counterpart in the reference (one MFiX run is one system); for
+   parameter sweeps and ensembles of pure-DEM systems of uneven size.  One
+   handle holds nsys systems, p[s] giving each its own parameters, box,
+   capacity (max_pip) and time step; every launch -- the step kernel and
+   each rebuild stage -- covers all systems at once, and the runs of steps
+   between rebuilds are replayed from one captured hipGraph.  Each system
+   evolves exactly as it would in its own handle.  All systems must share
+   precision, neighbor_search_n and remove_rogue, use the grid search and be
+   uncoupled (else MFIXDEM_ERR_UNSUPPORTED).  mfixdem_init / step / run /
+   synchronize / get_status / destroy work on the ensemble handle; the
+   single-system state calls return MFIXDEM_ERR_ARG on it. */
+int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_handle** out);
+/* Particles of system s, as mfixdem_set_particles (taken by mfixdem_init). */
+int32_t mfixdem_ensemble_set_particles(mfixdem_handle* h, int32_t s, int64_t n,
+                                       const double* pos, const double* vel,
+                                       const double* omega, const double* radius,
+                                       const double* ro_sol, const int32_t* phase, int64_t ld);
+/* State of system s in its particle-id order (ld >= its max_pip). */
+int32_t mfixdem_ensemble_get_particles(mfixdem_handle* h, int32_t s, double* pos, double* vel,
+                                       double* omega, int8_t* state, int64_t ld);
+
 /* --- measurement (no counterpart in the reference; its only throughput
    figure is the NITs/SEC log line, des_time_march.f:481-488) ------------- */
 /* The handle's HIP stream (hipStream_t), for callers that time or order

@@ -857,18 +857,40 @@ __device__ __forceinline__ void stencil_off(const GridDev& G, const double x[3],
   }
 }
 
-// DESGRID_PIC locate (desgrid_mod.f:900-915) + sort key on the search grid
-__global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G, const double* pos,
+template <bool BATCH>
+__device__ __forceinline__ const GridDev& sys_grid(const GridDev& G0, const BatchDev& B, int s) {
+  if constexpr (BATCH) return B.gtab[s];
+  else return G0;
+}
+// system owning global search cell c (ensembles): last s with cell_off[s] <= c
+__device__ __forceinline__ int cell_system(const BatchDev& B, int c) {
+  int lo = 0, hi = B.nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (B.cell_off[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// DESGRID_PIC locate (desgrid_mod.f:900-915) + sort key on the search grid.
+// Ensembles put the system id above the Morton code, so every system keeps
+// its slot range (its empty slots sort to the end of that range).
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const double* pos,
                                              const uint8_t* state, const int* gid, int* dgc,
-                                             uint64_t* keys, int* vals, int* flags) {
+                                             uint64_t* keys, int* vals, int* flags, BatchDev B) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   vals[i] = i;
+  const int s = BATCH ? B.sys[i] : 0;
+  const uint64_t sys_key = BATCH ? (uint64_t)s << B.sys_shift : 0ull;
   if (state[i] == 0) {
-    keys[i] = ~0ull;
+    keys[i] = BATCH ? sys_key | ((1ull << B.sys_shift) - 1) : ~0ull;
     dgc[i] = 0;
     return;
   }
+  const GridDev& G = sys_grid<BATCH>(G0, B, s);
   const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
   const int nn[3] = {G.dg_ni, G.dg_nj, G.dg_nk};
   int c3[3], f3[3];
@@ -880,11 +902,15 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G, const doub
   }
   dgc[i] = (c3[1] - 1) + (c3[0] - 1) * G.dg_nj + (c3[2] - 1) * G.dg_nj * G.dg_ni;
   if (dg_ghost(G, c3[0], c3[1], c3[2])) {
+    if (BATCH) {
+      atomicMin(&B.rogue[2 * s], gid[i]);
+      atomicAdd(&B.rogue[2 * s + 1], 1);
+    }
     atomicMin(&flags[kFlagRogueGid], gid[i]);
     atomicAdd(&flags[kFlagRogueCount], 1);
   }
   const uint64_t m = (uint64_t)(spread10(f3[0]) | (spread10(f3[1]) << 1) | (spread10(f3[2]) << 2));
-  keys[i] = (m << G.gid_bits) | (uint64_t)gid[i];
+  keys[i] = sys_key | (m << G.gid_bits) | (uint64_t)gid[i];
 }
 
 struct PermPtrs {
@@ -943,15 +969,19 @@ __device__ __forceinline__ uint32_t compact10(uint64_t v) {
 
 // search-grid cell ranges of the sorted particles (from the Morton part of
 // the sort key)
-__global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G, const uint64_t* keys,
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G0, const uint64_t* keys,
                                                      const uint8_t* state, int* cstart,
-                                                     int* cend) {
+                                                     int* cend, BatchDev B) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n || state[i] == 0) return;
-  const uint64_t m = keys[i] >> G.gid_bits;
+  const int s = BATCH ? B.sys[i] : 0;
+  const GridDev& G = sys_grid<BATCH>(G0, B, s);
+  const uint64_t m = keys[i] >> G.gid_bits;  // system id (ensembles) and Morton code
   const bool first = (i == 0) || (keys[i - 1] >> G.gid_bits) != m;
   const bool last = (i + 1 == n) || (keys[i + 1] >> G.gid_bits) != m;
-  const int c = (int)compact10(m) + G.fn[0] * ((int)compact10(m >> 1) + G.fn[1] * (int)compact10(m >> 2));
+  const int c = (BATCH ? B.cell_off[s] : 0) + (int)compact10(m) +
+                G.fn[0] * ((int)compact10(m >> 1) + G.fn[1] * (int)compact10(m >> 2));
   if (first) cstart[c] = i;
   if (last) cend[c] = i + 1;
 }
@@ -973,17 +1003,22 @@ struct BuildPtrs {
 // search walks the 27 search-grid cells (cell >= the largest cut-off), the
 // reference stencil is applied as a filter, and the row is put in the
 // reference's order (stencil lk,lj,li then ascending id).
-__global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, BuildPtrs b) {
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, BuildPtrs b,
+                                                  BatchDev B) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   if (b.state[i] != 1) {
     b.own_cnt[i] = 0;
     return;
   }
+  const int s = BATCH ? B.sys[i] : 0;
+  const GridDev& G = sys_grid<BATCH>(G0, B, s);
+  const int cbase = BATCH ? B.cell_off[s] : 0;
   const int gi = b.gid[i];
   int ci, cj, ck;
   dg_decode(G, b.dgc[i], ci, cj, ck);
-  const int g_r = b.flags[kFlagRogueGid];
+  const int g_r = BATCH ? B.rogue[2 * s] : b.flags[kFlagRogueGid];
   if (dg_ghost(G, ci, cj, ck) || gi >= g_r) {
     b.own_cnt[i] = 0;
     return;
@@ -1009,7 +1044,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, Build
       for (int dx = -1; dx <= 1; dx++) {
         const int fx = f3[0] + dx;
         if (fx < 0 || fx >= G.fn[0]) continue;
-        const int fc = fx + G.fn[0] * (fy + G.fn[1] * fz);
+        const int fc = cbase + fx + G.fn[0] * (fy + G.fn[1] * fz);
         const int t1 = b.fend[fc];
         for (int t = b.fstart[fc]; t < t1; t++) {
           if (t == i || b.state[t] == 0) continue;
@@ -1055,7 +1090,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G, Build
 // thread-per-particle kernel).
 constexpr int kRowCap = 192;  // candidates staged in LDS per wavefront
 
-__global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPtrs b, int ncells) {
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G0, BuildPtrs b, int ncells,
+                                                       BatchDev B) {
   __shared__ double s_x[kBS / 64][kRowCap], s_y[kBS / 64][kRowCap], s_z[kBS / 64][kRowCap],
       s_r[kBS / 64][kRowCap];
   __shared__ int s_j[kBS / 64][kRowCap], s_g[kBS / 64][kRowCap], s_d[kBS / 64][kRowCap];
@@ -1065,14 +1102,18 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
   if (c >= ncells) return;                       // whole wavefront leaves together
   const int h0 = b.fstart[c], h1 = b.fend[c];
   if (h1 <= h0) return;
-  const int fx = c % G.fn[0], fy = (c / G.fn[0]) % G.fn[1], fz = c / (G.fn[0] * G.fn[1]);
+  const int s = BATCH ? cell_system(B, c) : 0;
+  const GridDev& G = sys_grid<BATCH>(G0, B, s);
+  const int cbase = BATCH ? B.cell_off[s] : 0;
+  const int lc = c - cbase;
+  const int fx = lc % G.fn[0], fy = (lc / G.fn[0]) % G.fn[1], fz = lc / (G.fn[0] * G.fn[1]);
   // lane k < 27 holds neighbour cell k (x fastest): its first slot and the
   // inclusive prefix of the candidate counts
   int ns = 0, nc = 0;
   if (lane < 27) {
     const int x = fx + lane % 3 - 1, y = fy + (lane / 3) % 3 - 1, z = fz + lane / 9 - 1;
     if (x >= 0 && x < G.fn[0] && y >= 0 && y < G.fn[1] && z >= 0 && z < G.fn[2]) {
-      const int cc = x + G.fn[0] * (y + G.fn[1] * z);
+      const int cc = cbase + x + G.fn[0] * (y + G.fn[1] * z);
       ns = b.fstart[cc];
       nc = b.fend[cc] - ns;
     }
@@ -1122,7 +1163,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G, BuildPt
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int g_r = b.flags[kFlagRogueGid];
+  const int g_r = BATCH ? B.rogue[2 * s] : b.flags[kFlagRogueGid];
   for (int h = h0; h < h1; h++) {
     if (b.state[h] != 1) {
       if (lane == 0) b.own_cnt[h] = 0;
@@ -1381,11 +1422,14 @@ __global__ __launch_bounds__(kBS) void k_carry(int n, CarryPtrs c) {
 }
 
 // DELETE_PARTICLE of rogue particles (mass_outflow_dem.f:201-280)
-__global__ __launch_bounds__(kBS) void k_delete_rogue(int n, int S, GridDev G, const int* dgc,
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBS) void k_delete_rogue(int n, int S, GridDev G0, const int* dgc,
                                                       uint8_t* state, uint8_t* wmask, double* pos,
-                                                      double* vel, double* omega, double* radius) {
+                                                      double* vel, double* omega, double* radius,
+                                                      BatchDev B) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n || state[i] != 1) return;
+  const GridDev& G = sys_grid<BATCH>(G0, B, BATCH ? B.sys[i] : 0);
   int ci, cj, ck;
   dg_decode(G, dgc[i], ci, cj, ck);
   if (!dg_ghost(G, ci, cj, ck)) return;

@@ -49,6 +49,18 @@ struct mfixdem_handle {
   // ensembles (nsys > 1): per-system set-ups and device tables
   int nsys = 1;
   BatchDev B{};
+  int key_bits = 0;                       // significant bits of the rebuild sort key
+  std::vector<Setup> sys_setup;           // per system (ensembles)
+  std::vector<int> sys_off;               // first slot of each system (+ total)
+  std::vector<int> sys_n;                 // particles set per system
+  std::vector<int> rogue_init;            // {INT_MAX, 0} per system
+  uint16_t* d_sys = nullptr;
+  DevParams* d_ptab = nullptr;
+  GridDev* d_gtab = nullptr;
+  int *d_cell_off = nullptr, *d_dg_off = nullptr, *d_rogue = nullptr;
+  // host staging of an ensemble's particles until mfixdem_init
+  std::vector<double> st_pos, st_vel, st_om, st_r, st_rho;
+  std::vector<int> st_phase;
   hipStream_t stream = nullptr;
   int n = 0, stride = 0;
   int cur = 0, pcur = 0, tw = 0, lt = 0;
@@ -245,13 +257,23 @@ int32_t do_bin(mfixdem_handle* h) {
                        h->stream));
   HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, init + kFlagOwnOverflow, sizeof(int),
                        hipMemcpyHostToDevice, h->stream));
+  const bool batch = h->nsys > 1;
+  if (batch)
+    HIPOK(hipMemcpyAsync(h->d_rogue, h->rogue_init.data(), sizeof(int) * 2 * h->nsys,
+                         hipMemcpyHostToDevice, h->stream));
   const int t = h->tw, c = h->cur;
-  hipLaunchKernelGGL(k_bin, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, h->pos[c],
-                     h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0], h->flags);
+  if (batch)
+    hipLaunchKernelGGL((k_bin<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
+                       h->pos[c], h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0],
+                       h->flags, h->B);
+  else
+    hipLaunchKernelGGL((k_bin<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
+                       h->pos[c], h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0],
+                       h->flags, h->B);
   HIPOK(hipGetLastError());
   size_t tb = h->sort_tmp_bytes;
   HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
-                                           h->vals[1], n, 0, 30 + h->G.gid_bits, h->stream));
+                                           h->vals[1], n, 0, h->key_bits, h->stream));
   PermPtrs p{};
   p.perm = h->vals[1];
   p.pos = h->pos[c]; p.vel = h->vel[c]; p.omega = h->omega[c];
@@ -273,8 +295,12 @@ int32_t do_bin(mfixdem_handle* h) {
   h->tw = 1 - t;
   CHECKRC(fill_i32(h->stream, h->fstart, h->n_fine, 0));
   CHECKRC(fill_i32(h->stream, h->fend, h->n_fine, 0));
-  hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G, h->keys[1],
-                     h->state[h->tw], h->fstart, h->fend);
+  if (batch)
+    hipLaunchKernelGGL((k_cell_ranges<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
+                       h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
+  else
+    hipLaunchKernelGGL((k_cell_ranges<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
+                       h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
   HIPOK(hipGetLastError());
   if (h->S.p.coupled) {
     // deposition order for the steps up to the next rebuild (the key buffers
@@ -313,10 +339,19 @@ int32_t do_neighbour(mfixdem_handle* h) {
       // particles outside every cell range (state 0) get no row
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       const int64_t threads = (int64_t)h->n_fine * 64;
-      hipLaunchKernelGGL(k_own_rows_wave, dim3(nblk(threads)), dim3(kBS), 0, h->stream, S, h->G,
-                         b, h->n_fine);
+      if (h->nsys > 1)
+        hipLaunchKernelGGL((k_own_rows_wave<true>), dim3(nblk(threads)), dim3(kBS), 0, h->stream,
+                           S, h->G, b, h->n_fine, h->B);
+      else
+        hipLaunchKernelGGL((k_own_rows_wave<false>), dim3(nblk(threads)), dim3(kBS), 0,
+                           h->stream, S, h->G, b, h->n_fine, h->B);
     } else {
-      hipLaunchKernelGGL(k_own_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b);
+      if (h->nsys > 1)
+        hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
+                           h->G, b, h->B);
+      else
+        hipLaunchKernelGGL((k_own_rows<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
+                           h->G, b, h->B);
     }
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
@@ -380,9 +415,14 @@ int32_t do_neighbour(mfixdem_handle* h) {
   HIPOK(hipStreamSynchronize(h->stream));
   CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
   if (!nsq && fl[kFlagRogueCount] > 0) {
-    hipLaunchKernelGGL(k_delete_rogue, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
-                       h->dgc[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c],
-                       h->radius[t]);
+    if (h->nsys > 1)
+      hipLaunchKernelGGL((k_delete_rogue<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
+                         h->G, h->dgc[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c],
+                         h->omega[c], h->radius[t], h->B);
+    else
+      hipLaunchKernelGGL((k_delete_rogue<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
+                         h->G, h->dgc[t], h->state[t], h->wmask[t], h->pos[c], h->vel[c],
+                         h->omega[c], h->radius[t], h->B);
     HIPOK(hipGetLastError());
     h->st.rogue_deleted += fl[kFlagRogueCount];
   }
@@ -594,7 +634,126 @@ int32_t upload_setup(mfixdem_handle* h) {
   return MFIXDEM_OK;
 }
 
+// Upload an ensemble's staged particles: per system, ids 0..capacity-1 in
+// slot order, PVOL/PMASS/OMOI as make_arrays_des.f:288-291, NORMAL for the
+// particles set, NONEXISTENT for the rest of the system's slots.
+int32_t upload_ensemble(mfixdem_handle* h) {
+  const int64_t N = h->n, S = h->stride;
+  std::vector<double> R(S, 0.0), M(S, 3.4028234663852886e38), OI(S, 0.0),
+      PV(S, 3.4028234663852886e38);
+  std::vector<int> G(S, 0), FC(S, -1), DG(S, 0);
+  std::vector<uint8_t> ST(S, 0), PH(S, 0), WM(S, 0);
+  for (int s = 0; s < h->nsys; s++) {
+    const int off = h->sys_off[s];
+    for (int q = off; q < h->sys_off[s + 1]; q++) G[q] = q - off;
+    for (int q = off; q < off + h->sys_n[s]; q++) {
+      const double r = h->st_r[q];
+      R[q] = r;
+      PV[q] = (4.0 / 3.0) * kPi * (r * r * r);
+      M[q] = PV[q] * h->st_rho[q];
+      OI[q] = 2.5 / (M[q] * (r * r));
+      PH[q] = (uint8_t)(h->st_phase[q] - 1);
+      ST[q] = 1;
+    }
+  }
+  invalidate_graphs(h);
+  h->cur = h->pcur = h->tw = h->lt = 0;
+  h->fb = 0;
+  HIPOK(hipMemcpy(h->pos[0], h->st_pos.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->vel[0], h->st_vel.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->omega[0], h->st_om.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->radius[0], R.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->pmass[0], M.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->omoi[0], OI.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->pvol[0], PV.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->gid[0], G.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->fcell[0], FC.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->dgc[0], DG.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->state[0], ST.data(), S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->phase[0], PH.data(), S, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->wmask[0], WM.data(), S, hipMemcpyHostToDevice));
+  std::vector<int> Z((size_t)N + 1, 0);
+  for (int b = 0; b < 2; b++) {
+    HIPOK(hipMemcpy(h->nbr_start[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->nbr_split[b], Z.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  }
+  HIPOK(hipMemset(h->flags, 0, sizeof(int) * kFlagCount));
+  h->st = mfixdem_status{};
+  h->n_pairs = 0;
+  h->first_pass = true;
+  h->binned = false;
+  return MFIXDEM_OK;
+}
+
 void fill_devparams(mfixdem_handle* h) { fill_params(h->S, h->n, h->stride, h->P, h->G); }
+
+// Device buffers, stream and events of a handle whose set-up (S, P, G,
+// n, stride, n_fine, key_bits) is filled in; facet_cells = DES cells of
+// the per-cell wall lists.  On failure the caller destroys the handle.
+int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
+  const int64_t n = h->n, S = h->stride, nc = h->S.ncell;
+  int32_t rc = MFIXDEM_OK;
+  auto fail = [&](int32_t r) { return r; };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  if (hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  const int nf = std::max(1, h->P.nf);
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dalloc(&h->pos[b], 3 * S)) || (rc = dalloc(&h->vel[b], 3 * S)) ||
+        (rc = dalloc(&h->omega[b], 3 * S)) || (rc = dalloc(&h->radius[b], S)) ||
+        (rc = dalloc(&h->pmass[b], S)) || (rc = dalloc(&h->omoi[b], S)) ||
+        (rc = dalloc(&h->pvol[b], S)) || (rc = dalloc(&h->acc[b], 3 * S)) ||
+        (rc = dalloc(&h->racc[b], 3 * S)) || (rc = dalloc(&h->gid[b], S)) ||
+        (rc = dalloc(&h->fcell[b], S)) || (rc = dalloc(&h->dgc[b], S)) ||
+        (rc = dalloc(&h->state[b], S)) || (rc = dalloc(&h->phase[b], S)) ||
+        (rc = dalloc(&h->wmask[b], S)) || (rc = dalloc(&h->wpft[b], 3 * nf * S)) ||
+        (rc = dalloc(&h->keys[b], n)) || (rc = dalloc(&h->vals[b], n)) ||
+        (rc = dalloc(&h->nbr_start[b], n + 1)) || (rc = dalloc(&h->nbr_split[b], n)) ||
+        (rc = dalloc(&h->pair_base[b], n + 1)))
+      return fail(rc);
+  }
+  if ((rc = dalloc(&h->fc, 3 * S)) || (rc = dalloc(&h->tow, 3 * S)) ||
+      (rc = dalloc(&h->fc_id, 3 * S)) || (rc = dalloc(&h->tow_id, 3 * S)) ||
+      (rc = dalloc(&h->fstart, h->n_fine)) || (rc = dalloc(&h->fend, h->n_fine)) ||
+      (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
+      (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
+      (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
+      (rc = dalloc(&h->scan_bsum, std::max<int64_t>(n, nc) / kScanTile + 2)) ||
+      (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
+      (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
+      (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
+      (rc = dalloc(&h->ugc, 3 * nc)) ||
+      (rc = dalloc(&h->ep_g, nc)) || (rc = dalloc(&h->rop_s, nc * h->S.p.nphases)) ||
+      (rc = dalloc(&h->rops_node, nc * h->S.p.nphases)) || (rc = dalloc(&h->vol, nc)) ||
+      (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
+      (rc = dalloc(&h->fluid, nc)) || (rc = dalloc(&h->pic_cnt, nc)) ||
+      (rc = dalloc(&h->pic_off, nc + 1)) || (rc = dalloc(&h->pic_cur, nc)) ||
+      (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, facet_cells)) ||
+      (rc = dalloc(&h->stage, 3 * n)) ||
+      (rc = dalloc(&h->rops_fix[0], nc * h->S.p.nphases)) ||
+      (rc = dalloc(&h->rops_fix[1], nc * h->S.p.nphases)) || (rc = dalloc(&h->dep_list, n)))
+    return fail(rc);
+  for (int b = 0; b < 2; b++)
+    if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * h->S.p.nphases) != hipSuccess)
+      return fail(MFIXDEM_ERR_HIP);
+  size_t dep_tmp = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
+                                         h->vals[0], h->vals[1], (int)n, 0, h->key_bits,
+                                         h->stream) != hipSuccess ||
+      hipcub::DeviceRadixSort::SortPairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0],
+                                         h->dep_list, (int)n, 0, 30, h->stream) != hipSuccess)
+    return fail(MFIXDEM_ERR_HIP);
+  h->sort_tmp_bytes = std::max(h->sort_tmp_bytes, dep_tmp);
+  if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
+    return fail(MFIXDEM_ERR_OOM);
+  if ((rc = upload_setup(h))) return fail(rc);
+  if (hipMemset(h->flags, 0, sizeof(int) * kFlagCount) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
+  return MFIXDEM_OK;
+}
+
+
 
 }  // namespace
 
@@ -671,66 +830,11 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
     delete h;
     return MFIXDEM_ERR_UNSUPPORTED;
   }
-  const int64_t n = h->n, S = h->stride, nc = h->S.ncell;
-  auto fail = [&](int32_t r) {
+  h->key_bits = 30 + h->G.gid_bits;
+  if ((rc = alloc_buffers(h, h->S.dg_ncell))) {
     mfixdem_destroy(h);
-    return r;
-  };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(MFIXDEM_ERR_HIP);
-  if (hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess)
-    return fail(MFIXDEM_ERR_HIP);
-  const int nf = std::max(1, h->P.nf);
-  for (int b = 0; b < 2; b++) {
-    if ((rc = dalloc(&h->pos[b], 3 * S)) || (rc = dalloc(&h->vel[b], 3 * S)) ||
-        (rc = dalloc(&h->omega[b], 3 * S)) || (rc = dalloc(&h->radius[b], S)) ||
-        (rc = dalloc(&h->pmass[b], S)) || (rc = dalloc(&h->omoi[b], S)) ||
-        (rc = dalloc(&h->pvol[b], S)) || (rc = dalloc(&h->acc[b], 3 * S)) ||
-        (rc = dalloc(&h->racc[b], 3 * S)) || (rc = dalloc(&h->gid[b], S)) ||
-        (rc = dalloc(&h->fcell[b], S)) || (rc = dalloc(&h->dgc[b], S)) ||
-        (rc = dalloc(&h->state[b], S)) || (rc = dalloc(&h->phase[b], S)) ||
-        (rc = dalloc(&h->wmask[b], S)) || (rc = dalloc(&h->wpft[b], 3 * nf * S)) ||
-        (rc = dalloc(&h->keys[b], n)) || (rc = dalloc(&h->vals[b], n)) ||
-        (rc = dalloc(&h->nbr_start[b], n + 1)) || (rc = dalloc(&h->nbr_split[b], n)) ||
-        (rc = dalloc(&h->pair_base[b], n + 1)))
-      return fail(rc);
+    return rc;
   }
-  if ((rc = dalloc(&h->fc, 3 * S)) || (rc = dalloc(&h->tow, 3 * S)) ||
-      (rc = dalloc(&h->fc_id, 3 * S)) || (rc = dalloc(&h->tow_id, 3 * S)) ||
-      (rc = dalloc(&h->fstart, h->n_fine)) || (rc = dalloc(&h->fend, h->n_fine)) ||
-      (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
-      (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
-      (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
-      (rc = dalloc(&h->scan_bsum, std::max<int64_t>(n, nc) / kScanTile + 2)) ||
-      (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
-      (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
-      (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
-      (rc = dalloc(&h->ugc, 3 * nc)) ||
-      (rc = dalloc(&h->ep_g, nc)) || (rc = dalloc(&h->rop_s, nc * p->nphases)) ||
-      (rc = dalloc(&h->rops_node, nc * p->nphases)) || (rc = dalloc(&h->vol, nc)) ||
-      (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
-      (rc = dalloc(&h->fluid, nc)) || (rc = dalloc(&h->pic_cnt, nc)) ||
-      (rc = dalloc(&h->pic_off, nc + 1)) || (rc = dalloc(&h->pic_cur, nc)) ||
-      (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, h->S.dg_ncell)) ||
-      (rc = dalloc(&h->stage, 3 * n)) ||
-      (rc = dalloc(&h->rops_fix[0], nc * p->nphases)) ||
-      (rc = dalloc(&h->rops_fix[1], nc * p->nphases)) || (rc = dalloc(&h->dep_list, n)))
-    return fail(rc);
-  for (int b = 0; b < 2; b++)
-    if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * p->nphases) != hipSuccess)
-      return fail(MFIXDEM_ERR_HIP);
-  size_t dep_tmp = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
-                                         h->vals[0], h->vals[1], (int)n, 0, 30 + h->G.gid_bits,
-                                         h->stream) != hipSuccess ||
-      hipcub::DeviceRadixSort::SortPairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0],
-                                         h->dep_list, (int)n, 0, 30, h->stream) != hipSuccess)
-    return fail(MFIXDEM_ERR_HIP);
-  h->sort_tmp_bytes = std::max(h->sort_tmp_bytes, dep_tmp);
-  if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
-    return fail(MFIXDEM_ERR_OOM);
-  if ((rc = upload_setup(h))) return fail(rc);
-  if (hipMemset(h->flags, 0, sizeof(int) * kFlagCount) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
   *out = h;
   return MFIXDEM_OK;
 }
@@ -754,6 +858,8 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
   dfree(h->rops_node); dfree(h->vol); dfree(h->vol_surr); dfree(h->ro_s); dfree(h->fluid);
   dfree(h->pic_cnt); dfree(h->pic_off); dfree(h->pic_cur); dfree(h->pic);
   dfree(h->cell_facets); dfree(h->stage); dfree(h->rops_fix[0]); dfree(h->rops_fix[1]);
+  dfree(h->d_sys); dfree(h->d_ptab); dfree(h->d_gtab); dfree(h->d_cell_off); dfree(h->d_dg_off);
+  dfree(h->d_rogue);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -770,6 +876,7 @@ int32_t mfixdem_get_derived(const mfixdem_handle* h, mfixdem_derived* out) {
 int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, const double* vel,
                               const double* omega, const double* radius, const double* ro_sol,
                               const int32_t* phase, int64_t ld) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || n < 0 || n > h->n || ld < n || !pos || !radius || !ro_sol) return MFIXDEM_ERR_ARG;
   const int64_t N = h->n, S = h->stride;
   std::vector<double> P3((size_t)3 * S, 0.0), V3((size_t)3 * S, 0.0), W3((size_t)3 * S, 0.0);
@@ -846,6 +953,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
 
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, double* omega,
                               int8_t* state, int64_t ld) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
   if (pos) CHECKRC(scatter_d3(h, h->pos[h->cur], pos, ld));
   if (vel) CHECKRC(scatter_d3(h, h->vel[h->cur], vel, ld));
@@ -862,6 +970,7 @@ int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, doubl
 
 int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
                         const double* w_g, const double* p_force) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || !u_g || !v_g || !w_g) return MFIXDEM_ERR_ARG;
   const Setup& S = h->S;
   const mfixdem_params& p = S.p;
@@ -912,6 +1021,7 @@ int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
 }
 
 int32_t mfixdem_set_gas_pressure(mfixdem_handle* h, const double* p_g) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || !p_g) return MFIXDEM_ERR_ARG;
   const Setup& S = h->S;
   const int nc = S.ncell;
@@ -953,6 +1063,7 @@ int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s) {
 int32_t mfixdem_init(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
   if (h->S.p.coupled && !h->gas_set) return MFIXDEM_ERR_NOT_READY;
+  if (h->nsys > 1) CHECKRC(upload_ensemble(h));
   const int n = h->n;
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
   if (h->S.p.coupled) {
@@ -1063,6 +1174,7 @@ int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {
 }
 
 int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t ld) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
   const int64_t n = h->n, S = h->stride;
   std::vector<double> tmp;
@@ -1078,6 +1190,7 @@ int32_t mfixdem_get_forces(mfixdem_handle* h, double* fc, double* tow, int64_t l
 }
 
 int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk, int64_t ld) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || ld < h->n) return MFIXDEM_ERR_ARG;
   const int n = h->n;
   std::vector<int> g, d, f;
@@ -1106,6 +1219,7 @@ int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk, in
 
 int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index, int32_t* neighbors,
                           double* pft, int64_t capacity, int64_t* npairs) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h) return MFIXDEM_ERR_ARG;
   if (npairs) *npairs = h->n_pairs;
   if (capacity == 0) return MFIXDEM_OK;
@@ -1174,6 +1288,177 @@ int32_t mfixdem_reset_kernel_times(mfixdem_handle* h) {
     h->t_ms[i] = 0.0;
     h->t_calls[i] = 0;
   }
+  return MFIXDEM_OK;
+}
+
+// ================================================================ ensembles
+// One handle for nsys independent pure-DEM systems: every launch (the step
+// kernel, each rebuild stage) covers all systems; see BatchDev.
+
+int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_handle** out) {
+  if (!p || !out || nsys < 1 || nsys > 65535) return MFIXDEM_ERR_ARG;
+  *out = nullptr;
+  mfixdem_handle* h = new (std::nothrow) mfixdem_handle();
+  if (!h) return MFIXDEM_ERR_OOM;
+  h->nsys = nsys;
+  h->sys_setup.resize(nsys);
+  h->sys_off.assign(nsys + 1, 0);
+  h->sys_n.assign(nsys, 0);
+  for (int s = 0; s < nsys; s++) {
+    const mfixdem_params& q = p[s];
+    int32_t rc = build_setup(q, h->sys_setup[s]);
+    if (rc == MFIXDEM_OK &&
+        (q.coupled || q.des_neighbor_search != MFIXDEM_NSEARCH_GRID ||
+         q.precision != p[0].precision || q.neighbor_search_n != p[0].neighbor_search_n ||
+         q.remove_rogue != p[0].remove_rogue))
+      rc = MFIXDEM_ERR_UNSUPPORTED;  // per-launch choices must agree; no gas coupling
+    if (rc != MFIXDEM_OK) {
+      delete h;
+      return rc;
+    }
+    h->sys_off[s + 1] = h->sys_off[s] + q.max_pip;
+  }
+  if ((int64_t)h->sys_off[nsys] > INT_MAX / 4) {
+    delete h;
+    return MFIXDEM_ERR_ARG;
+  }
+  h->S = h->sys_setup[0];
+  h->n = h->sys_off[nsys];
+  h->stride = (h->n + 63) / 64 * 64;
+  fill_devparams(h);
+  std::vector<DevParams> ptab(nsys);
+  std::vector<GridDev> gtab(nsys);
+  std::vector<int> cell_off(nsys + 1, 0), dg_off(nsys + 1, 0);
+  int gid_bits = 1, nf = 0;
+  for (int s = 0; s < nsys; s++) {
+    const Setup& S = h->sys_setup[s];
+    fill_params(S, h->n, h->stride, ptab[s], gtab[s]);
+    int nfine = 0;
+    if (search_grid(S.p, S.p.max_pip, gtab[s], nfine) != MFIXDEM_OK) {
+      delete h;
+      return MFIXDEM_ERR_UNSUPPORTED;
+    }
+    gid_bits = std::max(gid_bits, gtab[s].gid_bits);
+    nf = std::max(nf, ptab[s].nf);
+    cell_off[s + 1] = cell_off[s] + nfine;
+    dg_off[s + 1] = dg_off[s] + S.dg_ncell;
+  }
+  int sys_bits = 0;
+  while ((1 << sys_bits) < nsys) sys_bits++;
+  if (30 + gid_bits + sys_bits > 64) {
+    delete h;
+    return MFIXDEM_ERR_UNSUPPORTED;
+  }
+  for (auto& g : gtab) g.gid_bits = gid_bits;
+  h->G = gtab[0];
+  // handle-level flags that rebuild stages apply to every slot: the largest
+  // facet count, and carry Adams-Bashforth history if any member uses it
+  h->P.nf = nf;
+  for (int s = 0; s < nsys; s++) h->P.ab2 = std::max(h->P.ab2, ptab[s].ab2);
+  h->n_fine = cell_off[nsys];
+  h->key_bits = 30 + gid_bits + sys_bits;
+  int32_t rc = alloc_buffers(h, dg_off[nsys]);
+  std::vector<uint16_t> sys(h->stride, 0);
+  for (int s = 0; s < nsys; s++)
+    for (int q = h->sys_off[s]; q < h->sys_off[s + 1]; q++) sys[q] = (uint16_t)s;
+  std::vector<uint16_t> facets(dg_off[nsys]);
+  for (int s = 0; s < nsys; s++)
+    std::copy(h->sys_setup[s].cell_facets.begin(), h->sys_setup[s].cell_facets.end(),
+              facets.begin() + dg_off[s]);
+  h->rogue_init.resize(2 * nsys);
+  for (int s = 0; s < nsys; s++) {
+    h->rogue_init[2 * s] = INT_MAX;
+    h->rogue_init[2 * s + 1] = 0;
+  }
+  if (rc == MFIXDEM_OK &&
+      ((rc = dalloc(&h->d_sys, h->stride)) || (rc = dalloc(&h->d_ptab, nsys)) ||
+       (rc = dalloc(&h->d_gtab, nsys)) || (rc = dalloc(&h->d_cell_off, nsys + 1)) ||
+       (rc = dalloc(&h->d_dg_off, nsys + 1)) || (rc = dalloc(&h->d_rogue, 2 * nsys))))
+    rc = MFIXDEM_ERR_OOM;
+  if (rc == MFIXDEM_OK &&
+      (hipMemcpy(h->d_sys, sys.data(), sizeof(uint16_t) * h->stride, hipMemcpyHostToDevice) ||
+       hipMemcpy(h->d_ptab, ptab.data(), sizeof(DevParams) * nsys, hipMemcpyHostToDevice) ||
+       hipMemcpy(h->d_gtab, gtab.data(), sizeof(GridDev) * nsys, hipMemcpyHostToDevice) ||
+       hipMemcpy(h->d_cell_off, cell_off.data(), sizeof(int) * (nsys + 1),
+                 hipMemcpyHostToDevice) ||
+       hipMemcpy(h->d_dg_off, dg_off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) ||
+       hipMemcpy(h->cell_facets, facets.data(), sizeof(uint16_t) * facets.size(),
+                 hipMemcpyHostToDevice)))
+    rc = MFIXDEM_ERR_HIP;
+  if (rc != MFIXDEM_OK) {
+    mfixdem_destroy(h);
+    return rc;
+  }
+  h->B.sys = h->d_sys;
+  h->B.ptab = h->d_ptab;
+  h->B.gtab = h->d_gtab;
+  h->B.cell_off = h->d_cell_off;
+  h->B.dg_off = h->d_dg_off;
+  h->B.rogue = h->d_rogue;
+  h->B.nsys = nsys;
+  h->B.sys_shift = 30 + gid_bits;
+  const size_t S = h->stride;
+  h->st_pos.assign(3 * S, 0.0);
+  h->st_vel.assign(3 * S, 0.0);
+  h->st_om.assign(3 * S, 0.0);
+  h->st_r.assign(S, 0.0);
+  h->st_rho.assign(S, 0.0);
+  h->st_phase.assign(S, 1);
+  *out = h;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_ensemble_set_particles(mfixdem_handle* h, int32_t s, int64_t n, const double* pos,
+                                       const double* vel, const double* omega,
+                                       const double* radius, const double* ro_sol,
+                                       const int32_t* phase, int64_t ld) {
+  if (!h || h->nsys < 2 || s < 0 || s >= h->nsys || !pos || !radius || !ro_sol) return MFIXDEM_ERR_ARG;
+  const int off = h->sys_off[s], cap = h->sys_off[s + 1] - off;
+  if (n < 0 || n > cap || ld < n) return MFIXDEM_ERR_ARG;
+  const size_t S = h->stride;
+  for (int64_t q = 0; q < n; q++) {
+    const int64_t t = off + q;
+    for (int c = 0; c < 3; c++) {
+      h->st_pos[c * S + t] = pos[c * ld + q];
+      h->st_vel[c * S + t] = vel ? vel[c * ld + q] : 0.0;
+      h->st_om[c * S + t] = omega ? omega[c * ld + q] : 0.0;
+    }
+    h->st_r[t] = radius[q];
+    h->st_rho[t] = ro_sol[q];
+    const int ph = phase ? phase[q] : 1;
+    if (ph < 1 || ph > h->sys_setup[s].p.nphases) return MFIXDEM_ERR_ARG;
+    h->st_phase[t] = ph;
+  }
+  h->sys_n[s] = (int)n;
+  h->inited = false;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_ensemble_get_particles(mfixdem_handle* h, int32_t s, double* pos, double* vel,
+                                       double* omega, int8_t* state, int64_t ld) {
+  if (!h || h->nsys < 2 || s < 0 || s >= h->nsys) return MFIXDEM_ERR_ARG;
+  const int off = h->sys_off[s], cap = h->sys_off[s + 1] - off;
+  if (ld < cap) return MFIXDEM_ERR_ARG;
+  const int64_t S = h->stride;
+  std::vector<int> g(cap);
+  std::vector<int8_t> st(cap);
+  std::vector<double> tmp(cap);
+  HIPOK(hipMemcpyAsync(g.data(), h->gid[h->tw] + off, sizeof(int) * cap, hipMemcpyDeviceToHost,
+                       h->stream));
+  HIPOK(hipMemcpyAsync(st.data(), h->state[h->tw] + off, cap, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  const double* src[3] = {h->pos[h->cur], h->vel[h->cur], h->omega[h->cur]};
+  double* dst[3] = {pos, vel, omega};
+  for (int a = 0; a < 3; a++) {
+    if (!dst[a]) continue;
+    for (int c = 0; c < 3; c++) {
+      HIPOK(hipMemcpy(tmp.data(), src[a] + c * S + off, sizeof(double) * cap,
+                      hipMemcpyDeviceToHost));
+      for (int q = 0; q < cap; q++) dst[a][c * ld + g[q]] = tmp[q];
+    }
+  }
+  if (state)
+    for (int q = 0; q < cap; q++) state[g[q]] = st[q];
   return MFIXDEM_OK;
 }
 

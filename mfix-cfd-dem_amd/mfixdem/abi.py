@@ -192,6 +192,10 @@ def _declare(lib):
         "mfixdem_get_status": ([_P, C.POINTER(Status)], _I),
         "mfixdem_get_stream": ([_P, C.POINTER(_P)], _I),
         "mfixdem_get_kernel_times": ([_P, _DP, C.POINTER(C.c_int64), _I], _I),
+        "mfixdem_ensemble_create": ([_I, C.POINTER(Params), C.POINTER(_P)], _I),
+        "mfixdem_ensemble_set_particles": ([_P, _I, C.c_int64, _DP, _DP, _DP, _DP, _DP, _IP,
+                                            C.c_int64], _I),
+        "mfixdem_ensemble_get_particles": ([_P, _I, _DP, _DP, _DP, _BP, C.c_int64], _I),
         "mfixdem_reset_kernel_times": ([_P], _I),
     }
     for name, (args, res) in sig.items():

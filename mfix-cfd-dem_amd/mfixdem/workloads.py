@@ -329,3 +329,49 @@ def dem05(rows, model="LSD", integrator=abi.ADAMS_BASHFORTH):
     phase = np.where(a[:, 4] > 4500.0, 2, 1).astype(np.int32)
     parts = make_particles(a[:, 0:3].T, a[:, 3], a[:, 4], vel=a[:, 5:8].T, phase=phase)
     return p, parts, None
+
+
+def ensemble_member(n, kn, en, mew, seed, phi=0.30):
+    """One member of a parameter-sweep ensemble: n spheres of d = 1 mm,
+    rho = 2500 (as the settling bed) placed by RSA in an X = Z, Y = 4X box
+    sized for solids fraction phi over the region the centres can reach;
+    LSD with KN = KN_W = kn, EN = EN_WALL = en, MEW = MEW_W = mew,
+    DTSOLID = TCOLL/50 (the reference's rule)."""
+    d = 1.0e-3
+    vol = n * (np.pi / 6.0) * d ** 3 / phi
+    lx = (vol / 4.0) ** (1.0 / 3.0) + d
+    for _ in range(50):  # (lx - d)^2 (4 lx - d) = vol
+        f = (lx - d) ** 2 * (4 * lx - d) - vol
+        df = 2 * (lx - d) * (4 * lx - d) + 4 * (lx - d) ** 2
+        lx -= f / df
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = abi.EULER
+    p.nphases = 1
+    p.max_pip = n
+    p.xlength, p.ylength, p.zlength = lx, 4.0 * lx, lx
+    p.imax = p.jmax = p.kmax = 1
+    p.grav[:] = [0.0, -9.81, 0.0]
+    p.kn = p.kn_w = kn
+    p.mew = p.mew_w = mew
+    p.des_en_input[0] = en
+    p.des_en_wall_input[0] = en
+    p.d_p0[0] = 1.0e-3
+    p.ro_s0[0] = 2500.0
+    p.max_radius = 0.5e-3
+    r = np.full(n, 0.5e-3)
+    pos = rsa_positions(seed, r, [0, 0, 0], [lx, 4.0 * lx, lx])
+    return p, make_particles(pos, r, 2500.0)
+
+
+def sweep_ensemble(nsys=256, seed=11, nmin=2000, nmax=200_000, phi=0.30):
+    """Ensemble of independent settling beds of uneven size: N log-uniform in
+    [nmin, nmax], KN log-uniform in [1e2, 1e4], EN ~ U[0.7, 0.95],
+    MEW ~ U[0.1, 0.5] (seeded)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys)).astype(int)
+    kns = np.exp(rng.uniform(np.log(1e2), np.log(1e4), nsys))
+    ens = rng.uniform(0.7, 0.95, nsys)
+    mews = rng.uniform(0.1, 0.5, nsys)
+    return [ensemble_member(int(ns[s]), float(kns[s]), float(ens[s]), float(mews[s]),
+                            seed * 1000 + s, phi) for s in range(nsys)]

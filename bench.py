@@ -107,6 +107,90 @@ def force_error(p, parts, gas):
     return float(np.max(np.abs(out[1] - out[0])) / np.max(np.abs(out[0])))
 
 
+def ensemble_bench(nsys, steps, log):
+    """Config C4: nsys independent settling beds of uneven size (N log-uniform
+    in [2k, 200k], d = 1 mm), KN / EN / MEW swept, DTSOLID = TCOLL/50 each, in
+    one ensemble handle; the result-invariant tuning is chosen by the
+    autotuner on a 32-member sample first."""
+    import functools
+    from mfixdem import autotune
+    from mfixdem.ensemble import Ensemble
+    member = functools.partial(workloads.lattice_member, precision=abi.FP64)
+    t0 = time.perf_counter()
+    ms = workloads.sweep_ensemble(nsys=nsys, seed=11, member=member)
+    log(f"ensemble: {nsys} systems, {sum(p.max_pip for p, _ in ms)} particles built in "
+        f"{time.perf_counter() - t0:.1f} s")
+
+    def make(sub):
+        e = Ensemble([p for p, _ in sub])
+        for s, (_, parts) in enumerate(sub):
+            e.set_particles(s, parts)
+        return e
+
+    sample = ms[:: max(1, nsys // 32)]
+    best, timings = autotune.autotune(
+        lambda: make(sample),
+        cands=autotune.candidates((1.0, 1.25), (abi.ROWS_WAVE128, abi.ROWS_WAVE192,
+                                                abi.ROWS_WAVE256)), log=log)
+    e = make(ms)
+    e.set_tuning(*best)
+    e.init()
+    e.run(1, 25)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.run(26, steps)
+    e.synchronize()
+    el = time.perf_counter() - t0
+    npart = sum(p.max_pip for p, _ in ms)
+    out = {"systems": nsys, "particles": npart, "steps": steps,
+           "particle_steps_per_s": npart * steps / el, "ms_per_step": 1e3 * el / steps,
+           "device_bytes": e.device_bytes(),
+           "tuning": {"cell_scale": best[0], "row_kernel": best[1],
+                      "candidates_s": timings},
+           "data": "synthetic (seeded jittered-lattice packings, swept KN/EN/MEW)"}
+    e.close()
+    log(f"ensemble: {out['particle_steps_per_s']:.4e} particle-steps/s, tuning {best}")
+    return out
+
+
+def serving_bench(n_req, log):
+    """Config C5, scaled to a bounded run: Poisson arrivals of parameter-sweep
+    requests (N log-uniform in [5k, 200k], steps uniform in [200, 2000],
+    mixed precision), admitted by HBM footprint within a 16 GiB budget.  The
+    arrival rate offers 70 % of the throughput measured for the same kind of
+    requests run one after another."""
+    from mfixdem import serving
+    reqs = serving.poisson_requests(n_req, rate=1.0, make_member=workloads.lattice_member,
+                                    nmin=5000, nmax=200_000, smin=200, smax=2000, seed=21)
+    model = serving.FootprintModel(lambda n: workloads.lattice_member(n, 1e3, 0.9, 0.3, 1))
+    for r in reqs:
+        r.footprint = model(r.params.max_pip)
+    # capacity probe: 6 requests back to back
+    t0 = time.perf_counter()
+    probe = serving.SweepServer(1 << 62, workers=1).serve(
+        [serving.Request(-i, 0.0, r.params, r.parts, r.nsteps, r.footprint)
+         for i, r in enumerate(reqs[:6])])
+    cap = sum(r.work for r in probe) / (time.perf_counter() - t0)
+    mean_work = float(np.mean([r.work for r in reqs]))
+    rate = 0.7 * cap / mean_work
+    rng = np.random.default_rng(22)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    for r, a in zip(reqs, t):
+        r.arrival = float(a)
+    srv = serving.SweepServer(16 << 30, workers=8)
+    done = srv.serve(reqs)
+    rep = serving.report(done, srv.makespan)
+    rep.update({"arrival_rate_per_s": rate, "offered_particle_steps_per_s": rate * mean_work,
+                "sequential_capacity_particle_steps_per_s": cap,
+                "hbm_budget_bytes": 16 << 30, "max_concurrent": srv.max_running,
+                "max_bytes_in_use": srv.max_in_use,
+                "workload": "scaled config C5: N log-uniform [5k, 200k], steps U[200, 2000], "
+                            "mixed precision, KN/EN/MEW swept, Poisson arrivals"})
+    log(f"serving: p50 {rep['p50_latency_s']:.3f} s, p99 {rep['p99_latency_s']:.3f} s, "
+        f"{rep['particle_steps_per_s']:.3e} particle-steps/s at rate {rate:.2f}/s")
+    return rep
+
+
 def cpu_baseline(p, parts, gas, seconds=15.0):
     """Times the OpenMP CPU restatement (oracle/liboracle_omp.so) on the same
     workload for as many whole steps as fit in about `seconds`."""
@@ -142,6 +226,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--ens-systems", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one DEM system per GPU on a single GPU; "
@@ -202,6 +291,9 @@ def main():
             f"{mixed['max_rel_force_error']:.2e}, KE diff {mixed['ke_rel_diff']:.2e}, "
             f"bed height {h64:.5f} / {hm:.5f} m")
 
+    ens = None if args.no_ensemble else ensemble_bench(args.ens_systems, args.ens_steps, log)
+    srv = None if args.no_serving else serving_bench(args.serve_requests, log)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p, parts, gas, args.cpu_seconds)
@@ -233,6 +325,8 @@ def main():
                      "kernel_us": t_step_ms * 1e3, "algorithmic_bytes": bytes_step},
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "ensemble": ens,
+        "serving": srv,
         "kernel_groups_us": {"step": kt[0] * 1e3, "rebuild": kt[1] * 1e3, "gas": kt[2] * 1e3},
     }
     print(json.dumps(out))

@@ -73,9 +73,14 @@ enum { MFIXDEM_NSEARCH_NSQUARE = 1, MFIXDEM_NSEARCH_GRID = 4 };
 enum {
   MFIXDEM_DEBUG_STORE_FORCES = 1, /* keep FC/TOW of the last step for mfixdem_get_forces */
   MFIXDEM_DEBUG_TIME_KERNELS = 2, /* eager launches, HIP-event time per kernel group */
-  MFIXDEM_DEBUG_REFERENCE_DEPOSITION = 4 /* eps_s deposition summed in the reference's
+  MFIXDEM_DEBUG_REFERENCE_DEPOSITION = 4, /* eps_s deposition summed in the reference's
                                      serial order (PIC lists, node gather) instead
                                      of order-independent fixed-point sums */
+  MFIXDEM_DEBUG_NO_GRAPH = 8      /* launch every step eagerly, no hipGraph capture:
+                                     for hosts that drive several handles from
+                                     different threads (a capture in progress on
+                                     one thread makes the HIP runtime refuse
+                                     legacy-stream copies and frees on others) */
 };
 /* kernel groups reported by mfixdem_get_kernel_times */
 enum { MFIXDEM_TIME_STEP = 0, MFIXDEM_TIME_REBUILD = 1, MFIXDEM_TIME_GAS = 2,
@@ -245,6 +250,22 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index,
                           int32_t* neighbors, double* pft, int64_t capacity,
                           int64_t* npairs);
 int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
+
+/* --- tuning ------------------------------------------------------------- */
+/* Result-invariant performance choices (no counterpart in the reference):
+   the neighbour-search cell size as a multiple (1..4) of the largest
+   cut-off FACTOR_RLM*2*MAX_RADIUS, and the kernel that builds the
+   neighbour rows: one thread per particle, or one wavefront per search cell
+   with an LDS tile of 128, 192 (default) or 256 candidates.  The pair sets
+   and every result are the same for all choices; FACTOR_RLM itself (the
+   neighbour-list skin) is a model parameter and stays as given.  May be
+   called at any time outside a step. */
+enum { MFIXDEM_ROWS_THREAD = 0, MFIXDEM_ROWS_WAVE128 = 1, MFIXDEM_ROWS_WAVE192 = 2,
+       MFIXDEM_ROWS_WAVE256 = 3 };
+int32_t mfixdem_set_tuning(mfixdem_handle* h, double cell_scale, int32_t row_kernel);
+/* Device memory the handle holds now, in bytes (for admission control by
+   HBM footprint; it grows when neighbour lists outgrow their capacity). */
+int32_t mfixdem_get_device_bytes(mfixdem_handle* h, int64_t* bytes);
 
 /* --- ensembles: many independent systems in one handle ------------------ */
 /* No counterpart in the reference (one MFiX run is one system); for

@@ -1088,9 +1088,8 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
 // per row) and finally put in the reference's order by a rank sort.  A row
 // longer than 64 raises the overflow flag (the host then reruns the
 // thread-per-particle kernel).
-constexpr int kRowCap = 192;  // candidates staged in LDS per wavefront
-
-template <bool BATCH = false>
+// kRowCap = candidates staged in LDS per wavefront (a tuning choice)
+template <bool BATCH = false, int kRowCap = 192>
 __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G0, BuildPtrs b, int ncells,
                                                        BatchDev B) {
   __shared__ double s_x[kBS / 64][kRowCap], s_y[kBS / 64][kRowCap], s_z[kBS / 64][kRowCap],

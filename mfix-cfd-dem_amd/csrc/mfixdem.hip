@@ -51,6 +51,7 @@ struct mfixdem_handle {
   BatchDev B{};
   int key_bits = 0;                       // significant bits of the rebuild sort key
   std::vector<Setup> sys_setup;           // per system (ensembles)
+  std::vector<GridDev> sys_grid;          // per-system grids (host copy of d_gtab)
   std::vector<int> sys_off;               // first slot of each system (+ total)
   std::vector<int> sys_n;                 // particles set per system
   std::vector<int> rogue_init;            // {INT_MAX, 0} per system
@@ -83,6 +84,9 @@ struct mfixdem_handle {
       *tot_cnt = nullptr;
   uint64_t* own_key = nullptr;
   int maxrow = 48;
+  // tuning (mfixdem_set_tuning): search-cell size / cut-off, row kernel
+  double cell_scale = 1.0;
+  int row_kernel = MFIXDEM_ROWS_WAVE192;
   int *nbr_start[2]{}, *nbr_split[2]{}, *pair_base[2]{};
   int *nbr[2]{}, *nbr_pid[2]{};
   int64_t nbr_cap[2]{0, 0};
@@ -317,6 +321,29 @@ int32_t do_bin(mfixdem_handle* h) {
   return MFIXDEM_OK;
 }
 
+template <bool BATCH>
+void launch_rows_wave_t(mfixdem_handle* h, const BuildPtrs& b, int64_t threads) {
+  const dim3 g(nblk(threads)), blk(kBS);
+  const int S = h->stride;
+  switch (h->row_kernel) {
+    case MFIXDEM_ROWS_WAVE128:
+      hipLaunchKernelGGL((k_own_rows_wave<BATCH, 128>), g, blk, 0, h->stream, S, h->G, b,
+                         h->n_fine, h->B);
+      break;
+    case MFIXDEM_ROWS_WAVE256:
+      hipLaunchKernelGGL((k_own_rows_wave<BATCH, 256>), g, blk, 0, h->stream, S, h->G, b,
+                         h->n_fine, h->B);
+      break;
+    default:
+      hipLaunchKernelGGL((k_own_rows_wave<BATCH, 192>), g, blk, 0, h->stream, S, h->G, b,
+                         h->n_fine, h->B);
+  }
+}
+void launch_rows_wave(mfixdem_handle* h, const BuildPtrs& b, int64_t threads) {
+  if (h->nsys > 1) launch_rows_wave_t<true>(h, b, threads);
+  else launch_rows_wave_t<false>(h, b, threads);
+}
+
 // NEIGHBOUR (neighbour.f:21-126) on the binned order.
 int32_t do_neighbour(mfixdem_handle* h) {
   if (!h->binned) CHECKRC(do_bin(h));
@@ -326,7 +353,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
   b.own_cnt = h->own_cnt; b.flags = h->flags;
   int fl[kFlagCount];
-  bool wave_rows = true;
+  bool wave_rows = h->row_kernel != MFIXDEM_ROWS_THREAD;
   // neighbour.f:60-66: NSQUARE for DES_NEIGHBOR_SEARCH = 1 (no rogue handling
   // there: that lives in DESGRID_NEIGH_BUILD)
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
@@ -339,12 +366,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
       // particles outside every cell range (state 0) get no row
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       const int64_t threads = (int64_t)h->n_fine * 64;
-      if (h->nsys > 1)
-        hipLaunchKernelGGL((k_own_rows_wave<true>), dim3(nblk(threads)), dim3(kBS), 0, h->stream,
-                           S, h->G, b, h->n_fine, h->B);
-      else
-        hipLaunchKernelGGL((k_own_rows_wave<false>), dim3(nblk(threads)), dim3(kBS), 0,
-                           h->stream, S, h->G, b, h->n_fine, h->B);
+      launch_rows_wave(h, b, threads);
     } else {
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
@@ -550,7 +572,7 @@ int32_t plain_step_body(mfixdem_handle* h) {
 int32_t one_step(mfixdem_handle* h, int32_t nn) {
   const bool rebuild = (nn == 1) || (nn % h->S.p.neighbor_search_n == 0);
   const bool use_graph = !rebuild && !h->first_pass && !timing(h) &&
-                         !(h->debug & MFIXDEM_DEBUG_STORE_FORCES);
+                         !(h->debug & (MFIXDEM_DEBUG_STORE_FORCES | MFIXDEM_DEBUG_NO_GRAPH));
   if (use_graph) {
     hipGraphExec_t& g = h->graph[h->cur][h->fb];
     if (!g) {
@@ -684,6 +706,30 @@ int32_t upload_ensemble(mfixdem_handle* h) {
   h->first_pass = true;
   h->binned = false;
   return MFIXDEM_OK;
+}
+
+// Calls f(ptr) for every device buffer a handle owns (the single list that
+// destruction and the footprint query share).
+template <class F>
+void for_each_buffer(mfixdem_handle* h, F f) {
+  for (int b = 0; b < 2; b++) {
+    f(h->pos[b]); f(h->vel[b]); f(h->omega[b]); f(h->radius[b]);
+    f(h->pmass[b]); f(h->omoi[b]); f(h->pvol[b]); f(h->acc[b]);
+    f(h->racc[b]); f(h->gid[b]); f(h->fcell[b]); f(h->dgc[b]);
+    f(h->state[b]); f(h->phase[b]); f(h->wmask[b]); f(h->wpft[b]);
+    f(h->keys[b]); f(h->vals[b]); f(h->nbr_start[b]); f(h->nbr_split[b]);
+    f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]);
+    f(h->rops_fix[b]);
+  }
+  f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->dep_list);
+  f(h->fstart); f(h->fend);
+  f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
+  f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->xe); f(h->yn);
+  f(h->zt); f(h->vnode); f(h->ugc); f(h->p_force); f(h->ep_g); f(h->rop_s);
+  f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
+  f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
+  f(h->cell_facets); f(h->stage);
+  f(h->d_sys); f(h->d_ptab); f(h->d_gtab); f(h->d_cell_off); f(h->d_dg_off); f(h->d_rogue);
 }
 
 void fill_devparams(mfixdem_handle* h) { fill_params(h->S, h->n, h->stride, h->P, h->G); }
@@ -843,23 +889,7 @@ int32_t mfixdem_destroy(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   invalidate_graphs(h);
-  for (int b = 0; b < 2; b++) {
-    dfree(h->pos[b]); dfree(h->vel[b]); dfree(h->omega[b]); dfree(h->radius[b]);
-    dfree(h->pmass[b]); dfree(h->omoi[b]); dfree(h->pvol[b]); dfree(h->acc[b]);
-    dfree(h->racc[b]); dfree(h->gid[b]); dfree(h->fcell[b]); dfree(h->dgc[b]);
-    dfree(h->state[b]); dfree(h->phase[b]); dfree(h->wmask[b]); dfree(h->wpft[b]);
-    dfree(h->keys[b]); dfree(h->vals[b]); dfree(h->nbr_start[b]); dfree(h->nbr_split[b]);
-    dfree(h->pair_base[b]); dfree(h->nbr[b]); dfree(h->nbr_pid[b]); dfree(h->pft[b]);
-  }
-  dfree(h->fc); dfree(h->tow); dfree(h->fc_id); dfree(h->tow_id); dfree(h->sort_tmp); dfree(h->dep_list); dfree(h->fstart); dfree(h->fend);
-  dfree(h->own_cnt); dfree(h->own_tmp); dfree(h->own_key); dfree(h->in_cnt); dfree(h->in_cur);
-  dfree(h->tot_cnt); dfree(h->scan_bsum); dfree(h->flags); dfree(h->xe); dfree(h->yn);
-  dfree(h->zt); dfree(h->vnode); dfree(h->ugc); dfree(h->p_force); dfree(h->ep_g); dfree(h->rop_s);
-  dfree(h->rops_node); dfree(h->vol); dfree(h->vol_surr); dfree(h->ro_s); dfree(h->fluid);
-  dfree(h->pic_cnt); dfree(h->pic_off); dfree(h->pic_cur); dfree(h->pic);
-  dfree(h->cell_facets); dfree(h->stage); dfree(h->rops_fix[0]); dfree(h->rops_fix[1]);
-  dfree(h->d_sys); dfree(h->d_ptab); dfree(h->d_gtab); dfree(h->d_cell_off); dfree(h->d_dg_off);
-  dfree(h->d_rogue);
+  for_each_buffer(h, [](auto*& p) { dfree(p); });
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1397,6 +1427,7 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
   h->B.rogue = h->d_rogue;
   h->B.nsys = nsys;
   h->B.sys_shift = 30 + gid_bits;
+  h->sys_grid = gtab;
   const size_t S = h->stride;
   h->st_pos.assign(3 * S, 0.0);
   h->st_vel.assign(3 * S, 0.0);
@@ -1460,6 +1491,62 @@ int32_t mfixdem_ensemble_get_particles(mfixdem_handle* h, int32_t s, double* pos
   if (state)
     for (int q = 0; q < cap; q++) state[g[q]] = st[q];
   return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_tuning(mfixdem_handle* h, double cell_scale, int32_t row_kernel) {
+  if (!h || !(cell_scale >= 1.0) || cell_scale > 4.0 || row_kernel < MFIXDEM_ROWS_THREAD ||
+      row_kernel > MFIXDEM_ROWS_WAVE256)
+    return MFIXDEM_ERR_ARG;
+  HIPOK(hipStreamSynchronize(h->stream));
+  int n_fine = 0;
+  if (h->nsys == 1) {
+    GridDev G = h->G;
+    if (search_grid(h->S.p, h->n, G, n_fine, cell_scale) != MFIXDEM_OK)
+      return MFIXDEM_ERR_UNSUPPORTED;
+    G.gid_bits = h->G.gid_bits;
+    h->G = G;
+  } else {
+    std::vector<GridDev> gt = h->sys_grid;
+    std::vector<int> off(h->nsys + 1, 0);
+    for (int s = 0; s < h->nsys; s++) {
+      int nf = 0;
+      const int gb = gt[s].gid_bits;
+      if (search_grid(h->sys_setup[s].p, h->sys_setup[s].p.max_pip, gt[s], nf, cell_scale) !=
+          MFIXDEM_OK)
+        return MFIXDEM_ERR_UNSUPPORTED;
+      gt[s].gid_bits = gb;
+      off[s + 1] = off[s] + nf;
+    }
+    n_fine = off[h->nsys];
+    HIPOK(hipMemcpy(h->d_gtab, gt.data(), sizeof(GridDev) * h->nsys, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->d_cell_off, off.data(), sizeof(int) * (h->nsys + 1),
+                    hipMemcpyHostToDevice));
+    h->sys_grid = gt;
+    h->G = gt[0];
+  }
+  dfree(h->fstart);
+  dfree(h->fend);
+  CHECKRC(dalloc(&h->fstart, n_fine));
+  CHECKRC(dalloc(&h->fend, n_fine));
+  h->n_fine = n_fine;
+  h->cell_scale = cell_scale;
+  h->row_kernel = row_kernel;
+  h->binned = false;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_device_bytes(mfixdem_handle* h, int64_t* bytes) {
+  if (!h || !bytes) return MFIXDEM_ERR_ARG;
+  int64_t total = 0;
+  int32_t rc = MFIXDEM_OK;
+  for_each_buffer(h, [&](auto*& p) {
+    if (!p) return;
+    size_t sz = 0;
+    if (hipMemPtrGetInfo((void*)p, &sz) != hipSuccess) rc = MFIXDEM_ERR_HIP;
+    total += (int64_t)sz;
+  });
+  *bytes = total;
+  return rc;
 }
 
 }  // extern "C"

@@ -66,11 +66,12 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
 }
 
 
-// Neighbour-search grid of one system: cubic cells >= the largest cut-off
-// FACTOR_RLM*(2 r_max), one cell of margin around the box; particle ids
-// take gid_bits bits of the sort key.
-inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& n_fine) {
-  const double hcut = p.factor_rlm * (2.0 * p.max_radius) * (1.0 + 1e-9);
+// Neighbour-search grid of one system: cubic cells of cell_scale (>= 1)
+// times the largest cut-off FACTOR_RLM*(2 r_max), one cell of margin around
+// the box; particle ids take gid_bits bits of the sort key.
+inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& n_fine,
+                           double cell_scale = 1.0) {
+  const double hcut = p.factor_rlm * (2.0 * p.max_radius) * (1.0 + 1e-9) * cell_scale;
   const double lo[3] = {p.x_min - hcut, p.y_min - hcut, p.z_min - hcut};
   const double len[3] = {p.xlength, p.ylength, p.zlength};
   for (int d = 0; d < 3; d++) {

@@ -26,7 +26,9 @@ NSEARCH_NSQUARE, NSEARCH_GRID = 1, 4
 DEBUG_STORE_FORCES = 1
 DEBUG_TIME_KERNELS = 2
 DEBUG_REFERENCE_DEPOSITION = 4
+DEBUG_NO_GRAPH = 8
 TIME_STEP, TIME_REBUILD, TIME_GAS, TIME_GROUPS = 0, 1, 2, 3
+ROWS_THREAD, ROWS_WAVE128, ROWS_WAVE192, ROWS_WAVE256 = 0, 1, 2, 3
 
 ERRORS = {
     0: "OK", 1: "ARG", 2: "HIP", 3: "OOM", 4: "ROGUE", 5: "EPG",
@@ -193,6 +195,8 @@ def _declare(lib):
         "mfixdem_get_stream": ([_P, C.POINTER(_P)], _I),
         "mfixdem_get_kernel_times": ([_P, _DP, C.POINTER(C.c_int64), _I], _I),
         "mfixdem_ensemble_create": ([_I, C.POINTER(Params), C.POINTER(_P)], _I),
+        "mfixdem_set_tuning": ([_P, C.c_double, _I], _I),
+        "mfixdem_get_device_bytes": ([_P, C.POINTER(C.c_int64)], _I),
         "mfixdem_ensemble_set_particles": ([_P, _I, C.c_int64, _DP, _DP, _DP, _DP, _DP, _IP,
                                             C.c_int64], _I),
         "mfixdem_ensemble_get_particles": ([_P, _I, _DP, _DP, _DP, _BP, C.c_int64], _I),

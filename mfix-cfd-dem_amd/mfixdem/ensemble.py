@@ -39,6 +39,15 @@ class Ensemble:
             abi.dptr(parts.radius), abi.dptr(parts.ro_sol), abi.iptr(parts.phase), n),
             "ensemble_set_particles")
 
+    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_WAVE192):
+        abi.check(self.lib.mfixdem_set_tuning(self.h, float(cell_scale), int(row_kernel)),
+                  "set_tuning")
+
+    def device_bytes(self) -> int:
+        b = C.c_int64(0)
+        abi.check(self.lib.mfixdem_get_device_bytes(self.h, C.byref(b)), "get_device_bytes")
+        return int(b.value)
+
     def init(self):
         abi.check(self.lib.mfixdem_init(self.h), "init")
 

@@ -60,6 +60,15 @@ class DemSystem:
             self.h, abi.dptr(gas.u_g), abi.dptr(gas.v_g), abi.dptr(gas.w_g),
             abi.dptr(gas.p_force)), "set_gas")
 
+    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_WAVE192):
+        abi.check(self.lib.mfixdem_set_tuning(self.h, float(cell_scale), int(row_kernel)),
+                  "set_tuning")
+
+    def device_bytes(self) -> int:
+        b = C.c_int64(0)
+        abi.check(self.lib.mfixdem_get_device_bytes(self.h, C.byref(b)), "get_device_bytes")
+        return int(b.value)
+
     def init(self):
         abi.check(self.lib.mfixdem_init(self.h), "init")
 

@@ -364,14 +364,51 @@ def ensemble_member(n, kn, en, mew, seed, phi=0.30):
     return p, make_particles(pos, r, 2500.0)
 
 
-def sweep_ensemble(nsys=256, seed=11, nmin=2000, nmax=200_000, phi=0.30):
+def sweep_ensemble(nsys=256, seed=11, nmin=2000, nmax=200_000, phi=0.30, member=None):
     """Ensemble of independent settling beds of uneven size: N log-uniform in
     [nmin, nmax], KN log-uniform in [1e2, 1e4], EN ~ U[0.7, 0.95],
-    MEW ~ U[0.1, 0.5] (seeded)."""
+    MEW ~ U[0.1, 0.5] (seeded).  member(n, kn, en, mew, seed) builds one
+    system (default: RSA packing, ensemble_member)."""
     rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys)).astype(int)
     kns = np.exp(rng.uniform(np.log(1e2), np.log(1e4), nsys))
     ens = rng.uniform(0.7, 0.95, nsys)
     mews = rng.uniform(0.1, 0.5, nsys)
-    return [ensemble_member(int(ns[s]), float(kns[s]), float(ens[s]), float(mews[s]),
-                            seed * 1000 + s, phi) for s in range(nsys)]
+    if member is None:
+        return [ensemble_member(int(ns[s]), float(kns[s]), float(ens[s]), float(mews[s]),
+                                seed * 1000 + s, phi) for s in range(nsys)]
+    return [member(int(ns[s]), float(kns[s]), float(ens[s]), float(mews[s]), seed * 1000 + s)
+            for s in range(nsys)]
+
+
+def lattice_member(n, kn, en, mew, seed, phi=0.30, precision=abi.MIXED):
+    """A sweep member whose spheres (d = 1 mm, rho = 2500) sit on a jittered
+    cubic lattice filling an X = Z, Y = 4X box at solids fraction phi (no
+    overlaps); O(n) to build, for request streams.  LSD, KN = KN_W = kn,
+    EN = EN_WALL = en, MEW = MEW_W = mew, DTSOLID = TCOLL/50."""
+    d = 1.0e-3
+    a = (np.pi / 6.0 * d ** 3 / phi) ** (1.0 / 3.0)   # lattice spacing
+    m = max(1, int(np.ceil((n / 4.0) ** (1.0 / 3.0))))
+    nx, ny, nz = m, int(np.ceil(n / (m * m))), m
+    lx, ly, lz = nx * a, max(ny * a, 4.0 * nx * a), nz * a
+    rng = np.random.default_rng(seed)
+    ii = np.arange(nx * ny * nz)[:n]
+    ix, iy, iz = ii % nx, (ii // nx) % ny, ii // (nx * ny)
+    jit = 0.5 * (a - d) * rng.uniform(-1.0, 1.0, (3, n))
+    pos = np.stack([(ix + 0.5) * a, (iy + 0.5) * a, (iz + 0.5) * a]) + jit
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.precision = precision
+    p.nphases = 1
+    p.max_pip = n
+    p.xlength, p.ylength, p.zlength = lx, ly, lz
+    p.imax = p.jmax = p.kmax = 1
+    p.grav[:] = [0.0, -9.81, 0.0]
+    p.kn = p.kn_w = kn
+    p.mew = p.mew_w = mew
+    p.des_en_input[0] = en
+    p.des_en_wall_input[0] = en
+    p.d_p0[0] = d
+    p.ro_s0[0] = 2500.0
+    p.max_radius = 0.5 * d
+    return p, make_particles(pos, 0.5 * d, 2500.0)

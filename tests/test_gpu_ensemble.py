@@ -72,3 +72,55 @@ def test_ensemble_rejects_mixed_launch_choices():
     ps[1].neighbor_search_n = 10
     with pytest.raises(abi.MfixDemError):
         Ensemble(ps)
+
+
+def test_tuning_choices_give_identical_results():
+    """Every (search-cell size, row kernel) choice gives the same state bit
+    for bit, for a single system and an ensemble."""
+    from mfixdem import autotune
+    p, parts, _ = workloads.settling_bed(n=2000, seed=3, box=(0.01, 0.035, 0.01))
+    p, parts = workloads.jostled(p, parts, seed=4)
+    ref = None
+    for c, r in autotune.candidates():
+        g = DemSystem(p)
+        g.set_particles(parts)
+        g.set_tuning(c, r)
+        g.init()
+        g.run(1, 55)
+        st = g.get_particles()
+        g.close()
+        if ref is None:
+            ref = st
+        else:
+            for a, b in zip(st, ref):
+                assert np.array_equal(a, b), (c, r)
+    ms = members(nsys=3, seed=13)
+    outs = []
+    for c, r in [(1.0, abi.ROWS_THREAD), (1.5, abi.ROWS_WAVE256)]:
+        e = Ensemble([q for q, _ in ms])
+        for s, (_, pts) in enumerate(ms):
+            e.set_particles(s, pts)
+        e.set_tuning(c, r)
+        e.init()
+        e.run(1, 55)
+        outs.append([e.get_particles(s) for s in range(len(ms))])
+        e.close()
+    for s in range(len(ms)):
+        for a, b in zip(outs[0][s], outs[1][s]):
+            assert np.array_equal(a, b)
+
+
+def test_autotuner_picks_a_candidate():
+    from mfixdem import autotune
+    ms = members(nsys=3, seed=17)
+
+    def make():
+        e = Ensemble([q for q, _ in ms])
+        for s, (_, pts) in enumerate(ms):
+            e.set_particles(s, pts)
+        return e
+
+    best, timings = autotune.autotune(make, steps=30,
+                                      cands=[(1.0, abi.ROWS_WAVE192), (1.25, abi.ROWS_WAVE128)])
+    assert best in [(1.0, abi.ROWS_WAVE192), (1.25, abi.ROWS_WAVE128)]
+    assert len(timings) == 2 and all(t > 0 for t in timings.values())

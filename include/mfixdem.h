@@ -295,7 +295,9 @@ int32_t mfixdem_ensemble_get_particles(mfixdem_handle* h, int32_t s, double* pos
    work against it. */
 int32_t mfixdem_get_stream(mfixdem_handle* h, void** stream);
 /* Accumulated milliseconds and call counts per kernel group (needs
-   MFIXDEM_DEBUG_TIME_KERNELS); n = number of groups requested. */
+   MFIXDEM_DEBUG_TIME_KERNELS); n = number of groups requested.  Each group
+   launch is bracketed by HIP events on the handle's stream without a host
+   wait; the intervals are read back here (this call waits for the stream). */
 int32_t mfixdem_get_kernel_times(mfixdem_handle* h, double* ms, int64_t* calls, int32_t n);
 int32_t mfixdem_reset_kernel_times(mfixdem_handle* h);
 

@@ -52,6 +52,18 @@ enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFla
 
 enum { kModeFused = 0, kModeForce = 1, kModeDrag = 2, kModeIntegrate = 3 };
 
+// XCD-aware tile order.  Workgroups are dealt to the 8 XCDs of the chip
+// round-robin (workgroup b runs on XCD b % 8), and each XCD has its own L2.
+// Particles are stored in spatial (Morton) order, so neighbouring tiles
+// share neighbour and gas-node data; mapping the workgroups of one XCD to a
+// contiguous range of tiles keeps that data in one L2 instead of eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Ensembles: many independent systems in one set of particle arrays.  The
 // slots of system s are a fixed range; each slot carries its system id, and
 // parameters, grids, search-grid cell offsets, DES-cell offsets and rogue
@@ -311,7 +323,7 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
 // GARG_2012), so that each kernel carries only one drag path.
 template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
   if (i >= P0.n) return;
   const int S = P0.stride;
   const DevParams& P = sys_params<BATCH>(P0, B, i);
@@ -543,7 +555,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
   __shared__ unsigned long long tile[kDepCap];
   __shared__ int box[6];
   const int S = P.stride;
-  const int k0 = (blockIdx.x * kBS + threadIdx.x) * kDepPPT;
+  const int k0 = (xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x) * kDepPPT;
   int pi[kDepPPT], pcell[kDepPPT];
   int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
 #pragma unroll
@@ -1097,7 +1109,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G0, BuildP
   __shared__ int s_j[kBS / 64][kRowCap], s_g[kBS / 64][kRowCap], s_d[kBS / 64][kRowCap];
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int c = (int)((blockIdx.x * (unsigned)kBS + threadIdx.x) >> 6);
+  const int c = (int)(((unsigned)xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x) >> 6);
   if (c >= ncells) return;                       // whole wavefront leaves together
   const int h0 = b.fstart[c], h1 = b.fend[c];
   if (h1 <= h0) return;

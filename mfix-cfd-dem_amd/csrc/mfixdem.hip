@@ -13,6 +13,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <vector>
 
 #include "../../include/mfixdem.h"
@@ -115,7 +116,11 @@ struct mfixdem_handle {
   // graphs: [cur parity][deposition buffer]
   hipGraphExec_t graph[2][2]{};
   // kernel timing
-  hipEvent_t ev[2]{nullptr, nullptr};
+  // kernel timing: a pool of events recorded around each timed launch and
+  // read back in one go, so launches queue back to back while timed
+  std::vector<hipEvent_t> ev;
+  int ev_next = 0, ev_open = -1;
+  std::vector<std::array<int, 3>> t_pending;   // {slot, begin event, end event}
   double t_ms[kTimeSlots]{};
   int64_t t_calls[kTimeSlots]{};
 };
@@ -194,17 +199,43 @@ bool ref_deposition(const mfixdem_handle* h) {
 }
 
 bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS) != 0; }
+constexpr int kTimeEvents = 2048;
+
+// Waits for the recorded timing events and adds their intervals to t_ms.
+void t_flush(mfixdem_handle* h) {
+  if (!h->t_pending.empty()) {
+    (void)hipEventSynchronize(h->ev[h->t_pending.back()[2]]);
+    for (const auto& t : h->t_pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, h->ev[t[1]], h->ev[t[2]]) == hipSuccess) {
+        h->t_ms[t[0]] += ms;
+        h->t_calls[t[0]] += 1;
+      }
+    }
+    h->t_pending.clear();
+  }
+  h->ev_next = 0;
+  h->ev_open = -1;
+}
 void t_begin(mfixdem_handle* h) {
-  if (timing(h)) (void)hipEventRecord(h->ev[0], h->stream);
+  if (!timing(h)) return;
+  if (h->ev.empty()) {
+    h->ev.assign(kTimeEvents, nullptr);
+    for (auto& e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  }
+  if (h->ev_next + 2 > kTimeEvents) t_flush(h);
+  h->ev_open = h->ev_next++;
+  if (h->ev[h->ev_open]) (void)hipEventRecord(h->ev[h->ev_open], h->stream);
 }
 void t_end(mfixdem_handle* h, int slot) {
-  if (!timing(h)) return;
-  (void)hipEventRecord(h->ev[1], h->stream);
-  (void)hipEventSynchronize(h->ev[1]);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
-  h->t_ms[slot] += ms;
-  h->t_calls[slot] += 1;
+  if (!timing(h) || h->ev_open < 0) return;
+  const int b = h->ev_next++;
+  if (h->ev[h->ev_open] && h->ev[b]) {
+    (void)hipEventRecord(h->ev[b], h->stream);
+    h->t_pending.push_back({slot, h->ev_open, b});
+  }
+  h->ev_open = -1;
 }
 
 template <int MODE>
@@ -742,8 +773,6 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
   int32_t rc = MFIXDEM_OK;
   auto fail = [&](int32_t r) { return r; };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(MFIXDEM_ERR_HIP);
-  if (hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
   const int nf = std::max(1, h->P.nf);
   for (int b = 0; b < 2; b++) {
@@ -1305,6 +1334,7 @@ int32_t mfixdem_get_stream(mfixdem_handle* h, void** stream) {
 
 int32_t mfixdem_get_kernel_times(mfixdem_handle* h, double* ms, int64_t* calls, int32_t n) {
   if (!h || n < 0) return MFIXDEM_ERR_ARG;
+  t_flush(h);
   for (int i = 0; i < n && i < kTimeSlots; i++) {
     if (ms) ms[i] = h->t_ms[i];
     if (calls) calls[i] = h->t_calls[i];
@@ -1314,6 +1344,7 @@ int32_t mfixdem_get_kernel_times(mfixdem_handle* h, double* ms, int64_t* calls, 
 
 int32_t mfixdem_reset_kernel_times(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
+  t_flush(h);
   for (int i = 0; i < kTimeSlots; i++) {
     h->t_ms[i] = 0.0;
     h->t_calls[i] = 0;

@@ -296,6 +296,59 @@ def dem04(mew=0.2, integrator=abi.ADAMS_BASHFORTH):
     return p, parts, None
 
 
+def dem06(scheme=abi.INTERP_NONE, width=None, u_gas=0.4, fluid_dt=1.0e-3):
+    """tests/dem/DEM06: one sphere (d = 0.1 mm, rho = 2000) released at rest
+    at (0.005, 0.0925, 0.005) in a 0.01 x 0.10 x 0.01 m column of 5 x 20 x 5
+    cells with a frozen upflow V_G = 0.4 m/s (rho_g = 1.2, mu_g = 1.8e-5),
+    one-way coupled; KN = KN_W = 0.1, EN = 1, no friction, Euler.  The
+    reference case uses a user drag law equal to Wen-Yu at EP_G = 1; here
+    the drag is WEN_YU at the cell voidage the deposition gives.  The gas
+    pressure is hydrostatic, so P_FORCE carries the buoyancy.
+
+    Returns (params, particles, gas, p_g, substeps): DTSOLID is the
+    reference's TCOLL/50 rounded so that `substeps` solid steps make one
+    fluid step of `fluid_dt` (des_time_march.f:120-126)."""
+    p = abi.default_params()
+    p.coll_model = abi.LSD
+    p.integrator = abi.EULER
+    p.nphases = 1
+    p.max_pip = 1
+    p.coupled = 1
+    p.drag_type = abi.DRAG_WEN_YU
+    p.interp_scheme = scheme
+    if scheme == abi.INTERP_NONE:
+        p.des_interp_on = p.des_interp_mean_fields = 0
+    else:
+        p.des_interp_on = p.des_interp_mean_fields = 1
+    p.des_interp_width = abi.UNDEFINED if width is None else width
+    p.xlength, p.ylength, p.zlength = 0.01, 0.10, 0.01
+    p.imax, p.jmax, p.kmax = 5, 20, 5
+    p.desgridsearch_imax, p.desgridsearch_jmax, p.desgridsearch_kmax = 5, 20, 5
+    p.kn = p.kn_w = 0.1
+    p.mew = p.mew_w = 0.0
+    p.des_en_input[0] = 1.0
+    p.des_en_wall_input[0] = 1.0
+    p.d_p0[0] = 1.0e-4
+    p.ro_s0[0] = 2000.0
+    p.max_radius = 5.0e-5
+    p.ro_g0 = 1.2
+    p.mu_g0 = 1.8e-5
+    # TCOLL (check_solids_dem.f:676-694 with e = 1): pi sqrt(m_eff / kn)
+    m = 2000.0 * np.pi / 6.0 * 1.0e-12
+    tcoll = min(np.pi * np.sqrt(0.5 * m / 0.1), np.pi * np.sqrt(m / 0.1))
+    substeps = int(np.ceil(np.float32(fluid_dt / (tcoll / 50.0))))
+    p.dtsolid = fluid_dt / substeps
+    parts = make_particles(np.array([[0.005], [0.0925], [0.005]]), 5.0e-5, 2000.0)
+    nc = n_cells(p)
+    gas = Gas(np.zeros(nc), np.full(nc, u_gas), np.zeros(nc), np.zeros((nc, 3)))
+    ni, nj, nk = p.imax + 2, p.jmax + 2, p.kmax + 2
+    dy = p.ylength / p.jmax
+    j = np.tile(np.arange(1, nj + 1), ni * nk)      # J fastest (MFiX cell order)
+    y = (j - 1.5) * dy
+    p_g = p.ro_g0 * -p.grav[1] * (p.ylength - y)
+    return p, parts, gas, np.ascontiguousarray(p_g), substeps
+
+
 def dem05(rows, model="LSD", integrator=abi.ADAMS_BASHFORTH):
     """tests/dem/DEM05: oblique impacts, no gravity.  `rows` are the 93
     lines of its particle_input.dat (x y z r rho u v w); rho 4000 is phase

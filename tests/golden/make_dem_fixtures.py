@@ -11,6 +11,9 @@ code) and keeps, per run, the solver columns the tests compare:
   DEM05  tests/dem/DEM05/particle_input.dat (the 93 particles) and
          AUTOTEST/POST_ALPHA.dat, POST_COEFF.dat, POST_OMEGA.dat
          (LSD, HERTZ) -> rows of (impact angle, part-part, part-wall)
+  DEM06  tests/dem/DEM06/AUTOTEST/POST_POS.dat, POST_VEL.dat: the three
+         one-way coupled runs (frozen gas; no interpolation, GARG_2012,
+         SQUARE_DPVM width 2 mm) -> rows of (time, y_mfix, v_mfix)
 Run from the repository root when /root/reference is mounted:
   python tests/golden/make_dem_fixtures.py
 """
@@ -131,7 +134,44 @@ def dem05():
     print("DEM05:", len(parts), "particles,", [(r["model"], len(r["alpha"])) for r in runs])
 
 
+def dem06_blocks(path):
+    """Per run: the rows (time, analytic, MFIX, %diff) of one AUTOTEST block."""
+    out, cur = [], None
+    for line in open(path):
+        if "Time" in line and "MFIX" in line:
+            cur = []
+            out.append(cur)
+            continue
+        parts = line.split()
+        if cur is not None and len(parts) == 4:
+            try:
+                cur.append([float(x) for x in parts])
+            except ValueError:
+                pass
+    return out
+
+
+def dem06():
+    d = os.path.join(REF, "DEM06", "AUTOTEST")
+    pos = dem06_blocks(os.path.join(d, "POST_POS.dat"))
+    vel = dem06_blocks(os.path.join(d, "POST_VEL.dat"))
+    # runtest order: the first three runs are DES_ONEWAY_COUPLED
+    runs = []
+    for (scheme, width), bp, bv in zip([("NONE", None), ("GARG_2012", None),
+                                        ("SQUARE_DPVM", 2.0e-3)], pos[:3], vel[:3]):
+        assert len(bp) == len(bv)
+        rows = [[a[0], a[2], b[2]] for a, b in zip(bp, bv)]
+        runs.append({"scheme": scheme, "width": width, "rows": rows})
+    with open(os.path.join(HERE, "dem06_post.json"), "w") as f:
+        json.dump({"source": "tests/dem/DEM06/AUTOTEST/POST_POS.dat, POST_VEL.dat "
+                             "(runs 1-3, DES_ONEWAY_COUPLED)",
+                   "columns": ["time", "y_mfix", "vy_mfix"], "runs": runs}, f,
+                  separators=(",", ":"))
+    print("DEM06:", len(runs), "runs,", sum(len(r["rows"]) for r in runs), "rows")
+
+
 def main():
+    dem06()
     dem03()
     dem05()
     dem04()

@@ -456,3 +456,29 @@ def test_pressure_gradient_and_coupled_run_bit_exact():
     gp, op = g.get_particles()[0], o.get_particles()[0]
     assert np.max(np.abs(gp - op)) <= 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("name", ["NONE", "GARG_2012", "SQUARE_DPVM"])
+def test_dem06_on_gpu_terminal_velocity(name):
+    """Single sphere settling against a frozen upflow (tests/dem/DEM06, the
+    three one-way coupled runs): the GPU follows the oracle (pow in the drag
+    law: 1e-12 relative on y and v at every printed time) and the
+    reference's printed MFIX columns to the bounds the oracle meets (3e-5
+    m/s, 1e-5 m; the reference's drag ignores the cell voidage)."""
+    from test_oracle_dem06 import SCHEMES, golden, trajectory
+    scheme, width = SCHEMES[name]
+    p, parts, gas, p_g, sub = workloads.dem06(scheme, width)
+    g, o = DemSystem(p), OracleSystem(p)
+    for s in (g, o):
+        s.set_particles(parts)
+        s.set_gas(gas)
+        s.set_gas_pressure(p_g)
+        s.init()
+    tg = trajectory(g, sub, 350)
+    to = trajectory(o, sub, 350)
+    g.close()
+    o.close()
+    assert np.max(np.abs(tg[:, 1:] - to[:, 1:]) / np.abs(to[:, 1:])) <= 1e-12
+    rows = np.array(next(r for r in golden() if r["scheme"] == name)["rows"])
+    assert np.max(np.abs(tg[:, 2] - rows[1:, 2])) <= 3.0e-5
+    assert np.max(np.abs(tg[:, 1] - rows[1:, 1])) <= 1.0e-5

@@ -108,7 +108,7 @@ def force_error(p, parts, gas):
 
 
 def ensemble_bench(nsys, steps, log):
-    """Config C4: nsys independent settling beds of uneven size (N log-uniform
+    """Parameter-sweep ensemble: nsys independent settling beds of uneven size (N log-uniform
     in [2k, 200k], d = 1 mm), KN / EN / MEW swept, DTSOLID = TCOLL/50 each, in
     one ensemble handle; the result-invariant tuning is chosen by the
     autotuner on a 32-member sample first."""
@@ -154,7 +154,7 @@ def ensemble_bench(nsys, steps, log):
 
 
 def serving_bench(n_req, log):
-    """Config C5, scaled to a bounded run: Poisson arrivals of parameter-sweep
+    """Sweep-request serving, scaled to a bounded run: Poisson arrivals of parameter-sweep
     requests (N log-uniform in [5k, 200k], steps uniform in [200, 2000],
     mixed precision), admitted by HBM footprint within a 16 GiB budget.  The
     arrival rate offers 70 % of the throughput measured for the same kind of
@@ -184,7 +184,7 @@ def serving_bench(n_req, log):
                 "sequential_capacity_particle_steps_per_s": cap,
                 "hbm_budget_bytes": 16 << 30, "max_concurrent": srv.max_running,
                 "max_bytes_in_use": srv.max_in_use,
-                "workload": "scaled config C5: N log-uniform [5k, 200k], steps U[200, 2000], "
+                "workload": "sweep requests: N log-uniform [5k, 200k], steps U[200, 2000], "
                             "mixed precision, KN/EN/MEW swept, Poisson arrivals"})
     log(f"serving: p50 {rep['p50_latency_s']:.3f} s, p99 {rep['p99_latency_s']:.3f} s, "
         f"{rep['particle_steps_per_s']:.3e} particle-steps/s at rate {rate:.2f}/s")

@@ -317,17 +317,207 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
   for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
 }
 
+// One pair entry of a row (calc_force_dem.f:210-412): particle ip is L of
+// the pair (owner, own row) or I (incoming row); partner j, pair id pid.
+// Writes the pair's new history when ip owns it and returns 1 with the
+// increments of ip's force and torque in d[0..5] when the pair is in
+// contact; returns 0 when it adds nothing.
+template <bool MIXED, bool BATCH>
+__device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a,
+                                          const BatchDev& B, int ip, int j, int pid, bool owner,
+                                          double d[6]) {
+  const int S = P0.stride;
+  const DevParams& P = sys_params<BATCH>(P0, B, ip);
+  const uint8_t stj = a.state[j];
+  double xj[3], xp[3];
+  load3(a.pos, S, j, xj);
+  load3(a.pos, S, ip, xp);
+  const double rj = a.radius[j], rp = a.radius[ip];
+  if (stj == 0) {  // IF(IS_NONEXISTENT(I)) CYCLE: history kept
+    if (owner)
+      for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+    return 0;
+  }
+  // contact test in the reference's orientation: |x_I - x_L|^2, r_L + r_I
+  const double r_lm = owner ? rp + rj : rj + rp;
+  const double d0 = owner ? xj[0] - xp[0] : xp[0] - xj[0];
+  const double d1 = owner ? xj[1] - xp[1] : xp[1] - xj[1];
+  const double d2 = owner ? xj[2] - xp[2] : xp[2] - xj[2];
+  const double dm = d0 * d0 + d1 * d1 + d2 * d2;
+  const double rs = r_lm - kSmall;
+  if (dm > rs * rs) {
+    if (owner) {
+      a.pft_out[3 * pid] = 0.0;
+      a.pft_out[3 * pid + 1] = 0.0;
+      a.pft_out[3 * pid + 2] = 0.0;
+    }
+    return 0;
+  }
+  PairIn in;
+  double vj[3], wj[3], vp[3], wp[3];
+  load3(a.vel, S, j, vj);
+  load3(a.omega, S, j, wj);
+  load3(a.vel, S, ip, vp);
+  load3(a.omega, S, ip, wp);
+  const double mj = a.pmass[j], mp = a.pmass[ip];
+  const int phj = a.phase[j], php = a.phase[ip];
+  for (int c = 0; c < 3; c++) {
+    in.xL[c] = owner ? xp[c] : xj[c];
+    in.xI[c] = owner ? xj[c] : xp[c];
+    in.vL[c] = owner ? vp[c] : vj[c];
+    in.vI[c] = owner ? vj[c] : vp[c];
+    in.wL[c] = owner ? wp[c] : wj[c];
+    in.wI[c] = owner ? wj[c] : wp[c];
+  }
+  in.rL = owner ? rp : rj;
+  in.rI = owner ? rj : rp;
+  in.mL = owner ? mp : mj;
+  in.mI = owner ? mj : mp;
+  in.phL = owner ? php : phj;
+  in.phI = owner ? phj : php;
+  const double hin[3] = {a.pft_in[3 * pid], a.pft_in[3 * pid + 1], a.pft_in[3 * pid + 2]};
+  PairOut o;
+  const int r = MIXED ? pair_force_mixed(P, in, hin, o) : pair_force_fp64(P, in, hin, o);
+  if (r == 2) {
+    atomicAdd(&a.flags[kFlagZeroDist], 1);
+    if (owner)
+      for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = hin[c];
+    return 0;
+  }
+  if (owner) {
+    for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = o.pft[c];
+    if (P0.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
+    for (int c = 0; c < 3; c++) d[c] = o.fct[c];
+    for (int c = 0; c < 3; c++) d[3 + c] = o.tl[c];
+  } else {  // FC(I) = FC(I) - FC_pair: adding the negation rounds the same
+    for (int c = 0; c < 3; c++) d[c] = -o.fct[c];
+    for (int c = 0; c < 3; c++) d[3 + c] = o.ti[c];
+  }
+  return 1;
+}
+
+// The pair part of CALC_FORCE_DEM for the 64 particles of a wavefront.
+// Their rows ([incoming | own], consecutive particles) are one contiguous
+// run of entries; the lanes take the entries 64 at a time (one pair entry
+// per lane, so a long row does not hold the other lanes back), put each
+// entry's force and torque increment in LDS, and then every lane adds the
+// increments of its own row in row order -- the reference's summation
+// order, so the sums are bit-identical to a per-particle loop.  Rows of a
+// nonexistent particle only carry their history over.  Every lane of the
+// wavefront must call this (lanes past n with live = false).
+template <bool MIXED, bool BATCH>
+__device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& a,
+                                           const BatchDev& B, int i, bool live, int st,
+                                           double fc[3], double tow[3]) {
+  __shared__ int s_row[kBS / 64][65];
+  __shared__ int s_split[kBS / 64][64];
+  __shared__ int s_st[kBS / 64][64];
+  __shared__ int s_hit[kBS / 64][64];
+  __shared__ double s_d[kBS / 64][6][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = P0.n;
+  const int r0 = a.nbr_start[live ? i : n];
+  s_row[w][lane] = r0;
+  s_split[w][lane] = live ? a.nbr_split[i] : r0;
+  s_st[w][lane] = st;
+  if (lane == 63) s_row[w][64] = a.nbr_start[live ? i + 1 : n];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int e_first = s_row[w][0], e_last = s_row[w][64];
+  const int my1 = s_row[w][lane + 1];
+  const int wbase = i - lane;
+  for (int base = e_first; base < e_last; base += 64) {
+    const int f = base + lane;
+    int hit = 0;
+    double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (f < e_last) {
+      // the lane whose row holds entry f: the last p with s_row[p] <= f
+      int p = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (s_row[w][p + step] <= f) p += step;
+      const bool owner = f >= s_split[w][p];
+      const int pid = a.nbr_pid[f];
+      if (s_st[w][p] == 0) {
+        if (owner)  // rows of a removed particle keep their history
+          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+      } else {
+        hit = pair_entry<MIXED, BATCH>(P0, a, B, wbase + p, a.nbr[f], pid, owner, d);
+      }
+    }
+    s_hit[w][lane] = hit;
+#pragma unroll
+    for (int k = 0; k < 6; k++) s_d[w][k][lane] = d[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int lo = r0 > base ? r0 : base, hi = my1 < base + 64 ? my1 : base + 64;
+    for (int g = lo; g < hi; g++) {
+      const int t = g - base;
+      if (s_hit[w][t]) {
+        for (int c = 0; c < 3; c++) fc[c] = fc[c] + s_d[w][c][t];
+        for (int c = 0; c < 3; c++) tow[c] = tow[c] + s_d[w][3 + c][t];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
 // unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES);
 // FILTER selects the DRAG_GS_DES1 drag (any interpolation scheme other than
 // GARG_2012), so that each kernel carries only one drag path.
 template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
+  constexpr bool kPairs = MODE == kModeFused || MODE == kModeForce;
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
-  if (i >= P0.n) return;
+  const bool live = i < P0.n;
+  if (!kPairs && !live) return;  // the pair modes keep every lane for the wave passes
   const int S = P0.stride;
-  const DevParams& P = sys_params<BATCH>(P0, B, i);
-  const int st = a.state[i];
+  const DevParams& P = sys_params<BATCH>(P0, B, live ? i : 0);
+  const int st = live ? a.state[i] : 0;
+  double fc[3] = {0.0, 0.0, 0.0}, tow[3] = {0.0, 0.0, 0.0};
+
+  if (kPairs) {
+    if (live && st != 0) {
+      double pos[3], vel[3], om[3];
+      load3(a.pos, S, i, pos);
+      load3(a.vel, S, i, vel);
+      load3(a.omega, S, i, om);
+      const double ri = a.radius[i], mi = a.pmass[i];
+      const int phi = a.phase[i];
+      // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
+      const uint16_t fm = a.cell_facets[(BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i]];
+      uint8_t wm = a.wmask[i];
+      const uint8_t wm0 = wm;
+      if (fm == 0) {
+        wm = 0;
+      } else {
+        for (int f = 0; f < P.nf; f++) {
+          if (!((fm >> f) & 1)) continue;
+          double hist[3];
+          const bool set = (wm >> f) & 1;
+          if (set)
+            for (int d = 0; d < 3; d++) hist[d] = a.wpft[(f * 3 + d) * S + i];
+          const int r = wall_contact<MIXED>(P, P.facets[f], pos, vel, om, ri, mi, phi, hist, set,
+                                            fc, tow);
+          if (r == 0) {
+            wm &= (uint8_t)~(1u << f);
+          } else if (r == 1) {
+            wm |= (uint8_t)(1u << f);
+            for (int d = 0; d < 3; d++) a.wpft[(f * 3 + d) * S + i] = hist[d];
+          }
+        }
+      }
+      if (wm != wm0) a.wmask[i] = wm;
+    }
+    // ---- pairs (calc_force_dem.f:180-412), summed after the walls
+    wave_pairs<MIXED, BATCH>(P0, a, B, i, live, st, fc, tow);
+    if (!live) return;
+  }
   if (st == 0) {
     if (MODE == kModeFused || MODE == kModeIntegrate) {
       for (int c = 0; c < 3; c++) {
@@ -336,124 +526,18 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
         a.omega_o[c * S + i] = a.omega[c * S + i];
       }
     }
-    if (MODE == kModeFused || MODE == kModeForce)  // rows of a removed particle keep their history
-      for (int e = a.nbr_split[i]; e < a.nbr_start[i + 1]; e++) {
-        const int pid = a.nbr_pid[e];
-        for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
-      }
+    return;
+  }
+  if (MODE == kModeForce) {
+    store3(a.fc, S, i, fc);
+    store3(a.tow, S, i, tow);
     return;
   }
   double pos[3], vel[3], om[3];
   load3(a.pos, S, i, pos);
   load3(a.vel, S, i, vel);
   load3(a.omega, S, i, om);
-  const double ri = a.radius[i], mi = a.pmass[i];
-  const int phi = a.phase[i];
-  double fc[3] = {0.0, 0.0, 0.0}, tow[3] = {0.0, 0.0, 0.0};
-
-  if (MODE == kModeFused || MODE == kModeForce) {
-    // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
-    const uint16_t fm = a.cell_facets[(BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i]];
-    uint8_t wm = a.wmask[i];
-    const uint8_t wm0 = wm;
-    if (fm == 0) {
-      wm = 0;
-    } else {
-      for (int f = 0; f < P.nf; f++) {
-        if (!((fm >> f) & 1)) continue;
-        double hist[3];
-        const bool set = (wm >> f) & 1;
-        if (set)
-          for (int d = 0; d < 3; d++) hist[d] = a.wpft[(f * 3 + d) * S + i];
-        const int r = wall_contact<MIXED>(P, P.facets[f], pos, vel, om, ri, mi, phi, hist, set,
-                                          fc, tow);
-        if (r == 0) {
-          wm &= (uint8_t)~(1u << f);
-        } else if (r == 1) {
-          wm |= (uint8_t)(1u << f);
-          for (int d = 0; d < 3; d++) a.wpft[(f * 3 + d) * S + i] = hist[d];
-        }
-      }
-    }
-    if (wm != wm0) a.wmask[i] = wm;
-
-    // ---- pairs (calc_force_dem.f:180-412): incoming rows first (this
-    // particle is I of an owner with a lower id, ascending owner id), then
-    // own rows in the reference's stencil order.
-    const int e0 = a.nbr_start[i], es = a.nbr_split[i], e1 = a.nbr_start[i + 1];
-    for (int e = e0; e < e1; e++) {
-      const int j = a.nbr[e];
-      const bool owner = e >= es;
-      const int pid = a.nbr_pid[e];
-      if (a.state[j] == 0) {  // IF(IS_NONEXISTENT(I)) CYCLE: history kept
-        if (owner)
-          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
-        continue;
-      }
-      double xj[3];
-      load3(a.pos, S, j, xj);
-      const double rj = a.radius[j];
-      // contact test (orientation independent: |x_I - x_L|^2 and r_L + r_I)
-      const double r_lm = owner ? ri + rj : rj + ri;
-      const double d0 = owner ? xj[0] - pos[0] : pos[0] - xj[0];
-      const double d1 = owner ? xj[1] - pos[1] : pos[1] - xj[1];
-      const double d2 = owner ? xj[2] - pos[2] : pos[2] - xj[2];
-      const double dm = d0 * d0 + d1 * d1 + d2 * d2;
-      const double rs = r_lm - kSmall;
-      if (dm > rs * rs) {
-        if (owner) {
-          a.pft_out[3 * pid] = 0.0;
-          a.pft_out[3 * pid + 1] = 0.0;
-          a.pft_out[3 * pid + 2] = 0.0;
-        }
-        continue;
-      }
-      PairIn in;
-      double vj[3], wj[3];
-      load3(a.vel, S, j, vj);
-      load3(a.omega, S, j, wj);
-      const double mj = a.pmass[j];
-      const int phj = a.phase[j];
-      if (owner) {
-        for (int c = 0; c < 3; c++) {
-          in.xL[c] = pos[c]; in.xI[c] = xj[c];
-          in.vL[c] = vel[c]; in.vI[c] = vj[c];
-          in.wL[c] = om[c];  in.wI[c] = wj[c];
-        }
-        in.rL = ri; in.rI = rj; in.mL = mi; in.mI = mj; in.phL = phi; in.phI = phj;
-      } else {
-        for (int c = 0; c < 3; c++) {
-          in.xL[c] = xj[c];  in.xI[c] = pos[c];
-          in.vL[c] = vj[c];  in.vI[c] = vel[c];
-          in.wL[c] = wj[c];  in.wI[c] = om[c];
-        }
-        in.rL = rj; in.rI = ri; in.mL = mj; in.mI = mi; in.phL = phj; in.phI = phi;
-      }
-      const double hin[3] = {a.pft_in[3 * pid], a.pft_in[3 * pid + 1], a.pft_in[3 * pid + 2]};
-      PairOut o;
-      const int r = MIXED ? pair_force_mixed(P, in, hin, o) : pair_force_fp64(P, in, hin, o);
-      if (r == 2) {
-        atomicAdd(&a.flags[kFlagZeroDist], 1);
-        if (owner)
-          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = hin[c];
-        continue;
-      }
-      if (owner) {
-        for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = o.pft[c];
-        for (int c = 0; c < 3; c++) fc[c] = fc[c] + o.fct[c];
-        for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.tl[c];
-        if (P0.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
-      } else {
-        for (int c = 0; c < 3; c++) fc[c] = fc[c] - o.fct[c];
-        for (int c = 0; c < 3; c++) tow[c] = tow[c] + o.ti[c];
-      }
-    }
-    if (MODE == kModeForce) {
-      store3(a.fc, S, i, fc);
-      store3(a.tow, S, i, tow);
-      return;
-    }
-  }
+  const double mi = a.pmass[i];
   if (MODE == kModeDrag) {
     load3(a.fc, S, i, fc);
     if (P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);

@@ -620,7 +620,8 @@ __global__ __launch_bounds__(kBS) void k_dep_keys(int n, const uint8_t* state, c
 // COMP_MEAN_FIELDS0 deposition (comp_mean_fields0.f:186-210) of the current
 // positions.  The particles are visited in the deposition order built at
 // the last rebuild (Morton order of their fluid cells then), kDepPPT
-// consecutive ones per thread: a run of particles in the same fluid cell
+// consecutive ones per thread (2 measured fastest at 1M spheres, against 1
+// and 4): a run of particles in the same fluid cell
 // is summed in registers, each run adds its 8 corner values into an int64
 // tile in LDS covering the bounding box of the workgroup's nodes (up to
 // kDepCap nodes), and each touched tile node then costs one global atomic.
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(kBS) void k_dep_keys(int n, const uint8_t* state, c
 // to global memory.  Particles that changed fluid cell since the rebuild
 // only widen the box.  Weights are rounded to fixed point per particle and
 // summed as integers, so the result does not depend on the order.
-constexpr int kDepPPT = 4;
+constexpr int kDepPPT = 2;
 constexpr int kDepCap = 4096;
 __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* list,
                                                       const double* pos, const double* pmass,
@@ -640,13 +641,24 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
   __shared__ int box[6];
   const int S = P.stride;
   const int k0 = (xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x) * kDepPPT;
-  int pi[kDepPPT], pcell[kDepPPT];
+  int pi[kDepPPT], pcell[kDepPPT], pph[kDepPPT];
+  double px[kDepPPT][3], pm[kDepPPT];
   int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
 #pragma unroll
   for (int u = 0; u < kDepPPT; u++) {
     const int k = k0 + u;
     pi[u] = k < P.n ? list[k] : -1;
     pcell[u] = -1;
+  }
+  // every per-particle load is issued here, ahead of the workgroup barriers
+#pragma unroll
+  for (int u = 0; u < kDepPPT; u++) {
+    const int i = pi[u] < 0 ? 0 : pi[u];
+    pph[u] = phase[i];
+    px[u][0] = pos[i];
+    px[u][1] = pos[S + i];
+    px[u][2] = pos[2 * S + i];
+    pm[u] = pmass[i];
   }
 #pragma unroll
   for (int u = 0; u < kDepPPT; u++) {
@@ -661,10 +673,21 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
     lo0 = min(lo0, I); lo1 = min(lo1, J); lo2 = min(lo2, K);
     hi0 = max(hi0, I); hi1 = max(hi1, J); hi2 = max(hi2, K);
   }
+  // bounding box of the workgroup's cells: reduce within the wavefront,
+  // then one LDS atomic per wavefront and bound
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo0 = min(lo0, __shfl_xor(lo0, o, 64));
+    lo1 = min(lo1, __shfl_xor(lo1, o, 64));
+    lo2 = min(lo2, __shfl_xor(lo2, o, 64));
+    hi0 = max(hi0, __shfl_xor(hi0, o, 64));
+    hi1 = max(hi1, __shfl_xor(hi1, o, 64));
+    hi2 = max(hi2, __shfl_xor(hi2, o, 64));
+  }
   if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
   else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
   __syncthreads();
-  if (lo0 != INT_MAX) {
+  if ((threadIdx.x & 63) == 0 && lo0 != INT_MAX) {
     atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
     atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
   }
@@ -700,10 +723,9 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
 #pragma unroll
   for (int u = 0; u < kDepPPT; u++) {
     if (pcell[u] < 0) continue;
-    const int i = pi[u];
-    const int ph = phase[i];
-    const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
-    const double m = pmass[i];
+    const int ph = pph[u];
+    const double x[3] = {px[u][0], px[u][1], px[u][2]};
+    const double m = pm[u];
     int ci, cj, ck;
     unpack_ijk(pcell[u], ci, cj, ck);
     const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
@@ -780,12 +802,24 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
     if (state[i] == 0 || pc < 0) pc = -1;
     else unpack_ijk(pc, ci, cj, ck);
   }
+  // bounding box: reduce within the wavefront, one LDS atomic per bound
+  int lo0 = pc >= 0 ? ci : INT_MAX, lo1 = pc >= 0 ? cj : INT_MAX, lo2 = pc >= 0 ? ck : INT_MAX;
+  int hi0 = pc >= 0 ? ci : INT_MIN, hi1 = pc >= 0 ? cj : INT_MIN, hi2 = pc >= 0 ? ck : INT_MIN;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo0 = min(lo0, __shfl_xor(lo0, o, 64));
+    lo1 = min(lo1, __shfl_xor(lo1, o, 64));
+    lo2 = min(lo2, __shfl_xor(lo2, o, 64));
+    hi0 = max(hi0, __shfl_xor(hi0, o, 64));
+    hi1 = max(hi1, __shfl_xor(hi1, o, 64));
+    hi2 = max(hi2, __shfl_xor(hi2, o, 64));
+  }
   if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
   else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
   __syncthreads();
-  if (pc >= 0) {
-    atomicMin(&box[0], ci); atomicMin(&box[1], cj); atomicMin(&box[2], ck);
-    atomicMax(&box[3], ci); atomicMax(&box[4], cj); atomicMax(&box[5], ck);
+  if ((threadIdx.x & 63) == 0 && lo0 != INT_MAX) {
+    atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+    atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
   }
   __syncthreads();
   // target cells lie in [min-1, max+1] per axis

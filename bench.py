@@ -44,12 +44,28 @@ def algorithmic_bytes_step_kernel(n, pairs):
     return 152.0 * n + 52.0 * pairs
 
 
-def make_system(p, parts, gas):
+def make_system(p, parts, gas, tuning=None):
     s = DemSystem(p)
     s.set_particles(parts)
     s.set_gas(gas)
+    if tuning is not None:
+        s.set_tuning(*tuning)
     s.init()
     return s
+
+
+def tune_single(p, parts, gas, log):
+    """Result-invariant tuning of the neighbour-row kernel for the workload
+    (every candidate gives bit-identical results)."""
+    from mfixdem import autotune
+
+    def make():
+        s = DemSystem(p)
+        s.set_particles(parts)
+        s.set_gas(gas)
+        return s
+    return autotune.autotune(make, cands=[(1.0, abi.ROWS_THREAD), (1.0, abi.ROWS_WAVE128),
+                                          (1.0, abi.ROWS_WAVE192)], log=log)
 
 
 def time_steps(s, nn0, steps):
@@ -242,8 +258,9 @@ def main():
     p, parts, gas = workloads.fluidised_bed(n=args.n)
     log(f"workload built in {time.perf_counter() - t0:.1f} s: n={parts.n}")
 
+    tuning, tune_s = tune_single(p, parts, gas, log)
     # FP64
-    s = make_system(p, parts, gas)
+    s = make_system(p, parts, gas, tuning)
     s.run(1, args.warmup)
     el = time_steps(s, 1 + args.warmup, args.steps)
     st = s.status()
@@ -270,7 +287,7 @@ def main():
     if not args.no_mixed:
         pm = abi.Params.from_buffer_copy(p)
         pm.precision = abi.MIXED
-        m = make_system(pm, parts, gas)
+        m = make_system(pm, parts, gas, tuning)
         m.run(1, args.warmup)
         elm = time_steps(m, 1 + args.warmup, args.steps)
         ktm, _ = kernel_times(m, 1 + args.warmup + args.steps, 50)
@@ -318,7 +335,9 @@ def main():
                    "des_grid": [p.desgridsearch_imax, p.desgridsearch_jmax,
                                 p.desgridsearch_kmax],
                    "dt": p.dtsolid, "neighbor_search_n": p.neighbor_search_n,
-                   "pairs_per_particle": pairs / parts.n},
+                   "pairs_per_particle": pairs / parts.n,
+                   "tuning": {"cell_scale": tuning[0], "row_kernel": tuning[1],
+                              "candidates_s": tune_s}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_step (fused walls+pairs+drag+integrate+locate)",

@@ -87,7 +87,7 @@ struct mfixdem_handle {
   int maxrow = 48;
   // tuning (mfixdem_set_tuning): search-cell size / cut-off, row kernel
   double cell_scale = 1.0;
-  int row_kernel = MFIXDEM_ROWS_WAVE192;
+  int row_kernel = MFIXDEM_ROWS_WAVE128;
   int *nbr_start[2]{}, *nbr_split[2]{}, *pair_base[2]{};
   int *nbr[2]{}, *nbr_pid[2]{};
   int64_t nbr_cap[2]{0, 0};
@@ -221,8 +221,10 @@ void t_begin(mfixdem_handle* h) {
   if (!timing(h)) return;
   if (h->ev.empty()) {
     h->ev.assign(kTimeEvents, nullptr);
+    // timing-only events: no system-scope fence (cache writeback and
+    // invalidate) when they are recorded
     for (auto& e : h->ev)
-      if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) e = nullptr;
   }
   if (h->ev_next + 2 > kTimeEvents) t_flush(h);
   h->ev_open = h->ev_next++;

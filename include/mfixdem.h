@@ -230,7 +230,11 @@ int32_t mfixdem_cfnewvalues(mfixdem_handle* h);      /* cfnewvalues.f:641    */
 int32_t mfixdem_desgrid_pic(mfixdem_handle* h);      /* desgrid_mod.f:877    */
 int32_t mfixdem_neighbour(mfixdem_handle* h);        /* neighbour.f:21       */
 int32_t mfixdem_particles_in_cell(mfixdem_handle* h);/* particles_in_cell.f:20 */
-int32_t mfixdem_calc_interp_weights(mfixdem_handle* h); /* calc_interp_weights.f:22 */
+/* CALC_INTERP_WEIGHTS (calc_interp_weights.f:22): the filter weights are
+   evaluated where they are used, inside the drag and deposition kernels,
+   from the particle's current position, and never stored; this entry only
+   checks the handle, so that a UDF override of the routine has a target. */
+int32_t mfixdem_calc_interp_weights(mfixdem_handle* h);
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
 
 /* --- inspection for parity and restart ------------------------------------ */

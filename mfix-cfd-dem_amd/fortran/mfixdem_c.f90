@@ -21,6 +21,8 @@ module mfixdem_c
   integer(c_int32_t), parameter :: MFIXDEM_INTERP_GARG_2012 = 0, MFIXDEM_INTERP_NONE = 1
   integer(c_int32_t), parameter :: MFIXDEM_INTERP_SQUARE_DPVM = 2, MFIXDEM_INTERP_GAUSS_DPVM = 3
   integer(c_int32_t), parameter :: MFIXDEM_INTERP_LINEAR_HAT = 4
+  integer(c_int32_t), parameter :: MFIXDEM_ROWS_THREAD = 0, MFIXDEM_ROWS_WAVE128 = 1
+  integer(c_int32_t), parameter :: MFIXDEM_ROWS_WAVE192 = 2, MFIXDEM_ROWS_WAVE256 = 3
   integer(c_int32_t), parameter :: MFIXDEM_NSEARCH_NSQUARE = 1, MFIXDEM_NSEARCH_GRID = 4
   integer(c_int32_t), parameter :: MFIXDEM_DEBUG_STORE_FORCES = 1
 
@@ -139,6 +141,12 @@ module mfixdem_c
        type(c_ptr), value :: h
        real(c_double), intent(in) :: p_g(*)
      end function
+     integer(c_int32_t) function mfixdem_get_p_force(h, p_force) &
+          bind(C, name="mfixdem_get_p_force")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: p_force(*)
+     end function
      integer(c_int32_t) function mfixdem_calc_interp_weights(h) &
           bind(C, name="mfixdem_calc_interp_weights")
        import :: c_int32_t, c_ptr
@@ -167,6 +175,26 @@ module mfixdem_c
        real(c_double), intent(out) :: pft(*)
        integer(c_int64_t), value :: capacity
        integer(c_int64_t), intent(out) :: npairs
+     end function
+     integer(c_int32_t) function mfixdem_get_cells(h, dg_pijk, pijk, ld) &
+          bind(C, name="mfixdem_get_cells")
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(out) :: dg_pijk(*), pijk(*)
+       integer(c_int64_t), value :: ld
+     end function
+     integer(c_int32_t) function mfixdem_set_tuning(h, cell_scale, row_kernel) &
+          bind(C, name="mfixdem_set_tuning")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: cell_scale
+       integer(c_int32_t), value :: row_kernel
+     end function
+     integer(c_int32_t) function mfixdem_get_device_bytes(h, bytes) &
+          bind(C, name="mfixdem_get_device_bytes")
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int64_t), intent(out) :: bytes
      end function
      integer(c_int32_t) function mfixdem_get_status(h, st) bind(C, name="mfixdem_get_status")
        import :: c_int32_t, c_ptr, mfixdem_status

@@ -602,14 +602,14 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
 
 // Sort key of the deposition order: Morton code of the packed fluid cell
 // (I, J, K < 1024); particles without a fluid cell go last.
-__global__ __launch_bounds__(kBS) void k_dep_keys(int n, const uint8_t* state, const int* fcell,
-                                                  uint64_t* keys, int* vals) {
+__global__ __launch_bounds__(kBS) void k_dep_keys(int n, int bits, const uint8_t* state,
+                                                  const int* fcell, uint64_t* keys, int* vals) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   vals[i] = i;
   const int pc = fcell[i];
-  if (state[i] == 0 || pc < 0) {
-    keys[i] = (1ull << 30) - 1;
+  if (state[i] == 0 || pc < 0) {  // all ones: above every cell's code
+    keys[i] = (1ull << bits) - 1;
     return;
   }
   int I, J, K;
@@ -956,7 +956,7 @@ struct GridDev {
   // neighbour-search grid
   int fn[3];
   double f_lo[3], f_inv;
-  int gid_bits;
+  int mbits;  // bits of the Morton code in the sort key
 };
 
 __device__ __forceinline__ int floor_sat_d(double v) {
@@ -1039,8 +1039,10 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const dou
     atomicMin(&flags[kFlagRogueGid], gid[i]);
     atomicAdd(&flags[kFlagRogueCount], 1);
   }
+  // the sort is stable, so particles keep their previous relative order
+  // within a search cell (no result depends on that order)
   const uint64_t m = (uint64_t)(spread10(f3[0]) | (spread10(f3[1]) << 1) | (spread10(f3[2]) << 2));
-  keys[i] = sys_key | (m << G.gid_bits) | (uint64_t)gid[i];
+  keys[i] = sys_key | m;
 }
 
 struct PermPtrs {
@@ -1107,9 +1109,10 @@ __global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G0, const ui
   if (i >= n || state[i] == 0) return;
   const int s = BATCH ? B.sys[i] : 0;
   const GridDev& G = sys_grid<BATCH>(G0, B, s);
-  const uint64_t m = keys[i] >> G.gid_bits;  // system id (ensembles) and Morton code
-  const bool first = (i == 0) || (keys[i - 1] >> G.gid_bits) != m;
-  const bool last = (i + 1 == n) || (keys[i + 1] >> G.gid_bits) != m;
+  const uint64_t k = keys[i];  // system id (ensembles) and Morton code
+  const bool first = (i == 0) || keys[i - 1] != k;
+  const bool last = (i + 1 == n) || keys[i + 1] != k;
+  const uint64_t m = k & ((1ull << G.mbits) - 1);
   const int c = (BATCH ? B.cell_off[s] : 0) + (int)compact10(m) +
                 G.fn[0] * ((int)compact10(m >> 1) + G.fn[1] * (int)compact10(m >> 2));
   if (first) cstart[c] = i;

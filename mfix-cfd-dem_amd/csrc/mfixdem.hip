@@ -342,12 +342,15 @@ int32_t do_bin(mfixdem_handle* h) {
   if (h->S.p.coupled) {
     // deposition order for the steps up to the next rebuild (the key buffers
     // are free again: the cell ranges above were their last reader)
-    hipLaunchKernelGGL(k_dep_keys, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->state[h->tw],
-                       h->fcell[h->tw], h->keys[0], h->vals[0]);
+    // fluid cells are 1-based with ghosts: values up to max(ni, nj, nk)
+    const int dep_bits =
+        3 * morton_axis_bits(std::max(h->S.ni, std::max(h->S.nj, h->S.nk)) + 1);
+    hipLaunchKernelGGL(k_dep_keys, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, dep_bits,
+                       h->state[h->tw], h->fcell[h->tw], h->keys[0], h->vals[0]);
     HIPOK(hipGetLastError());
     tb = h->sort_tmp_bytes;
     HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
-                                             h->dep_list, n, 0, 30, h->stream));
+                                             h->dep_list, n, 0, dep_bits, h->stream));
   }
   h->binned = true;
   invalidate_graphs(h);
@@ -907,7 +910,7 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
     delete h;
     return MFIXDEM_ERR_UNSUPPORTED;
   }
-  h->key_bits = 30 + h->G.gid_bits;
+  h->key_bits = h->G.mbits;
   if ((rc = alloc_buffers(h, h->S.dg_ncell))) {
     mfixdem_destroy(h);
     return rc;
@@ -1392,7 +1395,7 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
   std::vector<DevParams> ptab(nsys);
   std::vector<GridDev> gtab(nsys);
   std::vector<int> cell_off(nsys + 1, 0), dg_off(nsys + 1, 0);
-  int gid_bits = 1, nf = 0;
+  int mbits = 3, nf = 0;
   for (int s = 0; s < nsys; s++) {
     const Setup& S = h->sys_setup[s];
     fill_params(S, h->n, h->stride, ptab[s], gtab[s]);
@@ -1401,25 +1404,25 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
       delete h;
       return MFIXDEM_ERR_UNSUPPORTED;
     }
-    gid_bits = std::max(gid_bits, gtab[s].gid_bits);
+    mbits = std::max(mbits, gtab[s].mbits);
     nf = std::max(nf, ptab[s].nf);
     cell_off[s + 1] = cell_off[s] + nfine;
     dg_off[s + 1] = dg_off[s] + S.dg_ncell;
   }
   int sys_bits = 0;
   while ((1 << sys_bits) < nsys) sys_bits++;
-  if (30 + gid_bits + sys_bits > 64) {
+  if (mbits + sys_bits > 64) {
     delete h;
     return MFIXDEM_ERR_UNSUPPORTED;
   }
-  for (auto& g : gtab) g.gid_bits = gid_bits;
+  for (auto& g : gtab) g.mbits = mbits;
   h->G = gtab[0];
   // handle-level flags that rebuild stages apply to every slot: the largest
   // facet count, and carry Adams-Bashforth history if any member uses it
   h->P.nf = nf;
   for (int s = 0; s < nsys; s++) h->P.ab2 = std::max(h->P.ab2, ptab[s].ab2);
   h->n_fine = cell_off[nsys];
-  h->key_bits = 30 + gid_bits + sys_bits;
+  h->key_bits = mbits + sys_bits;
   int32_t rc = alloc_buffers(h, dg_off[nsys]);
   std::vector<uint16_t> sys(h->stride, 0);
   for (int s = 0; s < nsys; s++)
@@ -1459,7 +1462,7 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
   h->B.dg_off = h->d_dg_off;
   h->B.rogue = h->d_rogue;
   h->B.nsys = nsys;
-  h->B.sys_shift = 30 + gid_bits;
+  h->B.sys_shift = mbits;
   h->sys_grid = gtab;
   const size_t S = h->stride;
   h->st_pos.assign(3 * S, 0.0);
@@ -1536,18 +1539,18 @@ int32_t mfixdem_set_tuning(mfixdem_handle* h, double cell_scale, int32_t row_ker
     GridDev G = h->G;
     if (search_grid(h->S.p, h->n, G, n_fine, cell_scale) != MFIXDEM_OK)
       return MFIXDEM_ERR_UNSUPPORTED;
-    G.gid_bits = h->G.gid_bits;
+    G.mbits = h->G.mbits;  // the key layout stays (a coarser grid needs fewer bits)
     h->G = G;
   } else {
     std::vector<GridDev> gt = h->sys_grid;
     std::vector<int> off(h->nsys + 1, 0);
     for (int s = 0; s < h->nsys; s++) {
       int nf = 0;
-      const int gb = gt[s].gid_bits;
+      const int gb = gt[s].mbits;
       if (search_grid(h->sys_setup[s].p, h->sys_setup[s].p.max_pip, gt[s], nf, cell_scale) !=
           MFIXDEM_OK)
         return MFIXDEM_ERR_UNSUPPORTED;
-      gt[s].gid_bits = gb;
+      gt[s].mbits = gb;
       off[s + 1] = off[s] + nf;
     }
     n_fine = off[h->nsys];

@@ -66,9 +66,18 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
 }
 
 
+// bits b per axis of a Morton code over cells 0..count-1 with 2^b > count
+inline int morton_axis_bits(int count) {
+  int b = 1;
+  while ((1 << b) <= count) b++;
+  return b;
+}
+
 // Neighbour-search grid of one system: cubic cells of cell_scale (>= 1)
 // times the largest cut-off FACTOR_RLM*(2 r_max), one cell of margin around
-// the box; particle ids take gid_bits bits of the sort key.
+// the box.  The Morton code of a cell takes mbits = 3 b bits of the sort
+// key, with 2^b > every axis count, so no cell's code is all ones (the key
+// of an empty slot).
 inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& n_fine,
                            double cell_scale = 1.0) {
   const double hcut = p.factor_rlm * (2.0 * p.max_radius) * (1.0 + 1e-9) * cell_scale;
@@ -81,9 +90,8 @@ inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& 
   }
   G.f_inv = 1.0 / hcut;
   n_fine = G.fn[0] * G.fn[1] * G.fn[2];
-  int gb = 1;
-  while ((1ll << gb) < n) gb++;
-  G.gid_bits = gb;
+  (void)n;
+  G.mbits = 3 * morton_axis_bits(std::max(G.fn[0], std::max(G.fn[1], G.fn[2])));
   return MFIXDEM_OK;
 }
 

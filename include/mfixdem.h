@@ -221,6 +221,11 @@ int32_t mfixdem_step(mfixdem_handle* h, int32_t nn);
 /* nsteps consecutive DES_TIME_STEP calls, NN = nn_first .. nn_first+nsteps-1;
    the runs of steps between rebuilds are replayed from a captured hipGraph. */
 int32_t mfixdem_run(mfixdem_handle* h, int32_t nn_first, int32_t nsteps);
+/* Waits for the handle's stream.  Conditions the reference treats as fatal
+   that arise on the device (a coincident contact pair, calc_force_dem.f:
+   267-271; EP_G out of (0,1], calc_epg_des.f:146-203) are returned by the
+   next call that waits for the device: this one, or a step that rebuilds
+   the neighbour list; they stay set (counts in mfixdem_get_status). */
 int32_t mfixdem_synchronize(mfixdem_handle* h);
 
 /* --- finer entry points (one per reference routine, unfused kernels) ----- */

@@ -378,10 +378,11 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
   const double hin[3] = {a.pft_in[3 * pid], a.pft_in[3 * pid + 1], a.pft_in[3 * pid + 2]};
   PairOut o;
   const int r = MIXED ? pair_force_mixed(P, in, hin, o) : pair_force_fp64(P, in, hin, o);
-  if (r == 2) {
-    atomicAdd(&a.flags[kFlagZeroDist], 1);
-    if (owner)
+  if (r == 2) {  // counted once per pair, on the owner's row
+    if (owner) {
+      atomicAdd(&a.flags[kFlagZeroDist], 1);
       for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = hin[c];
+    }
     return 0;
   }
   if (owner) {

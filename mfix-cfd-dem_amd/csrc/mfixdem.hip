@@ -381,6 +381,16 @@ void launch_rows_wave(mfixdem_handle* h, const BuildPtrs& b, int64_t threads) {
 }
 
 // NEIGHBOUR (neighbour.f:21-126) on the binned order.
+// The reference's fatal device-side conditions, as counted in the flags:
+// a coincident pair (calc_force_dem.f:267-271) and EP_G out of (0, 1] or a
+// deposition sum out of the fixed-point range (calc_epg_des.f:146-203).
+// Sticky: the counts are cumulative, so every later check reports them.
+int32_t fatal_flags(const int* fl) {
+  if (fl[kFlagZeroDist] > 0) return MFIXDEM_ERR_ZERO_DISTANCE;
+  if (fl[kFlagEpg] != 0) return MFIXDEM_ERR_EPG;
+  return MFIXDEM_OK;
+}
+
 int32_t do_neighbour(mfixdem_handle* h) {
   if (!h->binned) CHECKRC(do_bin(h));
   const int n = h->n, S = h->stride, t = h->tw, c = h->cur;
@@ -430,6 +440,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
     CHECKRC(dalloc(&h->own_key, (int64_t)n * mr));
     h->maxrow = mr;
   }
+  CHECKRC(fatal_flags(fl));
   if (!nsq && fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
   const int nl = 1 - h->lt;
   // own rows -> pair ids
@@ -1145,11 +1156,13 @@ int32_t mfixdem_init(mfixdem_handle* h) {
       CHECKRC(gas_tail(h));
     }
   }
+  int fl[kFlagCount];
+  HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
   h->inited = true;
   h->first_pass = true;
   h->st.steps = 0;
-  return MFIXDEM_OK;
+  return fatal_flags(fl);  // EP_G of the initial deposition
 }
 
 int32_t mfixdem_step(mfixdem_handle* h, int32_t nn) {
@@ -1167,8 +1180,14 @@ int32_t mfixdem_run(mfixdem_handle* h, int32_t nn_first, int32_t nsteps) {
 
 int32_t mfixdem_synchronize(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
+  if (!h->inited) {
+    HIPOK(hipStreamSynchronize(h->stream));
+    return MFIXDEM_OK;
+  }
+  int fl[kFlagCount];
+  HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
   HIPOK(hipStreamSynchronize(h->stream));
-  return MFIXDEM_OK;
+  return fatal_flags(fl);
 }
 
 int32_t mfixdem_calc_force_dem(mfixdem_handle* h) {

@@ -34,6 +34,7 @@ ERRORS = {
     0: "OK", 1: "ARG", 2: "HIP", 3: "OOM", 4: "ROGUE", 5: "EPG",
     6: "ZERO_DISTANCE", 7: "CAPACITY", 8: "NOT_READY", 9: "UNSUPPORTED",
 }
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
 _D = C.c_double
 _I = C.c_int32
@@ -230,9 +231,13 @@ def bptr(a: np.ndarray | None):
 
 
 class MfixDemError(RuntimeError):
-    pass
+    """A non-zero status; .code is the MFIXDEM_ERR_* value."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc: int, what: str = ""):
     if rc != 0:
-        raise MfixDemError(f"{what}: status {rc} ({ERRORS.get(rc, '?')})")
+        raise MfixDemError(f"{what}: status {rc} ({ERRORS.get(rc, '?')})", rc)

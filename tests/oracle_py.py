@@ -90,7 +90,7 @@ class OracleSystem:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise abi.MfixDemError(f"oracle {what}: status {rc} ({abi.ERRORS.get(rc, '?')})")
+            raise abi.MfixDemError(f"oracle {what}: status {rc} ({abi.ERRORS.get(rc, '?')})", rc)
 
     def derived(self):
         d = abi.Derived()

@@ -124,3 +124,34 @@ def test_autotuner_picks_a_candidate():
                                       cands=[(1.0, abi.ROWS_WAVE192), (1.25, abi.ROWS_WAVE128)])
     assert best in [(1.0, abi.ROWS_WAVE192), (1.25, abi.ROWS_WAVE128)]
     assert len(timings) == 2 and all(t > 0 for t in timings.values())
+
+
+def test_skin_tuner_keeps_every_contact():
+    """The opt-in skin tuner (FACTOR_RLM with the longest safe rebuild
+    interval) on the GPU: its choice follows a run that rebuilds every step
+    (complete contact detection) to 1e-12 m over 200 steps; on this bed the
+    two are bit-identical (no DES-cell changes reorder the sums)."""
+    from mfixdem import autotune
+    p, parts, _ = workloads.settling_bed(n=2000, seed=3, box=(0.01, 0.035, 0.01))
+    p, parts = workloads.jostled(p, parts, seed=3)
+    vmax = float(np.sqrt((parts.vel ** 2).sum(0)).max())
+    r_min = float(parts.radius.min())
+
+    def make(q):
+        s = DemSystem(q)
+        s.set_particles(parts)
+        return s
+    (f, n), timings = autotune.autotune_skin(make, p, r_min, vmax, factors=(1.2, 1.35, 1.5),
+                                             steps=100)
+    assert len(timings) == 3 and n >= 1
+    runs = []
+    for ff, nn in [(f, n), (p.factor_rlm, 1)]:
+        q = abi.Params.from_buffer_copy(p)
+        q.factor_rlm = ff
+        q.neighbor_search_n = nn
+        s = make(q)
+        s.init()
+        s.run(1, 200)
+        runs.append(s.get_particles()[0])
+        s.close()
+    assert np.max(np.abs(runs[0] - runs[1])) <= 1e-12

@@ -123,6 +123,40 @@ def force_error(p, parts, gas):
     return float(np.max(np.abs(out[1] - out[0])) / np.max(np.abs(out[0])))
 
 
+def skin_bench(p, parts, gas, vel, tuning, steps, value_ref, log):
+    """The opt-in skin tuner on the same bed: FACTOR_RLM candidates, each
+    with the longest rebuild interval that keeps every contact for speeds
+    up to twice the largest one seen in the FP64 run, timed from the same
+    initial state.  Not the reference's schedule (results differ in the
+    last bits), so reported beside the headline value, not as it."""
+    from mfixdem import autotune
+    vmax = float(np.sqrt((vel ** 2).sum(0)).max())
+    r_min = float(parts.radius.min())
+
+    def make(q):
+        s = DemSystem(q)
+        s.set_particles(parts)
+        s.set_gas(gas)
+        s.set_tuning(*tuning)
+        return s
+    (f, n), timings = autotune.autotune_skin(make, p, r_min, vmax, factors=(1.2, 1.35, 1.5),
+                                             steps=100, log=log)
+    q = abi.Params.from_buffer_copy(p)
+    q.factor_rlm = f
+    q.neighbor_search_n = n
+    s = make(q)
+    s.init()
+    s.run(1, 50)
+    el = time_steps(s, 51, steps)
+    s.close()
+    v = parts.n * steps / el
+    log(f"skin: FACTOR_RLM {f}, NEIGHBOR_SEARCH_N {n}: {v:.4e} particle-steps/s "
+        f"(reference schedule {value_ref:.4e})")
+    return {"factor_rlm": f, "neighbor_search_n": n, "v_max_probe": vmax, "safety": 2.0,
+            "r_min": r_min, "particle_steps_per_s": v, "candidates_s": timings,
+            "reference_schedule_particle_steps_per_s": value_ref}
+
+
 def ensemble_bench(nsys, steps, log):
     """Parameter-sweep ensemble: nsys independent settling beds of uneven size (N log-uniform
     in [2k, 200k], d = 1 mm), KN / EN / MEW swept, DTSOLID = TCOLL/50 each, in
@@ -243,6 +277,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-skin", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--ens-systems", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=1000)
@@ -308,6 +343,8 @@ def main():
             f"{mixed['max_rel_force_error']:.2e}, KE diff {mixed['ke_rel_diff']:.2e}, "
             f"bed height {h64:.5f} / {hm:.5f} m")
 
+    skin = None if args.no_skin else skin_bench(p, parts, gas, vel64, tuning, args.steps,
+                                                value, log)
     ens = None if args.no_ensemble else ensemble_bench(args.ens_systems, args.ens_steps, log)
     srv = None if args.no_serving else serving_bench(args.serve_requests, log)
 
@@ -344,6 +381,7 @@ def main():
                      "kernel_us": t_step_ms * 1e3, "algorithmic_bytes": bytes_step},
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "skin": skin,
         "ensemble": ens,
         "serving": srv,
         "kernel_groups_us": {"step": kt[0] * 1e3, "rebuild": kt[1] * 1e3, "gas": kt[2] * 1e3},

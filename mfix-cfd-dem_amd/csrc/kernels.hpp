@@ -618,6 +618,27 @@ __global__ __launch_bounds__(kBS) void k_dep_keys(int n, int bits, const uint8_t
   keys[i] = (uint64_t)(spread10(I) | (spread10(J) << 1) | (spread10(K) << 2));
 }
 
+// PARTICLES_IN_CELL deletion of a particle outside the fluid grid
+// (particles_in_cell.f:175-184); the deposition kernels, which visit every
+// particle once, apply it.
+struct Removal {
+  uint8_t* state;
+  uint8_t* wmask;
+  double *pos, *vel, *omega, *radius;
+  int* flags;
+};
+__device__ __forceinline__ void remove_outside(const Removal& r, int S, int i) {
+  r.state[i] = 0;
+  r.wmask[i] = 0;
+  for (int d = 0; d < 3; d++) {
+    r.pos[d * S + i] = 0.0;
+    r.vel[d * S + i] = 0.0;
+    r.omega[d * S + i] = 0.0;
+  }
+  r.radius[i] = 0.0;
+  atomicAdd(&r.flags[kFlagOutOfDomain], 1);
+}
+
 // COMP_MEAN_FIELDS0 deposition (comp_mean_fields0.f:186-210) of the current
 // positions.  The particles are visited in the deposition order built at
 // the last rebuild (Morton order of their fluid cells then), kDepPPT
@@ -637,7 +658,8 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
                                                       const uint8_t* phase, const uint8_t* state,
                                                       const int* fcell, const uint8_t* fluid,
                                                       const double* xe, const double* yn,
-                                                      const double* zt, unsigned long long* fix) {
+                                                      const double* zt, unsigned long long* fix,
+                                                      Removal rm) {
   __shared__ unsigned long long tile[kDepCap];
   __shared__ int box[6];
   const int S = P.stride;
@@ -666,7 +688,11 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
     const int i = pi[u];
     if (i < 0) continue;
     const int pc = fcell[i];
-    if (state[i] == 0 || pc < 0) continue;
+    if (state[i] == 0) continue;
+    if (pc < 0) {
+      remove_outside(rm, S, i);
+      continue;
+    }
     int I, J, K;
     unpack_ijk(pc, I, J, K);
     if (!fluid[lin_ijk(I, J, K, P.ni, P.nj)]) continue;
@@ -791,7 +817,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
                                                         const int* fcell, const double* xe,
                                                         const double* yn, const double* zt,
                                                         const uint8_t* fluid,
-                                                        unsigned long long* fix) {
+                                                        unsigned long long* fix, Removal rm) {
   __shared__ unsigned long long tile[kDepCap];
   __shared__ int box[6];
   const int S = P.stride;
@@ -800,8 +826,13 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
   if (k < P.n) {
     i = list[k];
     pc = fcell[i];
-    if (state[i] == 0 || pc < 0) pc = -1;
-    else unpack_ijk(pc, ci, cj, ck);
+    if (state[i] == 0) {
+      pc = -1;
+    } else if (pc < 0) {
+      remove_outside(rm, S, i);
+    } else {
+      unpack_ijk(pc, ci, cj, ck);
+    }
   }
   // bounding box: reduce within the wavefront, one LDS atomic per bound
   int lo0 = pc >= 0 ? ci : INT_MAX, lo1 = pc >= 0 ? cj : INT_MAX, lo2 = pc >= 0 ? ck : INT_MAX;

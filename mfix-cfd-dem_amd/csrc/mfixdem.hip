@@ -583,27 +583,30 @@ int32_t ref_mean_fields(mfixdem_handle* h) {
 // deposition of the current positions into the fixed-point accumulators
 int32_t deposit_current(mfixdem_handle* h) {
   DevParams P = h->P;
+  const int t = h->tw, c = h->cur;
+  const Removal rm{h->state[t], h->wmask[t], h->pos[c], h->vel[c], h->omega[c], h->radius[t],
+                   h->flags};
   if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
     hipLaunchKernelGGL(k_deposit_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
                        h->dep_list, h->pos[h->cur], h->pvol[h->tw], h->phase[h->tw],
                        h->state[h->tw], h->fcell[h->tw], h->xe, h->yn, h->zt, h->fluid,
-                       h->rops_fix[h->fb]);
+                       h->rops_fix[h->fb], rm);
     HIPOK(hipGetLastError());
     return do_cell_fields(h, true);
   }
   hipLaunchKernelGGL(k_deposit_list, dim3(nblk(((int64_t)h->n + kDepPPT - 1) / kDepPPT)),
                      dim3(kBS), 0, h->stream, P, h->dep_list, h->pos[h->cur], h->pmass[h->tw],
                      h->phase[h->tw], h->state[h->tw], h->fcell[h->tw], h->fluid, h->xe, h->yn,
-                     h->zt, h->rops_fix[h->fb]);
+                     h->zt, h->rops_fix[h->fb], rm);
   HIPOK(hipGetLastError());
   return do_cell_fields(h, true);
 }
 
 // the gas-side tail of a step after the fused kernel (and any rebuild):
-// PARTICLES_IN_CELL deletions, deposition, ROP_S / EP_G
+// PARTICLES_IN_CELL deletions (inside the deposition kernels, which visit
+// every particle), deposition, ROP_S / EP_G
 int32_t gas_tail(mfixdem_handle* h) {
   if (ref_deposition(h)) return ref_mean_fields(h);  // deletion inside do_pic_lists
-  CHECKRC(delete_outside(h));
   return deposit_current(h);
 }
 

@@ -180,8 +180,8 @@ def ensemble_bench(nsys, steps, log):
     sample = ms[:: max(1, nsys // 32)]
     best, timings = autotune.autotune(
         lambda: make(sample),
-        cands=autotune.candidates((1.0, 1.25), (abi.ROWS_WAVE128, abi.ROWS_WAVE192,
-                                                abi.ROWS_WAVE256)), log=log)
+        cands=autotune.candidates((1.0, 1.25), (abi.ROWS_THREAD, abi.ROWS_WAVE128,
+                                                abi.ROWS_WAVE192)), log=log)
     e = make(ms)
     e.set_tuning(*best)
     e.init()

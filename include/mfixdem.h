@@ -264,8 +264,9 @@ int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
 /* Result-invariant performance choices (no counterpart in the reference):
    the neighbour-search cell size as a multiple (1..4) of the largest
    cut-off FACTOR_RLM*2*MAX_RADIUS, and the kernel that builds the
-   neighbour rows: one thread per particle, or one wavefront per search cell
-   with an LDS tile of 128 (default), 192 or 256 candidates.  The pair sets
+   neighbour rows: one thread per particle over a half stencil (default),
+   or one wavefront per search cell with an LDS tile of 128, 192 or 256
+   candidates.  The pair sets
    and every result are the same for all choices; FACTOR_RLM itself (the
    neighbour-list skin) is a model parameter and stays as given.  May be
    called at any time outside a step. */

@@ -1164,84 +1164,95 @@ struct BuildPtrs {
 };
 
 // Own rows of DESGRID_NEIGH_BUILD (desgrid_mod.f:1057-1214): j in the
-// reference stencil of i, j after i (ids), within FACTOR_RLM*(ri+rj).  The
-// search walks the 27 search-grid cells (cell >= the largest cut-off), the
-// reference stencil is applied as a filter, and the row is put in the
-// reference's order (stencil lk,lj,li then ascending id).
+// reference stencil of i, j after i (ids), within FACTOR_RLM*(ri+rj).
+// Half-stencil search: the thread of particle h tests each unordered pair
+// once -- partners in its own search cell after it (storage order) and in
+// the 13 neighbouring cells with a positive offset (z, then y, then x).  A
+// pair within the cut-off belongs to the row of its lower-id particle (the
+// owner), which must be normal, outside the ghost DES cells and before the
+// first rogue id; the partner must exist and lie in the owner's reference
+// stencil.  Entries are appended to the owner's row through an atomic slot
+// (own_cnt zeroed beforehand); k_sort_rows then puts each row in the
+// reference's order (stencil lk, lj, li, then ascending id).
 template <bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, BuildPtrs b,
                                                   BatchDev B) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  if (b.state[i] != 1) {
-    b.own_cnt[i] = 0;
-    return;
-  }
-  const int s = BATCH ? B.sys[i] : 0;
+  const int h = blockIdx.x * kBS + threadIdx.x;
+  if (h >= n || b.state[h] == 0) return;
+  const int s = BATCH ? B.sys[h] : 0;
   const GridDev& G = sys_grid<BATCH>(G0, B, s);
   const int cbase = BATCH ? B.cell_off[s] : 0;
-  const int gi = b.gid[i];
-  int ci, cj, ck;
-  dg_decode(G, b.dgc[i], ci, cj, ck);
   const int g_r = BATCH ? B.rogue[2 * s] : b.flags[kFlagRogueGid];
-  if (dg_ghost(G, ci, cj, ck) || gi >= g_r) {
-    b.own_cnt[i] = 0;
-    return;
-  }
-  const double x[3] = {b.pos[i], b.pos[S + i], b.pos[2 * S + i]};
-  const double r = b.radius[i];
-  int lo[3], hi[3];
-  stencil_off(G, x, lo, hi);
+  const double x[3] = {b.pos[h], b.pos[S + h], b.pos[2 * S + h]};
+  const double r = b.radius[h];
   int f3[3];
   for (int d = 0; d < 3; d++) {
     const int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
     f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
   }
-  int cnt = 0;
-  int* row = b.own_tmp + (size_t)i * b.maxrow;
-  uint64_t* key = b.own_key + (size_t)i * b.maxrow;
-  for (int dz = -1; dz <= 1; dz++) {
-    const int fz = f3[2] + dz;
-    if (fz < 0 || fz >= G.fn[2]) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int fy = f3[1] + dy;
-      if (fy < 0 || fy >= G.fn[1]) continue;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int fx = f3[0] + dx;
-        if (fx < 0 || fx >= G.fn[0]) continue;
-        const int fc = cbase + fx + G.fn[0] * (fy + G.fn[1] * fz);
-        const int t1 = b.fend[fc];
-        for (int t = b.fstart[fc]; t < t1; t++) {
-          if (t == i || b.state[t] == 0) continue;
-          const int gj = b.gid[t];
-          if (gj < gi) continue;
-          const double sr = G.factor_rlm * (r + b.radius[t]);
-          const double d0 = x[0] - b.pos[t], d1 = x[1] - b.pos[S + t], d2 = x[2] - b.pos[2 * S + t];
-          const double ds = d0 * d0 + d1 * d1 + d2 * d2;
-          if (ds > sr * sr) continue;
-          int li, lj, lk;
-          dg_decode(G, b.dgc[t], li, lj, lk);
-          if (li < ci - lo[0] || li > ci + hi[0] || lj < cj - lo[1] || lj > cj + hi[1] ||
-              lk < ck - lo[2] || lk > ck + hi[2])
-            continue;
-          if (cnt < b.maxrow) {
-            const uint64_t sk = ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gj;
-            int q = cnt;
-            while (q > 0 && key[q - 1] > sk) {
-              key[q] = key[q - 1];
-              row[q] = row[q - 1];
-              q--;
-            }
-            key[q] = sk;
-            row[q] = t;
-          }
-          cnt++;
-        }
+#pragma unroll 1
+  for (int q = 0; q < 14; q++) {
+    // q = 0: own cell; q = 1..13: offsets (dx, dy, dz) after (0, 0, 0)
+    const int o = q == 0 ? 13 : 13 + q;  // index in the 3x3x3 block, x fastest
+    const int fx = f3[0] + o % 3 - 1, fy = f3[1] + (o / 3) % 3 - 1, fz = f3[2] + o / 9 - 1;
+    if (fx < 0 || fx >= G.fn[0] || fy < 0 || fy >= G.fn[1] || fz < 0 || fz >= G.fn[2]) continue;
+    const int fc = cbase + fx + G.fn[0] * (fy + G.fn[1] * fz);
+    const int t0 = q == 0 ? h + 1 : b.fstart[fc];
+    const int t1 = b.fend[fc];
+#pragma unroll 1
+    for (int t = t0; t < t1; t++) {
+      if (b.state[t] == 0) continue;
+      const double sr = G.factor_rlm * (r + b.radius[t]);
+      const double d0 = x[0] - b.pos[t], d1 = x[1] - b.pos[S + t], d2 = x[2] - b.pos[2 * S + t];
+      const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+      if (ds > sr * sr) continue;
+      // owner = lower id
+      const int gh = b.gid[h], gt = b.gid[t];
+      const int own = gh < gt ? h : t, oth = gh < gt ? t : h;
+      const int go = gh < gt ? gh : gt, gp = gh < gt ? gt : gh;
+      if (b.state[own] != 1 || go >= g_r) continue;
+      int ci, cj, ck, li, lj, lk;
+      dg_decode(G, b.dgc[own], ci, cj, ck);
+      if (dg_ghost(G, ci, cj, ck)) continue;
+      const double xo[3] = {b.pos[own], b.pos[S + own], b.pos[2 * S + own]};
+      int lo[3], hi[3];
+      stencil_off(G, xo, lo, hi);
+      dg_decode(G, b.dgc[oth], li, lj, lk);
+      if (li < ci - lo[0] || li > ci + hi[0] || lj < cj - lo[1] || lj > cj + hi[1] ||
+          lk < ck - lo[2] || lk > ck + hi[2])
+        continue;
+      const int slot = atomicAdd(&b.own_cnt[own], 1);
+      if (slot < b.maxrow) {
+        b.own_tmp[(size_t)own * b.maxrow + slot] = oth;
+        b.own_key[(size_t)own * b.maxrow + slot] =
+            ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gp;
+      } else {
+        atomicMax(&b.flags[kFlagOwnOverflow], slot + 1);
       }
     }
   }
-  if (cnt > b.maxrow) atomicMax(&b.flags[kFlagOwnOverflow], cnt);
-  b.own_cnt[i] = cnt;
+}
+
+// Each row of k_own_rows in the reference's order (keys are distinct: the
+// partner id is part of the key); rows are short, so an insertion sort.
+__global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int cnt = min(b.own_cnt[i], b.maxrow);
+  int* row = b.own_tmp + (size_t)i * b.maxrow;
+  uint64_t* key = b.own_key + (size_t)i * b.maxrow;
+  for (int a = 1; a < cnt; a++) {
+    const uint64_t k = key[a];
+    const int v = row[a];
+    int q = a;
+    while (q > 0 && key[q - 1] > k) {
+      key[q] = key[q - 1];
+      row[q] = row[q - 1];
+      q--;
+    }
+    key[q] = k;
+    row[q] = v;
+  }
 }
 
 // Own rows, one wavefront per search-grid cell.  The candidates of the 27

@@ -87,7 +87,7 @@ struct mfixdem_handle {
   int maxrow = 48;
   // tuning (mfixdem_set_tuning): search-cell size / cut-off, row kernel
   double cell_scale = 1.0;
-  int row_kernel = MFIXDEM_ROWS_WAVE128;
+  int row_kernel = MFIXDEM_ROWS_THREAD;
   int *nbr_start[2]{}, *nbr_split[2]{}, *pair_base[2]{};
   int *nbr[2]{}, *nbr_pid[2]{};
   int64_t nbr_cap[2]{0, 0};
@@ -414,12 +414,15 @@ int32_t do_neighbour(mfixdem_handle* h) {
       const int64_t threads = (int64_t)h->n_fine * 64;
       launch_rows_wave(h, b, threads);
     } else {
+      // half-stencil rows appended through atomic slots, then sorted
+      CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                            h->G, b, h->B);
       else
         hipLaunchKernelGGL((k_own_rows<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                            h->G, b, h->B);
+      hipLaunchKernelGGL(k_sort_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, b);
     }
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));

@@ -39,7 +39,7 @@ class Ensemble:
             abi.dptr(parts.radius), abi.dptr(parts.ro_sol), abi.iptr(parts.phase), n),
             "ensemble_set_particles")
 
-    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_WAVE128):
+    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_THREAD):
         abi.check(self.lib.mfixdem_set_tuning(self.h, float(cell_scale), int(row_kernel)),
                   "set_tuning")
 

@@ -60,7 +60,7 @@ class DemSystem:
             self.h, abi.dptr(gas.u_g), abi.dptr(gas.v_g), abi.dptr(gas.w_g),
             abi.dptr(gas.p_force)), "set_gas")
 
-    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_WAVE128):
+    def set_tuning(self, cell_scale=1.0, row_kernel=abi.ROWS_THREAD):
         abi.check(self.lib.mfixdem_set_tuning(self.h, float(cell_scale), int(row_kernel)),
                   "set_tuning")
 

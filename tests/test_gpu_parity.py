@@ -482,3 +482,30 @@ def test_dem06_on_gpu_terminal_velocity(name):
     rows = np.array(next(r for r in golden() if r["scheme"] == name)["rows"])
     assert np.max(np.abs(tg[:, 2] - rows[1:, 2])) <= 3.0e-5
     assert np.max(np.abs(tg[:, 1] - rows[1:, 1])) <= 1.0e-5
+
+
+@pytest.mark.parametrize("rows", [abi.ROWS_THREAD, abi.ROWS_WAVE192], ids=["thread", "wave192"])
+@pytest.mark.parametrize("skin", [1.2, 3.0], ids=["skin1.2", "skin3.0-long-rows"])
+def test_row_kernels_bit_exact(rows, skin):
+    """Every neighbour-row kernel gives the oracle's pair set, row order and
+    trajectory: the half-stencil thread kernel (atomic slots + row sort) and
+    the wavefront kernel; FACTOR_RLM = 3 makes rows longer than the initial
+    row capacity (48), so the capacity-growth path runs too."""
+    p, parts, _ = settling()
+    p.factor_rlm = skin
+    g, o = DemSystem(p), OracleSystem(p)
+    g.set_tuning(1.0, rows)
+    for s in (g, o):
+        s.set_particles(parts)
+        s.init()
+    gp, op = assert_same_pairs(g, o)
+    if skin > 2.0:
+        idx = o.get_pairs()[0]
+        assert np.max(np.diff(np.concatenate([[1], idx]))) > 48
+    for s in (g, o):
+        s.run(1, 30)
+    for a, b in zip(g.get_particles(), o.get_particles()):
+        assert np.array_equal(a, b)
+    assert_same_pairs(g, o)
+    g.close()
+    o.close()

@@ -1157,8 +1157,9 @@ struct BuildPtrs {
   const uint8_t* state;
   const int *fstart, *fend;     // search-grid cell ranges
   int* own_cnt;
-  int* own_tmp;                 // [i*maxrow + k]
-  uint64_t* own_key;
+  int* own_tmp;                 // own row entry k of slot i at [k*ld + i] (slot-major:
+  uint64_t* own_key;            // consecutive slots are adjacent, so row walks coalesce)
+  int ld;
   int maxrow;
   int* flags;
 };
@@ -1223,8 +1224,8 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
         continue;
       const int slot = atomicAdd(&b.own_cnt[own], 1);
       if (slot < b.maxrow) {
-        b.own_tmp[(size_t)own * b.maxrow + slot] = oth;
-        b.own_key[(size_t)own * b.maxrow + slot] =
+        b.own_tmp[(size_t)slot * b.ld + own] = oth;
+        b.own_key[(size_t)slot * b.ld + own] =
             ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gp;
       } else {
         atomicMax(&b.flags[kFlagOwnOverflow], slot + 1);
@@ -1239,19 +1240,20 @@ __global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int cnt = min(b.own_cnt[i], b.maxrow);
-  int* row = b.own_tmp + (size_t)i * b.maxrow;
-  uint64_t* key = b.own_key + (size_t)i * b.maxrow;
+  int* row = b.own_tmp + i;
+  uint64_t* key = b.own_key + i;
+  const size_t ld = b.ld;
   for (int a = 1; a < cnt; a++) {
-    const uint64_t k = key[a];
-    const int v = row[a];
+    const uint64_t k = key[a * ld];
+    const int v = row[a * ld];
     int q = a;
-    while (q > 0 && key[q - 1] > k) {
-      key[q] = key[q - 1];
-      row[q] = row[q - 1];
+    while (q > 0 && key[(q - 1) * ld] > k) {
+      key[q * ld] = key[(q - 1) * ld];
+      row[q * ld] = row[(q - 1) * ld];
       q--;
     }
-    key[q] = k;
-    row[q] = v;
+    key[q * ld] = k;
+    row[q * ld] = v;
   }
 }
 
@@ -1433,8 +1435,8 @@ __global__ __launch_bounds__(kBS) void k_own_rows_wave(int S, GridDev G0, BuildP
       rank += (km < row_key) ? 1 : 0;
     }
     if (lane < cnt) {
-      b.own_tmp[(size_t)h * b.maxrow + rank] = row_j;
-      b.own_key[(size_t)h * b.maxrow + rank] = row_key;
+      b.own_tmp[(size_t)rank * b.ld + h] = row_j;
+      b.own_key[(size_t)rank * b.ld + h] = row_key;
     }
     if (lane == 0) b.own_cnt[h] = cnt;
   }
@@ -1459,8 +1461,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows_nsquare(int n, int S, double f
     gi = b.gid[i];
   }
   int cnt = 0;
-  int* row = b.own_tmp + (size_t)(home ? i : 0) * b.maxrow;
-  uint64_t* key = b.own_key + (size_t)(home ? i : 0) * b.maxrow;
+  int* row = b.own_tmp + (home ? i : 0);
+  uint64_t* key = b.own_key + (home ? i : 0);
+  const size_t ld = b.ld;
   for (int base = 0; base < n; base += kBS) {
     const int j = base + threadIdx.x;
     if (j < n) {
@@ -1486,13 +1489,13 @@ __global__ __launch_bounds__(kBS) void k_own_rows_nsquare(int n, int S, double f
         if (cnt < b.maxrow) {
           const uint64_t sk = (uint64_t)(uint32_t)gj;
           int q = cnt;
-          while (q > 0 && key[q - 1] > sk) {
-            key[q] = key[q - 1];
-            row[q] = row[q - 1];
+          while (q > 0 && key[(q - 1) * ld] > sk) {
+            key[q * ld] = key[(q - 1) * ld];
+            row[q * ld] = row[(q - 1) * ld];
             q--;
           }
-          key[q] = sk;
-          row[q] = base + t;
+          key[q * ld] = sk;
+          row[q * ld] = base + t;
         }
         cnt++;
       }
@@ -1510,7 +1513,7 @@ __global__ __launch_bounds__(kBS) void k_count_in(int n, const int* own_cnt, con
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int c = own_cnt[i];
-  for (int k = 0; k < c; k++) atomicAdd(&in_cnt[own_tmp[(size_t)i * maxrow + k]], 1);
+  for (int k = 0; k < c; k++) atomicAdd(&in_cnt[own_tmp[(size_t)k * n + i]], 1);
 }
 
 __global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const int* b, int* o) {
@@ -1531,7 +1534,7 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, const 
   const int c = own_cnt[i];
   const int pb = pair_base[i];
   for (int k = 0; k < c; k++) {
-    const int j = own_tmp[(size_t)i * maxrow + k];
+    const int j = own_tmp[(size_t)k * n + i];
     nbr[split + k] = j;
     nbr_pid[split + k] = pb + k;
     const int slot = atomicAdd(&in_cur[j], 1);

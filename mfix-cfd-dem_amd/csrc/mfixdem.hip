@@ -404,7 +404,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   // there: that lives in DESGRID_NEIGH_BUILD)
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
   for (;;) {
-    b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow;
+    b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow; b.ld = n;
     if (nsq) {
       hipLaunchKernelGGL(k_own_rows_nsquare, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                          h->S.p.factor_rlm, b);

@@ -7,7 +7,8 @@
 // neighbour list rebuilt.  Runs of steps between rebuilds are replayed from
 // captured hipGraphs.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -128,6 +129,18 @@ struct mfixdem_handle {
 namespace {
 
 inline int nblk(int64_t n) { return (int)((n + kBS - 1) / kBS); }
+
+// Stable LSD radix sort of (key, slot) pairs on key bits [0, bits).  The
+// library default switches to a merge sort up to 2^20 items; for the few
+// key bits used here the onesweep radix passes are several times faster,
+// so the merge-sort limit is set to 0.
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+hipError_t sort_pairs(void* tmp, size_t& bytes, const uint64_t* kin, uint64_t* kout,
+                      const int* vin, int* vout, int n, int bits, hipStream_t s) {
+  return rocprim::radix_sort_pairs<SortConfig>(tmp, bytes, kin, kout, vin, vout, (size_t)n, 0u,
+                                               (unsigned)bits, s);
+}
 
 template <typename T>
 int32_t dalloc(T** p, int64_t count) {
@@ -309,8 +322,8 @@ int32_t do_bin(mfixdem_handle* h) {
                        h->flags, h->B);
   HIPOK(hipGetLastError());
   size_t tb = h->sort_tmp_bytes;
-  HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
-                                           h->vals[1], n, 0, h->key_bits, h->stream));
+  HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->vals[1], n,
+                   h->key_bits, h->stream));
   PermPtrs p{};
   p.perm = h->vals[1];
   p.pos = h->pos[c]; p.vel = h->vel[c]; p.omega = h->omega[c];
@@ -349,8 +362,8 @@ int32_t do_bin(mfixdem_handle* h) {
                        h->state[h->tw], h->fcell[h->tw], h->keys[0], h->vals[0]);
     HIPOK(hipGetLastError());
     tb = h->sort_tmp_bytes;
-    HIPOK(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0],
-                                             h->dep_list, n, 0, dep_bits, h->stream));
+    HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->dep_list, n,
+                     dep_bits, h->stream));
   }
   h->binned = true;
   invalidate_graphs(h);
@@ -836,11 +849,10 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
     if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * h->S.p.nphases) != hipSuccess)
       return fail(MFIXDEM_ERR_HIP);
   size_t dep_tmp = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1],
-                                         h->vals[0], h->vals[1], (int)n, 0, h->key_bits,
-                                         h->stream) != hipSuccess ||
-      hipcub::DeviceRadixSort::SortPairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0],
-                                         h->dep_list, (int)n, 0, 30, h->stream) != hipSuccess)
+  if (sort_pairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1], h->vals[0], h->vals[1],
+                 (int)n, h->key_bits, h->stream) != hipSuccess ||
+      sort_pairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0], h->dep_list, (int)n, 30,
+                 h->stream) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
   h->sort_tmp_bytes = std::max(h->sort_tmp_bytes, dep_tmp);
   if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)

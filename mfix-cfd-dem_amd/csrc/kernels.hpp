@@ -34,8 +34,10 @@ struct StepPtrs {
   uint8_t* wmask;
   double* wpft;                 // [(f*3 + d)*stride + i]
   const int *nbr_start, *nbr_split, *nbr, *nbr_pid;
-  const double* pft_in;
+  const double* pft_in;         // tangential history [3*pair + d], valid where pfl is 1
   double* pft_out;
+  const uint8_t* pfl_in;        // 1: the pair's history is set; 0: it is zero (pft unread)
+  uint8_t* pfl_out;
   double *fc, *tow;             // FC/TOW in slot order (unfused path)
   double *fc_id, *tow_id;       // FC/TOW in global-id order (inspection)
   double *acc_old, *rot_acc_old;
@@ -317,6 +319,14 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
   for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
 }
 
+// The pair's history carried unchanged into the next step.
+__device__ __forceinline__ void keep_history(const StepPtrs& a, int pid) {
+  const uint8_t f = a.pfl_in[pid];
+  a.pfl_out[pid] = f;
+  if (f)
+    for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+}
+
 // One pair entry of a row (calc_force_dem.f:210-412): particle ip is L of
 // the pair (owner, own row) or I (incoming row); partner j, pair id pid.
 // Writes the pair's new history when ip owns it and returns 1 with the
@@ -334,8 +344,7 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
   load3(a.pos, S, ip, xp);
   const double rj = a.radius[j], rp = a.radius[ip];
   if (stj == 0) {  // IF(IS_NONEXISTENT(I)) CYCLE: history kept
-    if (owner)
-      for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+    if (owner) keep_history(a, pid);
     return 0;
   }
   // contact test in the reference's orientation: |x_I - x_L|^2, r_L + r_I
@@ -345,12 +354,8 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
   const double d2 = owner ? xj[2] - xp[2] : xp[2] - xj[2];
   const double dm = d0 * d0 + d1 * d1 + d2 * d2;
   const double rs = r_lm - kSmall;
-  if (dm > rs * rs) {
-    if (owner) {
-      a.pft_out[3 * pid] = 0.0;
-      a.pft_out[3 * pid + 1] = 0.0;
-      a.pft_out[3 * pid + 2] = 0.0;
-    }
+  if (dm > rs * rs) {  // PFT_NEIGHBOR(:,CC) = 0: the flag says so, no 24-byte write
+    if (owner) a.pfl_out[pid] = 0;
     return 0;
   }
   PairIn in;
@@ -375,18 +380,21 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
   in.mI = owner ? mj : mp;
   in.phL = owner ? php : phj;
   in.phI = owner ? phj : php;
-  const double hin[3] = {a.pft_in[3 * pid], a.pft_in[3 * pid + 1], a.pft_in[3 * pid + 2]};
+  double hin[3] = {0.0, 0.0, 0.0};
+  if (a.pfl_in[pid])
+    for (int c = 0; c < 3; c++) hin[c] = a.pft_in[3 * pid + c];
   PairOut o;
   const int r = MIXED ? pair_force_mixed(P, in, hin, o) : pair_force_fp64(P, in, hin, o);
   if (r == 2) {  // counted once per pair, on the owner's row
     if (owner) {
       atomicAdd(&a.flags[kFlagZeroDist], 1);
-      for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = hin[c];
+      keep_history(a, pid);
     }
     return 0;
   }
   if (owner) {
     for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = o.pft[c];
+    a.pfl_out[pid] = 1;
     if (P0.store_forces) atomicAdd(&a.flags[kFlagContacts], 1);
     for (int c = 0; c < 3; c++) d[c] = o.fct[c];
     for (int c = 0; c < 3; c++) d[3 + c] = o.tl[c];
@@ -441,8 +449,7 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
       const bool owner = f >= s_split[w][p];
       const int pid = a.nbr_pid[f];
       if (s_st[w][p] == 0) {
-        if (owner)  // rows of a removed particle keep their history
-          for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
+        if (owner) keep_history(a, pid);  // rows of a removed particle keep their history
       } else {
         hit = pair_entry<MIXED, BATCH>(P0, a, B, wbase + p, a.nbr[f], pid, owner, d);
       }
@@ -1571,6 +1578,8 @@ struct CarryPtrs {
   const int *old_start, *old_split, *old_nbr, *old_pair_base;
   const double* pft_old;
   double* pft_new;
+  const uint8_t* pfl_old;       // history-set flags of pft_old / pft_new
+  uint8_t* pfl_new;
 };
 __global__ __launch_bounds__(kBS) void k_carry(int n, CarryPtrs c) {
   const int i = blockIdx.x * kBS + threadIdx.x;
@@ -1582,20 +1591,18 @@ __global__ __launch_bounds__(kBS) void k_carry(int n, CarryPtrs c) {
   const int pb = c.pair_base[i], opb = c.old_pair_base[o];
   for (int q = s; q < e; q++) {
     const int gj = c.gid[c.nbr[q]];
-    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    int op = -1;
     for (int r = os; r < oe; r++) {
       if (c.old_gid[c.old_nbr[r]] == gj) {
-        const int op = opb + (r - os);
-        h0 = c.pft_old[3 * op];
-        h1 = c.pft_old[3 * op + 1];
-        h2 = c.pft_old[3 * op + 2];
+        op = opb + (r - os);
         break;
       }
     }
     const int p = pb + (q - s);
-    c.pft_new[3 * p] = h0;
-    c.pft_new[3 * p + 1] = h1;
-    c.pft_new[3 * p + 2] = h2;
+    const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
+    c.pfl_new[p] = f;
+    if (f)
+      for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
   }
 }
 

@@ -93,6 +93,7 @@ struct mfixdem_handle {
   int *nbr[2]{}, *nbr_pid[2]{};
   int64_t nbr_cap[2]{0, 0};
   double* pft[2]{};
+  uint8_t* pfl[2]{};            // per pair: 1 when pft holds the history, 0 when it is zero
   int64_t pft_cap[2]{0, 0};
   int64_t n_pairs = 0;
   int* scan_bsum = nullptr;
@@ -186,6 +187,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.nbr_start = h->nbr_start[l]; a.nbr_split = h->nbr_split[l];
   a.nbr = h->nbr[l]; a.nbr_pid = h->nbr_pid[l];
   a.pft_in = h->pft[p]; a.pft_out = h->pft[1 - p];
+  a.pfl_in = h->pfl[p]; a.pfl_out = h->pfl[1 - p];
   a.fc = h->fc; a.tow = h->tow; a.fc_id = h->fc_id; a.tow_id = h->tow_id;
   a.acc_old = h->acc[t]; a.rot_acc_old = h->racc[t];
   a.fcell = h->fcell[t];
@@ -292,7 +294,9 @@ int32_t ensure_pft_capacity(mfixdem_handle* h, int b, int64_t pairs) {
   if (h->pft_cap[b] >= pairs && h->pft[b]) return MFIXDEM_OK;
   const int64_t cap = std::max<int64_t>(pairs + pairs / 4, 1024);
   dfree(h->pft[b]);
+  dfree(h->pfl[b]);
   CHECKRC(dalloc(&h->pft[b], 3 * cap));
+  CHECKRC(dalloc(&h->pfl[b], cap));
   h->pft_cap[b] = cap;
   return MFIXDEM_OK;
 }
@@ -491,6 +495,8 @@ int32_t do_neighbour(mfixdem_handle* h) {
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
   cp.pft_old = h->pft[h->pcur];
   cp.pft_new = h->pft[1 - h->pcur];
+  cp.pfl_old = h->pfl[h->pcur];
+  cp.pfl_new = h->pfl[1 - h->pcur];
   hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, cp);
   HIPOK(hipGetLastError());
   h->lt = nl;
@@ -784,7 +790,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
     f(h->racc[b]); f(h->gid[b]); f(h->fcell[b]); f(h->dgc[b]);
     f(h->state[b]); f(h->phase[b]); f(h->wmask[b]); f(h->wpft[b]);
     f(h->keys[b]); f(h->vals[b]); f(h->nbr_start[b]); f(h->nbr_split[b]);
-    f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]);
+    f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]); f(h->pfl[b]);
     f(h->rops_fix[b]);
   }
   f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->dep_list);
@@ -1334,6 +1340,8 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index, int32_t* n
   CHECKRC(download(h, nb, h->nbr[l], st[n]));
   CHECKRC(download(h, pid, h->nbr_pid[l], st[n]));
   CHECKRC(download(h, pf, h->pft[h->pcur], 3 * h->n_pairs));
+  std::vector<uint8_t> pfl;
+  CHECKRC(download(h, pfl, h->pfl[h->pcur], h->n_pairs));
   std::vector<int> slot_of(n);
   for (int i = 0; i < n; i++) slot_of[g[i]] = i;
   int64_t w = 0;
@@ -1342,7 +1350,8 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index, int32_t* n
     for (int e = sp[i]; e < st[i + 1]; e++) {
       if (neighbors) neighbors[w] = g[nb[e]] + 1;
       if (pft)
-        for (int d = 0; d < 3; d++) pft[3 * w + d] = pf[3 * (size_t)pid[e] + d];
+        for (int d = 0; d < 3; d++)
+          pft[3 * w + d] = pfl[pid[e]] ? pf[3 * (size_t)pid[e] + d] : 0.0;
       w++;
     }
     if (neighbor_index) neighbor_index[q] = (int32_t)(w + 1);

@@ -509,3 +509,32 @@ def test_row_kernels_bit_exact(rows, skin):
     assert_same_pairs(g, o)
     g.close()
     o.close()
+
+
+def test_kernel_timing_mode_matches_plain_run():
+    """MFIXDEM_DEBUG_TIME_KERNELS (eager launches bracketed by pooled HIP
+    events) steps exactly like a plain run with graph replay, coupled gas
+    included, and reports every kernel group."""
+    import ctypes as C
+    p, parts, gas = fluidised(n=2000)
+    a, _ = both(p, parts, gas)
+    c = DemSystem(p)
+    c.set_particles(parts)
+    c.set_gas(gas)
+    c.init()
+    c.set_debug(abi.DEBUG_TIME_KERNELS)
+    a.run(1, 80)
+    c.run(1, 80)
+    for x, y in zip(a.get_particles(), c.get_particles()):
+        assert np.array_equal(x, y)
+    ea, _ = a.get_gas()
+    ec, _ = c.get_gas()
+    assert np.array_equal(ea, ec)
+    ms = (C.c_double * abi.TIME_GROUPS)()
+    calls = (C.c_int64 * abi.TIME_GROUPS)()
+    assert c.lib.mfixdem_get_kernel_times(c.h, ms, calls, abi.TIME_GROUPS) == 0
+    assert calls[abi.TIME_STEP] == 80 and calls[abi.TIME_GAS] == 80
+    assert calls[abi.TIME_REBUILD] == 4   # NN = 1, 25, 50, 75
+    assert min(ms[abi.TIME_STEP], ms[abi.TIME_GAS], ms[abi.TIME_REBUILD]) > 0.0
+    a.close()
+    c.close()

@@ -319,6 +319,75 @@ __device__ __forceinline__ void add_drag(const DevParams& P, const StepPtrs& a, 
   for (int d = 0; d < 3; d++) fc[d] = fc[d] + a.p_force[3 * c + d] * pvol;
 }
 
+// The same GARG_2012 drag, evaluated ahead of the pair pass: returns the
+// two increments add_drag adds, D_FORCE in t and P_FORCE*PVOL in q (same
+// operations, so adding t and then q later rounds exactly as add_drag),
+// or false when the particle adds none.  Every gas load is issued in one
+// batch before the fluid test: indices are clamped into the grid, so the
+// loads stay in bounds for a particle outside the domain or in a ghost
+// cell, whose values are then dropped.  pc is the particle's packed fluid
+// cell (FCELL), pvol its volume, dpm its diameter.
+template <bool MIXED>
+__device__ __forceinline__ bool garg_drag_terms(const DevParams& P, const StepPtrs& a, int pc,
+                                                const double pos[3], const double vel[3],
+                                                double pvol, double dpm, double t[3],
+                                                double q[3]) {
+  int ci = 2, cj = 2, ck = 2;
+  if (pc >= 0) unpack_ijk(pc, ci, cj, ck);
+  const int c = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  const int qi = ci > 2 ? ci : 2, qj = cj > 2 ? cj : 2, qk = ck > 2 ? ck : 2;
+  const uint8_t fl = a.fluid[c];
+  const double epg = a.ep_g[c];
+  double pf[3];
+  for (int d = 0; d < 3; d++) pf[d] = a.p_force[3 * c + d];
+  const double x0 = a.xe[qi - 1], y0 = a.yn[qj - 1], z0 = a.zt[qk - 1];
+  const double x1 = a.xe[qi], y1 = a.yn[qj], z1 = a.zt[qk];
+  const int base = (qj - 2) + (qi - 2) * P.nj + (qk - 2) * P.nj * P.ni;
+  double vn[8][3];
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const double* v = a.vnode + 3 * (base + jj + ii * P.nj + kk * P.nj * P.ni);
+        for (int d = 0; d < 3; d++) vn[(kk * 2 + jj) * 2 + ii][d] = v[d];
+      }
+  if (pc < 0 || !fl) return false;
+  const double dxx = x1 - x0, dyy = y1 - y0, dzz = z1 - z0;
+  double zx = pos[0] - x0, zy = pos[1] - y0;
+  zx = zx / dxx;
+  zy = zy / dyy;
+  double zz = pos[2] - z0;
+  zz = zz / dzz;
+  const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  double velfp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const double w = xv[ii] * yv[jj] * zv[kk];
+        for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + vn[(kk * 2 + jj) * 2 + ii][d] * w;
+      }
+  if (!MIXED) {
+    const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+    const double vrel = sqrt(dot3(vs, vs));
+    const double fgp = drag_fgp_fp64(P.drag_type, epg, P.mu_g0, P.ro_g0, vrel, dpm, pvol);
+    for (int d = 0; d < 3; d++) t[d] = fgp * (velfp[d] - vel[d]);
+  } else {
+    const float vs[3] = {(float)(velfp[0] - vel[0]), (float)(velfp[1] - vel[1]),
+                         (float)(velfp[2] - vel[2])};
+    const float vrel = fsqrt(dot3f(vs, vs));
+    const float fgp = drag_fgp_fp32(P.drag_type, (float)epg, (float)P.mu_g0, (float)P.ro_g0,
+                                    vrel, (float)dpm, (float)pvol);
+    for (int d = 0; d < 3; d++) t[d] = (double)(fgp * vs[d]);
+  }
+  for (int d = 0; d < 3; d++) q[d] = pf[d] * pvol;
+  return true;
+}
+
 // The pair's history carried unchanged into the next step.
 __device__ __forceinline__ void keep_history(const StepPtrs& a, int pid) {
   const uint8_t f = a.pfl_in[pid];
@@ -413,23 +482,22 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
 // increments of its own row in row order -- the reference's summation
 // order, so the sums are bit-identical to a per-particle loop.  Rows of a
 // nonexistent particle only carry their history over.  Every lane of the
-// wavefront must call this (lanes past n with live = false).
+// wavefront must call this (lanes past n with live = false).  The lane's
+// row is [r0, r1) with its own entries from sp on (row_bounds).
 template <bool MIXED, bool BATCH>
 __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& a,
-                                           const BatchDev& B, int i, bool live, int st,
-                                           double fc[3], double tow[3]) {
+                                           const BatchDev& B, int i, int st, int r0, int sp,
+                                           int r1, double fc[3], double tow[3]) {
   __shared__ int s_row[kBS / 64][65];
   __shared__ int s_split[kBS / 64][64];
   __shared__ int s_st[kBS / 64][64];
   __shared__ int s_hit[kBS / 64][64];
   __shared__ double s_d[kBS / 64][6][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = P0.n;
-  const int r0 = a.nbr_start[live ? i : n];
   s_row[w][lane] = r0;
-  s_split[w][lane] = live ? a.nbr_split[i] : r0;
+  s_split[w][lane] = sp;
   s_st[w][lane] = st;
-  if (lane == 63) s_row[w][64] = a.nbr_start[live ? i + 1 : n];
+  if (lane == 63) s_row[w][64] = r1;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -478,9 +546,17 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
 // unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES);
 // FILTER selects the DRAG_GS_DES1 drag (any interpolation scheme other than
 // GARG_2012), so that each kernel carries only one drag path.
+//
+// The fused GARG_2012 step evaluates the drag before the walls and the
+// pair pass (garg_drag_terms), so that its gas loads travel with the
+// particle's own loads instead of forming a second chain of dependent
+// loads after the pairs; the two increments wait in LDS and are added
+// after the pair forces, where CALC_DRAG_DES adds them.
 template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
   constexpr bool kPairs = MODE == kModeFused || MODE == kModeForce;
+  constexpr bool kEarlyDrag = MODE == kModeFused && !FILTER && !BATCH;
+  __shared__ double s_drag[kEarlyDrag ? 6 : 1][kBS];
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
   const bool live = i < P0.n;
   if (!kPairs && !live) return;  // the pair modes keep every lane for the wave passes
@@ -488,8 +564,14 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
   const DevParams& P = sys_params<BATCH>(P0, B, live ? i : 0);
   const int st = live ? a.state[i] : 0;
   double fc[3] = {0.0, 0.0, 0.0}, tow[3] = {0.0, 0.0, 0.0};
+  bool drag = false;
 
   if (kPairs) {
+    // the row bounds go out with the particle's own loads
+    const int n = P0.n;
+    const int r0 = a.nbr_start[live ? i : n];
+    const int sp = live ? a.nbr_split[i] : r0;
+    const int r1 = a.nbr_start[live ? i + 1 : n];
     if (live && st != 0) {
       double pos[3], vel[3], om[3];
       load3(a.pos, S, i, pos);
@@ -497,9 +579,24 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
       load3(a.omega, S, i, om);
       const double ri = a.radius[i], mi = a.pmass[i];
       const int phi = a.phase[i];
+      int pc = -1;
+      double pv = 0.0;
+      if (kEarlyDrag && P.coupled) {
+        pc = a.fcell[i];
+        pv = a.pvol[i];
+      }
       // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
       const uint16_t fm = a.cell_facets[(BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i]];
       uint8_t wm = a.wmask[i];
+      if (kEarlyDrag && P.coupled) {
+        double t[3], q[3];
+        drag = garg_drag_terms<MIXED>(P, a, pc, pos, vel, pv, 2.0 * ri, t, q);
+        if (drag)
+          for (int d = 0; d < 3; d++) {
+            s_drag[d][threadIdx.x] = t[d];
+            s_drag[3 + d][threadIdx.x] = q[d];
+          }
+      }
       const uint8_t wm0 = wm;
       if (fm == 0) {
         wm = 0;
@@ -523,7 +620,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
       if (wm != wm0) a.wmask[i] = wm;
     }
     // ---- pairs (calc_force_dem.f:180-412), summed after the walls
-    wave_pairs<MIXED, BATCH>(P0, a, B, i, live, st, fc, tow);
+    wave_pairs<MIXED, BATCH>(P0, a, B, i, st, r0, sp, r1, fc, tow);
     if (!live) return;
   }
   if (st == 0) {
@@ -552,7 +649,14 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
     store3(a.fc, S, i, fc);
     return;
   }
-  if (MODE == kModeFused && P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
+  if (kEarlyDrag) {
+    if (drag) {
+      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[d][threadIdx.x];
+      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[3 + d][threadIdx.x];
+    }
+  } else if (MODE == kModeFused && P.coupled) {
+    add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
+  }
   if (MODE == kModeIntegrate) {
     load3(a.fc, S, i, fc);
     load3(a.tow, S, i, tow);

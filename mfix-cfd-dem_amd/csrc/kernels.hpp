@@ -1286,9 +1286,54 @@ struct BuildPtrs {
 // stencil.  Entries are appended to the owner's row through an atomic slot
 // (own_cnt zeroed beforehand); k_sort_rows then puts each row in the
 // reference's order (stencil lk, lj, li, then ascending id).
+//
+// Search cells hold about one particle, so a loop per cell would cost a
+// wavefront the slowest lane's count in every one of the 14 cells.  The
+// thread instead looks up all 14 cell ranges at once, keeps the non-empty
+// ones in LDS and walks its candidates as one sequence, loading the next
+// candidate before it tests the current one.  Candidates within the
+// cut-off (about one in ten) wait in a short LDS list and go through the
+// owner and stencil checks afterwards, so the wavefront runs that long path
+// as often as its busiest lane has hits rather than in almost every step
+// of the walk.
+constexpr int kHalfCells = 14;
+constexpr int kHitCap = 8;
+
+// The owner and stencil checks of a pair within the cut-off (h, t) and the
+// append to the owner's row.
+template <bool BATCH>
+__device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b, int S, int g_r,
+                                            int h, int gh, int t) {
+  const int gt = b.gid[t];
+  // owner = lower id
+  const int own = gh < gt ? h : t, oth = gh < gt ? t : h;
+  const int go = gh < gt ? gh : gt, gp = gh < gt ? gt : gh;
+  if (b.state[own] != 1 || go >= g_r) return;
+  int ci, cj, ck, li, lj, lk;
+  dg_decode(G, b.dgc[own], ci, cj, ck);
+  if (dg_ghost(G, ci, cj, ck)) return;
+  const double xo[3] = {b.pos[own], b.pos[S + own], b.pos[2 * S + own]};
+  int slo[3], shi[3];
+  stencil_off(G, xo, slo, shi);
+  dg_decode(G, b.dgc[oth], li, lj, lk);
+  if (li < ci - slo[0] || li > ci + shi[0] || lj < cj - slo[1] || lj > cj + shi[1] ||
+      lk < ck - slo[2] || lk > ck + shi[2])
+    return;
+  const int slot = atomicAdd(&b.own_cnt[own], 1);
+  if (slot < b.maxrow) {
+    b.own_tmp[(size_t)slot * b.ld + own] = oth;
+    b.own_key[(size_t)slot * b.ld + own] =
+        ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gp;
+  } else {
+    atomicMax(&b.flags[kFlagOwnOverflow], slot + 1);
+  }
+}
+
 template <bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, BuildPtrs b,
                                                   BatchDev B) {
+  __shared__ int s_lo[kHalfCells][kBS], s_hi[kHalfCells][kBS];
+  __shared__ int s_hit[kHitCap][kBS];
   const int h = blockIdx.x * kBS + threadIdx.x;
   if (h >= n || b.state[h] == 0) return;
   const int s = BATCH ? B.sys[h] : 0;
@@ -1297,52 +1342,60 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   const int g_r = BATCH ? B.rogue[2 * s] : b.flags[kFlagRogueGid];
   const double x[3] = {b.pos[h], b.pos[S + h], b.pos[2 * S + h]};
   const double r = b.radius[h];
+  const int gh = b.gid[h];
   int f3[3];
   for (int d = 0; d < 3; d++) {
     const int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
     f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
   }
-#pragma unroll 1
-  for (int q = 0; q < 14; q++) {
+  int lo[kHalfCells], hi[kHalfCells];
+#pragma unroll
+  for (int q = 0; q < kHalfCells; q++) {
     // q = 0: own cell; q = 1..13: offsets (dx, dy, dz) after (0, 0, 0)
     const int o = q == 0 ? 13 : 13 + q;  // index in the 3x3x3 block, x fastest
     const int fx = f3[0] + o % 3 - 1, fy = f3[1] + (o / 3) % 3 - 1, fz = f3[2] + o / 9 - 1;
-    if (fx < 0 || fx >= G.fn[0] || fy < 0 || fy >= G.fn[1] || fz < 0 || fz >= G.fn[2]) continue;
-    const int fc = cbase + fx + G.fn[0] * (fy + G.fn[1] * fz);
-    const int t0 = q == 0 ? h + 1 : b.fstart[fc];
-    const int t1 = b.fend[fc];
-#pragma unroll 1
-    for (int t = t0; t < t1; t++) {
-      if (b.state[t] == 0) continue;
-      const double sr = G.factor_rlm * (r + b.radius[t]);
-      const double d0 = x[0] - b.pos[t], d1 = x[1] - b.pos[S + t], d2 = x[2] - b.pos[2 * S + t];
-      const double ds = d0 * d0 + d1 * d1 + d2 * d2;
-      if (ds > sr * sr) continue;
-      // owner = lower id
-      const int gh = b.gid[h], gt = b.gid[t];
-      const int own = gh < gt ? h : t, oth = gh < gt ? t : h;
-      const int go = gh < gt ? gh : gt, gp = gh < gt ? gt : gh;
-      if (b.state[own] != 1 || go >= g_r) continue;
-      int ci, cj, ck, li, lj, lk;
-      dg_decode(G, b.dgc[own], ci, cj, ck);
-      if (dg_ghost(G, ci, cj, ck)) continue;
-      const double xo[3] = {b.pos[own], b.pos[S + own], b.pos[2 * S + own]};
-      int lo[3], hi[3];
-      stencil_off(G, xo, lo, hi);
-      dg_decode(G, b.dgc[oth], li, lj, lk);
-      if (li < ci - lo[0] || li > ci + hi[0] || lj < cj - lo[1] || lj > cj + hi[1] ||
-          lk < ck - lo[2] || lk > ck + hi[2])
-        continue;
-      const int slot = atomicAdd(&b.own_cnt[own], 1);
-      if (slot < b.maxrow) {
-        b.own_tmp[(size_t)slot * b.ld + own] = oth;
-        b.own_key[(size_t)slot * b.ld + own] =
-            ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gp;
-      } else {
-        atomicMax(&b.flags[kFlagOwnOverflow], slot + 1);
-      }
-    }
+    const bool in = fx >= 0 && fx < G.fn[0] && fy >= 0 && fy < G.fn[1] && fz >= 0 &&
+                    fz < G.fn[2];
+    const int fc = in ? cbase + fx + G.fn[0] * (fy + G.fn[1] * fz) : cbase;
+    lo[q] = q == 0 ? h + 1 : b.fstart[fc];
+    hi[q] = in ? b.fend[fc] : 0;
   }
+  int m = 0;
+#pragma unroll
+  for (int q = 0; q < kHalfCells; q++)
+    if (hi[q] > lo[q]) {
+      s_lo[m][threadIdx.x] = lo[q];
+      s_hi[m][threadIdx.x] = hi[q];
+      m++;
+    }
+  if (m == 0) return;
+  int k = 0, t = s_lo[0][threadIdx.x], t1 = s_hi[0][threadIdx.x], nh = 0;
+  // candidate t's data, loaded one step ahead
+  uint8_t st_t = b.state[t];
+  double r_t = b.radius[t], p_t[3] = {b.pos[t], b.pos[S + t], b.pos[2 * S + t]};
+#pragma unroll 1
+  while (k < m) {
+    const int tc = t;
+    const uint8_t stc = st_t;
+    const double rc = r_t, pc[3] = {p_t[0], p_t[1], p_t[2]};
+    if (++t == t1 && ++k < m) {
+      t = s_lo[k][threadIdx.x];
+      t1 = s_hi[k][threadIdx.x];
+    }
+    const int tn = k < m ? t : tc;  // in range either way
+    st_t = b.state[tn];
+    r_t = b.radius[tn];
+    for (int d = 0; d < 3; d++) p_t[d] = b.pos[d * S + tn];
+    if (stc == 0) continue;
+    const double sr = G.factor_rlm * (r + rc);
+    const double d0 = x[0] - pc[0], d1 = x[1] - pc[1], d2 = x[2] - pc[2];
+    const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+    if (ds > sr * sr) continue;
+    if (nh < kHitCap) s_hit[nh++][threadIdx.x] = tc;
+    else own_row_hit<BATCH>(G, b, S, g_r, h, gh, tc);  // a crowded stencil: at once
+  }
+#pragma unroll 1
+  for (int e = 0; e < nh; e++) own_row_hit<BATCH>(G, b, S, g_r, h, gh, s_hit[e][threadIdx.x]);
 }
 
 // Each row of k_own_rows in the reference's order (keys are distinct: the

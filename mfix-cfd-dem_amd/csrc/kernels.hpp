@@ -1264,6 +1264,7 @@ __global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G0, const ui
 
 struct BuildPtrs {
   const double *pos, *radius;
+  const double* xr;             // [4*i + c]: x, y, z, radius (k_pack_xr), read per candidate
   const int *gid, *dgc;
   const uint8_t* state;
   const int *fstart, *fend;     // search-grid cell ranges
@@ -1371,21 +1372,22 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   if (m == 0) return;
   int k = 0, t = s_lo[0][threadIdx.x], t1 = s_hi[0][threadIdx.x], nh = 0;
   // candidate t's data, loaded one step ahead
+  const double2* xr2 = reinterpret_cast<const double2*>(b.xr);
   uint8_t st_t = b.state[t];
-  double r_t = b.radius[t], p_t[3] = {b.pos[t], b.pos[S + t], b.pos[2 * S + t]};
+  double2 u_t = xr2[2 * t], v_t = xr2[2 * t + 1];  // x, y | z, radius: one 32-byte line
 #pragma unroll 1
   while (k < m) {
     const int tc = t;
     const uint8_t stc = st_t;
-    const double rc = r_t, pc[3] = {p_t[0], p_t[1], p_t[2]};
+    const double rc = v_t.y, pc[3] = {u_t.x, u_t.y, v_t.x};
     if (++t == t1 && ++k < m) {
       t = s_lo[k][threadIdx.x];
       t1 = s_hi[k][threadIdx.x];
     }
     const int tn = k < m ? t : tc;  // in range either way
     st_t = b.state[tn];
-    r_t = b.radius[tn];
-    for (int d = 0; d < 3; d++) p_t[d] = b.pos[d * S + tn];
+    u_t = xr2[2 * tn];
+    v_t = xr2[2 * tn + 1];
     if (stc == 0) continue;
     const double sr = G.factor_rlm * (r + rc);
     const double d0 = x[0] - pc[0], d1 = x[1] - pc[1], d2 = x[2] - pc[2];
@@ -1396,6 +1398,17 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   }
 #pragma unroll 1
   for (int e = 0; e < nh; e++) own_row_hit<BATCH>(G, b, S, g_r, h, gh, s_hit[e][threadIdx.x]);
+}
+
+// Position and radius of particles [0, n) packed per particle for the
+// candidate walk of k_own_rows (one cache line per candidate instead of four).
+__global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, const double* pos,
+                                                 const double* radius, double* xr) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  double2* o = reinterpret_cast<double2*>(xr) + 2 * i;
+  o[0] = make_double2(pos[i], pos[S + i]);
+  o[1] = make_double2(pos[2 * S + i], radius[i]);
 }
 
 // Each row of k_own_rows in the reference's order (keys are distinct: the

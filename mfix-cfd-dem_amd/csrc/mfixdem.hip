@@ -69,6 +69,7 @@ struct mfixdem_handle {
   int cur = 0, pcur = 0, tw = 0, lt = 0;
   // particles (cur / tw twins)
   double *pos[2]{}, *vel[2]{}, *omega[2]{};
+  double* xr_pack = nullptr;  // (x, y, z, radius) per particle, written before each row build
   double *radius[2]{}, *pmass[2]{}, *omoi[2]{}, *pvol[2]{};
   double *acc[2]{}, *racc[2]{};
   int *gid[2]{}, *fcell[2]{}, *dgc[2]{};
@@ -413,6 +414,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   const int n = h->n, S = h->stride, t = h->tw, c = h->cur;
   BuildPtrs b{};
   b.pos = h->pos[c]; b.radius = h->radius[t]; b.gid = h->gid[t]; b.dgc = h->dgc[t];
+  b.xr = h->xr_pack;
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
   b.own_cnt = h->own_cnt; b.flags = h->flags;
   int fl[kFlagCount];
@@ -433,6 +435,8 @@ int32_t do_neighbour(mfixdem_handle* h) {
     } else {
       // half-stencil rows appended through atomic slots, then sorted
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
+      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, b.pos,
+                         b.radius, h->xr_pack);
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                            h->G, b, h->B);
@@ -794,6 +798,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
     f(h->rops_fix[b]);
   }
   f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->dep_list);
+  f(h->xr_pack);
   f(h->fstart); f(h->fend);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
   f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->xe); f(h->yn);
@@ -831,6 +836,7 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
       return fail(rc);
   }
   if ((rc = dalloc(&h->fc, 3 * S)) || (rc = dalloc(&h->tow, 3 * S)) ||
+      (rc = dalloc(&h->xr_pack, 4 * S)) ||
       (rc = dalloc(&h->fc_id, 3 * S)) || (rc = dalloc(&h->tow_id, 3 * S)) ||
       (rc = dalloc(&h->fstart, h->n_fine)) || (rc = dalloc(&h->fend, h->n_fine)) ||
       (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||

@@ -1305,6 +1305,7 @@ constexpr int kHitCap = 8;
 template <bool BATCH>
 __device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b, int S, int g_r,
                                             int h, int gh, int t) {
+  if (b.state[t] == 0) return;
   const int gt = b.gid[t];
   // owner = lower id
   const int own = gh < gt ? h : t, oth = gh < gt ? t : h;
@@ -1373,23 +1374,19 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   int k = 0, t = s_lo[0][threadIdx.x], t1 = s_hi[0][threadIdx.x], nh = 0;
   // candidate t's data, loaded one step ahead
   const double2* xr2 = reinterpret_cast<const double2*>(b.xr);
-  uint8_t st_t = b.state[t];
   double2 u_t = xr2[2 * t], v_t = xr2[2 * t + 1];  // x, y | z, radius: one 32-byte line
 #pragma unroll 1
   while (k < m) {
     const int tc = t;
-    const uint8_t stc = st_t;
     const double rc = v_t.y, pc[3] = {u_t.x, u_t.y, v_t.x};
     if (++t == t1 && ++k < m) {
       t = s_lo[k][threadIdx.x];
       t1 = s_hi[k][threadIdx.x];
     }
     const int tn = k < m ? t : tc;  // in range either way
-    st_t = b.state[tn];
     u_t = xr2[2 * tn];
     v_t = xr2[2 * tn + 1];
-    if (stc == 0) continue;
-    const double sr = G.factor_rlm * (r + rc);
+    const double sr = G.factor_rlm * (r + rc);  // a removed candidate fails the test below
     const double d0 = x[0] - pc[0], d1 = x[1] - pc[1], d2 = x[2] - pc[2];
     const double ds = d0 * d0 + d1 * d1 + d2 * d2;
     if (ds > sr * sr) continue;
@@ -1401,14 +1398,23 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
 }
 
 // Position and radius of particles [0, n) packed per particle for the
-// candidate walk of k_own_rows (one cache line per candidate instead of four).
+// candidate walk of k_own_rows (one cache line per candidate instead of five).
+// A nonexistent particle gets a position no cut-off reaches, so the walk
+// needs no state load; the hit path checks the state itself.
+constexpr double kFarAway = 1.0e300;
 __global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, const double* pos,
-                                                 const double* radius, double* xr) {
+                                                 const double* radius, const uint8_t* state,
+                                                 double* xr) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   double2* o = reinterpret_cast<double2*>(xr) + 2 * i;
-  o[0] = make_double2(pos[i], pos[S + i]);
-  o[1] = make_double2(pos[2 * S + i], radius[i]);
+  if (state[i] == 0) {
+    o[0] = make_double2(kFarAway, kFarAway);
+    o[1] = make_double2(kFarAway, radius[i]);
+  } else {
+    o[0] = make_double2(pos[i], pos[S + i]);
+    o[1] = make_double2(pos[2 * S + i], radius[i]);
+  }
 }
 
 // Each row of k_own_rows in the reference's order (keys are distinct: the

@@ -436,7 +436,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
       // half-stencil rows appended through atomic slots, then sorted
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, b.pos,
-                         b.radius, h->xr_pack);
+                         b.radius, b.state, h->xr_pack);
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                            h->G, b, h->B);

@@ -1938,72 +1938,120 @@ struct FieldPtrs {
   int* flags;
 };
 // Node sums -> ROP_S, EP_G per fluid cell (comp_mean_fields0.f:339-432,
-// calc_epg_des.f:133-145).  All corner loads are issued before any is
-// used (the kernel is one short pass over the grid, so it is bound by load
-// latency, not bandwidth); the sum runs over the corners in the reference's
-// order (K, J, I) and skips nodes with VOL_SURR = 0 exactly as it does.
+// calc_epg_des.f:133-145).  A workgroup takes kFieldCells consecutive
+// cells (J fastest), kFieldCPT per thread (two per thread measured the same
+// as one at 64x128x64 cells).  The corner nodes of those cells form four runs of
+// kFieldCells + 1 consecutive nodes, one per (I, K) corner pair, so the
+// workgroup reads each node once, coalesced, and computes its density (node
+// sum / VOL_SURR) once into LDS instead of once per adjacent cell.  Each
+// cell then sums its corners in the reference's order (I fastest, then J,
+// then K), skipping corners outside the node range and nodes with VOL_SURR
+// = 0 (their density is stored as 0, and adding 0 leaves the non-negative
+// sum as is).
+constexpr int kFieldCPT = 1, kFieldCells = kBS * kFieldCPT;
 template <bool FIXED>
 __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kmax, int ni, int nj,
                                                      int nk, int nph, double packed_eps,
                                                      double inv_scale0, double inv_scale1,
                                                      double inv_scale2, double inv_scale3,
                                                      FieldPtrs f) {
-  const int c = blockIdx.x * kBS + threadIdx.x;
-  const int ncell = ni * nj * nk;
-  if (c >= ncell) return;
+  constexpr int kRun = kFieldCells + 1, kNodes = 4 * kRun, kPer = (kNodes + kBS - 1) / kBS;
+  __shared__ double s_dens[4][kRun];
+  const int ncell = ni * nj * nk, plane = nj * ni;
+  const int c0 = blockIdx.x * kFieldCells;
   const double inv[kMaxPh] = {inv_scale0, inv_scale1, inv_scale2, inv_scale3};
-  const int cj = c % nj + 1;
-  const int ci = (c / nj) % ni + 1;
-  const int ck = c / (nj * ni) + 1;
-  int node[8];
-  double vs[8];
+  // this thread's cells c0 + tid + v kBS: corner mask (bit q: the corner is
+  // inside the node range), volume, fluid flag
+  unsigned okm[kFieldCPT];
+  double vc[kFieldCPT];
+  uint8_t fl[kFieldCPT];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int I = ci - 1 + (q & 1), J = cj - 1 + ((q >> 1) & 1), K = ck - 1 + (q >> 2);
-    const bool ok = I >= 1 && I <= imax + 1 && J >= 1 && J <= jmax + 1 && K >= 1 && K <= kmax + 1;
-    node[q] = ok ? (J - 1) + (I - 1) * nj + (K - 1) * nj * ni : -1;
+  for (int v = 0; v < kFieldCPT; v++) {
+    const int c = c0 + threadIdx.x + v * kBS;
+    const bool live = c < ncell;
+    const int cj = c % nj + 1;
+    const int ci = (c / nj) % ni + 1;
+    const int ck = c / plane + 1;
+    okm[v] = 0u;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int I = ci - 1 + (q & 1), J = cj - 1 + ((q >> 1) & 1), K = ck - 1 + (q >> 2);
+      if (live && I >= 1 && I <= imax + 1 && J >= 1 && J <= jmax + 1 && K >= 1 &&
+          K <= kmax + 1)
+        okm[v] |= 1u << q;
+    }
+    vc[v] = live ? f.vol[c] : 1.0;
+    fl[v] = live ? f.fluid[c] : 0;
   }
+  // loader nodes: entry e of run r = qi + 2 qk is node
+  // c0 - 1 - (1 - qi) nj - (1 - qk) ni nj + e, the (qi, qj, qk) corner of
+  // cell c0 + e - qj
+  int nidx[kPer];
+  double vsl[kPer];
 #pragma unroll
-  for (int q = 0; q < 8; q++) vs[q] = node[q] >= 0 ? f.vol_surr[node[q]] : 0.0;
-  const double vc = f.vol[c];
-  const uint8_t fl = f.fluid[c];
-  double acc[kMaxPh] = {0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < kPer; u++) {
+    const int e = threadIdx.x + u * kBS;
+    nidx[u] = -1;
+    if (e < kNodes) {
+      const int r = e / kRun, qi = r & 1, qk = r >> 1;
+      const int nd = c0 - 1 - (1 - qi) * nj - (1 - qk) * plane + (e - r * kRun);
+      if (nd >= 0 && nd < ncell) nidx[u] = nd;
+    }
+    vsl[u] = nidx[u] >= 0 ? f.vol_surr[nidx[u]] : 0.0;
+  }
+  double acc[kFieldCPT][kMaxPh] = {};
   for (int m = 0; m < nph; m++) {
-    double nodev[8];
+    if (m > 0) __syncthreads();  // the previous phase's densities are read
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      nodev[q] = 0.0;
-      if (node[q] < 0) continue;
-      if (FIXED) {
-        const unsigned long long u = f.fix[(size_t)m * ncell + node[q]];
-        if (u >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
-        nodev[q] = (double)u * inv[m];
-      } else {
-        nodev[q] = f.rops_node[(size_t)m * ncell + node[q]];
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + u * kBS;
+      if (e >= kNodes) continue;
+      double d = 0.0;
+      if (nidx[u] >= 0) {
+        double nodev;
+        if (FIXED) {
+          const unsigned long long x = f.fix[(size_t)m * ncell + nidx[u]];
+          if (x >> 62) atomicOr(&f.flags[kFlagEpg], 1 << 30);  // accumulator range exceeded
+          nodev = (double)x * inv[m];
+        } else {
+          nodev = f.rops_node[(size_t)m * ncell + nidx[u]];
+        }
+        if (vsl[u] != 0.0) d = nodev / vsl[u];
       }
+      s_dens[e / kRun][e % kRun] = d;
     }
-    // corner order q = I fastest, then J, then K: the reference's loop order
-    double a = 0.0;
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      if (node[q] < 0 || vs[q] == 0.0) continue;
-      const double dens = nodev[q] / vs[q];
-      a = a + dens * vc;
+    for (int v = 0; v < kFieldCPT; v++) {
+      const int o = threadIdx.x + v * kBS;
+      double a = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (!((okm[v] >> q) & 1u)) continue;
+        const double dens = s_dens[(q & 1) + 2 * (q >> 2)][o + ((q >> 1) & 1)];
+        a = a + dens * vc[v];
+      }
+      acc[v][m] = a;
     }
-    acc[m] = a;
   }
-  if (FIXED)
-    for (int m = 0; m < nph; m++) f.fix_clear[(size_t)m * ncell + c] = 0ull;
-  if (fl)
-    for (int m = 0; m < nph; m++)
-      if (acc[m] > 0.0) acc[m] = acc[m] / vc;
-  for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = acc[m];
-  if (fl) {
-    double sum_eps = 0.0;
-    for (int m = 0; m < nph; m++) sum_eps = sum_eps + acc[m] / f.ro_s[m];
-    const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);
-    f.ep_g[c] = e;
-    if (e <= 0.0 || e > 1.0) atomicAdd(&f.flags[kFlagEpg], 1);
+#pragma unroll
+  for (int v = 0; v < kFieldCPT; v++) {
+    const int c = c0 + threadIdx.x + v * kBS;
+    if (c >= ncell) continue;
+    double* am = acc[v];
+    if (FIXED)
+      for (int m = 0; m < nph; m++) f.fix_clear[(size_t)m * ncell + c] = 0ull;
+    if (fl[v])
+      for (int m = 0; m < nph; m++)
+        if (am[m] > 0.0) am[m] = am[m] / vc[v];
+    for (int m = 0; m < nph; m++) f.rop_s[(size_t)m * ncell + c] = am[m];
+    if (fl[v]) {
+      double sum_eps = 0.0;
+      for (int m = 0; m < nph; m++) sum_eps = sum_eps + am[m] / f.ro_s[m];
+      const double e = 1.0 - (packed_eps < sum_eps ? packed_eps : sum_eps);
+      f.ep_g[c] = e;
+      if (e <= 0.0 || e > 1.0) atomicAdd(&f.flags[kFlagEpg], 1);
+    }
   }
 }
 

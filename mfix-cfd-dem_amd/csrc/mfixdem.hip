@@ -131,6 +131,9 @@ struct mfixdem_handle {
 namespace {
 
 inline int nblk(int64_t n) { return (int)((n + kBS - 1) / kBS); }
+inline int field_blocks(int64_t ncell) {
+  return (int)((ncell + kFieldCells - 1) / kFieldCells);
+}
 
 // Stable LSD radix sort of (key, slot) pairs on key bits [0, bits).  The
 // library default switches to a merge sort up to 2^20 items; for the few
@@ -422,6 +425,8 @@ int32_t do_neighbour(mfixdem_handle* h) {
   // neighbour.f:60-66: NSQUARE for DES_NEIGHBOR_SEARCH = 1 (no rogue handling
   // there: that lives in DESGRID_NEIGH_BUILD)
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
+  const int nl = 1 - h->lt;
+  int npairs = 0;
   for (;;) {
     b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow; b.ld = n;
     if (nsq) {
@@ -446,6 +451,11 @@ int32_t do_neighbour(mfixdem_handle* h) {
       hipLaunchKernelGGL(k_sort_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, b);
     }
     HIPOK(hipGetLastError());
+    // own rows -> pair ids, read back with the flags in the same wait (an
+    // overflowing pass is built again and its pair ids with it)
+    CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
+    HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
+                         h->stream));
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
     HIPOK(hipStreamSynchronize(h->stream));
     if (fl[kFlagOwnOverflow] <= h->maxrow) break;
@@ -466,13 +476,6 @@ int32_t do_neighbour(mfixdem_handle* h) {
   }
   CHECKRC(fatal_flags(fl));
   if (!nsq && fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
-  const int nl = 1 - h->lt;
-  // own rows -> pair ids
-  CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
-  int npairs = 0;
-  HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
-                       h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
   CHECKRC(ensure_list_capacity(h, nl, 2 * (int64_t)npairs));
   CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
   CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
@@ -506,9 +509,12 @@ int32_t do_neighbour(mfixdem_handle* h) {
   h->lt = nl;
   h->pcur = 1 - h->pcur;
   h->n_pairs = npairs;
-  // the old history buffer becomes the next step's output
-  HIPOK(hipStreamSynchronize(h->stream));
-  CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
+  // the old history buffer becomes the next step's output; k_carry reads it,
+  // so a buffer too small is replaced only once the carry is done
+  if (h->pft_cap[1 - h->pcur] < npairs || !h->pft[1 - h->pcur]) {
+    HIPOK(hipStreamSynchronize(h->stream));
+    CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
+  }
   if (!nsq && fl[kFlagRogueCount] > 0) {
     if (h->nsys > 1)
       hipLaunchKernelGGL((k_delete_rogue<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
@@ -571,14 +577,14 @@ int32_t do_cell_fields(mfixdem_handle* h, bool fixed) {
     return MFIXDEM_OK;
   }
   if (fixed) {
-    hipLaunchKernelGGL(k_cell_fields<true>, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
-                       p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, packed,
-                       iv[0], iv[1], iv[2], iv[3], f);
+    hipLaunchKernelGGL(k_cell_fields<true>, dim3(field_blocks(h->S.ncell)), dim3(kBS), 0,
+                       h->stream, p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases,
+                       packed, iv[0], iv[1], iv[2], iv[3], f);
     h->fb = 1 - h->fb;
   } else {
-    hipLaunchKernelGGL(k_cell_fields<false>, dim3(nblk(h->S.ncell)), dim3(kBS), 0, h->stream,
-                       p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases, packed,
-                       iv[0], iv[1], iv[2], iv[3], f);
+    hipLaunchKernelGGL(k_cell_fields<false>, dim3(field_blocks(h->S.ncell)), dim3(kBS), 0,
+                       h->stream, p.imax, p.jmax, p.kmax, h->S.ni, h->S.nj, h->S.nk, p.nphases,
+                       packed, iv[0], iv[1], iv[2], iv[3], f);
   }
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;

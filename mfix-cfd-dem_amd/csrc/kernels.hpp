@@ -1418,14 +1418,48 @@ __global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, const double* pos
 }
 
 // Each row of k_own_rows in the reference's order (keys are distinct: the
-// partner id is part of the key); rows are short, so an insertion sort.
-__global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b) {
+// partner id is part of the key), and each entry counted in its partner's
+// incoming row (in_cnt zeroed beforehand).  Rows are short: up to kRegRow
+// entries are sorted in registers, longer ones by insertion in place.
+constexpr int kRegRow = 8;
+__global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b, int* in_cnt) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int cnt = min(b.own_cnt[i], b.maxrow);
   int* row = b.own_tmp + i;
   uint64_t* key = b.own_key + i;
   const size_t ld = b.ld;
+  if (cnt <= kRegRow) {
+    uint64_t k[kRegRow];
+    int v[kRegRow];
+#pragma unroll
+    for (int a = 0; a < kRegRow; a++)
+      if (a < cnt) {
+        k[a] = key[a * ld];
+        v[a] = row[a * ld];
+      }
+    // insertion sort, unrolled: entry a sinks past larger keys before it
+#pragma unroll
+    for (int a = 1; a < kRegRow; a++)
+#pragma unroll
+      for (int q = a; q > 0; q--)
+        if (a < cnt && k[q - 1] > k[q]) {
+          const uint64_t tk = k[q - 1];
+          k[q - 1] = k[q];
+          k[q] = tk;
+          const int tv = v[q - 1];
+          v[q - 1] = v[q];
+          v[q] = tv;
+        }
+#pragma unroll
+    for (int a = 0; a < kRegRow; a++)
+      if (a < cnt) {
+        key[a * ld] = k[a];
+        row[a * ld] = v[a];
+        atomicAdd(&in_cnt[v[a]], 1);
+      }
+    return;
+  }
   for (int a = 1; a < cnt; a++) {
     const uint64_t k = key[a * ld];
     const int v = row[a * ld];
@@ -1438,6 +1472,7 @@ __global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b) {
     key[q * ld] = k;
     row[q * ld] = v;
   }
+  for (int a = 0; a < cnt; a++) atomicAdd(&in_cnt[row[a * ld]], 1);
 }
 
 // Own rows, one wavefront per search-grid cell.  The candidates of the 27
@@ -1704,34 +1739,91 @@ __global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const i
   if (i < n) o[i] = a[i] + b[i];
 }
 
-// place own entries and scatter the transposed (incoming) entries
+// history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
+struct CarryPtrs {
+  const int* perm;              // new slot -> old slot
+  const int* gid;               // new order
+  const int* old_gid;           // old order
+  const int *old_start, *old_split, *old_nbr, *old_pair_base;
+  const double* pft_old;
+  double* pft_new;
+  const uint8_t* pfl_old;       // history-set flags of pft_old / pft_new
+  uint8_t* pfl_new;
+};
+
+// Places the own entries of each row and scatters the transposed
+// (incoming) entries; carries each own pair's history from the particle's
+// old row (the same owner and partner ids), or starts it at zero.
 __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, const int* own_tmp,
-                                               int maxrow, const int* pair_base,
-                                               const int* nbr_start, const int* in_cnt,
-                                               int* nbr_split, int* in_cur, int* nbr,
-                                               int* nbr_pid) {
+                                               const int* pair_base, const int* nbr_start,
+                                               const int* in_cnt, int* nbr_split, int* in_cur,
+                                               int* nbr, int* nbr_pid, CarryPtrs c) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int split = nbr_start[i] + in_cnt[i];
   nbr_split[i] = split;
-  const int c = own_cnt[i];
+  const int cnt = own_cnt[i];
+  if (cnt == 0) return;
   const int pb = pair_base[i];
-  for (int k = 0; k < c; k++) {
+  const int o = c.perm[i];
+  const int os = c.old_split[o], oe = c.old_start[o + 1], opb = c.old_pair_base[o];
+  for (int k = 0; k < cnt; k++) {
     const int j = own_tmp[(size_t)k * n + i];
     nbr[split + k] = j;
     nbr_pid[split + k] = pb + k;
     const int slot = atomicAdd(&in_cur[j], 1);
     nbr[nbr_start[j] + slot] = i;
     nbr_pid[nbr_start[j] + slot] = pb + k;
+    const int gj = c.gid[j];
+    int op = -1;
+    for (int r = os; r < oe; r++)
+      if (c.old_gid[c.old_nbr[r]] == gj) {
+        op = opb + (r - os);
+        break;
+      }
+    const int p = pb + k;
+    const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
+    c.pfl_new[p] = f;
+    if (f)
+      for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
   }
 }
 
-// incoming entries in ascending owner id
+// incoming entries in ascending owner id (short rows in registers)
 __global__ __launch_bounds__(kBS) void k_sort_in(int n, const int* gid, const int* nbr_start,
                                                  const int* nbr_split, int* nbr, int* nbr_pid) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
-  const int s = nbr_start[i], e = nbr_split[i];
+  const int s = nbr_start[i], e = nbr_split[i], cnt = e - s;
+  if (cnt < 2) return;
+  if (cnt <= kRegRow) {
+    int g[kRegRow], v[kRegRow], pv[kRegRow];
+#pragma unroll
+    for (int a = 0; a < kRegRow; a++)
+      if (a < cnt) {
+        v[a] = nbr[s + a];
+        pv[a] = nbr_pid[s + a];
+      }
+#pragma unroll
+    for (int a = 0; a < kRegRow; a++)
+      if (a < cnt) g[a] = gid[v[a]];
+#pragma unroll
+    for (int a = 1; a < kRegRow; a++)
+#pragma unroll
+      for (int q = a; q > 0; q--)
+        if (a < cnt && g[q - 1] > g[q]) {
+          int t = g[q - 1]; g[q - 1] = g[q]; g[q] = t;
+          t = v[q - 1]; v[q - 1] = v[q]; v[q] = t;
+          t = pv[q - 1]; pv[q - 1] = pv[q]; pv[q] = t;
+        }
+#pragma unroll
+    for (int a = 0; a < kRegRow; a++)
+      if (a < cnt) {
+        nbr[s + a] = v[a];
+        nbr_pid[s + a] = pv[a];
+      }
+    return;
+  }
   for (int a = s + 1; a < e; a++) {
     const int j = nbr[a], pj = nbr_pid[a], gj = gid[j];
     int q = a;
@@ -1742,43 +1834,6 @@ __global__ __launch_bounds__(kBS) void k_sort_in(int n, const int* gid, const in
     }
     nbr[q] = j;
     nbr_pid[q] = pj;
-  }
-}
-
-// history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
-struct CarryPtrs {
-  const int* perm;              // new slot -> old slot
-  const int* gid;               // new order
-  const int *nbr_start, *nbr_split, *nbr, *pair_base;
-  const int* old_gid;           // old order
-  const int *old_start, *old_split, *old_nbr, *old_pair_base;
-  const double* pft_old;
-  double* pft_new;
-  const uint8_t* pfl_old;       // history-set flags of pft_old / pft_new
-  uint8_t* pfl_new;
-};
-__global__ __launch_bounds__(kBS) void k_carry(int n, CarryPtrs c) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  const int s = c.nbr_split[i], e = c.nbr_start[i + 1];
-  if (s == e) return;
-  const int o = c.perm[i];
-  const int os = c.old_split[o], oe = c.old_start[o + 1];
-  const int pb = c.pair_base[i], opb = c.old_pair_base[o];
-  for (int q = s; q < e; q++) {
-    const int gj = c.gid[c.nbr[q]];
-    int op = -1;
-    for (int r = os; r < oe; r++) {
-      if (c.old_gid[c.old_nbr[r]] == gj) {
-        op = opb + (r - os);
-        break;
-      }
-    }
-    const int p = pb + (q - s);
-    const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
-    c.pfl_new[p] = f;
-    if (f)
-      for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
   }
 }
 

@@ -438,8 +438,10 @@ int32_t do_neighbour(mfixdem_handle* h) {
       const int64_t threads = (int64_t)h->n_fine * 64;
       launch_rows_wave(h, b, threads);
     } else {
-      // half-stencil rows appended through atomic slots, then sorted
+      // half-stencil rows appended through atomic slots, then sorted (the
+      // sort also counts the incoming entries)
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
+      CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
       hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, b.pos,
                          b.radius, b.state, h->xr_pack);
       if (h->nsys > 1)
@@ -448,7 +450,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
       else
         hipLaunchKernelGGL((k_own_rows<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
                            h->G, b, h->B);
-      hipLaunchKernelGGL(k_sort_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, b);
+      hipLaunchKernelGGL(k_sort_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, b, h->in_cnt);
     }
     HIPOK(hipGetLastError());
     // own rows -> pair ids, read back with the flags in the same wait (an
@@ -478,25 +480,19 @@ int32_t do_neighbour(mfixdem_handle* h) {
   if (!nsq && fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
   CHECKRC(ensure_list_capacity(h, nl, 2 * (int64_t)npairs));
   CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
-  CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
-  hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
-                     h->own_tmp, h->maxrow, h->in_cnt);
+  if (nsq || wave_rows) {  // rows built sorted: count the incoming entries here
+    CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
+    hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
+                       h->own_tmp, h->maxrow, h->in_cnt);
+  }
   hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
                      h->own_cnt, h->tot_cnt);
   HIPOK(hipGetLastError());
   CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
   CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
-  hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
-                     h->maxrow, h->pair_base[nl], h->nbr_start[nl], h->in_cnt, h->nbr_split[nl],
-                     h->in_cur, h->nbr[nl], h->nbr_pid[nl]);
-  hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
-                     h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
-  HIPOK(hipGetLastError());
   CarryPtrs cp{};
   cp.perm = h->vals[1];
   cp.gid = h->gid[t];
-  cp.nbr_start = h->nbr_start[nl]; cp.nbr_split = h->nbr_split[nl]; cp.nbr = h->nbr[nl];
-  cp.pair_base = h->pair_base[nl];
   cp.old_gid = h->gid[1 - t];
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
@@ -504,12 +500,16 @@ int32_t do_neighbour(mfixdem_handle* h) {
   cp.pft_new = h->pft[1 - h->pcur];
   cp.pfl_old = h->pfl[h->pcur];
   cp.pfl_new = h->pfl[1 - h->pcur];
-  hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, cp);
+  hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
+                     h->pair_base[nl], h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur,
+                     h->nbr[nl], h->nbr_pid[nl], cp);
+  hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
+                     h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
   HIPOK(hipGetLastError());
   h->lt = nl;
   h->pcur = 1 - h->pcur;
   h->n_pairs = npairs;
-  // the old history buffer becomes the next step's output; k_carry reads it,
+  // the old history buffer becomes the next step's output; k_place reads it,
   // so a buffer too small is replaced only once the carry is done
   if (h->pft_cap[1 - h->pcur] < npairs || !h->pft[1 - h->pcur]) {
     HIPOK(hipStreamSynchronize(h->stream));

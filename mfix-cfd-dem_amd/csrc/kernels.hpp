@@ -396,22 +396,22 @@ __device__ __forceinline__ void keep_history(const StepPtrs& a, int pid) {
     for (int c = 0; c < 3; c++) a.pft_out[3 * pid + c] = a.pft_in[3 * pid + c];
 }
 
-// One pair entry of a row (calc_force_dem.f:210-412): particle ip is L of
-// the pair (owner, own row) or I (incoming row); partner j, pair id pid.
+// One pair entry of a row (calc_force_dem.f:210-412): particle ip (position
+// xp, radius rp) is L of the pair (owner, own row) or I (incoming row);
+// partner j, pair id pid.
 // Writes the pair's new history when ip owns it and returns 1 with the
 // increments of ip's force and torque in d[0..5] when the pair is in
 // contact; returns 0 when it adds nothing.
 template <bool MIXED, bool BATCH>
 __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a,
-                                          const BatchDev& B, int ip, int j, int pid, bool owner,
-                                          double d[6]) {
+                                          const BatchDev& B, int ip, const double xp[3],
+                                          double rp, int j, int pid, bool owner, double d[6]) {
   const int S = P0.stride;
   const DevParams& P = sys_params<BATCH>(P0, B, ip);
   const uint8_t stj = a.state[j];
-  double xj[3], xp[3];
+  double xj[3];
   load3(a.pos, S, j, xj);
-  load3(a.pos, S, ip, xp);
-  const double rj = a.radius[j], rp = a.radius[ip];
+  const double rj = a.radius[j];
   if (stj == 0) {  // IF(IS_NONEXISTENT(I)) CYCLE: history kept
     if (owner) keep_history(a, pid);
     return 0;
@@ -483,11 +483,14 @@ __device__ __forceinline__ int pair_entry(const DevParams& P0, const StepPtrs& a
 // order, so the sums are bit-identical to a per-particle loop.  Rows of a
 // nonexistent particle only carry their history over.  Every lane of the
 // wavefront must call this (lanes past n with live = false).  The lane's
-// row is [r0, r1) with its own entries from sp on (row_bounds).
+// row is [r0, r1) with its own entries from sp on; xr is the lane's own
+// position and radius, which the lanes working on its row read from LDS.
 template <bool MIXED, bool BATCH>
 __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& a,
                                            const BatchDev& B, int i, int st, int r0, int sp,
-                                           int r1, double fc[3], double tow[3]) {
+                                           int r1, const double xr[4], double fc[3],
+                                           double tow[3]) {
+  __shared__ double s_xr[kBS / 64][4][64];
   __shared__ int s_row[kBS / 64][65];
   __shared__ int s_split[kBS / 64][64];
   __shared__ int s_st[kBS / 64][64];
@@ -497,6 +500,7 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
   s_row[w][lane] = r0;
   s_split[w][lane] = sp;
   s_st[w][lane] = st;
+  for (int c = 0; c < 4; c++) s_xr[w][c][lane] = xr[c];
   if (lane == 63) s_row[w][64] = r1;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -519,7 +523,9 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
       if (s_st[w][p] == 0) {
         if (owner) keep_history(a, pid);  // rows of a removed particle keep their history
       } else {
-        hit = pair_entry<MIXED, BATCH>(P0, a, B, wbase + p, a.nbr[f], pid, owner, d);
+        const double xp[3] = {s_xr[w][0][p], s_xr[w][1][p], s_xr[w][2][p]};
+        hit = pair_entry<MIXED, BATCH>(P0, a, B, wbase + p, xp, s_xr[w][3][p], a.nbr[f], pid,
+                                       owner, d);
       }
     }
     s_hit[w][lane] = hit;
@@ -572,12 +578,15 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
     const int r0 = a.nbr_start[live ? i : n];
     const int sp = live ? a.nbr_split[i] : r0;
     const int r1 = a.nbr_start[live ? i + 1 : n];
+    double xr[4] = {0.0, 0.0, 0.0, 0.0};
     if (live && st != 0) {
       double pos[3], vel[3], om[3];
       load3(a.pos, S, i, pos);
       load3(a.vel, S, i, vel);
       load3(a.omega, S, i, om);
       const double ri = a.radius[i], mi = a.pmass[i];
+      for (int c = 0; c < 3; c++) xr[c] = pos[c];
+      xr[3] = ri;
       const int phi = a.phase[i];
       int pc = -1;
       double pv = 0.0;
@@ -620,7 +629,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
       if (wm != wm0) a.wmask[i] = wm;
     }
     // ---- pairs (calc_force_dem.f:180-412), summed after the walls
-    wave_pairs<MIXED, BATCH>(P0, a, B, i, st, r0, sp, r1, fc, tow);
+    wave_pairs<MIXED, BATCH>(P0, a, B, i, st, r0, sp, r1, xr, fc, tow);
     if (!live) return;
   }
   if (st == 0) {

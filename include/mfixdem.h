@@ -242,6 +242,22 @@ int32_t mfixdem_particles_in_cell(mfixdem_handle* h);/* particles_in_cell.f:20 *
 int32_t mfixdem_calc_interp_weights(mfixdem_handle* h);
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
 
+/* --- gas side of the drag (two-way coupling) ------------------------------ */
+/* DRAG_GS_GAS0 (drag_gs_des0.f:235-497), GARG_2012 only (else
+   MFIXDEM_ERR_UNSUPPORTED; single systems, after mfixdem_init): from the
+   current particles and gas fields, DRAG_AM and DRAG_BM at the grid nodes
+   (F_GP*w/DES_VOL_NODE and F_GP*v_p*w/DES_VOL_NODE summed in the reference's
+   cell and particle order) and F_GDS at the fluid cells (1/8 of the
+   DRAG_AM of the 8 corner nodes, 0 elsewhere).  Asynchronous on the
+   handle's stream. */
+int32_t mfixdem_drag_gs_gas(mfixdem_handle* h);
+/* The last mfixdem_drag_gs_gas results (NOT_READY before the first):
+   f_gds[ncell], drag_am[ncell] (node (i,j,k) at the index of cell (i,j,k)),
+   drag_bm[3*ncell] as DRAG_BM(IJK,1:3) (component planes); any may be
+   NULL. */
+int32_t mfixdem_get_gas_drag(mfixdem_handle* h, double* f_gds, double* drag_am,
+                             double* drag_bm);
+
 /* --- inspection for parity and restart ------------------------------------ */
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags);
 /* FC and TOW as they stood just before the integrator of the last step

@@ -1987,6 +1987,124 @@ __global__ __launch_bounds__(kBS) void k_deposit_nodes(int imax, int jmax, int k
   for (int m = 0; m < nph; m++) rops_node[(size_t)m * ncell + node] = sum[m];
 }
 
+// DRAG_GS_GAS0 (drag_gs_des0.f:235-497), the gas side of the GARG_2012 drag,
+// in three passes.  k_gas_fgp: each particle's drag coefficient F_GP
+// (DES_DRAG_GP with the gas velocity interpolated from the node velocities
+// and EP_G of its fluid cell, as on the particle side), 0 outside a fluid
+// cell.
+__global__ __launch_bounds__(kBS) void k_gas_fgp(DevParams P, StepPtrs a, double* fgp) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= P.n) return;
+  const int S = P.stride;
+  const int pc = a.fcell[i];
+  double f = 0.0;
+  if (a.state[i] != 0 && pc >= 0) {
+    int ci, cj, ck;
+    unpack_ijk(pc, ci, cj, ck);
+    const int c = lin_ijk(ci, cj, ck, P.ni, P.nj);
+    if (a.fluid[c]) {
+      double pos[3], vel[3];
+      load3(a.pos, S, i, pos);
+      load3(a.vel, S, i, vel);
+      const double x0 = a.xe[ci - 1], y0 = a.yn[cj - 1], z0 = a.zt[ck - 1];
+      const double dxx = a.xe[ci] - x0, dyy = a.yn[cj] - y0, dzz = a.zt[ck] - z0;
+      double zx = pos[0] - x0, zy = pos[1] - y0;
+      zx = zx / dxx;
+      zy = zy / dyy;
+      double zz = pos[2] - z0;
+      zz = zz / dzz;
+      const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+      double velfp[3] = {0.0, 0.0, 0.0};
+      const int base = (cj - 2) + (ci - 2) * P.nj + (ck - 2) * P.nj * P.ni;
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++) {
+            const double w = xv[ii] * yv[jj] * zv[kk];
+            const double* vn = a.vnode + 3 * (base + jj + ii * P.nj + kk * P.nj * P.ni);
+            for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + vn[d] * w;
+          }
+      const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+      const double vrel = sqrt(dot3(vs, vs));
+      f = drag_fgp_fp64(P.drag_type, a.ep_g[c], P.mu_g0, P.ro_g0, vrel, 2.0 * a.radius[i],
+                        a.pvol[i]);
+    }
+  }
+  fgp[i] = f;
+}
+
+// k_gas_nodes: DRAG_AM and DRAG_BM of every node, gathered in the
+// reference's order -- the particle's cells in ascending IJK (K, then I,
+// then J fastest), each cell's particles in PIC order (ascending id) --
+// so each node's sum is the reference's: F_GP*w*OVOL and F_GP*v*w*OVOL
+// with OVOL = 1/DES_VOL_NODE.  Nodes outside [1, imax+1] x [1, jmax+1] x
+// [1, kmax+1] get 0.
+__global__ __launch_bounds__(kBS) void k_gas_nodes(int imax, int jmax, int kmax, int ni, int nj,
+                                                   int nk, int S, const double* xe,
+                                                   const double* yn, const double* zt,
+                                                   const uint8_t* fluid, const int* off,
+                                                   const int* pic, const uint8_t* state,
+                                                   const double* pos, const double* vel,
+                                                   const double* fgp, const double* vol_node,
+                                                   double* drag_am, double* drag_bm) {
+  const int node = blockIdx.x * kBS + threadIdx.x;
+  const int ncell = ni * nj * nk;
+  if (node >= ncell) return;
+  const int J = node % nj + 1;
+  const int I = (node / nj) % ni + 1;
+  const int K = node / (nj * ni) + 1;
+  double am = 0.0, bm[3] = {0.0, 0.0, 0.0};
+  if (I <= imax + 1 && J <= jmax + 1 && K <= kmax + 1) {
+    const double ovol = 1.0 / vol_node[node];
+    for (int c = 0; c < 2; c++)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+          const int ci = I + a, cj = J + b, ck = K + c;
+          const int cell = (cj - 1) + (ci - 1) * nj + (ck - 1) * nj * ni;
+          if (!fluid[cell]) continue;
+          const int s0 = off[cell], e0 = off[cell + 1];
+          if (s0 == e0) continue;
+          const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
+          const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
+          for (int q = s0; q < e0; q++) {
+            const int p = pic[q];
+            if (state[p] == 0) continue;
+            double zx = pos[p] - x0, zy = pos[S + p] - y0;
+            zx = zx / dxx;
+            zy = zy / dyy;
+            double zz = pos[2 * S + p] - z0;
+            zz = zz / dzz;
+            // this node is corner (1-a, 1-b, 1-c) of the cell
+            const double xv = a ? 1.0 - zx : zx;
+            const double yv = b ? 1.0 - zy : zy;
+            const double zv = c ? 1.0 - zz : zz;
+            const double w = xv * yv * zv;
+            const double f = fgp[p];
+            am = am + f * w * ovol;
+            for (int d = 0; d < 3; d++) bm[d] = bm[d] + f * vel[d * S + p] * w * ovol;
+          }
+        }
+  }
+  drag_am[node] = am;
+  for (int d = 0; d < 3; d++) drag_bm[(size_t)d * ncell + node] = bm[d];
+}
+
+// k_gas_fgds: F_GDS of a fluid cell, 1/8 of the DRAG_AM of its 8 corner
+// nodes in the reference's sum order (drag_gs_des0.f:455-494), else 0.
+__global__ __launch_bounds__(kBS) void k_gas_fgds(int ni, int nj, int nk, const uint8_t* fluid,
+                                                  const double* am, double* f_gds) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  if (c >= ni * nj * nk) return;
+  if (!fluid[c]) {
+    f_gds[c] = 0.0;
+    return;
+  }
+  // a fluid cell is interior, so its west, south and bottom nodes exist
+  const int di = nj, dk = nj * ni;
+  double f = 0.125 * (am[c] + am[c - 1] + am[c - di - 1] + am[c - di]);
+  f = f + 0.125 * (am[c - dk] + am[c - dk - 1] + am[c - dk - di - 1] + am[c - dk - di]);
+  f_gds[c] = f;
+}
+
 // node -> cell ROP_S (comp_mean_fields0.f:298-345), ROP_S/VOL (:347-370) and
 // CALC_EPG_DES (calc_epg_des.f:95-150).  Nodes visited K, J, I ascending.
 // FIXED: node sums come from the fixed-point accumulators (converted exactly

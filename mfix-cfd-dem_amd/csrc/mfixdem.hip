@@ -106,6 +106,11 @@ struct mfixdem_handle {
          *vol_surr = nullptr, *ro_s = nullptr;
   uint8_t* fluid = nullptr;
   int *pic_cnt = nullptr, *pic_off = nullptr, *pic_cur = nullptr, *pic = nullptr;
+  // gas side of the drag (DRAG_GS_GAS0), allocated on first use: F_GP per
+  // particle slot, DES_VOL_NODE, DRAG_AM, DRAG_BM (3 planes), F_GDS
+  double *gd_fgp = nullptr, *gd_vol_node = nullptr, *gd_am = nullptr, *gd_bm = nullptr,
+         *gd_fgds = nullptr;
+  bool gas_drag_ready = false;
   unsigned long long* rops_fix[2]{};  // fixed-point node sums, double-buffered
   int* dep_list = nullptr;            // deposition order (slots by fluid-cell Morton code)
   int fb = 0;
@@ -811,6 +816,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->zt); f(h->vnode); f(h->ugc); f(h->p_force); f(h->ep_g); f(h->rop_s);
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
+  f(h->gd_fgp); f(h->gd_vol_node); f(h->gd_am); f(h->gd_bm); f(h->gd_fgds);
   f(h->cell_facets); f(h->stage);
   f(h->d_sys); f(h->d_ptab); f(h->d_gtab); f(h->d_cell_off); f(h->d_dg_off); f(h->d_rogue);
 }
@@ -1283,6 +1289,77 @@ int32_t mfixdem_comp_mean_fields(mfixdem_handle* h) {
     return do_cell_fields(h, false);
   }
   return deposit_current(h);
+}
+
+// DES_VOL_NODE (compute_volume_of_nodes, cfassign.f:100-230) of the uniform
+// grid, in the reference's sum order, and the buffers of the gas-side drag.
+int32_t ensure_gas_drag(mfixdem_handle* h) {
+  if (h->gd_am) return MFIXDEM_OK;
+  const mfixdem_params& p = h->S.p;
+  const int ni = h->S.ni, nj = h->S.nj, nc = h->S.ncell;
+  const double dx = p.xlength / (double)p.imax, dy = p.ylength / (double)p.jmax,
+               dz = p.zlength / (double)p.kmax;
+  const double v = dx * dy * dz, af = 0.125;
+  auto cell = [&](int i, int j, int k) { return (j - 1) + (i - 1) * nj + (k - 1) * nj * ni; };
+  auto volf = [&](int i, int j, int k) { return h->S.fluid[cell(i, j, k)] ? v : 0.0; };
+  std::vector<double> vn((size_t)nc, 0.0);
+  for (int k = 1; k <= p.kmax + 1; k++)
+    for (int j = 1; j <= p.jmax + 1; j++)
+      for (int i = 1; i <= p.imax + 1; i++) {
+        double a = af * (volf(i, j, k) + volf(i + 1, j, k) + volf(i, j + 1, k) +
+                         volf(i + 1, j + 1, k));
+        a = a + af * (volf(i, j, k + 1) + volf(i + 1, j + 1, k + 1) + volf(i, j + 1, k + 1) +
+                      volf(i + 1, j, k + 1));
+        vn[cell(i, j, k)] = a;
+      }
+  int32_t rc;
+  if ((rc = dalloc(&h->gd_fgp, h->stride)) || (rc = dalloc(&h->gd_vol_node, nc)) ||
+      (rc = dalloc(&h->gd_am, nc)) || (rc = dalloc(&h->gd_bm, 3 * (int64_t)nc)) ||
+      (rc = dalloc(&h->gd_fgds, nc)))
+    return rc;
+  HIPOK(hipMemcpy(h->gd_vol_node, vn.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_drag_gs_gas(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  const mfixdem_params& p = h->S.p;
+  if (h->nsys > 1 || !p.coupled || p.interp_scheme != MFIXDEM_INTERP_GARG_2012)
+    return MFIXDEM_ERR_UNSUPPORTED;
+  if (!h->inited) return MFIXDEM_ERR_NOT_READY;
+  CHECKRC(ensure_gas_drag(h));
+  CHECKRC(do_pic_lists(h));
+  const int nc = h->S.ncell;
+  if (h->n > 0)
+    hipLaunchKernelGGL(k_gas_fgp, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h),
+                       h->gd_fgp);
+  hipLaunchKernelGGL(k_gas_nodes, dim3(nblk(nc)), dim3(kBS), 0, h->stream, p.imax, p.jmax, p.kmax,
+                     h->S.ni, h->S.nj, h->S.nk, h->stride, h->xe, h->yn, h->zt, h->fluid,
+                     h->pic_off, h->pic, h->state[h->tw], h->pos[h->cur], h->vel[h->cur],
+                     h->gd_fgp, h->gd_vol_node, h->gd_am, h->gd_bm);
+  hipLaunchKernelGGL(k_gas_fgds, dim3(nblk(nc)), dim3(kBS), 0, h->stream, h->S.ni, h->S.nj,
+                     h->S.nk, h->fluid, h->gd_am, h->gd_fgds);
+  HIPOK(hipGetLastError());
+  h->gas_drag_ready = true;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_get_gas_drag(mfixdem_handle* h, double* f_gds, double* drag_am,
+                             double* drag_bm) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (!h->gas_drag_ready) return MFIXDEM_ERR_NOT_READY;
+  const size_t nc = (size_t)h->S.ncell;
+  if (f_gds)
+    HIPOK(hipMemcpyAsync(f_gds, h->gd_fgds, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                         h->stream));
+  if (drag_am)
+    HIPOK(hipMemcpyAsync(drag_am, h->gd_am, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                         h->stream));
+  if (drag_bm)
+    HIPOK(hipMemcpyAsync(drag_bm, h->gd_bm, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
+                         h->stream));
+  HIPOK(hipStreamSynchronize(h->stream));
+  return MFIXDEM_OK;
 }
 
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {

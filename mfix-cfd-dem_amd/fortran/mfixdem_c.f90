@@ -156,6 +156,17 @@ module mfixdem_c
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function
+     ! DRAG_GS_GAS0 (drag_gs_des0.f:235): F_GDS, DRAG_AM, DRAG_BM for the gas solver
+     integer(c_int32_t) function mfixdem_drag_gs_gas(h) bind(C, name="mfixdem_drag_gs_gas")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function
+     integer(c_int32_t) function mfixdem_get_gas_drag(h, f_gds, drag_am, drag_bm) &
+          bind(C, name="mfixdem_get_gas_drag")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: f_gds(*), drag_am(*), drag_bm(*)
+     end function
      integer(c_int32_t) function mfixdem_set_debug(h, flags) bind(C, name="mfixdem_set_debug")
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h

@@ -202,6 +202,8 @@ def _declare(lib):
                                             C.c_int64], _I),
         "mfixdem_ensemble_get_particles": ([_P, _I, _DP, _DP, _DP, _BP, C.c_int64], _I),
         "mfixdem_reset_kernel_times": ([_P], _I),
+        "mfixdem_drag_gs_gas": ([_P], _I),
+        "mfixdem_get_gas_drag": ([_P, _DP, _DP, _DP], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -112,6 +112,17 @@ class DemSystem:
         abi.check(self.lib.mfixdem_get_p_force(self.h, abi.dptr(out)), "get_p_force")
         return out.reshape(self.ncell, 3)
 
+    def drag_gs_gas(self):
+        """DRAG_GS_GAS0 (drag_gs_des0.f:235-497) on the current particles:
+        returns (F_GDS[ncell], DRAG_AM[ncell], DRAG_BM[3, ncell])."""
+        abi.check(self.lib.mfixdem_drag_gs_gas(self.h), "drag_gs_gas")
+        f = np.zeros(self.ncell)
+        am = np.zeros(self.ncell)
+        bm = np.zeros(3 * self.ncell)
+        abi.check(self.lib.mfixdem_get_gas_drag(self.h, abi.dptr(f), abi.dptr(am), abi.dptr(bm)),
+                  "get_gas_drag")
+        return f, am, bm.reshape(3, self.ncell)
+
     def calc_interp_weights(self):
         abi.check(self.lib.mfixdem_calc_interp_weights(self.h), "calc_interp_weights")
 

@@ -2029,6 +2029,105 @@ int orc_comp_mean_fields(orc_sys* s) {
   return calc_epg_des(s);
 }
 
+/* DES_VOL_NODE, compute_volume_of_nodes (cfassign.f:100-230): the volume
+   around each grid node (i, j, k) in [1, imax+1] x [1, jmax+1] x [1, kmax+1]
+   from the fluid cells of its 2x2x2 block, in the reference's sum order
+   (the upper K layer is added as ijkp, ipjpkp, ijpkp, ipjkp).  Uniform grid:
+   every cell has the volume dx*dy*dz. */
+static void des_vol_node(const orc_sys* s, double* vn) {
+  const mfixdem_params* p = &s->p;
+  const double af = 0.125;
+  memset(vn, 0, sizeof(double) * (size_t)s->ncell);
+  for (int k = 1; k <= p->kmax + 1; k++)
+    for (int j = 1; j <= p->jmax + 1; j++)
+      for (int i = 1; i <= p->imax + 1; i++) {
+        const double v = s->dx * s->dy * s->dz;
+#define VOLF(I, J, K) (fluid_at(s, (I), (J), (K)) ? v : 0.0)
+        double a = af * (VOLF(i, j, k) + VOLF(i + 1, j, k) + VOLF(i, j + 1, k) +
+                         VOLF(i + 1, j + 1, k));
+        a = a + af * (VOLF(i, j, k + 1) + VOLF(i + 1, j + 1, k + 1) + VOLF(i, j + 1, k + 1) +
+                      VOLF(i + 1, j, k + 1));
+#undef VOLF
+        vn[fluid_ijk(s, i, j, k) - 1] = a;
+      }
+}
+
+/* DRAG_GS_GAS0 (drag_gs_des0.f:235-497): the gas side of the GARG_2012
+   drag.  For every particle of every fluid cell (cell order, then PIC
+   order) the drag coefficient F_GP of DES_DRAG_GP with the interpolated
+   gas velocity, and its trilinear node weights; each node gets
+   F_GP*w/DES_VOL_NODE in DRAG_AM and F_GP*v_p*w/DES_VOL_NODE in DRAG_BM
+   (WTP = 1 for DEM).  F_GDS of a fluid cell is 1/8 of the DRAG_AM of its 8
+   corner nodes (:455-494), 0 elsewhere.  Outputs: drag_am[ncell] and
+   f_gds[ncell] in IJK order, drag_bm[3*ncell] as planes (d*ncell + ijk). */
+int orc_drag_gs_gas0(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm) {
+  if (!s->p.coupled || s->p.interp_scheme != MFIXDEM_INTERP_GARG_2012)
+    return MFIXDEM_ERR_UNSUPPORTED;
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+  size_t C = (size_t)s->ncell;
+  double* vn = malloc(sizeof(double) * C);
+  if (!vn) return MFIXDEM_ERR_OOM;
+  des_vol_node(s, vn);
+  memset(drag_am, 0, sizeof(double) * C);
+  memset(drag_bm, 0, sizeof(double) * 3 * C);
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk] || s->pinc[ijk] == 0) continue;
+    int z = ijk;
+    int j = z % s->nj + 1;
+    z /= s->nj;
+    int i = z % s->ni + 1;
+    int k = z / s->ni + 1;
+    double gst[2][2][2][3], vst[2][2][2][3];
+    cell_nodes(s, i, j, k, gst, vst);
+    for (int t = s->pic_off[ijk]; t < s->pic_off[ijk + 1]; t++) {
+      int np = s->pic_list[t];
+      if (s->state[np] == NONEXISTENT) continue;
+      double pos[3] = {s->pos[np], s->pos[L + np], s->pos[2 * L + np]};
+      double w[2][2][2];
+      trilinear_weights(gst, pos, w);
+      double velfp[3] = {0.0, 0.0, 0.0};
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++)
+            for (int c = 0; c < 3; c++) velfp[c] = velfp[c] + vst[kk][jj][ii][c] * w[kk][jj][ii];
+      double vel[3] = {s->vel[np], s->vel[L + np], s->vel[2 * L + np]};
+      double vslp[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+      double vrel = sqrt(dot3(vslp, vslp));
+      double fgp = orc_drag_fgp(p->drag_type, s->ep_g[ijk], p->mu_g0, p->ro_g0, vrel,
+                                2.0 * s->radius[np], s->pvol[np]);
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++) {
+            int cur = fluid_ijk(s, i - 1 + ii, j - 1 + jj, k - 1 + kk) - 1;
+            double ovol = 1.0 / vn[cur];
+            drag_am[cur] = drag_am[cur] + fgp * w[kk][jj][ii] * ovol;
+            for (int d = 0; d < 3; d++)
+              drag_bm[d * C + cur] = drag_bm[d * C + cur] + fgp * vel[d] * w[kk][jj][ii] * ovol;
+          }
+    }
+  }
+  for (int ijk = 0; ijk < s->ncell; ijk++) {
+    if (!s->fluid[ijk]) {
+      f_gds[ijk] = 0.0;
+      continue;
+    }
+    int z = ijk;
+    int j = z % s->nj + 1;
+    z /= s->nj;
+    int i = z % s->ni + 1;
+    int k = z / s->ni + 1;
+#define AM(I, J, K) drag_am[fluid_ijk(s, (I), (J), (K)) - 1]
+    double f = 0.125 * (AM(i, j, k) + AM(i, j - 1, k) + AM(i - 1, j - 1, k) + AM(i - 1, j, k));
+    f = f + 0.125 * (AM(i, j, k - 1) + AM(i, j - 1, k - 1) + AM(i - 1, j - 1, k - 1) +
+                     AM(i - 1, j, k - 1));
+#undef AM
+    f_gds[ijk] = f;
+  }
+  free(vn);
+  return 0;
+}
+
 /* ------------------------------------------------------- driver */
 int orc_desgrid_and_neighbour(orc_sys* s) {
   orc_desgrid_pic(s);

@@ -538,3 +538,42 @@ def test_kernel_timing_mode_matches_plain_run():
     assert min(ms[abi.TIME_STEP], ms[abi.TIME_GAS], ms[abi.TIME_REBUILD]) > 0.0
     a.close()
     c.close()
+
+
+def test_gas_side_drag_matches_oracle():
+    """DRAG_GS_GAS0 (mfixdem_drag_gs_gas): F_GDS, DRAG_AM and DRAG_BM against
+    the oracle.  At set-up with the reference-order deposition EP_G is
+    bit-exact, so the node sums (gathered in the reference's cell and
+    particle order) differ only where Wen-Yu's pow differs by an ulp:
+    1e-13 of the largest value.  After 30 coupled steps the particles
+    differ by <= 1e-12 m, and so do the drag fields, to 1e-9."""
+    p, parts, gas = fluidised(n=3000)
+    g, o = both(p, parts, gas, debug=abi.DEBUG_REFERENCE_DEPOSITION)
+    f = np.zeros(g.ncell)
+    rc = g.lib.mfixdem_get_gas_drag(g.h, abi.dptr(f), None, None)
+    assert rc == abi.ERRORS_BY_NAME["NOT_READY"]
+    assert np.array_equal(g.get_gas()[0], o.get_gas()[0])
+    for tol, steps in ((1e-13, 0), (1e-9, 30)):
+        if steps:
+            g.run(1, steps)
+            o.run(1, steps)
+        fg, ag, bg = g.drag_gs_gas()
+        fo, ao, bo = o.drag_gs_gas()
+        assert fo.max() > 0.0 and ao.max() > 0.0
+        assert np.max(np.abs(fg - fo)) <= tol * fo.max()
+        assert np.max(np.abs(ag - ao)) <= tol * ao.max()
+        assert np.max(np.abs(bg - bo)) <= tol * np.max(np.abs(bo))
+        assert np.array_equal(fg == 0.0, fo == 0.0)
+    g.close()
+    o.close()
+    # two-way coupling is built for GARG_2012 only
+    p, parts, gas = fluidised(n=500)
+    p.interp_scheme = abi.INTERP_SQUARE_DPVM
+    p.des_interp_on = 1
+    p.des_interp_width = 1.6e-3
+    g = DemSystem(p)
+    g.set_particles(parts)
+    g.set_gas(gas)
+    g.init()
+    assert g.lib.mfixdem_drag_gs_gas(g.h) == abi.ERRORS_BY_NAME["UNSUPPORTED"]
+    g.close()

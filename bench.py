@@ -241,6 +241,22 @@ def serving_bench(n_req, log):
     return rep
 
 
+def two_way_bench(s, reps=20):
+    """Gas side of the drag (DRAG_GS_GAS0: PARTICLES_IN_CELL lists, F_GP per
+    particle, DRAG_AM/DRAG_BM node gather in the reference's order, F_GDS)
+    on the current state: host wall time per call, stream synchronised."""
+    abi.check(s.lib.mfixdem_drag_gs_gas(s.h), "drag_gs_gas")   # buffers, node volumes
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.check(s.lib.mfixdem_drag_gs_gas(s.h), "drag_gs_gas")
+    s.synchronize()
+    us = (time.perf_counter() - t0) / reps * 1e6
+    f_gds, _, _ = s.drag_gs_gas()
+    return {"us_per_call": us, "calls": reps, "f_gds_max": float(f_gds.max()),
+            "routine": "DRAG_GS_GAS0 (F_GDS, DRAG_AM, DRAG_BM), reference summation order"}
+
+
 def cpu_baseline(p, parts, gas, seconds=15.0):
     """Times the OpenMP CPU restatement (oracle/liboracle_omp.so) on the same
     workload for as many whole steps as fit in about `seconds`."""
@@ -311,6 +327,8 @@ def main():
         f"gas {kt[abi.TIME_GAS] * 1e3:.1f} us")
     nsteps_total = args.warmup + args.steps + 50
     pos64, vel64, om64, _ = s.get_particles()
+    two_way = two_way_bench(s)
+    log(f"two-way coupling: DRAG_GS_GAS0 {two_way['us_per_call']:.1f} us per call")
     s.close()
 
     traffic = None
@@ -385,6 +403,7 @@ def main():
         "ensemble": ens,
         "serving": srv,
         "kernel_groups_us": {"step": kt[0] * 1e3, "rebuild": kt[1] * 1e3, "gas": kt[2] * 1e3},
+        "two_way": two_way,
     }
     print(json.dumps(out))
 

@@ -1990,14 +1990,17 @@ __global__ __launch_bounds__(kBS) void k_deposit_nodes(int imax, int jmax, int k
 // DRAG_GS_GAS0 (drag_gs_des0.f:235-497), the gas side of the GARG_2012 drag,
 // in three passes.  k_gas_fgp: each particle's drag coefficient F_GP
 // (DES_DRAG_GP with the gas velocity interpolated from the node velocities
-// and EP_G of its fluid cell, as on the particle side), 0 outside a fluid
-// cell.
-__global__ __launch_bounds__(kBS) void k_gas_fgp(DevParams P, StepPtrs a, double* fgp) {
+// and EP_G of its fluid cell, as on the particle side) into a 64-byte record
+// rec[8 i ..]: the normalised position in its cell (zx, zy, zz), F_GP, and
+// F_GP * v_p (the first product of the reference's DRAG_BM term); all 0
+// for a particle outside a fluid cell or removed, which adds nothing.
+constexpr int kGasRec = 8;
+__global__ __launch_bounds__(kBS) void k_gas_fgp(DevParams P, StepPtrs a, double* rec) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= P.n) return;
   const int S = P.stride;
   const int pc = a.fcell[i];
-  double f = 0.0;
+  double r[kGasRec] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (a.state[i] != 0 && pc >= 0) {
     int ci, cj, ck;
     unpack_ijk(pc, ci, cj, ck);
@@ -2025,27 +2028,30 @@ __global__ __launch_bounds__(kBS) void k_gas_fgp(DevParams P, StepPtrs a, double
           }
       const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
       const double vrel = sqrt(dot3(vs, vs));
-      f = drag_fgp_fp64(P.drag_type, a.ep_g[c], P.mu_g0, P.ro_g0, vrel, 2.0 * a.radius[i],
-                        a.pvol[i]);
+      const double f = drag_fgp_fp64(P.drag_type, a.ep_g[c], P.mu_g0, P.ro_g0, vrel,
+                                     2.0 * a.radius[i], a.pvol[i]);
+      r[0] = zx;
+      r[1] = zy;
+      r[2] = zz;
+      r[3] = f;
+      for (int d = 0; d < 3; d++) r[4 + d] = f * vel[d];
     }
   }
-  fgp[i] = f;
+  double2* o = reinterpret_cast<double2*>(rec) + (kGasRec / 2) * (size_t)i;
+  for (int k = 0; k < kGasRec / 2; k++) o[k] = make_double2(r[2 * k], r[2 * k + 1]);
 }
 
 // k_gas_nodes: DRAG_AM and DRAG_BM of every node, gathered in the
 // reference's order -- the particle's cells in ascending IJK (K, then I,
 // then J fastest), each cell's particles in PIC order (ascending id) --
-// so each node's sum is the reference's: F_GP*w*OVOL and F_GP*v*w*OVOL
-// with OVOL = 1/DES_VOL_NODE.  Nodes outside [1, imax+1] x [1, jmax+1] x
-// [1, kmax+1] get 0.
+// so each node's sum is the reference's: F_GP*w*OVOL and (F_GP*v)*w*OVOL
+// with OVOL = 1/DES_VOL_NODE, w from the particle's record.  Nodes outside
+// [1, imax+1] x [1, jmax+1] x [1, kmax+1] get 0.
 __global__ __launch_bounds__(kBS) void k_gas_nodes(int imax, int jmax, int kmax, int ni, int nj,
-                                                   int nk, int S, const double* xe,
-                                                   const double* yn, const double* zt,
-                                                   const uint8_t* fluid, const int* off,
-                                                   const int* pic, const uint8_t* state,
-                                                   const double* pos, const double* vel,
-                                                   const double* fgp, const double* vol_node,
-                                                   double* drag_am, double* drag_bm) {
+                                                   int nk, const uint8_t* fluid, const int* off,
+                                                   const int* pic, const double* rec,
+                                                   const double* vol_node, double* drag_am,
+                                                   double* drag_bm) {
   const int node = blockIdx.x * kBS + threadIdx.x;
   const int ncell = ni * nj * nk;
   if (node >= ncell) return;
@@ -2062,25 +2068,19 @@ __global__ __launch_bounds__(kBS) void k_gas_nodes(int imax, int jmax, int kmax,
           const int cell = (cj - 1) + (ci - 1) * nj + (ck - 1) * nj * ni;
           if (!fluid[cell]) continue;
           const int s0 = off[cell], e0 = off[cell + 1];
-          if (s0 == e0) continue;
-          const double x0 = xe[ci - 1], y0 = yn[cj - 1], z0 = zt[ck - 1];
-          const double dxx = xe[ci] - x0, dyy = yn[cj] - y0, dzz = zt[ck] - z0;
           for (int q = s0; q < e0; q++) {
-            const int p = pic[q];
-            if (state[p] == 0) continue;
-            double zx = pos[p] - x0, zy = pos[S + p] - y0;
-            zx = zx / dxx;
-            zy = zy / dyy;
-            double zz = pos[2 * S + p] - z0;
-            zz = zz / dzz;
+            const double2* r = reinterpret_cast<const double2*>(rec) + (kGasRec / 2) *
+                                                                        (size_t)pic[q];
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
             // this node is corner (1-a, 1-b, 1-c) of the cell
-            const double xv = a ? 1.0 - zx : zx;
-            const double yv = b ? 1.0 - zy : zy;
-            const double zv = c ? 1.0 - zz : zz;
+            const double xv = a ? 1.0 - r0.x : r0.x;
+            const double yv = b ? 1.0 - r0.y : r0.y;
+            const double zv = c ? 1.0 - r1.x : r1.x;
             const double w = xv * yv * zv;
-            const double f = fgp[p];
-            am = am + f * w * ovol;
-            for (int d = 0; d < 3; d++) bm[d] = bm[d] + f * vel[d * S + p] * w * ovol;
+            am = am + r1.y * w * ovol;
+            bm[0] = bm[0] + r2.x * w * ovol;
+            bm[1] = bm[1] + r2.y * w * ovol;
+            bm[2] = bm[2] + r3.x * w * ovol;
           }
         }
   }

@@ -106,8 +106,8 @@ struct mfixdem_handle {
          *vol_surr = nullptr, *ro_s = nullptr;
   uint8_t* fluid = nullptr;
   int *pic_cnt = nullptr, *pic_off = nullptr, *pic_cur = nullptr, *pic = nullptr;
-  // gas side of the drag (DRAG_GS_GAS0), allocated on first use: F_GP per
-  // particle slot, DES_VOL_NODE, DRAG_AM, DRAG_BM (3 planes), F_GDS
+  // gas side of the drag (DRAG_GS_GAS0), allocated on first use: the F_GP
+  // record per particle slot, DES_VOL_NODE, DRAG_AM, DRAG_BM (3 planes), F_GDS
   double *gd_fgp = nullptr, *gd_vol_node = nullptr, *gd_am = nullptr, *gd_bm = nullptr,
          *gd_fgds = nullptr;
   bool gas_drag_ready = false;
@@ -1313,7 +1313,8 @@ int32_t ensure_gas_drag(mfixdem_handle* h) {
         vn[cell(i, j, k)] = a;
       }
   int32_t rc;
-  if ((rc = dalloc(&h->gd_fgp, h->stride)) || (rc = dalloc(&h->gd_vol_node, nc)) ||
+  if ((rc = dalloc(&h->gd_fgp, (int64_t)kGasRec * h->stride)) ||
+      (rc = dalloc(&h->gd_vol_node, nc)) ||
       (rc = dalloc(&h->gd_am, nc)) || (rc = dalloc(&h->gd_bm, 3 * (int64_t)nc)) ||
       (rc = dalloc(&h->gd_fgds, nc)))
     return rc;
@@ -1334,9 +1335,8 @@ int32_t mfixdem_drag_gs_gas(mfixdem_handle* h) {
     hipLaunchKernelGGL(k_gas_fgp, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h),
                        h->gd_fgp);
   hipLaunchKernelGGL(k_gas_nodes, dim3(nblk(nc)), dim3(kBS), 0, h->stream, p.imax, p.jmax, p.kmax,
-                     h->S.ni, h->S.nj, h->S.nk, h->stride, h->xe, h->yn, h->zt, h->fluid,
-                     h->pic_off, h->pic, h->state[h->tw], h->pos[h->cur], h->vel[h->cur],
-                     h->gd_fgp, h->gd_vol_node, h->gd_am, h->gd_bm);
+                     h->S.ni, h->S.nj, h->S.nk, h->fluid, h->pic_off, h->pic, h->gd_fgp,
+                     h->gd_vol_node, h->gd_am, h->gd_bm);
   hipLaunchKernelGGL(k_gas_fgds, dim3(nblk(nc)), dim3(kBS), 0, h->stream, h->S.ni, h->S.nj,
                      h->S.nk, h->fluid, h->gd_am, h->gd_fgds);
   HIPOK(hipGetLastError());

@@ -243,13 +243,18 @@ int32_t mfixdem_calc_interp_weights(mfixdem_handle* h);
 int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
 
 /* --- gas side of the drag (two-way coupling) ------------------------------ */
-/* DRAG_GS_GAS0 (drag_gs_des0.f:235-497), GARG_2012 only (else
-   MFIXDEM_ERR_UNSUPPORTED; single systems, after mfixdem_init): from the
-   current particles and gas fields, DRAG_AM and DRAG_BM at the grid nodes
-   (F_GP*w/DES_VOL_NODE and F_GP*v_p*w/DES_VOL_NODE summed in the reference's
-   cell and particle order) and F_GDS at the fluid cells (1/8 of the
-   DRAG_AM of the 8 corner nodes, 0 elsewhere).  Asynchronous on the
-   handle's stream. */
+/* The drag call of CALC_DES_2FLUID (calc_des_2fluid.f:70-73), for a
+   coupled single system after mfixdem_init (else MFIXDEM_ERR_UNSUPPORTED /
+   NOT_READY), from the current particles and gas fields; asynchronous on
+   the handle's stream.
+   GARG_2012: DRAG_GS_GAS0 (drag_gs_des0.f:235-497): DRAG_AM and DRAG_BM at
+   the grid nodes (F_GP*w/DES_VOL_NODE and F_GP*v_p*w/DES_VOL_NODE summed in
+   the reference's cell and particle order) and F_GDS at the fluid cells
+   (1/8 of the DRAG_AM of the 8 corner nodes, 0 elsewhere).
+   Other schemes: DRAG_GS_GAS1 (drag_gs_des1.f:179-301): F_GDS and DRAG_BM
+   at the cells (F_GP*w/VOL and F_GP*v_p*w/VOL over each particle's filter
+   cells, added with atomics as the reference's OpenMP loop does); DRAG_AM
+   is 0. */
 int32_t mfixdem_drag_gs_gas(mfixdem_handle* h);
 /* The last mfixdem_drag_gs_gas results (NOT_READY before the first):
    f_gds[ncell], drag_am[ncell] (node (i,j,k) at the index of cell (i,j,k)),

@@ -2088,6 +2088,56 @@ __global__ __launch_bounds__(kBS) void k_gas_nodes(int imax, int jmax, int kmax,
   for (int d = 0; d < 3; d++) drag_bm[(size_t)d * ncell + node] = bm[d];
 }
 
+// DRAG_GS_GAS1 (drag_gs_des1.f:179-301), the gas side of the drag for the
+// filter schemes: each normal particle in a fluid cell interpolates EP_G and
+// the cell-centre gas velocity with its filter weights (as add_drag_des1
+// does), takes F_GP from DES_DRAG_GP, and adds F_GP*w/VOL to F_GDS and
+// (F_GP*v_p)*w/VOL to DRAG_BM of each filter cell (its own cell with weight
+// 1 without DES_INTERP_ON).  The reference adds with OpenMP atomics, in no
+// fixed order; so does this kernel, with FP64 atomics, so the cell sums
+// equal the reference's to rounding.
+__global__ __launch_bounds__(kBS) void k_gas_filter(DevParams P, StepPtrs a, const double* vol,
+                                                    int ncell, double* f_gds, double* drag_bm) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= P.n || a.state[i] != 1) return;  // IS_NORMAL
+  const int pc = a.fcell[i];
+  if (pc < 0) return;
+  int ci, cj, ck;
+  unpack_ijk(pc, ci, cj, ck);
+  const int c0 = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  if (!a.fluid[c0]) return;
+  const int S = P.stride;
+  double pos[3], vel[3];
+  load3(a.pos, S, i, pos);
+  load3(a.vel, S, i, vel);
+  const bool filt = P.interp_on && filter_scheme(P.interp);
+  double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0};
+  if (filt) {
+    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+      lepg = lepg + a.ep_g[c] * w;
+      for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
+    });
+  } else {
+    lepg = a.ep_g[c0];
+    for (int d = 0; d < 3; d++) velfp[d] = a.ugc[3 * c0 + d];
+  }
+  if (lepg < 1.0e-6) lepg = a.ep_g[c0];
+  const double vs[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+  const double vrel = sqrt(dot3(vs, vs));
+  const double f = drag_fgp_fp64(P.drag_type, lepg, P.mu_g0, P.ro_g0, vrel, 2.0 * a.radius[i],
+                                 a.pvol[i]);
+  const double lbm[3] = {f * vel[0], f * vel[1], f * vel[2]};
+  auto add = [&](int c, double wt) {
+    for (int d = 0; d < 3; d++) atomicAdd(&drag_bm[(size_t)d * ncell + c], lbm[d] * wt);
+    atomicAdd(&f_gds[c], f * wt);
+  };
+  if (filt)
+    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos,
+               [&](int c, double w) { add(c, w / vol[c]); });
+  else
+    add(c0, 1.0 / vol[c0]);
+}
+
 // k_gas_fgds: F_GDS of a fluid cell, 1/8 of the DRAG_AM of its 8 corner
 // nodes in the reference's sum order (drag_gs_des0.f:455-494), else 0.
 __global__ __launch_bounds__(kBS) void k_gas_fgds(int ni, int nj, int nk, const uint8_t* fluid,

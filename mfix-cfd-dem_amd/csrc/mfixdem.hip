@@ -1325,12 +1325,23 @@ int32_t ensure_gas_drag(mfixdem_handle* h) {
 int32_t mfixdem_drag_gs_gas(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
   const mfixdem_params& p = h->S.p;
-  if (h->nsys > 1 || !p.coupled || p.interp_scheme != MFIXDEM_INTERP_GARG_2012)
-    return MFIXDEM_ERR_UNSUPPORTED;
+  if (h->nsys > 1 || !p.coupled) return MFIXDEM_ERR_UNSUPPORTED;
   if (!h->inited) return MFIXDEM_ERR_NOT_READY;
   CHECKRC(ensure_gas_drag(h));
-  CHECKRC(do_pic_lists(h));
   const int nc = h->S.ncell;
+  if (p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {  // DRAG_GS_GAS1: no DRAG_AM
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nc)), dim3(kBS), 0, h->stream, h->gd_am, nc, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nc)), dim3(kBS), 0, h->stream, h->gd_fgds, nc, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * (int64_t)nc)), dim3(kBS), 0, h->stream,
+                       h->gd_bm, 3 * nc, 0.0);
+    if (h->n > 0)
+      hipLaunchKernelGGL(k_gas_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P,
+                         step_ptrs(h), h->vol, nc, h->gd_fgds, h->gd_bm);
+    HIPOK(hipGetLastError());
+    h->gas_drag_ready = true;
+    return MFIXDEM_OK;
+  }
+  CHECKRC(do_pic_lists(h));
   if (h->n > 0)
     hipLaunchKernelGGL(k_gas_fgp, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h),
                        h->gd_fgp);

@@ -2060,9 +2060,7 @@ static void des_vol_node(const orc_sys* s, double* vn) {
    (WTP = 1 for DEM).  F_GDS of a fluid cell is 1/8 of the DRAG_AM of its 8
    corner nodes (:455-494), 0 elsewhere.  Outputs: drag_am[ncell] and
    f_gds[ncell] in IJK order, drag_bm[3*ncell] as planes (d*ncell + ijk). */
-int orc_drag_gs_gas0(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm) {
-  if (!s->p.coupled || s->p.interp_scheme != MFIXDEM_INTERP_GARG_2012)
-    return MFIXDEM_ERR_UNSUPPORTED;
+static int drag_gs_gas0(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm) {
   int64_t L = s->ld;
   const mfixdem_params* p = &s->p;
   size_t C = (size_t)s->ncell;
@@ -2126,6 +2124,67 @@ int orc_drag_gs_gas0(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm
   }
   free(vn);
   return 0;
+}
+
+/* DRAG_GS_GAS1 (drag_gs_des1.f:179-301), the gas side of the drag for the
+   filter schemes: every normal particle in a fluid cell (ascending id)
+   interpolates EP_G and the cell-centre gas velocity with its filter
+   weights as DRAG_GS_DES1 does, takes F_GP from DES_DRAG_GP, and adds
+   F_GP*w/VOL to F_GDS and F_GP*v_p*w/VOL to DRAG_BM of each filter cell
+   (its own cell with weight 1 when DES_INTERP_ON is off).  There is no
+   DRAG_AM in this scheme; DES_COLLECT_gDATA is a halo exchange with no
+   partner on one process. */
+static int drag_gs_gas1(orc_sys* s, double* f_gds, double* drag_bm) {
+  int64_t L = s->ld;
+  const mfixdem_params* p = &s->p;
+  size_t C = (size_t)s->ncell;
+  memset(f_gds, 0, sizeof(double) * C);
+  memset(drag_bm, 0, sizeof(double) * 3 * C);
+  if (p->des_interp_on) orc_calc_interp_weights(s);
+  int nf = filter_size(s);
+  for (int64_t np = 0; np < s->max_pip; np++) {
+    if (s->state[np] != NORMAL_PARTICLE) continue;
+    int ijk0 = s->pijk[3 * L + np] - 1;
+    if (!s->fluid[ijk0]) continue;
+    double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0};
+    if (p->des_interp_on) {
+      for (int lc = 0; lc < nf; lc++) {
+        int ijk = s->filter_cell[lc * L + np];
+        double w = s->filter_weight[lc * L + np];
+        lepg = lepg + s->ep_g[ijk] * w;
+        for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + s->ugc[3 * ijk + d] * w;
+      }
+    } else {
+      lepg = s->ep_g[ijk0];
+      for (int d = 0; d < 3; d++) velfp[d] = s->ugc[3 * ijk0 + d];
+    }
+    if (lepg < 1.0e-6) lepg = s->ep_g[ijk0];
+    double vel[3] = {s->vel[np], s->vel[L + np], s->vel[2 * L + np]};
+    double vslp[3] = {velfp[0] - vel[0], velfp[1] - vel[1], velfp[2] - vel[2]};
+    double vrel = sqrt(dot3(vslp, vslp));
+    double lforce = orc_drag_fgp(p->drag_type, lepg, p->mu_g0, p->ro_g0, vrel,
+                                 2.0 * s->radius[np], s->pvol[np]);
+    double lbm[3] = {lforce * vel[0], lforce * vel[1], lforce * vel[2]};
+    int nl = p->des_interp_on ? nf : 1;
+    for (int lc = 0; lc < nl; lc++) {
+      int ijk = p->des_interp_on ? s->filter_cell[lc * L + np] : ijk0;
+      double w = p->des_interp_on ? s->filter_weight[lc * L + np] / s->vol[ijk]
+                                  : 1.0 / s->vol[ijk];
+      for (int d = 0; d < 3; d++) drag_bm[d * C + ijk] = drag_bm[d * C + ijk] + lbm[d] * w;
+      f_gds[ijk] = f_gds[ijk] + lforce * w;
+    }
+  }
+  return 0;
+}
+
+/* CALC_DES_2FLUID's drag call (calc_des_2fluid.f:70-73): DRAG_GS_GAS0 for
+   GARG_2012, DRAG_GS_GAS1 for the other schemes (drag_am all 0 there). */
+int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm) {
+  if (!s->p.coupled) return MFIXDEM_ERR_UNSUPPORTED;
+  if (s->p.interp_scheme == MFIXDEM_INTERP_GARG_2012)
+    return drag_gs_gas0(s, f_gds, drag_am, drag_bm);
+  memset(drag_am, 0, sizeof(double) * (size_t)s->ncell);
+  return drag_gs_gas1(s, f_gds, drag_bm);
 }
 
 /* ------------------------------------------------------- driver */

@@ -56,9 +56,11 @@ int orc_calc_interp_weights(orc_sys* s);
 int orc_get_filter(const orc_sys* s, int64_t q, int32_t* cells, double* weights);
 int orc_get_ugc(const orc_sys* s, double* ugc);
 int orc_comp_mean_fields(orc_sys* s);
-/* DRAG_GS_GAS0 (drag_gs_des0.f:235-497): DRAG_AM, DRAG_BM at the nodes and
-   F_GDS at the cells from the current particles (GARG_2012 only). */
-int orc_drag_gs_gas0(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm);
+/* The drag call of CALC_DES_2FLUID (calc_des_2fluid.f:70-73): DRAG_GS_GAS0
+   (drag_gs_des0.f:235-497; DRAG_AM, DRAG_BM at the nodes, F_GDS at the
+   cells) for GARG_2012, DRAG_GS_GAS1 (drag_gs_des1.f:179-301; F_GDS and
+   DRAG_BM at the cells, DRAG_AM 0) for the other schemes. */
+int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm);
 
 /* inspection */
 int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);

@@ -61,7 +61,7 @@ def load_oracle(omp: bool = False):
         "orc_get_ugc": ([P, D], C.c_int),
         "orc_set_gas_pressure": ([P, D], C.c_int),
         "orc_get_p_force": ([P, D], C.c_int),
-        "orc_drag_gs_gas0": ([P, D, D, D], C.c_int),
+        "orc_drag_gs_gas": ([P, D, D, D], C.c_int),
     }
     for n, (a, r) in sig.items():
         f = getattr(lib, n)
@@ -196,12 +196,13 @@ class OracleSystem:
         return cells[:nf], w[:nf]
 
     def drag_gs_gas(self):
-        """DRAG_GS_GAS0 on the current particles: (F_GDS, DRAG_AM, DRAG_BM[3])."""
+        """CALC_DES_2FLUID's drag (DRAG_GS_GAS0 for GARG_2012, else
+        DRAG_GS_GAS1) on the current particles: (F_GDS, DRAG_AM, DRAG_BM[3])."""
         f = np.zeros(self.ncell)
         am = np.zeros(self.ncell)
         bm = np.zeros(3 * self.ncell)
-        self._chk(self.lib.orc_drag_gs_gas0(self.h, abi.dptr(f), abi.dptr(am), abi.dptr(bm)),
-                  "drag_gs_gas0")
+        self._chk(self.lib.orc_drag_gs_gas(self.h, abi.dptr(f), abi.dptr(am), abi.dptr(bm)),
+                  "drag_gs_gas")
         return f, am, bm.reshape(3, self.ncell)
 
     def set_gas_pressure(self, p_g):

@@ -107,3 +107,90 @@ def test_node_sums_conserve_drag_and_momentum():
     assert np.all(am[vn == 0.0] == 0.0)
     assert np.all(f_gds >= 0.0)
     o.close()
+
+
+def filter_params(p, scheme, interp_on, width):
+    from mfixdem import abi
+    p.interp_scheme = scheme
+    p.des_interp_on = interp_on
+    p.des_interp_mean_fields = interp_on
+    p.des_interp_width = abi.UNDEFINED if width is None else width
+    return p
+
+
+def test_filter_gas_drag_single_particle_without_interpolation():
+    """DRAG_GS_GAS1 with DES_INTERP_ON off: everything lands in the
+    particle's own cell, F_GP/VOL and F_GP v_p/VOL; no DRAG_AM."""
+    from mfixdem import abi
+    x = np.array([0.0051, 0.00737, 0.0069])
+    p, parts = garg_single(x)
+    filter_params(p, abi.INTERP_NONE, 0, None)
+    vp = np.array([0.05, -0.4, 0.12])
+    parts.vel[:, 0] = vp
+    nc = workloads.n_cells(p)
+    o = OracleSystem(p)
+    o.set_particles(parts)
+    o.set_gas(uniform_gas(nc))
+    o.init()
+    ep, _ = o.get_gas()
+    _, pijk = o.get_cells()
+    own = pijk[3, 0] - 1
+    f_gds, am, bm = o.drag_gs_gas()
+    vol = (p.xlength / p.imax) * (p.ylength / p.jmax) * (p.zlength / p.kmax)
+    fgp = o.lib.orc_drag_fgp(p.drag_type, ep[own], p.mu_g0, p.ro_g0,
+                             np.linalg.norm(U_GAS - vp), 1.0e-3, np.pi / 6.0 * 1.0e-9)
+    assert f_gds[own] == fgp * (1.0 / vol)
+    assert np.count_nonzero(f_gds) == 1 and not am.any()
+    for dd in range(3):
+        assert bm[dd, own] == fgp * vp[dd] * (1.0 / vol)
+    o.close()
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("scheme,interp_on,width", [
+    (1, 0, None), (2, 1, 1.6e-3), (3, 1, 1.6e-3), (4, 1, None)],
+    ids=["none", "square_dpvm", "gauss_dpvm", "linear_hat"])
+def test_filter_gas_drag_conserves_drag_and_momentum(scheme, interp_on, width):
+    """DRAG_GS_GAS1: each particle's filter weights sum to 1, so the cell sums
+    weighted by VOL give back sum F_GP and sum F_GP v_p, with F_GP from the
+    particle's own filter interpolation of EP_G (uniform gas velocity)."""
+    p, parts, _ = workloads.fluidised_bed(n=1500, seed=5, lx=0.02, ly=0.04, lz=0.02,
+                                          cells=(10, 20, 10), dg=(0, 0, 0))
+    filter_params(p, scheme, interp_on, width)
+    nc = workloads.n_cells(p)
+    o = OracleSystem(p)
+    o.set_particles(parts)
+    o.set_gas(uniform_gas(nc))
+    o.init()
+    o.run(1, 20)
+    _, vel, _, state = o.get_particles()
+    ep, _ = o.get_gas()
+    _, pijk = o.get_cells()
+    f_gds, am, bm = o.drag_gs_gas()
+    vol = (p.xlength / p.imax) * (p.ylength / p.jmax) * (p.zlength / p.kmax)
+    fgp, fv = 0.0, np.zeros(3)
+    scale = np.zeros(3)
+    for q in np.flatnonzero(state == 1):
+        own = pijk[3, q] - 1
+        if interp_on:
+            cells, w = o.filter(q)
+            lepg = 0.0
+            for c, wc in zip(cells, w):
+                lepg = lepg + ep[c] * wc
+            if lepg < 1.0e-6:
+                lepg = ep[own]
+        else:
+            lepg = ep[own]
+        f = o.lib.orc_drag_fgp(p.drag_type, lepg, p.mu_g0, p.ro_g0,
+                               np.linalg.norm(U_GAS - vel[:, q]), 2.0 * parts.radius[q],
+                               np.pi / 6.0 * (2.0 * parts.radius[q]) ** 3)
+        fgp += f
+        fv += f * vel[:, q]
+        scale += f * np.abs(vel[:, q])
+    assert not am.any()
+    assert abs(f_gds.sum() * vol - fgp) <= 1e-12 * fgp
+    for dd in range(3):
+        assert abs(bm[dd].sum() * vol - fv[dd]) <= 1e-12 * scale[dd]
+    o.close()

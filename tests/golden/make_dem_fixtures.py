@@ -14,6 +14,10 @@ code) and keeps, per run, the solver columns the tests compare:
   DEM06  tests/dem/DEM06/AUTOTEST/POST_POS.dat, POST_VEL.dat: the three
          one-way coupled runs (frozen gas; no interpolation, GARG_2012,
          SQUARE_DPVM width 2 mm) -> rows of (time, y_mfix, v_mfix)
+  particle_input/DEMnn.dat  the particle_input.dat files of DEM03, DEM05
+         (Version 1.0) and DEM07, DEM08 (Version 2.0), copied as they are
+         (input data of the reference's tests, read by the particle-input
+         reader's tests)
 Run from the repository root when /root/reference is mounted:
   python tests/golden/make_dem_fixtures.py
 """
@@ -190,5 +194,15 @@ def main():
     print(len(out), "runs,", sum(len(b["rows"]) for b in out), "rows")
 
 
+def particle_inputs():
+    import shutil
+    out = os.path.join(HERE, "particle_input")
+    os.makedirs(out, exist_ok=True)
+    for t in ("DEM03", "DEM05", "DEM07", "DEM08"):
+        shutil.copyfile(os.path.join(REF, t, "particle_input.dat"), os.path.join(out, f"{t}.dat"))
+    print("particle_input: DEM03 DEM05 DEM07 DEM08")
+
+
 if __name__ == "__main__":
     main()
+    particle_inputs()

@@ -279,6 +279,16 @@ int32_t mfixdem_get_cells(mfixdem_handle* h, int32_t* dg_pijk, int32_t* pijk,
 int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index,
                           int32_t* neighbors, double* pft, int64_t capacity,
                           int64_t* npairs);
+/* Restart of the pair history (the NEIGHBORS, NEIGHBOR_INDEX and
+   PFT_NEIGHBOR a DES restart file holds, read_res1_des_mod.f:108), in the
+   layout mfixdem_get_pairs writes: after mfixdem_set_particles and before
+   mfixdem_init, whose first neighbour build then carries each pair's
+   tangential history over as NEIGHBOUR does after a rebuild
+   (neighbour.f:77-117); pairs it no longer lists are dropped, new ones start
+   at zero.  MFIXDEM_ERR_ARG for a malformed half list (partner ids must
+   exceed the row's) or after mfixdem_init. */
+int32_t mfixdem_set_pairs(mfixdem_handle* h, const int32_t* neighbor_index,
+                          const int32_t* neighbors, const double* pft, int64_t npairs);
 int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out);
 
 /* --- tuning ------------------------------------------------------------- */

@@ -1459,6 +1459,54 @@ int32_t mfixdem_get_pairs(mfixdem_handle* h, int32_t* neighbor_index, int32_t* n
   return MFIXDEM_OK;
 }
 
+// Restart of the pair history (read_res1_des_mod.f:108 reads NEIGHBORS,
+// NEIGHBOR_INDEX and PFT_NEIGHBOR back; neighbour.f:77-117 carries them):
+// the rows become the previous neighbour list, own entries only, in the
+// slot order that mfixdem_set_particles leaves (global-id order), with the
+// histories as the previous history buffer, so that the first neighbour
+// build (mfixdem_init) carries each surviving pair's history over.
+int32_t mfixdem_set_pairs(mfixdem_handle* h, const int32_t* neighbor_index,
+                          const int32_t* neighbors, const double* pft, int64_t npairs) {
+  if (!h || h->nsys > 1 || h->inited || npairs < 0 || npairs > INT_MAX / 2)
+    return MFIXDEM_ERR_ARG;
+  const int n = h->n;
+  if (n > 0 && !neighbor_index) return MFIXDEM_ERR_ARG;
+  if (npairs > 0 && (!neighbors || !pft)) return MFIXDEM_ERR_ARG;
+  std::vector<int> st((size_t)n + 1), nb((size_t)npairs), pid((size_t)npairs);
+  std::vector<uint8_t> fl((size_t)npairs);
+  int64_t prev = 0;
+  for (int q = 0; q < n; q++) {
+    const int64_t end = (int64_t)neighbor_index[q] - 1;  // 1-based exclusive end
+    if (end < prev || end > npairs) return MFIXDEM_ERR_ARG;
+    st[q] = (int)prev;
+    for (int64_t e = prev; e < end; e++) {
+      const int j = neighbors[e] - 1;
+      if (j <= q || j >= n) return MFIXDEM_ERR_ARG;  // a half list: partner id above the row's
+      nb[e] = j;
+      pid[e] = (int)e;
+      fl[e] = (pft[3 * e] != 0.0 || pft[3 * e + 1] != 0.0 || pft[3 * e + 2] != 0.0);
+    }
+    prev = end;
+  }
+  if (prev != npairs) return MFIXDEM_ERR_ARG;
+  st[n] = (int)npairs;
+  const int l = h->lt, pc = h->pcur;
+  CHECKRC(ensure_list_capacity(h, l, std::max<int64_t>(npairs, 1)));
+  CHECKRC(ensure_pft_capacity(h, pc, std::max<int64_t>(npairs, 1)));
+  HIPOK(hipMemcpy(h->nbr_start[l], st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->pair_base[l], st.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  if (n > 0)
+    HIPOK(hipMemcpy(h->nbr_split[l], st.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (npairs > 0) {
+    HIPOK(hipMemcpy(h->nbr[l], nb.data(), sizeof(int) * npairs, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->nbr_pid[l], pid.data(), sizeof(int) * npairs, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->pft[pc], pft, sizeof(double) * 3 * npairs, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(h->pfl[pc], fl.data(), npairs, hipMemcpyHostToDevice));
+  }
+  h->n_pairs = npairs;
+  return MFIXDEM_OK;
+}
+
 int32_t mfixdem_get_status(mfixdem_handle* h, mfixdem_status* out) {
   if (!h || !out) return MFIXDEM_ERR_ARG;
   int fl[kFlagCount];

@@ -187,6 +187,15 @@ module mfixdem_c
        integer(c_int64_t), value :: capacity
        integer(c_int64_t), intent(out) :: npairs
      end function
+     ! restart of NEIGHBORS, NEIGHBOR_INDEX, PFT_NEIGHBOR (read_res1_des_mod.f:108)
+     integer(c_int32_t) function mfixdem_set_pairs(h, neighbor_index, neighbors, pft, npairs) &
+          bind(C, name="mfixdem_set_pairs")
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(in) :: neighbor_index(*), neighbors(*)
+       real(c_double), intent(in) :: pft(*)
+       integer(c_int64_t), value :: npairs
+     end function
      integer(c_int32_t) function mfixdem_get_cells(h, dg_pijk, pijk, ld) &
           bind(C, name="mfixdem_get_cells")
        import :: c_int32_t, c_int64_t, c_ptr

@@ -203,6 +203,7 @@ def _declare(lib):
         "mfixdem_ensemble_get_particles": ([_P, _I, _DP, _DP, _DP, _BP, C.c_int64], _I),
         "mfixdem_reset_kernel_times": ([_P], _I),
         "mfixdem_drag_gs_gas": ([_P], _I),
+        "mfixdem_set_pairs": ([_P, _IP, _IP, _DP, C.c_int64], _I),
         "mfixdem_get_gas_drag": ([_P, _DP, _DP, _DP], _I),
     }
     for name, (args, res) in sig.items():

@@ -164,6 +164,14 @@ class DemSystem:
         k = int(npairs.value)
         return idx, nb[:k], pft[:3 * k].reshape(k, 3)
 
+    def set_pairs(self, neighbor_index, neighbors, pft):
+        """Restart of the pair history (get_pairs layout) before init()."""
+        idx = np.ascontiguousarray(neighbor_index, dtype=np.int32)
+        nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+        pf = np.ascontiguousarray(np.asarray(pft, dtype=np.float64).reshape(-1))
+        abi.check(self.lib.mfixdem_set_pairs(self.h, abi.iptr(idx), abi.iptr(nb), abi.dptr(pf),
+                                             nb.size), "set_pairs")
+
     def get_gas(self):
         ep = np.zeros(self.ncell)
         rops = np.zeros(self.ncell * self.params.nphases)

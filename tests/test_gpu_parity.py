@@ -594,3 +594,54 @@ def test_filter_gas_side_drag_matches_oracle(scheme, interp_on, width):
         assert np.max(np.abs(bg - bo)) <= tol * np.max(np.abs(bo))
     g.close()
     o.close()
+
+
+def _rows(idx, nb, pft):
+    """{(row id, partner id): PFT} of a half list in reference layout."""
+    out, start = {}, 0
+    for q in range(len(idx)):
+        for e in range(start, idx[q] - 1):
+            out[(q, int(nb[e]) - 1)] = pft[e]
+        start = idx[q] - 1
+    return out
+
+
+def test_restart_pair_history():
+    """mfixdem_set_pairs: a restart from the NEIGHBOR_INDEX, NEIGHBORS and
+    PFT that mfixdem_get_pairs writes (what a DES restart file holds), given
+    before init, carries every pair's tangential history into the new
+    neighbour list bit for bit; pairs the new list gains start at zero.
+    Without it the restarted run starts with no history."""
+    p, parts, _ = settling()
+    a = DemSystem(p)
+    a.set_particles(parts)
+    a.init()
+    a.run(1, 60)
+    pos, vel, om, state = a.get_particles()
+    assert np.all(state == 1)
+    ai, an, ap = a.get_pairs()
+    old = _rows(ai, an, ap)
+    assert sum(bool(np.any(v != 0.0)) for v in old.values()) > 0
+    rp = workloads.make_particles(pos, parts.radius, parts.ro_sol, vel=vel, omega=om,
+                                  phase=parts.phase)
+    b = DemSystem(p)
+    b.set_particles(rp)
+    b.set_pairs(ai, an, ap)
+    b.init()
+    new = _rows(*b.get_pairs())
+    carried = 0
+    for key, h in new.items():
+        if key in old:
+            assert np.array_equal(h, old[key]), key
+            carried += bool(np.any(h != 0.0))
+        else:
+            assert not np.any(h), key
+    assert carried > 0
+    with pytest.raises(abi.MfixDemError):  # the history is restarted before init only
+        b.set_pairs(ai, an, ap)
+    c = DemSystem(p)
+    c.set_particles(rp)
+    c.init()
+    assert not c.get_pairs()[2].any()
+    for s in (a, b, c):
+        s.close()

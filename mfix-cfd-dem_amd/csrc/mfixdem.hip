@@ -497,8 +497,6 @@ int32_t do_neighbour(mfixdem_handle* h) {
   CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
   CarryPtrs cp{};
   cp.perm = h->vals[1];
-  cp.gid = h->gid[t];
-  cp.old_gid = h->gid[1 - t];
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
   cp.pft_old = h->pft[h->pcur];

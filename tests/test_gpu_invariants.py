@@ -114,3 +114,49 @@ def test_garg_single_particle_on_gpu_matches_closed_forms():
     assert np.allclose(fg[:, 0], fo[:, 0], rtol=1e-12, atol=0)
     g.close()
     o.close()
+
+
+def test_gas_side_drag_conserved_at_full_size():
+    """DRAG_GS_GAS0 at the benchmark's size (1M spheres, 64x128x64 cells):
+    each particle's node weights sum to 1, so the node sums weighted by
+    DES_VOL_NODE give back sum F_GP and sum F_GP v_p (to 1e-11), with F_GP
+    from the drag law (orc_drag_fgp, pinned by the textbook laws) at each
+    particle's cell voidage in a uniform gas field."""
+    from oracle_py import load_oracle
+    p, parts, _ = workloads.fluidised_bed(n=1_000_000)
+    nc = workloads.n_cells(p)
+    u = np.array([0.3, 1.1, -0.2])
+    gas = workloads.Gas(np.full(nc, u[0]), np.full(nc, u[1]), np.full(nc, u[2]),
+                        np.zeros((nc, 3)))
+    g = DemSystem(p)
+    g.set_particles(parts)
+    g.set_gas(gas)
+    g.init()
+    g.run(1, 20)
+    f_gds, am, bm = g.drag_gs_gas()
+    _, vel, _, state = g.get_particles()
+    ep, _ = g.get_gas()
+    _, pijk = g.get_cells()
+    g.close()
+    live = np.flatnonzero(state == 1)
+    eps = ep[pijk[3, live] - 1]
+    vrel = np.linalg.norm(u[:, None] - vel[:, live], axis=0)
+    d = 2.0 * parts.radius[live]
+    fgp_law = load_oracle().orc_drag_fgp
+    fgp = np.array([fgp_law(p.drag_type, e, p.mu_g0, p.ro_g0, w, dd, np.pi / 6.0 * dd ** 3)
+                    for e, w, dd in zip(eps.tolist(), vrel.tolist(), d.tolist())])
+    # DES_VOL_NODE of a box: VOL/8 per fluid cell of the node's 2x2x2 block
+    ni, nj, nk = p.imax + 2, p.jmax + 2, p.kmax + 2
+    vol = (p.xlength / p.imax) * (p.ylength / p.jmax) * (p.zlength / p.kmax)
+    fl = np.zeros((nk, ni, nj))
+    fl[1:p.kmax + 1, 1:p.imax + 1, 1:p.jmax + 1] = 1.0
+    cnt = sum(fl[a:nk - 1 + a, b:ni - 1 + b, c:nj - 1 + c]
+              for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    vn = np.zeros((nk, ni, nj))
+    vn[:nk - 1, :ni - 1, :nj - 1] = cnt * vol / 8.0
+    vn = vn.ravel()
+    assert np.all(am[vn == 0.0] == 0.0) and np.all(f_gds >= 0.0)
+    assert abs((am * vn).sum() - fgp.sum()) <= 1e-11 * fgp.sum()
+    for k in range(3):
+        want = (fgp * vel[k, live]).sum()
+        assert abs((bm[k] * vn).sum() - want) <= 1e-11 * (fgp * np.abs(vel[k, live])).sum()

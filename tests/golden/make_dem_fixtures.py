@@ -4,6 +4,8 @@ Reads the AUTOTEST files the reference ships for its DEM tests (data, not
 code) and keeps, per run, the solver columns the tests compare:
   DEM01  tests/dem/DEM01/AUTOTEST/POST_POS.dat, POST_VEL.dat
          (scheme, KN, EN) -> rows of (time, y_mfix, v_mfix)
+  DEM02  tests/dem/DEM02/AUTOTEST/POST_HEIGHT.dat
+         (KN, EN; Adams-Bashforth) -> rows of (bounce, h_closed_form, h_mfix)
   DEM03  tests/dem/DEM03/AUTOTEST/POST_POS1.dat, POST_POS2.dat
          (scheme, KN, EN) -> rows of (time, y1_mfix, y2_mfix)
   DEM04  tests/dem/DEM04/AUTOTEST/POST_TIME.dat, POST_TVEL.dat, POST_AVEL.dat
@@ -68,6 +70,30 @@ def parse_table(path, ncol=4):
         except ValueError:
             continue
     return rows
+
+
+def dem02():
+    """POST_HEIGHT.dat: per run a header naming KN and DES_EN_INPUT, then one
+    line per bounce apex (bounce, closed-form height, MFiX height, %diff)."""
+    runs, cur = [], None
+    for line in open(os.path.join(REF, "DEM02", "AUTOTEST", "POST_HEIGHT.dat")):
+        m = re.match(r"\s*KN =\s*(\S+)", line)
+        if m:
+            cur = {"scheme": "ADAMS_BASHFORTH", "kn": float(m.group(1)), "rows": []}
+            runs.append(cur)
+            continue
+        m = re.match(r"\s*DES_EN_INPUT =\s*(\S+)", line)
+        if m:
+            cur["en"] = float(m.group(1))
+            continue
+        f = line.split()
+        if cur is not None and len(f) == 4 and re.fullmatch(r"\d+", f[0]):
+            cur["rows"].append([int(f[0]), float(f[1]), float(f[2])])
+    with open(os.path.join(HERE, "dem02_post.json"), "w") as f:
+        json.dump({"source": "tests/dem/DEM02/AUTOTEST/POST_HEIGHT.dat",
+                   "columns": ["bounce", "h_closed_form", "h_mfix"], "runs": runs}, f,
+                  separators=(",", ":"))
+    print("DEM02:", len(runs), "runs,", sum(len(r["rows"]) for r in runs), "rows")
 
 
 def dem03():
@@ -175,6 +201,7 @@ def dem06():
 
 
 def main():
+    dem02()
     dem06()
     dem03()
     dem05()

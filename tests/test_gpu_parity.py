@@ -230,6 +230,33 @@ def test_dem01_on_gpu_matches_reference_output(k):
     g.close()
 
 
+@pytest.mark.parametrize("k", [0, 3, 11], ids=lambda k: f"run{k}")
+def test_dem02_on_gpu_matches_reference_output(k):
+    """DEM02 bounce apexes (tests/dem/DEM02/AUTOTEST/POST_HEIGHT.dat) from
+    the HIP library, stepped one DES_TIME_STEP at a time for the whole 10 s
+    run: same apex count, heights to the last printed digit, and the same
+    FP64 heights as the oracle."""
+    import json
+    from dem02_usr import bounce_heights, dem02_steps
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "dem02_post.json")))
+    run = gold["runs"][k]
+    p, parts, _ = workloads.dem01(kn=run["kn"], en=run["en"], integrator=abi.ADAMS_BASHFORTH)
+    g = DemSystem(p)
+    g.set_particles(parts)
+    g.init()
+    o = OracleSystem(p)
+    o.set_particles(parts)
+    o.init()
+    nsteps = dem02_steps(g.derived().dtsolid)
+    assert nsteps == dem02_steps(o.dtsolid())
+    h = bounce_heights(g, nsteps)
+    rows = np.array(run["rows"])
+    assert len(h) == len(rows)
+    assert np.max(np.abs(h - rows[:, 2])) <= 1.0e-8
+    assert np.array_equal(h, bounce_heights(o, nsteps))
+    g.close()
+
+
 def test_fortran_driver_through_c_abi():
     """The Fortran ISO_C_BINDING shim drives DEM01 to the same trajectory."""
     exe = os.path.join(_paths.PKG, "lib", "dem01_driver")

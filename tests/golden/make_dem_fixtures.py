@@ -16,8 +16,10 @@ code) and keeps, per run, the solver columns the tests compare:
   DEM06  tests/dem/DEM06/AUTOTEST/POST_POS.dat, POST_VEL.dat: the three
          one-way coupled runs (frozen gas; no interpolation, GARG_2012,
          SQUARE_DPVM width 2 mm) -> rows of (time, y_mfix, v_mfix)
+  CGDEM03 tests/cgdem/CGDEM03/POST_VOL.dat -> rows of (time, solids volume,
+         fluid volume, |1 - sum|) of the LINEAR_HAT deposition of 480 spheres
   particle_input/DEMnn.dat  the particle_input.dat files of DEM03, DEM05
-         (Version 1.0) and DEM07, DEM08 (Version 2.0), copied as they are
+         (Version 1.0), DEM07, DEM08 and CGDEM03 (Version 2.0), copied as they are
          (input data of the reference's tests, read by the particle-input
          reader's tests)
 Run from the repository root when /root/reference is mounted:
@@ -227,9 +229,21 @@ def particle_inputs():
     os.makedirs(out, exist_ok=True)
     for t in ("DEM03", "DEM05", "DEM07", "DEM08"):
         shutil.copyfile(os.path.join(REF, t, "particle_input.dat"), os.path.join(out, f"{t}.dat"))
-    print("particle_input: DEM03 DEM05 DEM07 DEM08")
+    cg = os.path.join(os.path.dirname(REF), "cgdem", "CGDEM03")
+    shutil.copyfile(os.path.join(cg, "particle_input.dat"), os.path.join(out, "CGDEM03.dat"))
+    print("particle_input: DEM03 DEM05 DEM07 DEM08 CGDEM03")
+
+
+def cgdem03():
+    rows = parse_table(os.path.join(os.path.dirname(REF), "cgdem", "CGDEM03", "POST_VOL.dat"))
+    with open(os.path.join(HERE, "cgdem03_post_vol.json"), "w") as f:
+        json.dump({"source": "tests/cgdem/CGDEM03/POST_VOL.dat",
+                   "columns": ["time", "solids_volume", "fluid_volume", "abs_error"],
+                   "rows": rows}, f, separators=(",", ":"))
+    print("CGDEM03:", len(rows), "rows")
 
 
 if __name__ == "__main__":
     main()
     particle_inputs()
+    cgdem03()

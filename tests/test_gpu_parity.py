@@ -257,6 +257,27 @@ def test_dem02_on_gpu_matches_reference_output(k):
     g.close()
 
 
+def test_cgdem03_linear_hat_deposition_on_gpu():
+    """LINEAR_HAT deposition of the reference's CGDEM03 spheres by the HIP
+    library: EP_G and ROP_S equal the oracle's to 1e-14 (FP64 atomics add in
+    another order), and the solids and fluid volumes are those of
+    tests/cgdem/CGDEM03/POST_VOL.dat at t = 0 to the printed digits."""
+    import json
+    from cgdem03_case import cgdem03_case, volumes
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "cgdem03_post_vol.json")))["rows"][0]
+    p, parts, gas = cgdem03_case()
+    g, o = both(p, parts, gas)
+    ge, gr = g.get_gas()
+    oe, orr = o.get_gas()
+    assert np.max(np.abs(ge - oe)) <= 1e-14
+    assert np.max(np.abs(gr - orr)) <= 1e-14 * np.max(np.abs(orr))
+    solids, fluid = volumes(ge, gr)
+    assert abs(solids - gold[1]) <= 5e-12 and abs(fluid - gold[2]) <= 5e-9
+    assert abs(solids + fluid - 1.0) <= 1e-14
+    g.close()
+
+
 def test_fortran_driver_through_c_abi():
     """The Fortran ISO_C_BINDING shim drives DEM01 to the same trajectory."""
     exe = os.path.join(_paths.PKG, "lib", "dem01_driver")

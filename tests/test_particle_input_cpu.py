@@ -1,6 +1,6 @@
 """particle_input.dat reader and writer (mfixdem.particle_input) against the
 reference's own input files (tests/golden/particle_input/, copied from its
-DEM03, DEM05, DEM07 and DEM08 tests).  DEM05's 93 particles were also
+DEM03, DEM05, DEM07, DEM08 and CGDEM03 tests).  DEM05's 93 particles were also
 extracted independently into dem05_post.json, whose DEM05 oracle runs match
 the reference's POST files, so the Version 1.0 columns are pinned by that
 regression."""
@@ -50,6 +50,17 @@ def test_version2_defaults_and_columns():
     assert np.array_equal(bed.pos[:, 0], [0.48914431e-02, 0.15488297e+00, 0.50084351e-02])
     assert np.all(bed.radius == 0.5 * 0.01) and np.all(bed.ro_sol == 2500.0)
     assert not bed.vel.any() and np.all(bed.phase == 1)
+
+
+def test_version2_tabs_and_user_scalars():
+    """CGDEM03's file: tab-separated columns, velocities read, three user
+    scalars after them (not kept), diameter and density from the defaults."""
+    cg = read_particle_input(os.path.join(INP, "CGDEM03.dat"))
+    assert cg.n == 480
+    assert np.array_equal(cg.pos[:, 0], [9.92e-01, 1.31e-02, 1.49e-01])
+    assert np.array_equal(cg.pos[:, 2], [9.33e-03, 2.87e-02, 1.47e-01])
+    assert np.all(cg.radius == 0.005) and np.all(cg.ro_sol == 2700.0)
+    assert not cg.vel.any() and np.all(cg.phase == 1)
 
 
 def test_version2_read_columns(tmp_path):

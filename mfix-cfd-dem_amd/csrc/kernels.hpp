@@ -2317,6 +2317,56 @@ __global__ __launch_bounds__(kBS) void k_cell_fields_des1(int ncell, int nph, do
   }
 }
 
+// Gas velocities the drag reads, from the staggered U_G/V_G/W_G (one thread
+// per cell, MFiX serial layout):
+//  * node velocities of DRAG_GS_DES0 (drag_gs_des0.f:148-179) at nodes
+//    (I,J,K) = (1..IMAX+1, 1..JMAX+1, 1..KMAX+1), averages of four face
+//    values with AVG_FACTOR = 1/4, in the reference's sum order; 0 at the
+//    other indices;
+//  * cell-centre velocities of CALC_CELL_CENTER_GAS_VEL
+//    (drag_gs_des1.f:313-370) in fluid cells, AVG_X_E with FX = 1/2 and
+//    HALF*(m + p) for AVG_Y_N / AVG_Z_T (fun_avg.inc, set_geometry.f:344);
+//    0 outside fluid cells.
+__global__ __launch_bounds__(kBS) void k_gas_velocities(int ni, int nj, int nk, int imax,
+                                                        int jmax, int kmax,
+                                                        const uint8_t* fluid, const double* ug,
+                                                        const double* vg, const double* wg,
+                                                        double* vnode, double* ugc) {
+  const int c = blockIdx.x * kBS + threadIdx.x;
+  if (c >= ni * nj * nk) return;
+  const int J = c % nj + 1, I = (c / nj) % ni + 1, K = c / (nj * ni) + 1;
+  const int si = nj, sk = nj * ni;  // strides of I and K (J is 1)
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  if (I <= imax + 1 && J <= jmax + 1 && K <= kmax + 1) {
+    const double af = 0.25;
+    const int ipjk = c + si, ijpk = c + 1, ipjpk = c + si + 1;
+    const int ijkp = c + sk, ijpkp = c + 1 + sk, ipjkp = c + si + sk;
+    vx = af * (ug[c] + ug[ijpk]);
+    vy = af * (vg[c] + vg[ipjk]);
+    vx = vx + af * (ug[ijkp] + ug[ijpkp]);
+    vy = vy + af * (vg[ijkp] + vg[ipjkp]);
+    vz = af * (wg[c] + wg[ijpk] + wg[ipjk] + wg[ipjpk]);
+  }
+  vnode[3 * c] = vx;
+  vnode[3 * c + 1] = vy;
+  vnode[3 * c + 2] = vz;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (fluid[c]) {  // fluid cells are interior: the west/south/bottom faces exist
+    gx = 0.5 * ug[c - si] + (1.0 - 0.5) * ug[c];
+    gy = 0.5 * (vg[c - 1] + vg[c]);
+    gz = 0.5 * (wg[c - sk] + wg[c]);
+  }
+  ugc[3 * c] = gx;
+  ugc[3 * c + 1] = gy;
+  ugc[3 * c + 2] = gz;
+}
+
+// P_FORCE(3, IJK) from the caller's array, or 0 when none is given.
+__global__ void k_copy_or_zero_f64(double* dst, const double* src, int n) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) dst[i] = src ? src[i] : 0.0;
+}
+
 // calc_del_phi (calc_grad_des.f:104-133): 3-point derivative, a neighbour
 // that is not a valid pressure cell replaced by extrapolation
 __device__ __forceinline__ double calc_del_phi(bool valid_lo, bool valid_hi, double phi,

@@ -101,6 +101,7 @@ struct mfixdem_handle {
   int* flags = nullptr;
   // gas
   double* ugc = nullptr;  // cell-centre gas velocity (filter schemes)
+  double* gas_stage = nullptr;  // 3*ncell: staged U_G/V_G/W_G, P_G or P_FORCE uploads
   double *xe = nullptr, *yn = nullptr, *zt = nullptr, *vnode = nullptr, *p_force = nullptr,
          *ep_g = nullptr, *rop_s = nullptr, *rops_node = nullptr, *vol = nullptr,
          *vol_surr = nullptr, *ro_s = nullptr;
@@ -123,6 +124,9 @@ struct mfixdem_handle {
   int debug = 0;
   // graphs: [cur parity][deposition buffer]
   hipGraphExec_t graph[2][2]{};
+  // executable graphs replaced while launches of them may still be queued on
+  // the stream; destroyed once the stream has drained (reap_graphs)
+  std::vector<hipGraphExec_t> retired;
   // kernel timing
   // kernel timing: a pool of events recorded around each timed launch and
   // read back in one go, so launches queue back to back while timed
@@ -207,13 +211,26 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   return a;
 }
 
+// The captured graphs no longer match the handle's buffers: they are
+// retired, not destroyed, because launches of them may still be queued on
+// the handle's stream; sync_stream destroys them once the stream has drained.
 void invalidate_graphs(mfixdem_handle* h) {
   for (auto& row : h->graph)
     for (auto& g : row)
       if (g) {
-        (void)hipGraphExecDestroy(g);
+        h->retired.push_back(g);
         g = nullptr;
       }
+}
+void reap_graphs(mfixdem_handle* h) {
+  for (hipGraphExec_t g : h->retired) (void)hipGraphExecDestroy(g);
+  h->retired.clear();
+}
+// Waits for the handle's stream; every wait goes through here.
+hipError_t sync_stream(mfixdem_handle* h) {
+  const hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) reap_graphs(h);
+  return e;
 }
 
 // the reference-order deposition exists for the GARG_2012 node sums
@@ -464,7 +481,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
     HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
                          h->stream));
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
-    HIPOK(hipStreamSynchronize(h->stream));
+    HIPOK(sync_stream(h));
     if (fl[kFlagOwnOverflow] <= h->maxrow) break;
     int z = 0;
     HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, &z, sizeof(int), hipMemcpyHostToDevice,
@@ -515,7 +532,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   // the old history buffer becomes the next step's output; k_place reads it,
   // so a buffer too small is replaced only once the carry is done
   if (h->pft_cap[1 - h->pcur] < npairs || !h->pft[1 - h->pcur]) {
-    HIPOK(hipStreamSynchronize(h->stream));
+    HIPOK(sync_stream(h));
     CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
   }
   if (!nsq && fl[kFlagRogueCount] > 0) {
@@ -709,7 +726,7 @@ int32_t scatter_d3(mfixdem_handle* h, const double* src, double* host, int64_t l
   std::vector<double> tmp((size_t)3 * n);
   HIPOK(hipMemcpyAsync(tmp.data(), h->stage, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
                        h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   for (int c = 0; c < 3; c++) std::memcpy(host + c * ld, tmp.data() + (size_t)c * n, sizeof(double) * n);
   return MFIXDEM_OK;
 }
@@ -719,7 +736,7 @@ int32_t download(mfixdem_handle* h, std::vector<T>& v, const T* d, int64_t count
   v.resize((size_t)std::max<int64_t>(count, 0));
   if (count > 0)
     HIPOK(hipMemcpyAsync(v.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   return MFIXDEM_OK;
 }
 
@@ -811,7 +828,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->fstart); f(h->fend);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
   f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->xe); f(h->yn);
-  f(h->zt); f(h->vnode); f(h->ugc); f(h->p_force); f(h->ep_g); f(h->rop_s);
+  f(h->zt); f(h->vnode); f(h->ugc); f(h->gas_stage); f(h->p_force); f(h->ep_g); f(h->rop_s);
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
   f(h->gd_fgp); f(h->gd_vol_node); f(h->gd_am); f(h->gd_bm); f(h->gd_fgds);
@@ -856,7 +873,7 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
       (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
       (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
       (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
-      (rc = dalloc(&h->ugc, 3 * nc)) ||
+      (rc = dalloc(&h->ugc, 3 * nc)) || (rc = dalloc(&h->gas_stage, 3 * nc)) ||
       (rc = dalloc(&h->ep_g, nc)) || (rc = dalloc(&h->rop_s, nc * h->S.p.nphases)) ||
       (rc = dalloc(&h->rops_node, nc * h->S.p.nphases)) || (rc = dalloc(&h->vol, nc)) ||
       (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
@@ -972,8 +989,9 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
 
 int32_t mfixdem_destroy(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream) (void)sync_stream(h);
   invalidate_graphs(h);
+  reap_graphs(h);
   for_each_buffer(h, [](auto*& p) { dfree(p); });
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1033,6 +1051,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
       h->dep_inv[m] = std::ldexp(1.0, -e);
     }
   }
+  HIPOK(sync_stream(h));  // steps already queued still use the buffers
   invalidate_graphs(h);
   h->cur = h->pcur = h->tw = h->lt = 0;
   h->fb = 0;
@@ -1078,7 +1097,7 @@ int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, doubl
                        h->gid[h->tw], h->state[h->tw], (int8_t*)h->stage);
     HIPOK(hipGetLastError());
     HIPOK(hipMemcpyAsync(state, h->stage, h->n, hipMemcpyDeviceToHost, h->stream));
-    HIPOK(hipStreamSynchronize(h->stream));
+    HIPOK(sync_stream(h));
   }
   return MFIXDEM_OK;
 }
@@ -1090,46 +1109,27 @@ int32_t mfixdem_set_gas(mfixdem_handle* h, const double* u_g, const double* v_g,
   const Setup& S = h->S;
   const mfixdem_params& p = S.p;
   const int nc = S.ncell;
-  // node velocities of DRAG_GS_DES0 (drag_gs_des0.f:148-179) for every node
-  // (I,J,K) = (1..IMAX+1, 1..JMAX+1, 1..KMAX+1); frozen field -> computed once
-  std::vector<double> vn((size_t)3 * nc, 0.0);
-  const double af = 0.25;
-  for (int K = 1; K <= p.kmax + 1; K++)
-    for (int I = 1; I <= p.imax + 1; I++)
-      for (int J = 1; J <= p.jmax + 1; J++) {
-        const int cur = fluid_ijk(S, I, J, K), ipjk = fluid_ijk(S, I + 1, J, K);
-        const int ijpk = fluid_ijk(S, I, J + 1, K), ipjpk = fluid_ijk(S, I + 1, J + 1, K);
-        const int ijpkp = fluid_ijk(S, I, J + 1, K + 1), ipjkp = fluid_ijk(S, I + 1, J, K + 1);
-        const int ijkp = fluid_ijk(S, I, J, K + 1);
-        double vx = af * (u_g[cur] + u_g[ijpk]);
-        double vy = af * (v_g[cur] + v_g[ipjk]);
-        vx = vx + af * (u_g[ijkp] + u_g[ijpkp]);
-        vy = vy + af * (v_g[ijkp] + v_g[ipjkp]);
-        const double vz = af * (w_g[cur] + w_g[ijpk] + w_g[ipjk] + w_g[ipjpk]);
-        vn[3 * cur] = vx;
-        vn[3 * cur + 1] = vy;
-        vn[3 * cur + 2] = vz;
-      }
-  HIPOK(hipMemcpy(h->vnode, vn.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
-  // cell-centre gas velocity of DRAG_GS_DES1 (CALC_CELL_CENTER_GAS_VEL,
-  // drag_gs_des1.f:313-370) on a uniform grid: FX = 1/2 in AVG_X_E
-  // (fun_avg.inc:38-44), HALF*(m + p) in AVG_Y_N / AVG_Z_T; 0 outside fluid
-  std::vector<double> gc((size_t)3 * nc, 0.0);
-  for (int K = 1; K <= S.nk; K++)
-    for (int I = 1; I <= S.ni; I++)
-      for (int J = 1; J <= S.nj; J++) {
-        const int c = fluid_ijk(S, I, J, K);
-        if (!S.fluid[c]) continue;
-        const int imjk = fluid_ijk(S, I - 1, J, K), ijmk = fluid_ijk(S, I, J - 1, K);
-        const int ijkm = fluid_ijk(S, I, J, K - 1);
-        gc[3 * c] = 0.5 * u_g[imjk] + (1.0 - 0.5) * u_g[c];
-        gc[3 * c + 1] = 0.5 * (v_g[ijmk] + v_g[c]);
-        gc[3 * c + 2] = 0.5 * (w_g[ijkm] + w_g[c]);
-      }
-  HIPOK(hipMemcpy(h->ugc, gc.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
-  std::vector<double> pf((size_t)3 * nc, 0.0);
-  if (p_force) std::memcpy(pf.data(), p_force, sizeof(double) * 3 * nc);
-  HIPOK(hipMemcpy(h->p_force, pf.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
+  // steps already queued may still read the current fields
+  HIPOK(sync_stream(h));
+  double* st = h->gas_stage;
+  HIPOK(hipMemcpyAsync(st, u_g, sizeof(double) * nc, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(st + nc, v_g, sizeof(double) * nc, hipMemcpyHostToDevice, h->stream));
+  HIPOK(hipMemcpyAsync(st + 2 * (size_t)nc, w_g, sizeof(double) * nc, hipMemcpyHostToDevice,
+                       h->stream));
+  // node velocities (GARG_2012) and cell-centre velocities (filter schemes);
+  // the field is frozen until the next call, so they are formed once here
+  hipLaunchKernelGGL(k_gas_velocities, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk,
+                     p.imax, p.jmax, p.kmax, h->fluid, st, st + nc, st + 2 * (size_t)nc,
+                     h->vnode, h->ugc);
+  HIPOK(hipGetLastError());
+  // P_FORCE after the velocities have been formed (the stage is reused)
+  if (p_force)
+    HIPOK(hipMemcpyAsync(st, p_force, sizeof(double) * 3 * nc, hipMemcpyHostToDevice,
+                         h->stream));
+  hipLaunchKernelGGL(k_copy_or_zero_f64, dim3(nblk(3 * (int64_t)nc)), dim3(kBS), 0, h->stream,
+                     h->p_force, p_force ? st : nullptr, 3 * nc);
+  HIPOK(hipGetLastError());
+  HIPOK(sync_stream(h));  // the caller's arrays are free again on return
   h->gas_set = true;
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -1140,27 +1140,22 @@ int32_t mfixdem_set_gas_pressure(mfixdem_handle* h, const double* p_g) {
   if (!h || !p_g) return MFIXDEM_ERR_ARG;
   const Setup& S = h->S;
   const int nc = S.ncell;
-  double* d = nullptr;
-  if (hipMalloc(&d, sizeof(double) * nc) != hipSuccess) return MFIXDEM_ERR_OOM;
-  int32_t rc = MFIXDEM_OK;
-  if (hipMemcpy(d, p_g, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess) {
-    rc = MFIXDEM_ERR_HIP;
-  } else {
-    hipLaunchKernelGGL(k_pg_grad, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk,
-                       S.dx, S.dy, S.dz, h->fluid, d, h->p_force);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
-      rc = MFIXDEM_ERR_HIP;
-  }
-  hipFree(d);
+  HIPOK(sync_stream(h));  // queued steps may still read P_FORCE
+  HIPOK(hipMemcpyAsync(h->gas_stage, p_g, sizeof(double) * nc, hipMemcpyHostToDevice,
+                       h->stream));
+  hipLaunchKernelGGL(k_pg_grad, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk, S.dx,
+                     S.dy, S.dz, h->fluid, h->gas_stage, h->p_force);
+  HIPOK(hipGetLastError());
+  HIPOK(sync_stream(h));
   invalidate_graphs(h);
-  return rc;
+  return MFIXDEM_OK;
 }
 
 int32_t mfixdem_get_p_force(mfixdem_handle* h, double* p_force) {
   if (!h || !p_force) return MFIXDEM_ERR_ARG;
   HIPOK(hipMemcpyAsync(p_force, h->p_force, sizeof(double) * 3 * h->S.ncell,
                        hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   return MFIXDEM_OK;
 }
 
@@ -1171,13 +1166,14 @@ int32_t mfixdem_get_gas(mfixdem_handle* h, double* ep_g, double* rop_s) {
   if (rop_s)
     HIPOK(hipMemcpyAsync(rop_s, h->rop_s, sizeof(double) * nc * h->S.p.nphases,
                          hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   return MFIXDEM_OK;
 }
 
 int32_t mfixdem_init(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
   if (h->S.p.coupled && !h->gas_set) return MFIXDEM_ERR_NOT_READY;
+  HIPOK(sync_stream(h));
   if (h->nsys > 1) CHECKRC(upload_ensemble(h));
   const int n = h->n;
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
@@ -1198,7 +1194,7 @@ int32_t mfixdem_init(mfixdem_handle* h) {
   }
   int fl[kFlagCount];
   HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   h->inited = true;
   h->first_pass = true;
   h->st.steps = 0;
@@ -1221,12 +1217,12 @@ int32_t mfixdem_run(mfixdem_handle* h, int32_t nn_first, int32_t nsteps) {
 int32_t mfixdem_synchronize(mfixdem_handle* h) {
   if (!h) return MFIXDEM_ERR_ARG;
   if (!h->inited) {
-    HIPOK(hipStreamSynchronize(h->stream));
+    HIPOK(sync_stream(h));
     return MFIXDEM_OK;
   }
   int fl[kFlagCount];
   HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   return fatal_flags(fl);
 }
 
@@ -1367,12 +1363,13 @@ int32_t mfixdem_get_gas_drag(mfixdem_handle* h, double* f_gds, double* drag_am,
   if (drag_bm)
     HIPOK(hipMemcpyAsync(drag_bm, h->gd_bm, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
                          h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   return MFIXDEM_OK;
 }
 
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags) {
   if (!h) return MFIXDEM_ERR_ARG;
+  HIPOK(sync_stream(h));  // queued graph launches finish under the old flags
   h->debug = flags;
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -1488,6 +1485,7 @@ int32_t mfixdem_set_pairs(mfixdem_handle* h, const int32_t* neighbor_index,
   }
   if (prev != npairs) return MFIXDEM_ERR_ARG;
   st[n] = (int)npairs;
+  HIPOK(sync_stream(h));
   const int l = h->lt, pc = h->pcur;
   CHECKRC(ensure_list_capacity(h, l, std::max<int64_t>(npairs, 1)));
   CHECKRC(ensure_pft_capacity(h, pc, std::max<int64_t>(npairs, 1)));
@@ -1705,7 +1703,7 @@ int32_t mfixdem_ensemble_get_particles(mfixdem_handle* h, int32_t s, double* pos
   HIPOK(hipMemcpyAsync(g.data(), h->gid[h->tw] + off, sizeof(int) * cap, hipMemcpyDeviceToHost,
                        h->stream));
   HIPOK(hipMemcpyAsync(st.data(), h->state[h->tw] + off, cap, hipMemcpyDeviceToHost, h->stream));
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   const double* src[3] = {h->pos[h->cur], h->vel[h->cur], h->omega[h->cur]};
   double* dst[3] = {pos, vel, omega};
   for (int a = 0; a < 3; a++) {
@@ -1725,7 +1723,7 @@ int32_t mfixdem_set_tuning(mfixdem_handle* h, double cell_scale, int32_t row_ker
   if (!h || !(cell_scale >= 1.0) || cell_scale > 4.0 || row_kernel < MFIXDEM_ROWS_THREAD ||
       row_kernel > MFIXDEM_ROWS_WAVE256)
     return MFIXDEM_ERR_ARG;
-  HIPOK(hipStreamSynchronize(h->stream));
+  HIPOK(sync_stream(h));
   int n_fine = 0;
   if (h->nsys == 1) {
     GridDev G = h->G;

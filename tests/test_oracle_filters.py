@@ -13,6 +13,12 @@ fixes, independently of the restated code:
   conserves solids volume; the NONE scheme deposits PVOL into the own cell.
 * DES1 drag with a uniform gas field equals the textbook Gidaspow / Wen-Yu
   force on the interpolated voidage (drag_gs.f:536-758 describe the laws).
+* DES1 drag with a gas field linear in space (staggered U_G/V_G/W_G, cell
+  P_FORCE) equals the law at the closed-form interpolated velocity plus the
+  closed-form interpolated pressure force (linear_gas.py): pins the
+  cell-centre face averages of CALC_CELL_CENTER_GAS_VEL and the cells the
+  27 weights are applied to, for every scheme, with and without
+  DES_INTERP_ON.
 """
 import warnings
 
@@ -22,6 +28,8 @@ from scipy import integrate
 
 import _paths  # noqa: F401
 from mfixdem import abi, workloads
+from linear_gas import (LINEAR_CASES, LINEAR_IDS, linear_case, linear_expectation,
+                        textbook_fgp)
 from oracle_py import OracleSystem
 
 LX, LY, LZ, CELLS = 0.02, 0.04, 0.02, (10, 20, 10)
@@ -150,21 +158,6 @@ def test_none_scheme_deposits_into_own_cell():
     o.close()
 
 
-def textbook_fgp(drag, eps, mu, rho, vrel, dp, pvol):
-    """Drag coefficient times particle volume: Ergun (eps <= 0.8, Gidaspow
-    only) or Wen-Yu with Cd = 24/Re (1 + 0.15 Re^0.687), Re = eps rho dp |u|/mu,
-    0.44 above Re = 1000; both per unit solids volume (beta / eps_s).  Like
-    the reference (drag_gs.f DRAG_GIDASPOW / DRAG_WEN_YU), Re = 0 gives 0 in
-    either branch; the force F_gp (U - v) is then zero anyway."""
-    if vrel == 0.0:
-        return 0.0
-    if drag == abi.DRAG_GIDASPOW and eps <= 0.8:
-        return (150.0 * (1.0 - eps) * mu / (eps * dp * dp) + 1.75 * rho * vrel / dp) * pvol
-    re = dp * vrel * rho * eps / mu
-    if re == 0.0:
-        return 0.0
-    cd = 24.0 / (re + 1e-15) * (1.0 + 0.15 * re ** 0.687) if re <= 1000.0 else 0.44
-    return 0.75 * cd * vrel * rho * eps * eps ** (-2.65) / dp * pvol
 
 
 @pytest.mark.parametrize("scheme,drag", [(abi.INTERP_SQUARE_DPVM, abi.DRAG_GIDASPOW),
@@ -264,3 +257,39 @@ def test_pressure_gradient_linear_and_quadratic_fields():
     inner = fluid & (jj > 2) & (jj < ny + 1)
     assert np.allclose(pf[inner, 1], -6.0e4 * y[inner], rtol=1e-9, atol=1e-6)
     o.close()
+
+
+# --------------------------------------------------------------------------
+# DES1 drag in a gas field linear in space (closed forms in linear_gas.py)
+
+
+@pytest.mark.parametrize("drag", [abi.DRAG_GIDASPOW, abi.DRAG_WEN_YU], ids=["gidaspow", "wenyu"])
+@pytest.mark.parametrize("scheme,interp_on,width", LINEAR_CASES, ids=LINEAR_IDS)
+def test_des1_drag_linear_field_closed_form(scheme, interp_on, width, drag):
+    """CALC_CELL_CENTER_GAS_VEL's face averages and the 27-point
+    interpolation of the cell-centre velocity and P_FORCE
+    (drag_gs_des1.f:96-116,313-370) in a linear gas field: FC of each
+    sphere equals the drag law at the closed-form interpolated velocity plus
+    the closed-form interpolated pressure force, first with the spheres'
+    own velocities, then with spheres moving at the interpolated gas
+    velocity (drag ~ 0, the pressure force alone)."""
+    p, parts, gas = linear_case(scheme, interp_on, width, drag)
+    for match in (False, True):
+        o = OracleSystem(p)
+        o.set_particles(parts)
+        o.set_gas(gas)
+        o.init()
+        ep, _ = o.get_gas()
+        vel, want = linear_expectation(p, parts, ep, o.filter, match_gas_velocity=match)
+        if match:
+            o.close()
+            parts = workloads.make_particles(parts.pos, parts.radius, parts.ro_sol, vel=vel)
+            o = OracleSystem(p)
+            o.set_particles(parts)
+            o.set_gas(gas)
+            o.init()
+        o.step(1)
+        fc, _ = o.get_forces()
+        o.close()
+        scale = np.max(np.abs(want))
+        assert np.max(np.abs(fc - want)) <= 1e-12 * scale, (match, np.max(np.abs(fc - want)) / scale)

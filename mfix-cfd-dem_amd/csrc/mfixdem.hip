@@ -233,6 +233,17 @@ hipError_t sync_stream(mfixdem_handle* h) {
   return e;
 }
 
+// New particle state: deposition restarts from buffer 0 with both
+// fixed-point node-sum buffers zero (the last step left its sums in one of
+// them; only the other is cleared by the cell-field kernel).
+int32_t reset_deposit(mfixdem_handle* h) {
+  h->fb = 0;
+  const size_t nb = sizeof(unsigned long long) * (size_t)h->S.ncell * h->S.p.nphases;
+  for (int b = 0; b < 2; b++)
+    if (h->rops_fix[b]) HIPOK(hipMemsetAsync(h->rops_fix[b], 0, nb, h->stream));
+  return MFIXDEM_OK;
+}
+
 // the reference-order deposition exists for the GARG_2012 node sums
 bool ref_deposition(const mfixdem_handle* h) {
   return (h->debug & MFIXDEM_DEBUG_REFERENCE_DEPOSITION) != 0 &&
@@ -782,7 +793,7 @@ int32_t upload_ensemble(mfixdem_handle* h) {
   }
   invalidate_graphs(h);
   h->cur = h->pcur = h->tw = h->lt = 0;
-  h->fb = 0;
+  CHECKRC(reset_deposit(h));
   HIPOK(hipMemcpy(h->pos[0], h->st_pos.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vel[0], h->st_vel.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->omega[0], h->st_om.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
@@ -802,7 +813,7 @@ int32_t upload_ensemble(mfixdem_handle* h) {
     HIPOK(hipMemcpy(h->nbr_split[b], Z.data(), sizeof(int) * N, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   }
-  HIPOK(hipMemset(h->flags, 0, sizeof(int) * kFlagCount));
+  HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
   h->st = mfixdem_status{};
   h->n_pairs = 0;
   h->first_pass = true;
@@ -1054,7 +1065,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   HIPOK(sync_stream(h));  // steps already queued still use the buffers
   invalidate_graphs(h);
   h->cur = h->pcur = h->tw = h->lt = 0;
-  h->fb = 0;
+  CHECKRC(reset_deposit(h));
   HIPOK(hipMemcpy(h->pos[0], P3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vel[0], V3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->omega[0], W3.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice));
@@ -1074,9 +1085,10 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
     HIPOK(hipMemcpy(h->nbr_split[b], Z.data(), sizeof(int) * N, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   }
-  HIPOK(hipMemset(h->flags, 0, sizeof(int) * kFlagCount));
-  HIPOK(hipMemset(h->fc_id, 0, sizeof(double) * 3 * S));
-  HIPOK(hipMemset(h->tow_id, 0, sizeof(double) * 3 * S));
+  // on the handle's stream, which every later call on the handle waits for
+  HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
+  HIPOK(hipMemsetAsync(h->fc_id, 0, sizeof(double) * 3 * S, h->stream));
+  HIPOK(hipMemsetAsync(h->tow_id, 0, sizeof(double) * 3 * S, h->stream));
   h->st = mfixdem_status{};
   h->n_pairs = 0;
   h->inited = false;

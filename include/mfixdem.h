@@ -56,7 +56,9 @@ enum {
                                     (calc_force_dem.f:267-271)             */
   MFIXDEM_ERR_CAPACITY = 7,      /* caller buffer too small                */
   MFIXDEM_ERR_NOT_READY = 8,     /* mfixdem_init not called / no gas field */
-  MFIXDEM_ERR_UNSUPPORTED = 9    /* option outside this build's scope      */
+  MFIXDEM_ERR_UNSUPPORTED = 9,   /* option outside this build's scope      */
+  MFIXDEM_ERR_SCRATCH = 10       /* input larger than the fixed device
+                                    scratch of a no-allocation call        */
 };
 
 enum { MFIXDEM_LSD = 0, MFIXDEM_HERTZ = 1 };                 /* DES_COLL_MODEL   */
@@ -262,6 +264,22 @@ int32_t mfixdem_drag_gs_gas(mfixdem_handle* h);
    NULL. */
 int32_t mfixdem_get_gas_drag(mfixdem_handle* h, double* f_gds, double* drag_am,
                              double* drag_bm);
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  mfixdem_drag_gs_gas with the results written to the
+   caller's DEVICE buffers f_gds[ncell], drag_am[ncell], drag_bm[3*ncell]
+   (layouts as in mfixdem_get_gas_drag) instead of the handle's, and no
+   hipMalloc, hipFree or host<->device copy: the per-particle F_GP records
+   and DES_VOL_NODE live in one fixed device scratch array per GPU (80 MiB,
+   held until the process ends).  GARG_2012 needs 8*max_pip + ncell doubles
+   of it, at most 10*2^20 (about 1.24M particles with a 64x128x64 grid);
+   larger handles get MFIXDEM_ERR_SCRATCH.  Asynchronous on the handle's
+   stream; results equal mfixdem_drag_gs_gas's bit for bit.  Every call on
+   one GPU shares that GPU's scratch, so calls whose work overlaps on one
+   GPU (handles on different streams, or host threads) race: order them
+   (e.g. mfixdem_synchronize between them). */
+int32_t mfixdem_drag_gs_gas_noalloc(mfixdem_handle* h, double* f_gds, double* drag_am,
+                                    double* drag_bm);
 
 /* --- inspection for parity and restart ------------------------------------ */
 int32_t mfixdem_set_debug(mfixdem_handle* h, int32_t flags);

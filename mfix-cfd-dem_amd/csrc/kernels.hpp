@@ -2156,6 +2156,31 @@ __global__ __launch_bounds__(kBS) void k_gas_fgds(int ni, int nj, int nk, const 
   f_gds[c] = f;
 }
 
+// DES_VOL_NODE of the uniform grid (compute_volume_of_nodes,
+// cfassign.f:100-230): 1/8 of the fluid volume of the 8 cells sharing node
+// (I, J, K) for I <= imax+1, J <= jmax+1, K <= kmax+1, summed in the
+// reference's order; 0 at every other index.
+__global__ __launch_bounds__(kBS) void k_vol_node(int imax, int jmax, int kmax, int ni, int nj,
+                                                  int nk, const uint8_t* fluid, double v,
+                                                  double* vol_node) {
+  const int node = blockIdx.x * kBS + threadIdx.x;
+  if (node >= ni * nj * nk) return;
+  const int J = node % nj + 1;
+  const int I = (node / nj) % ni + 1;
+  const int K = node / (nj * ni) + 1;
+  double a = 0.0;
+  if (I <= imax + 1 && J <= jmax + 1 && K <= kmax + 1) {
+    auto volf = [&](int i, int j, int k) {
+      return fluid[(j - 1) + (i - 1) * nj + (k - 1) * nj * ni] ? v : 0.0;
+    };
+    const double af = 0.125;
+    a = af * (volf(I, J, K) + volf(I + 1, J, K) + volf(I, J + 1, K) + volf(I + 1, J + 1, K));
+    a = a + af * (volf(I, J, K + 1) + volf(I + 1, J + 1, K + 1) + volf(I, J + 1, K + 1) +
+                  volf(I + 1, J, K + 1));
+  }
+  vol_node[node] = a;
+}
+
 // node -> cell ROP_S (comp_mean_fields0.f:298-345), ROP_S/VOL (:347-370) and
 // CALC_EPG_DES (calc_epg_des.f:95-150).  Nodes visited K, J, I ascending.
 // FIXED: node sums come from the fixed-point accumulators (converted exactly

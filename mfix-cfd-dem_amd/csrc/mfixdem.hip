@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mfixdem.h"
@@ -65,6 +66,8 @@ struct mfixdem_handle {
   std::vector<double> st_pos, st_vel, st_om, st_r, st_rho;
   std::vector<int> st_phase;
   hipStream_t stream = nullptr;
+  int device = 0;              // HIP device the handle was created on
+  double* scratch_ptr = nullptr;  // that device's address of `scratch`
   int n = 0, stride = 0;
   int cur = 0, pcur = 0, tw = 0, lt = 0;
   // particles (cur / tw twins)
@@ -136,6 +139,14 @@ struct mfixdem_handle {
   double t_ms[kTimeSlots]{};
   int64_t t_calls[kTimeSlots]{};
 };
+
+// Device scratch of mfixdem_drag_gs_gas_noalloc, one copy per GPU, held
+// until the process ends: the per-particle F_GP records (kGasRec doubles per
+// particle slot) followed by DES_VOL_NODE (one double per cell).  Host code
+// never passes `scratch` itself to a kernel (in host code the name is a
+// host-side stand-in); it uses the device address resolved by bind_scratch.
+constexpr int64_t kScratchDoubles = 10LL << 20;  // 80 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -933,6 +944,7 @@ const char* mfixdem_error_string(int32_t code) {
     case MFIXDEM_ERR_CAPACITY: return "buffer too small";
     case MFIXDEM_ERR_NOT_READY: return "handle not initialised";
     case MFIXDEM_ERR_UNSUPPORTED: return "unsupported option";
+    case MFIXDEM_ERR_SCRATCH: return "input exceeds the fixed device scratch of a no-allocation call";
     default: return "unknown error";
   }
 }
@@ -972,6 +984,26 @@ int32_t mfixdem_params_default(mfixdem_params* p) {
   return MFIXDEM_OK;
 }
 
+// Device address of `scratch` on each device, resolved once per device with
+// that device current (each GPU holds its own copy at its own address).
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;
+int32_t bind_scratch(mfixdem_handle* h) {
+  int dev = 0;
+  HIPOK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIPOK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return MFIXDEM_ERR_HIP;
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  h->device = dev;
+  h->scratch_ptr = g_scratch_dev[dev];
+  return MFIXDEM_OK;
+}
+
 int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
   if (!p || !out) return MFIXDEM_ERR_ARG;
   *out = nullptr;
@@ -990,7 +1022,7 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
     return MFIXDEM_ERR_UNSUPPORTED;
   }
   h->key_bits = h->G.mbits;
-  if ((rc = alloc_buffers(h, h->S.dg_ncell))) {
+  if ((rc = alloc_buffers(h, h->S.dg_ncell)) || (rc = bind_scratch(h))) {
     mfixdem_destroy(h);
     return rc;
   }
@@ -1376,6 +1408,49 @@ int32_t mfixdem_get_gas_drag(mfixdem_handle* h, double* f_gds, double* drag_am,
     HIPOK(hipMemcpyAsync(drag_bm, h->gd_bm, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
                          h->stream));
   HIPOK(sync_stream(h));
+  return MFIXDEM_OK;
+}
+
+// mfixdem_drag_gs_gas without allocation or host-device copies: results go
+// to the caller's device buffers; the F_GP records and DES_VOL_NODE live in
+// the device's `scratch` (shared by every call on that device).
+int32_t mfixdem_drag_gs_gas_noalloc(mfixdem_handle* h, double* f_gds, double* drag_am,
+                                    double* drag_bm) {
+  if (!h || !f_gds || !drag_am || !drag_bm) return MFIXDEM_ERR_ARG;
+  const mfixdem_params& p = h->S.p;
+  if (h->nsys > 1 || !p.coupled) return MFIXDEM_ERR_UNSUPPORTED;
+  const int nc = h->S.ncell;
+  const bool garg = p.interp_scheme == MFIXDEM_INTERP_GARG_2012;
+  if (garg && (int64_t)kGasRec * h->n + nc > kScratchDoubles) return MFIXDEM_ERR_SCRATCH;
+  if (!h->inited) return MFIXDEM_ERR_NOT_READY;
+  if (!h->scratch_ptr) return MFIXDEM_ERR_HIP;
+  if (!garg) {  // DRAG_GS_GAS1: no DRAG_AM, no records
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nc)), dim3(kBS), 0, h->stream, drag_am, nc, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nc)), dim3(kBS), 0, h->stream, f_gds, nc, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * (int64_t)nc)), dim3(kBS), 0, h->stream, drag_bm,
+                       3 * nc, 0.0);
+    if (h->n > 0)
+      hipLaunchKernelGGL(k_gas_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P,
+                         step_ptrs(h), h->vol, nc, f_gds, drag_bm);
+    HIPOK(hipGetLastError());
+    return MFIXDEM_OK;
+  }
+  double* rec = h->scratch_ptr;                          // kGasRec * n, all written
+  double* vol_node = h->scratch_ptr + (int64_t)kGasRec * h->n;  // nc, all written
+  const double v = (p.xlength / (double)p.imax) * (p.ylength / (double)p.jmax) *
+                   (p.zlength / (double)p.kmax);
+  hipLaunchKernelGGL(k_vol_node, dim3(nblk(nc)), dim3(kBS), 0, h->stream, p.imax, p.jmax, p.kmax,
+                     h->S.ni, h->S.nj, h->S.nk, h->fluid, v, vol_node);
+  CHECKRC(do_pic_lists(h));
+  if (h->n > 0)
+    hipLaunchKernelGGL(k_gas_fgp, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h),
+                       rec);
+  hipLaunchKernelGGL(k_gas_nodes, dim3(nblk(nc)), dim3(kBS), 0, h->stream, p.imax, p.jmax, p.kmax,
+                     h->S.ni, h->S.nj, h->S.nk, h->fluid, h->pic_off, h->pic, rec, vol_node,
+                     drag_am, drag_bm);
+  hipLaunchKernelGGL(k_gas_fgds, dim3(nblk(nc)), dim3(kBS), 0, h->stream, h->S.ni, h->S.nj,
+                     h->S.nk, h->fluid, drag_am, f_gds);
+  HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
 

@@ -32,7 +32,7 @@ ROWS_THREAD, ROWS_WAVE128, ROWS_WAVE192, ROWS_WAVE256 = 0, 1, 2, 3
 
 ERRORS = {
     0: "OK", 1: "ARG", 2: "HIP", 3: "OOM", 4: "ROGUE", 5: "EPG",
-    6: "ZERO_DISTANCE", 7: "CAPACITY", 8: "NOT_READY", 9: "UNSUPPORTED",
+    6: "ZERO_DISTANCE", 7: "CAPACITY", 8: "NOT_READY", 9: "UNSUPPORTED", 10: "SCRATCH",
 }
 ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
@@ -205,6 +205,7 @@ def _declare(lib):
         "mfixdem_drag_gs_gas": ([_P], _I),
         "mfixdem_set_pairs": ([_P, _IP, _IP, _DP, C.c_int64], _I),
         "mfixdem_get_gas_drag": ([_P, _DP, _DP, _DP], _I),
+        "mfixdem_drag_gs_gas_noalloc": ([_P, _P, _P, _P], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

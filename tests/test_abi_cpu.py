@@ -50,7 +50,7 @@ def test_params_layout_and_defaults_match_c():
 
 def test_error_strings():
     lib = abi.load_library()
-    for code in range(10):
+    for code in range(11):
         assert lib.mfixdem_error_string(code)
 
 

@@ -76,6 +76,23 @@ def time_steps(s, nn0, steps):
     return time.perf_counter() - t0
 
 
+def timed_repeats(s, nn0, steps, repeats, k):
+    """`repeats` timed windows of exactly `steps` steps.  Window r starts at
+    NN = nn0 + r*L with L the smallest multiple of the rebuild interval k
+    that is >= steps (untimed steps fill the gaps), so every window holds
+    the same number of neighbour rebuilds.  Returns (seconds per window,
+    next NN)."""
+    period = -(-steps // k) * k
+    out, nn = [], nn0
+    for r in range(max(1, repeats)):
+        if r:
+            s.run(nn, period - steps)
+            nn += period - steps
+        out.append(time_steps(s, nn, steps))
+        nn += steps
+    return out, nn
+
+
 def kernel_times(s, nn0, steps):
     """Per-kernel-group HIP event times over `steps` eager steps."""
     s.set_debug(abi.DEBUG_TIME_KERNELS)
@@ -257,30 +274,71 @@ def two_way_bench(s, reps=20):
             "routine": "DRAG_GS_GAS0 (F_GDS, DRAG_AM, DRAG_BM), reference summation order"}
 
 
-def cpu_baseline(p, parts, gas, seconds=15.0):
+def cpu_model():
+    """CPU model and logical CPU count of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    model, cpus = None, os.cpu_count()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                model = v.strip()
+            elif k.strip() == "CPU(s)":
+                cpus = int(v.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    return model, cpus
+
+
+def cpu_baseline(p, parts, gas, cycles_all=3, cycles_one=1):
     """Times the OpenMP CPU restatement (oracle/liboracle_omp.so) on the same
-    workload for as many whole steps as fit in about `seconds`."""
+    workload over whole rebuild cycles: steps NN = K+1 .. 2K are K steps
+    with exactly one neighbour rebuild (at NN = 2K), as in the GPU run.
+    All threads (OMP_NUM_THREADS, else every CPU): median over `cycles_all`
+    consecutive cycles; 1 thread: `cycles_one` cycle(s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import OracleSystem
 
-    o = OracleSystem(p, omp=True)
-    o.set_particles(parts)
-    o.set_gas(gas)
-    o.init()
-    o.step(1)  # rebuild step, untimed
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        o.step(2 + steps)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds or steps >= 200:
-            break
-    threads = o.lib.orc_omp_threads()
-    o.close()
-    return {"value": p.max_pip * steps / el, "unit": "particle-steps/s", "cores": int(threads),
-            "kind": "port",
-            "sample": f"{steps} DES_TIME_STEP calls (no rebuild) of the same {p.max_pip}-sphere "
-                      f"workload, OpenMP CPU restatement, {threads} threads, {el:.1f} s"}
+    K = p.neighbor_search_n
+    model, cpus = cpu_model()
+
+    def timed(threads, cycles):
+        o = OracleSystem(p, omp=True)
+        o.lib.orc_set_omp_threads(threads)
+        used = int(o.lib.orc_omp_threads())
+        o.set_particles(parts)
+        o.set_gas(gas)
+        o.init()
+        o.run(1, K)                      # NN = 1 .. K, untimed
+        per_cycle = []
+        for c in range(cycles):
+            t0 = time.perf_counter()
+            o.run((c + 1) * K + 1, K)    # NN = (c+1)K+1 .. (c+2)K, one rebuild
+            per_cycle.append(time.perf_counter() - t0)
+        o.close()
+        return used, per_cycle
+
+    n_all = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (cpus or 1)
+    used_all, cyc_all = timed(n_all, cycles_all)
+    used_one, cyc_one = timed(1, cycles_one)
+    med = float(np.median(cyc_all))
+    v_all = p.max_pip * K / med
+    v_one = p.max_pip * K / float(np.median(cyc_one))
+    return {"value": v_all, "unit": "particle-steps/s", "cores": used_all, "kind": "port",
+            "sample": f"{p.max_pip}-sphere workload, OpenMP CPU restatement, {used_all} threads: "
+                      f"median of {cycles_all} whole {K}-step cycles (each with one neighbour "
+                      f"rebuild), {sum(cyc_all):.1f} s in total",
+            "cpu_model": model, "host_cpus": cpus,
+            "cycle_s": cyc_all,
+            "single_thread": {"value": v_one, "cores": used_one, "cycle_s": cyc_one,
+                              "sample": f"{cycles_one} whole {K}-step cycle(s) with one "
+                                        f"rebuild each, 1 thread"}}
 
 
 def main():
@@ -289,7 +347,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -313,28 +372,40 @@ def main():
     # FP64
     s = make_system(p, parts, gas, tuning)
     s.run(1, args.warmup)
-    el = time_steps(s, 1 + args.warmup, args.steps)
+    reps, nn = timed_repeats(s, 1 + args.warmup, args.steps, args.repeats,
+                             p.neighbor_search_n)
+    el = float(np.median(reps))
     st = s.status()
     pairs = st.n_pairs
     value = parts.n * args.steps / el
-    log(f"fp64: {args.steps} steps in {el:.3f} s -> {value:.4e} particle-steps/s, "
+    log(f"fp64: {args.steps} steps, median of {len(reps)} repeats {el:.4f} s "
+        f"({', '.join(f'{r:.4f}' for r in reps)}) -> {value:.4e} particle-steps/s, "
         f"pairs/particle {pairs / parts.n:.2f}")
-    kt, calls = kernel_times(s, 1 + args.warmup + args.steps, 50)
+    kt, calls = kernel_times(s, nn, 50)
     t_step_ms = kt[abi.TIME_STEP]
     bytes_step = algorithmic_bytes_step_kernel(parts.n, pairs)
     achieved = bytes_step / (t_step_ms * 1e-3) / 1e9
     log(f"fused step kernel {t_step_ms * 1e3:.1f} us, rebuild {kt[abi.TIME_REBUILD] * 1e3:.1f} us, "
         f"gas {kt[abi.TIME_GAS] * 1e3:.1f} us")
-    nsteps_total = args.warmup + args.steps + 50
+    nsteps_total = args.warmup + len(reps) * args.steps + 50
     pos64, vel64, om64, _ = s.get_particles()
     two_way = two_way_bench(s)
     log(f"two-way coupling: DRAG_GS_GAS0 {two_way['us_per_call']:.1f} us per call")
     s.close()
 
-    traffic = None
+    # HBM bytes per k_step launch from the committed rocprofv3 counter runs
+    # (FETCH_SIZE, WRITE_SIZE; a profiler run of its own), with the date they
+    # were measured and whether they describe the kernel sources built here
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_step_fluidised_1M.json")
     if os.path.exists(tpath) and args.n == 1_000_000:
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        from mfixdem import traffic as traffic_mod
+        tj = json.load(open(tpath))
+        traffic = tj.get("bytes_per_launch")
+        traffic_src = {"file": "profiles/traffic_k_step_fluidised_1M.json",
+                       "measured_utc": tj.get("measured_utc"),
+                       "same_kernel_sources": tj.get("kernel_source_sha256_16") ==
+                       traffic_mod.kernel_source_digest()}
 
     mixed = None
     if not args.no_mixed:
@@ -342,8 +413,10 @@ def main():
         pm.precision = abi.MIXED
         m = make_system(pm, parts, gas, tuning)
         m.run(1, args.warmup)
-        elm = time_steps(m, 1 + args.warmup, args.steps)
-        ktm, _ = kernel_times(m, 1 + args.warmup + args.steps, 50)
+        repm, nnm = timed_repeats(m, 1 + args.warmup, args.steps, args.repeats,
+                                  pm.neighbor_search_n)
+        elm = float(np.median(repm))
+        ktm, _ = kernel_times(m, nnm, 50)
         posm, velm, omm, _ = m.get_particles()
         m.close()
         ke64, kem = kinetic_energy(parts, vel64, om64), kinetic_energy(parts, velm, omm)
@@ -368,8 +441,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(p, parts, gas, args.cpu_seconds)
-        log(f"cpu baseline: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(p, parts, gas, cycles_all=args.cpu_cycles)
+        log(f"cpu baseline: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads, "
+            f"{cpu['single_thread']['value']:.4e} on 1 thread ({cpu['cpu_model']})")
 
     out = {
         "metric": METRIC,
@@ -379,6 +453,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * el / args.steps,
+        "repeats_s": reps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -395,6 +470,7 @@ def main():
                               "candidates_s": tune_s}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_step (fused walls+pairs+drag+integrate+locate)",
                      "kernel_us": t_step_ms * 1e3, "algorithmic_bytes": bytes_step},
         "cpu_baseline": cpu,

@@ -17,10 +17,27 @@ from __future__ import annotations
 
 import csv
 import json
+import os
 import sys
+import time
 
 COPY_BYTES = 1 << 30   # 1 GiB read + 1 GiB written by the calibration copy
 STEP_KERNEL = "k_step<false,0,false,false>"   # FP64, fused, GARG, single system
+
+
+KERNEL_SOURCES = ("csrc/kernels.hpp", "csrc/device_physics.hpp", "csrc/params.hpp")
+
+
+def kernel_source_digest() -> str:
+    """sha256 over the device sources of the step kernel: the counter figures
+    describe the kernel built from exactly these sources."""
+    import hashlib
+    import os
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update(open(os.path.join(here, rel), "rb").read())
+    return h.hexdigest()[:16]
 
 
 def run(steps=12):
@@ -92,6 +109,8 @@ def summarize(fetch_csv, write_csv, out_json):
         "copy_fetch_kib": fc, "copy_write_kib": wc,
         "read_scale": scale_r, "write_scale": scale_w,
         "bytes_per_launch": fk * 1024.0 * scale_r + wk * 1024.0 * scale_w,
+        "measured_utc": time.strftime("%Y-%m-%dT%H:%MZ", time.gmtime(os.path.getmtime(fetch_csv))),
+        "kernel_source_sha256_16": kernel_source_digest(),
     }
     json.dump(res, open(out_json, "w"), indent=1)
     print(json.dumps(res, indent=1))

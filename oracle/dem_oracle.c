@@ -154,6 +154,16 @@ int orc_omp_threads(void) {
 #endif
 }
 
+/* Thread count of the OpenMP build's later parallel regions (the CPU
+   baseline times 1 thread and all threads); no effect on the serial build. */
+void orc_set_omp_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------ parameter derivation */
 /* CHECK_SOLIDS_DEM_COLL_LSD, check_data/check_solids_dem.f:519-782 and
    REF_MASS / REF_RADIUS at :1162-1163. */

@@ -80,6 +80,7 @@ double orc_drag_fgp(int32_t drag_type, double epg, double mu, double rog,
                     double vrel, double dpm, double pvol);
 
 int orc_omp_threads(void);
+void orc_set_omp_threads(int n);
 
 #ifdef __cplusplus
 }

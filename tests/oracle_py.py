@@ -56,6 +56,7 @@ def load_oracle(omp: bool = False):
         "orc_drag_fgp": ([I, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                           C.c_double], C.c_double),
         "orc_omp_threads": ([], C.c_int),
+        "orc_set_omp_threads": ([C.c_int], None),
         "orc_user_set_particles": ([P, L, L, D, D, D], C.c_int),
         "orc_get_filter": ([P, L, IP, D], C.c_int),
         "orc_get_ugc": ([P, D], C.c_int),

@@ -35,6 +35,7 @@ struct DevParams {
   int hertz, ab2, coupled, drag_type, nph, nf;
   int first_pass;   // Adams-Bashforth first step
   int store_forces; // debug: keep FC/TOW
+  int fuse_dep;     // fused step: GARG_2012 deposition in the step kernel
   double dt;
   double grav[3];
   double kn, kt, kn_w, kt_w, mew, mew_w;

@@ -47,6 +47,12 @@ struct StepPtrs {
   const double* ugc;            // cell-centre gas velocity [3*ijk + d] (filter schemes)
   const uint8_t* fluid;
   int* flags;
+  // fused deposition (DevParams::fuse_dep): fixed-point node sums of this
+  // step, and the queue of particles that left the fluid grid (slot ids;
+  // count at rm_cnt[0]), whose removal the next kernel completes
+  unsigned long long* dep_fix;
+  int* rm_cnt;
+  int* rm_list;
 };
 
 enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
@@ -548,6 +554,246 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
   }
 }
 
+// Inputs of the fused deposition from one particle's step: its new
+// position, mass and phase and the fluid cell it is deposited into (-1:
+// none), or a removal (the particle left the fluid grid).
+struct DepIn {
+  int pc = -1;
+  int ph = 0;
+  bool remove = false;
+  double x[3] = {0.0, 0.0, 0.0};
+  double m = 0.0;
+};
+
+// COMP_MEAN_FIELDS0 deposition (comp_mean_fields0.f:186-210) fused into the
+// step kernel, one particle per thread: trilinear node weights x PMASS are
+// rounded to int64 fixed point (as in k_deposit_list, so the integer sums
+// and hence ROP_S / EP_G are identical) and summed in an LDS tile over the
+// bounding box of the workgroup's nodes (tile, cap entries, which aliases
+// LDS that the step no longer uses), then one global atomic per non-zero
+// node; a larger box or a multi-phase system adds straight to fix.  A
+// particle that left the grid is queued (slot id) for the next kernel,
+// which sets its state and radius to 0: other lanes of this launch may
+// still read them as a partner's.  Every thread of the workgroup calls
+// this (it holds barriers).
+__device__ __forceinline__ void fused_deposit(const DevParams& P, const StepPtrs& a, int i,
+                                              const DepIn& d, unsigned long long* tile,
+                                              int cap) {
+  __shared__ int box[6];
+  if (d.remove) {
+    const int q = atomicAdd(a.rm_cnt, 1);
+    a.rm_list[q] = i;
+    atomicAdd(&a.flags[kFlagOutOfDomain], 1);
+  }
+  int ci = 0, cj = 0, ck = 0;
+  if (d.pc >= 0) unpack_ijk(d.pc, ci, cj, ck);
+  int lo0 = d.pc >= 0 ? ci : INT_MAX, lo1 = d.pc >= 0 ? cj : INT_MAX;
+  int lo2 = d.pc >= 0 ? ck : INT_MAX;
+  int hi0 = d.pc >= 0 ? ci : INT_MIN, hi1 = d.pc >= 0 ? cj : INT_MIN;
+  int hi2 = d.pc >= 0 ? ck : INT_MIN;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo0 = min(lo0, __shfl_xor(lo0, o, 64));
+    lo1 = min(lo1, __shfl_xor(lo1, o, 64));
+    lo2 = min(lo2, __shfl_xor(lo2, o, 64));
+    hi0 = max(hi0, __shfl_xor(hi0, o, 64));
+    hi1 = max(hi1, __shfl_xor(hi1, o, 64));
+    hi2 = max(hi2, __shfl_xor(hi2, o, 64));
+  }
+  if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
+  else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
+  __syncthreads();  // also: every lane is done with the LDS that tile aliases
+  if ((threadIdx.x & 63) == 0 && lo0 != INT_MAX) {
+    atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+    atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
+  }
+  __syncthreads();
+  // tile nodes I in [bi-1, box_hi]: a cell I touches nodes I-1 and I
+  const int bi = box[0], bj = box[1], bk = box[2];
+  const int ti_n = box[3] - bi + 2, tj_n = box[4] - bj + 2, tk_n = box[5] - bk + 2;
+  const bool any = bi != INT_MAX;
+  const int vol = any ? ti_n * tj_n * tk_n : 0;
+  const bool tiled = any && P.nph == 1 && vol <= cap;
+  if (tiled)
+    for (int q = threadIdx.x; q < vol; q += kBS) tile[q] = 0ull;
+  __syncthreads();
+  if (d.pc >= 0) {
+    const double x0 = a.xe[ci - 1], y0 = a.yn[cj - 1], z0 = a.zt[ck - 1];
+    const double dxx = a.xe[ci] - x0, dyy = a.yn[cj] - y0, dzz = a.zt[ck] - z0;
+    double zx = d.x[0] - x0, zy = d.x[1] - y0;
+    zx = zx / dxx;
+    zy = zy / dyy;
+    double zz = d.x[2] - z0;
+    zz = zz / dzz;
+    const double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+    const double sc = P.dep_scale[d.ph];
+    if (tiled) {
+      const int tb = (ci - bi) + ti_n * ((cj - bj) + tj_n * (ck - bk));
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const double w = xv[q & 1] * yv[(q >> 1) & 1] * zv[q >> 2];
+        const double v = w * d.m;
+        const unsigned long long qv = (unsigned long long)__double2ll_rn(v * sc);
+        atomicAdd(&tile[tb + (q & 1) + ti_n * (((q >> 1) & 1) + tj_n * (q >> 2))], qv);
+      }
+    } else {
+      unsigned long long* f = a.dep_fix + (size_t)d.ph * P.ncell;
+      const int base = lin_ijk(ci - 1, cj - 1, ck - 1, P.ni, P.nj);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const double w = xv[q & 1] * yv[(q >> 1) & 1] * zv[q >> 2];
+        const double v = w * d.m;
+        const unsigned long long qv = (unsigned long long)__double2ll_rn(v * sc);
+        atomicAdd(&f[base + ((q >> 1) & 1) + (q & 1) * P.nj + (q >> 2) * P.nj * P.ni], qv);
+      }
+    }
+  }
+  if (!tiled) return;  // uniform over the workgroup
+  __syncthreads();
+  int ti = threadIdx.x % ti_n, tj = threadIdx.x / ti_n, tk = tj / tj_n;
+  tj -= tk * tj_n;
+  const int step_i = kBS % ti_n, step_r = kBS / ti_n;
+  for (int q = threadIdx.x; q < vol; q += kBS) {
+    const unsigned long long v = tile[q];
+    if (v != 0ull) {
+      const int I = bi - 1 + ti, J = bj - 1 + tj, K = bk - 1 + tk;  // node (I,J,K), 1-based
+      atomicAdd(&a.dep_fix[lin_ijk(I, J, K, P.ni, P.nj)], v);
+    }
+    ti += step_i;
+    int rows = step_r;
+    if (ti >= ti_n) {
+      ti -= ti_n;
+      rows++;
+    }
+    tj += rows;
+    while (tj >= tj_n) {
+      tj -= tj_n;
+      tk++;
+    }
+  }
+}
+
+// The per-particle part of a step after the pair pass (drag, integrator,
+// fluid-cell locate) for a live particle i; fills dep for the fused
+// deposition when P0.fuse_dep.
+template <bool MIXED, int MODE, bool FILTER, bool BATCH, bool kEarlyDrag>
+__device__ __forceinline__ void step_tail(const DevParams& P0, const DevParams& P,
+                                          const StepPtrs& a, int i, int st, double fc[3],
+                                          double tow[3], bool drag, const double* s_drag,
+                                          DepIn& dep) {
+  const int S = P0.stride;
+  if (st == 0) {
+    if (MODE == kModeFused || MODE == kModeIntegrate) {
+      for (int c = 0; c < 3; c++) {
+        a.pos_o[c * S + i] = a.pos[c * S + i];
+        a.vel_o[c * S + i] = a.vel[c * S + i];
+        a.omega_o[c * S + i] = a.omega[c * S + i];
+      }
+    }
+    return;
+  }
+  if (MODE == kModeForce) {
+    store3(a.fc, S, i, fc);
+    store3(a.tow, S, i, tow);
+    return;
+  }
+  double pos[3], vel[3], om[3];
+  load3(a.pos, S, i, pos);
+  load3(a.vel, S, i, vel);
+  load3(a.omega, S, i, om);
+  const double mi = a.pmass[i];
+  if (MODE == kModeDrag) {
+    load3(a.fc, S, i, fc);
+    if (P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
+    store3(a.fc, S, i, fc);
+    return;
+  }
+  if (kEarlyDrag) {
+    if (drag) {
+      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[d * kBS + threadIdx.x];
+      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[(3 + d) * kBS + threadIdx.x];
+    }
+  } else if (MODE == kModeFused && P.coupled) {
+    add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
+  }
+  if (MODE == kModeIntegrate) {
+    load3(a.fc, S, i, fc);
+    load3(a.tow, S, i, tow);
+  }
+  if (P0.store_forces) {
+    const int g = a.gid[i];
+    store3(a.fc_id, S, g, fc);
+    store3(a.tow_id, S, g, tow);
+  }
+
+  // ---- CFNEWVALUES (cfnewvalues.f:700-811)
+  const double dt = P.dt;
+  const double omoi = a.omoi[i];
+  if (!P.ab2) {
+    for (int c = 0; c < 3; c++) {
+      vel[c] = vel[c] + dt * (fc[c] / mi + P.grav[c]);
+      pos[c] = pos[c] + vel[c] * dt;
+    }
+    for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
+  } else {
+    double acc[3], racc[3];
+    if (P0.first_pass) {
+      for (int c = 0; c < 3; c++) acc[c] = fc[c] / mi + P.grav[c];
+      for (int c = 0; c < 3; c++) racc[c] = tow[c];
+    } else {
+      load3(a.acc_old, S, i, acc);
+      load3(a.rot_acc_old, S, i, racc);
+    }
+    for (int c = 0; c < 3; c++) {
+      const double f = fc[c] / mi + P.grav[c];
+      const double vn = vel[c] + 0.5 * (3.0 * f - acc[c]) * dt;
+      acc[c] = f;
+      pos[c] = pos[c] + 0.5 * (vel[c] + vn) * dt;
+      vel[c] = vn;
+    }
+    for (int c = 0; c < 3; c++) {
+      om[c] = om[c] + 0.5 * (3.0 * tow[c] * omoi - racc[c]) * dt;
+      racc[c] = tow[c] * omoi;
+    }
+    store3(a.acc_old, S, i, acc);
+    store3(a.rot_acc_old, S, i, racc);
+  }
+
+  // ---- PARTICLES_IN_CELL locate, fused (particles_in_cell.f:87-154)
+  int pc = 0;
+  if (MODE == kModeFused && P.coupled) {
+    const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
+    const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
+    const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
+    pc = (I && J && K) ? pack_ijk(I, J, K) : -1;
+    a.fcell[i] = pc;
+  }
+  if (kEarlyDrag && P0.fuse_dep && pc < 0) {
+    // PARTICLES_IN_CELL deletion (particles_in_cell.f:175-184): state and
+    // radius follow in the next kernel
+    const double z[3] = {0.0, 0.0, 0.0};
+    store3(a.pos_o, S, i, z);
+    store3(a.vel_o, S, i, z);
+    store3(a.omega_o, S, i, z);
+    a.wmask[i] = 0;
+    dep.remove = true;
+    return;
+  }
+  store3(a.pos_o, S, i, pos);
+  store3(a.vel_o, S, i, vel);
+  store3(a.omega_o, S, i, om);
+  if (kEarlyDrag && P0.fuse_dep) {
+    int I, J, K;
+    unpack_ijk(pc, I, J, K);
+    if (a.fluid[lin_ijk(I, J, K, P.ni, P.nj)]) {
+      dep.pc = pc;
+      dep.ph = a.phase[i];
+      dep.m = mi;
+      for (int c = 0; c < 3; c++) dep.x[c] = pos[c];
+    }
+  }
+}
+
 // One DES_TIME_STEP particle body.  MODE selects the fused step or one of the
 // unfused reference routines (CALC_FORCE_DEM / CALC_DRAG_DES / CFNEWVALUES);
 // FILTER selects the DRAG_GS_DES1 drag (any interpolation scheme other than
@@ -557,7 +803,9 @@ __device__ __forceinline__ void wave_pairs(const DevParams& P0, const StepPtrs& 
 // pair pass (garg_drag_terms), so that its gas loads travel with the
 // particle's own loads instead of forming a second chain of dependent
 // loads after the pairs; the two increments wait in LDS and are added
-// after the pair forces, where CALC_DRAG_DES adds them.
+// after the pair forces, where CALC_DRAG_DES adds them.  With
+// P0.fuse_dep it also deposits the particles' new positions
+// (COMP_MEAN_FIELDS0 node sums) before it ends.
 template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
   constexpr bool kPairs = MODE == kModeFused || MODE == kModeForce;
@@ -630,112 +878,16 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
     }
     // ---- pairs (calc_force_dem.f:180-412), summed after the walls
     wave_pairs<MIXED, BATCH>(P0, a, B, i, st, r0, sp, r1, xr, fc, tow);
-    if (!live) return;
   }
-  if (st == 0) {
-    if (MODE == kModeFused || MODE == kModeIntegrate) {
-      for (int c = 0; c < 3; c++) {
-        a.pos_o[c * S + i] = a.pos[c * S + i];
-        a.vel_o[c * S + i] = a.vel[c * S + i];
-        a.omega_o[c * S + i] = a.omega[c * S + i];
-      }
-    }
-    return;
+  DepIn dep;
+  if (live)
+    step_tail<MIXED, MODE, FILTER, BATCH, kEarlyDrag>(P0, P, a, i, st, fc, tow, drag,
+                                                      &s_drag[0][0], dep);
+  if constexpr (kEarlyDrag) {
+    // s_drag is read above and free once the first barrier inside has passed
+    if (P0.fuse_dep)
+      fused_deposit(P0, a, i, dep, reinterpret_cast<unsigned long long*>(&s_drag[0][0]), 6 * kBS);
   }
-  if (MODE == kModeForce) {
-    store3(a.fc, S, i, fc);
-    store3(a.tow, S, i, tow);
-    return;
-  }
-  double pos[3], vel[3], om[3];
-  load3(a.pos, S, i, pos);
-  load3(a.vel, S, i, vel);
-  load3(a.omega, S, i, om);
-  const double mi = a.pmass[i];
-  if (MODE == kModeDrag) {
-    load3(a.fc, S, i, fc);
-    if (P.coupled) add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
-    store3(a.fc, S, i, fc);
-    return;
-  }
-  if (kEarlyDrag) {
-    if (drag) {
-      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[d][threadIdx.x];
-      for (int d = 0; d < 3; d++) fc[d] = fc[d] + s_drag[3 + d][threadIdx.x];
-    }
-  } else if (MODE == kModeFused && P.coupled) {
-    add_drag<MIXED, FILTER>(P, a, i, pos, vel, fc);
-  }
-  if (MODE == kModeIntegrate) {
-    load3(a.fc, S, i, fc);
-    load3(a.tow, S, i, tow);
-  }
-  if (P0.store_forces) {
-    const int g = a.gid[i];
-    store3(a.fc_id, S, g, fc);
-    store3(a.tow_id, S, g, tow);
-  }
-
-  // ---- CFNEWVALUES (cfnewvalues.f:700-811)
-  const double dt = P.dt;
-  const double omoi = a.omoi[i];
-  if (!P.ab2) {
-    for (int c = 0; c < 3; c++) {
-      vel[c] = vel[c] + dt * (fc[c] / mi + P.grav[c]);
-      pos[c] = pos[c] + vel[c] * dt;
-    }
-    for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
-  } else {
-    double acc[3], racc[3];
-    if (P0.first_pass) {
-      for (int c = 0; c < 3; c++) acc[c] = fc[c] / mi + P.grav[c];
-      for (int c = 0; c < 3; c++) racc[c] = tow[c];
-    } else {
-      load3(a.acc_old, S, i, acc);
-      load3(a.rot_acc_old, S, i, racc);
-    }
-    for (int c = 0; c < 3; c++) {
-      const double f = fc[c] / mi + P.grav[c];
-      const double vn = vel[c] + 0.5 * (3.0 * f - acc[c]) * dt;
-      acc[c] = f;
-      pos[c] = pos[c] + 0.5 * (vel[c] + vn) * dt;
-      vel[c] = vn;
-    }
-    for (int c = 0; c < 3; c++) {
-      om[c] = om[c] + 0.5 * (3.0 * tow[c] * omoi - racc[c]) * dt;
-      racc[c] = tow[c] * omoi;
-    }
-    store3(a.acc_old, S, i, acc);
-    store3(a.rot_acc_old, S, i, racc);
-  }
-  store3(a.pos_o, S, i, pos);
-  store3(a.vel_o, S, i, vel);
-  store3(a.omega_o, S, i, om);
-
-  // ---- PARTICLES_IN_CELL locate, fused (particles_in_cell.f:87-154)
-  if (MODE == kModeFused && P.coupled) {
-    const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
-    const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
-    const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
-    a.fcell[i] = (I && J && K) ? pack_ijk(I, J, K) : -1;
-  }
-}
-
-// Sort key of the deposition order: Morton code of the packed fluid cell
-// (I, J, K < 1024); particles without a fluid cell go last.
-__global__ __launch_bounds__(kBS) void k_dep_keys(int n, int bits, const uint8_t* state,
-                                                  const int* fcell, uint64_t* keys, int* vals) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  vals[i] = i;
-  const int pc = fcell[i];
-  if (state[i] == 0 || pc < 0) {  // all ones: above every cell's code
-    keys[i] = (1ull << bits) - 1;
-    return;
-  }
-  int I, J, K;
-  unpack_ijk(pc, I, J, K);
-  keys[i] = (uint64_t)(spread10(I) | (spread10(J) << 1) | (spread10(K) << 2));
 }
 
 // PARTICLES_IN_CELL deletion of a particle outside the fluid grid
@@ -760,8 +912,9 @@ __device__ __forceinline__ void remove_outside(const Removal& r, int S, int i) {
 }
 
 // COMP_MEAN_FIELDS0 deposition (comp_mean_fields0.f:186-210) of the current
-// positions.  The particles are visited in the deposition order built at
-// the last rebuild (Morton order of their fluid cells then), kDepPPT
+// positions (at set-up and for the unfused routines; the fused step kernel
+// deposits itself).  The particles are visited in slot order (list null,
+// the Morton order of their search cells) or in list order, kDepPPT
 // consecutive ones per thread (2 measured fastest at 1M spheres, against 1
 // and 4): a run of particles in the same fluid cell
 // is summed in registers, each run adds its 8 corner values into an int64
@@ -790,7 +943,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
 #pragma unroll
   for (int u = 0; u < kDepPPT; u++) {
     const int k = k0 + u;
-    pi[u] = k < P.n ? list[k] : -1;
+    pi[u] = k < P.n ? (list ? list[k] : k) : -1;
     pcell[u] = -1;
   }
   // every per-particle load is issued here, ahead of the workgroup barriers
@@ -928,8 +1081,8 @@ __global__ __launch_bounds__(kBS) void k_deposit_list(DevParams P, const int* li
 
 // COMP_MEAN_FIELDS1 deposition (comp_mean_fields1.f:42-64): PVOL times the
 // filter weight into the 27 filter cells (or PVOL into the own cell for the
-// NONE scheme), fixed-point int64 sums per cell.  Visits particles in the
-// deposition order; the workgroup's cells are summed in an LDS tile over
+// NONE scheme), fixed-point int64 sums per cell.  Visits particles in slot
+// order (list null) or list order; the workgroup's cells are summed in an LDS tile over
 // the bounding box of its targets (up to kDepCap cells), else directly.
 __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* list,
                                                         const double* pos, const double* pvol,
@@ -944,7 +1097,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
   const int k = blockIdx.x * kBS + threadIdx.x;
   int i = -1, pc = -1, ci = 0, cj = 0, ck = 0;
   if (k < P.n) {
-    i = list[k];
+    i = list ? list[k] : k;
     pc = fcell[i];
     if (state[i] == 0) {
       pc = -1;
@@ -2194,7 +2347,36 @@ struct FieldPtrs {
   const uint8_t* fluid;
   double *rop_s, *ep_g;
   int* flags;
+  // removals queued by the fused step (null: none): count, slot ids, and
+  // the other step's count, cleared for the next step
+  const int* rm_cnt;
+  int* rm_cnt_clear;
+  const int* rm_list;
+  uint8_t* state;
+  double* radius;
 };
+
+// Completes the removals the fused step queued (particles that left the
+// fluid grid): NONEXISTENT state and zero radius.  Grid-stride over the
+// queue; any thread may run it.
+__device__ __forceinline__ void apply_removals(const int* cnt, const int* list, uint8_t* state,
+                                               double* radius, int tid, int nthreads) {
+  const int m = *cnt;
+  for (int q = tid; q < m; q += nthreads) {
+    const int i = list[q];
+    state[i] = 0;
+    radius[i] = 0.0;
+  }
+}
+// The same as a kernel of its own, ahead of a neighbour rebuild (which
+// reorders the slots the queue names); one workgroup, which then empties
+// the queue.
+__global__ __launch_bounds__(kBS) void k_apply_removals(int* cnt, const int* list,
+                                                        uint8_t* state, double* radius) {
+  apply_removals(cnt, list, state, radius, threadIdx.x, kBS);
+  __syncthreads();
+  if (threadIdx.x == 0) *cnt = 0;
+}
 // Node sums -> ROP_S, EP_G per fluid cell (comp_mean_fields0.f:339-432,
 // calc_epg_des.f:133-145).  A workgroup takes kFieldCells consecutive
 // cells (J fastest), kFieldCPT per thread (two per thread measured the same
@@ -2217,6 +2399,11 @@ __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kma
   __shared__ double s_dens[4][kRun];
   const int ncell = ni * nj * nk, plane = nj * ni;
   const int c0 = blockIdx.x * kFieldCells;
+  if (FIXED && f.rm_cnt) {
+    apply_removals(f.rm_cnt, f.rm_list, f.state, f.radius, blockIdx.x * kBS + threadIdx.x,
+                   gridDim.x * kBS);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *f.rm_cnt_clear = 0;
+  }
   const double inv[kMaxPh] = {inv_scale0, inv_scale1, inv_scale2, inv_scale3};
   // this thread's cells c0 + tid + v kBS: corner mask (bit q: the corner is
   // inside the node range), volume, fluid flag

@@ -116,7 +116,10 @@ struct mfixdem_handle {
          *gd_fgds = nullptr;
   bool gas_drag_ready = false;
   unsigned long long* rops_fix[2]{};  // fixed-point node sums, double-buffered
-  int* dep_list = nullptr;            // deposition order (slots by fluid-cell Morton code)
+  // removals queued by the fused step: two counters (used by turns with
+  // the node-sum buffers) and the slot list
+  int* rm_cnt = nullptr;
+  int* rm_list = nullptr;
   int fb = 0;
   double dep_inv[MFIXDEM_MAX_PHASES]{};
   uint16_t* cell_facets = nullptr;
@@ -219,6 +222,9 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
   a.ugc = h->ugc;
   a.flags = h->flags;
+  a.dep_fix = h->rops_fix[h->fb];
+  a.rm_cnt = h->rm_cnt ? h->rm_cnt + h->fb : nullptr;
+  a.rm_list = h->rm_list;
   return a;
 }
 
@@ -252,6 +258,7 @@ int32_t reset_deposit(mfixdem_handle* h) {
   const size_t nb = sizeof(unsigned long long) * (size_t)h->S.ncell * h->S.p.nphases;
   for (int b = 0; b < 2; b++)
     if (h->rops_fix[b]) HIPOK(hipMemsetAsync(h->rops_fix[b], 0, nb, h->stream));
+  if (h->rm_cnt) HIPOK(hipMemsetAsync(h->rm_cnt, 0, 2 * sizeof(int), h->stream));
   return MFIXDEM_OK;
 }
 
@@ -259,6 +266,13 @@ int32_t reset_deposit(mfixdem_handle* h) {
 bool ref_deposition(const mfixdem_handle* h) {
   return (h->debug & MFIXDEM_DEBUG_REFERENCE_DEPOSITION) != 0 &&
          h->S.p.interp_scheme == MFIXDEM_INTERP_GARG_2012;
+}
+// The fused step kernel deposits (GARG_2012 node sums) for a coupled
+// single system; the filter schemes and the reference-order deposition
+// keep their own kernels.
+bool fused_dep(const mfixdem_handle* h) {
+  return h->nsys == 1 && h->S.p.coupled && h->rm_cnt &&
+         h->S.p.interp_scheme == MFIXDEM_INTERP_GARG_2012 && !ref_deposition(h);
 }
 
 bool timing(const mfixdem_handle* h) { return (h->debug & MFIXDEM_DEBUG_TIME_KERNELS) != 0; }
@@ -308,6 +322,7 @@ int32_t launch_step(mfixdem_handle* h) {
   DevParams P = h->P;
   P.first_pass = h->first_pass ? 1 : 0;
   P.store_forces = (h->debug & MFIXDEM_DEBUG_STORE_FORCES) ? 1 : 0;
+  P.fuse_dep = (MODE == kModeFused && fused_dep(h)) ? 1 : 0;
   StepPtrs a = step_ptrs(h);
   const dim3 grid(nblk(h->n)), blk(kBS);
   const bool mixed = h->S.p.precision == MFIXDEM_MIXED;
@@ -404,19 +419,6 @@ int32_t do_bin(mfixdem_handle* h) {
     hipLaunchKernelGGL((k_cell_ranges<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
                        h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
   HIPOK(hipGetLastError());
-  if (h->S.p.coupled) {
-    // deposition order for the steps up to the next rebuild (the key buffers
-    // are free again: the cell ranges above were their last reader)
-    // fluid cells are 1-based with ghosts: values up to max(ni, nj, nk)
-    const int dep_bits =
-        3 * morton_axis_bits(std::max(h->S.ni, std::max(h->S.nj, h->S.nk)) + 1);
-    hipLaunchKernelGGL(k_dep_keys, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, dep_bits,
-                       h->state[h->tw], h->fcell[h->tw], h->keys[0], h->vals[0]);
-    HIPOK(hipGetLastError());
-    tb = h->sort_tmp_bytes;
-    HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->dep_list, n,
-                     dep_bits, h->stream));
-  }
   h->binned = true;
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -602,6 +604,13 @@ FieldPtrs field_ptrs(mfixdem_handle* h) {
   f.fix_clear = h->rops_fix[1 - h->fb];
   f.vol = h->vol; f.vol_surr = h->vol_surr; f.ro_s = h->ro_s; f.fluid = h->fluid;
   f.rop_s = h->rop_s; f.ep_g = h->ep_g; f.flags = h->flags;
+  if (fused_dep(h)) {
+    f.rm_cnt = h->rm_cnt + h->fb;
+    f.rm_cnt_clear = h->rm_cnt + (1 - h->fb);
+    f.rm_list = h->rm_list;
+    f.state = h->state[h->tw];
+    f.radius = h->radius[h->tw];
+  }
   return f;
 }
 
@@ -662,14 +671,14 @@ int32_t deposit_current(mfixdem_handle* h) {
                    h->flags};
   if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
     hipLaunchKernelGGL(k_deposit_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
-                       h->dep_list, h->pos[h->cur], h->pvol[h->tw], h->phase[h->tw],
+                       nullptr, h->pos[h->cur], h->pvol[h->tw], h->phase[h->tw],
                        h->state[h->tw], h->fcell[h->tw], h->xe, h->yn, h->zt, h->fluid,
                        h->rops_fix[h->fb], rm);
     HIPOK(hipGetLastError());
     return do_cell_fields(h, true);
   }
   hipLaunchKernelGGL(k_deposit_list, dim3(nblk(((int64_t)h->n + kDepPPT - 1) / kDepPPT)),
-                     dim3(kBS), 0, h->stream, P, h->dep_list, h->pos[h->cur], h->pmass[h->tw],
+                     dim3(kBS), 0, h->stream, P, nullptr, h->pos[h->cur], h->pmass[h->tw],
                      h->phase[h->tw], h->state[h->tw], h->fcell[h->tw], h->fluid, h->xe, h->yn,
                      h->zt, h->rops_fix[h->fb], rm);
   HIPOK(hipGetLastError());
@@ -681,6 +690,7 @@ int32_t deposit_current(mfixdem_handle* h) {
 // every particle), deposition, ROP_S / EP_G
 int32_t gas_tail(mfixdem_handle* h) {
   if (ref_deposition(h)) return ref_mean_fields(h);  // deletion inside do_pic_lists
+  if (fused_dep(h)) return do_cell_fields(h, true);  // the step kernel deposited
   return deposit_current(h);
 }
 
@@ -725,6 +735,11 @@ int32_t one_step(mfixdem_handle* h, int32_t nn) {
     h->pcur = 1 - h->pcur;
     if (rebuild) {
       t_begin(h);
+      if (fused_dep(h)) {  // the rebuild reorders the slots the removal queue names
+        hipLaunchKernelGGL(k_apply_removals, dim3(1), dim3(kBS), 0, h->stream, h->rm_cnt + h->fb,
+                           h->rm_list, h->state[h->tw], h->radius[h->tw]);
+        HIPOK(hipGetLastError());
+      }
       CHECKRC(do_bin(h));
       CHECKRC(do_neighbour(h));
       t_end(h, kTimeRebuild);
@@ -845,7 +860,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
     f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]); f(h->pfl[b]);
     f(h->rops_fix[b]);
   }
-  f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->dep_list);
+  f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->rm_cnt); f(h->rm_list);
   f(h->xr_pack);
   f(h->fstart); f(h->fend);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
@@ -904,18 +919,16 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
       (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, facet_cells)) ||
       (rc = dalloc(&h->stage, 3 * n)) ||
       (rc = dalloc(&h->rops_fix[0], nc * h->S.p.nphases)) ||
-      (rc = dalloc(&h->rops_fix[1], nc * h->S.p.nphases)) || (rc = dalloc(&h->dep_list, n)))
+      (rc = dalloc(&h->rops_fix[1], nc * h->S.p.nphases)) || (rc = dalloc(&h->rm_cnt, 2)) ||
+      (rc = dalloc(&h->rm_list, n)))
     return fail(rc);
   for (int b = 0; b < 2; b++)
     if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * h->S.p.nphases) != hipSuccess)
       return fail(MFIXDEM_ERR_HIP);
-  size_t dep_tmp = 0;
+  if (hipMemset(h->rm_cnt, 0, 2 * sizeof(int)) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
   if (sort_pairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1], h->vals[0], h->vals[1],
-                 (int)n, h->key_bits, h->stream) != hipSuccess ||
-      sort_pairs(nullptr, dep_tmp, h->keys[0], h->keys[1], h->vals[0], h->dep_list, (int)n, 30,
-                 h->stream) != hipSuccess)
+                 (int)n, h->key_bits, h->stream) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
-  h->sort_tmp_bytes = std::max(h->sort_tmp_bytes, dep_tmp);
   if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
     return fail(MFIXDEM_ERR_OOM);
   if ((rc = upload_setup(h))) return fail(rc);
@@ -1233,7 +1246,7 @@ int32_t mfixdem_init(mfixdem_handle* h) {
     if (ref_deposition(h)) {
       CHECKRC(ref_mean_fields(h));
     } else {
-      CHECKRC(gas_tail(h));
+      CHECKRC(deposit_current(h));  // no step has deposited yet
     }
   }
   int fl[kFlagCount];

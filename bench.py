@@ -220,41 +220,36 @@ def ensemble_bench(nsys, steps, log):
     return out
 
 
-def serving_bench(n_req, log):
-    """Sweep-request serving, scaled to a bounded run: Poisson arrivals of parameter-sweep
-    requests (N log-uniform in [5k, 200k], steps uniform in [200, 2000],
-    mixed precision), admitted by HBM footprint within a 16 GiB budget.  The
-    arrival rate offers 70 % of the throughput measured for the same kind of
-    requests run one after another."""
+def serving_bench(n_req, log, rate=20.0, workers=8, budget=64 << 30):
+    """The sweep-request stream: Poisson arrivals at `rate` per second of
+    DEM runs with N log-uniform in [5k, 500k] and steps uniform in [200,
+    5000], mixed precision, KN/EN/MEW swept; each request is built when it
+    is admitted (lazily), admitted by HBM footprint within `budget`, and
+    served alongside the running ones.  The first n_req requests of the
+    seeded stream (2,000 for the full stream)."""
     from mfixdem import serving
-    reqs = serving.poisson_requests(n_req, rate=1.0, make_member=workloads.lattice_member,
-                                    nmin=5000, nmax=200_000, smin=200, smax=2000, seed=21)
+    reqs = serving.poisson_requests(n_req, rate=rate, make_member=workloads.lattice_member,
+                                    nmin=5000, nmax=500_000, smin=200, smax=5000, seed=21,
+                                    lazy=True)
     model = serving.FootprintModel(lambda n: workloads.lattice_member(n, 1e3, 0.9, 0.3, 1))
     for r in reqs:
-        r.footprint = model(r.params.max_pip)
-    # capacity probe: 6 requests back to back
-    t0 = time.perf_counter()
-    probe = serving.SweepServer(1 << 62, workers=1).serve(
-        [serving.Request(-i, 0.0, r.params, r.parts, r.nsteps, r.footprint)
-         for i, r in enumerate(reqs[:6])])
-    cap = sum(r.work for r in probe) / (time.perf_counter() - t0)
-    mean_work = float(np.mean([r.work for r in reqs]))
-    rate = 0.7 * cap / mean_work
-    rng = np.random.default_rng(22)
-    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    for r, a in zip(reqs, t):
-        r.arrival = float(a)
-    srv = serving.SweepServer(16 << 30, workers=8)
+        r.footprint = model(r.n)
+    work = sum(r.work for r in reqs)
+    span = reqs[-1].arrival
+    srv = serving.SweepServer(budget, workers=workers)
     done = srv.serve(reqs)
     rep = serving.report(done, srv.makespan)
-    rep.update({"arrival_rate_per_s": rate, "offered_particle_steps_per_s": rate * mean_work,
-                "sequential_capacity_particle_steps_per_s": cap,
-                "hbm_budget_bytes": 16 << 30, "max_concurrent": srv.max_running,
-                "max_bytes_in_use": srv.max_in_use,
-                "workload": "sweep requests: N log-uniform [5k, 200k], steps U[200, 2000], "
-                            "mixed precision, KN/EN/MEW swept, Poisson arrivals"})
-    log(f"serving: p50 {rep['p50_latency_s']:.3f} s, p99 {rep['p99_latency_s']:.3f} s, "
-        f"{rep['particle_steps_per_s']:.3e} particle-steps/s at rate {rate:.2f}/s")
+    rep.update({"arrival_rate_per_s": rate, "offered_particle_steps_per_s": work / span,
+                "last_arrival_s": span, "drain_after_last_arrival_s": srv.makespan - span,
+                "hbm_budget_bytes": budget, "workers": workers,
+                "max_concurrent": srv.max_running, "max_bytes_in_use": srv.max_in_use,
+                "workload": f"{n_req} sweep requests: N log-uniform [5k, 500k], steps U[200, "
+                            f"5000], mixed precision, KN/EN/MEW swept, Poisson arrivals at "
+                            f"{rate}/s, built on admission"})
+    log(f"serving: {n_req} requests, p50 {rep['p50_latency_s']:.3f} s, p99 "
+        f"{rep['p99_latency_s']:.3f} s, served {rep['particle_steps_per_s']:.3e} of offered "
+        f"{rep['offered_particle_steps_per_s']:.3e} particle-steps/s, max {srv.max_running} "
+        f"running")
     return rep
 
 
@@ -356,7 +351,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--ens-systems", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one DEM system per GPU on a single GPU; "

@@ -28,18 +28,24 @@ from .system import DemSystem
 class Request:
     rid: int
     arrival: float            # seconds after the start of the run
-    params: abi.Params
-    parts: object             # workloads.Particles
+    params: abi.Params        # None until built (lazy requests)
+    parts: object             # workloads.Particles; None until built
     nsteps: int
     footprint: int = 0        # estimated device bytes
     admitted: float = float("nan")
     finished: float = float("nan")
     error: str = ""
     result: object = None     # final (pos, vel, omega, state) when kept
+    n: int = 0                # particles (params.max_pip once built)
+    make: object = None       # () -> (params, parts), called by the worker that runs it
+
+    def __post_init__(self):
+        if self.params is not None and not self.n:
+            self.n = int(self.params.max_pip)
 
     @property
     def work(self) -> float:
-        return float(self.params.max_pip) * self.nsteps
+        return float(self.n) * self.nsteps
 
 
 class FootprintModel:
@@ -80,6 +86,8 @@ class SweepServer:
     # ------------------------------------------------------------ worker side
     def _execute(self, req: Request, t0: float):
         try:
+            if req.params is None:   # built on admission: the queue holds no particle data
+                req.params, req.parts = req.make()
             s = DemSystem(req.params)
             try:
                 # several handles are driven from different threads: no
@@ -98,6 +106,8 @@ class SweepServer:
                     req.result = s.get_particles()
             finally:
                 s.close()
+            if not self.keep_results:
+                req.params, req.parts = None, None   # a long stream keeps no particle data
         except Exception as e:  # a failed request is reported, the server goes on
             req.error = str(e)
         req.finished = time.perf_counter() - t0
@@ -165,10 +175,12 @@ def report(reqs, makespan):
     }
 
 
-def poisson_requests(n_req, rate, make_member, nmin, nmax, smin, smax, seed=21):
+def poisson_requests(n_req, rate, make_member, nmin, nmax, smin, smax, seed=21, lazy=False):
     """n_req requests with Poisson arrivals at `rate` per second; N
     log-uniform in [nmin, nmax], steps uniform in [smin, smax], KN
-    log-uniform in [1e2, 1e4], EN ~ U[0.7, 0.95], MEW ~ U[0.1, 0.5]."""
+    log-uniform in [1e2, 1e4], EN ~ U[0.7, 0.95], MEW ~ U[0.1, 0.5].
+    lazy: each request's system is built by the worker that runs it (a
+    long stream holds only its parameters until then)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n_req))
     out = []
@@ -177,6 +189,12 @@ def poisson_requests(n_req, rate, make_member, nmin, nmax, smin, smax, seed=21):
         kn = float(np.exp(rng.uniform(np.log(1e2), np.log(1e4))))
         en = float(rng.uniform(0.7, 0.95))
         mew = float(rng.uniform(0.1, 0.5))
-        p, parts = make_member(n, kn, en, mew, seed * 100000 + i)
-        out.append(Request(i, float(t[i]), p, parts, int(rng.integers(smin, smax + 1))))
+        steps = int(rng.integers(smin, smax + 1))
+        args = (n, kn, en, mew, seed * 100000 + i)
+        if lazy:
+            out.append(Request(i, float(t[i]), None, None, steps, n=n,
+                               make=lambda a=args: make_member(*a)))
+        else:
+            p, parts = make_member(*args)
+            out.append(Request(i, float(t[i]), p, parts, steps))
     return out

@@ -138,7 +138,8 @@ typedef struct mfixdem_params {
   double des_epg_clip;        /* DES_EPG_CLIP (UNDEFINED)                   */
   double des_interp_width;    /* DES_INTERP_WIDTH (UNDEFINED; DPVM needs it) */
   int32_t des_neighbor_search;/* MFIXDEM_NSEARCH_* (4)                      */
-  int32_t reserved2;
+  int32_t des_explicitly_coupled; /* DES_EXPLICITLY_COUPLED (0): drag formed once
+                                  per fluid step, mfixdem_des_time_init     */
 } mfixdem_params;
 
 /* Quantities derived once at set-up (check_solids_dem.f:519-1122,
@@ -258,6 +259,17 @@ int32_t mfixdem_comp_mean_fields(mfixdem_handle* h); /* comp_mean_fields.f:22 */
    cells, added with atomics as the reference's OpenMP loop does); DRAG_AM
    is 0. */
 int32_t mfixdem_drag_gs_gas(mfixdem_handle* h);
+/* The coupled part of DES_TIME_INIT (des_time_march.f:155-161, 271) with
+   DES_EXPLICITLY_COUPLED (filter schemes; GARG_2012 with explicit coupling
+   is MFIXDEM_ERR_UNSUPPORTED at create): DRAG_GS_DES1's explicit branch
+   DRAG_FC = F_GP*(VELFP - v_p) (drag_gs_des1.f:126-137), with the F_GP the
+   last mfixdem_drag_gs_gas stored and VELFP from the current gas field,
+   then CALC_PG_GRAD's DRAG_FC += P_FORCE*PVOL (calc_pg_grad.f:74-105);
+   DRAG_FC = F_GP = 0 for a particle outside a fluid cell.  The steps that
+   follow add DRAG_FC to FC (calc_drag_des.f:34-43) and neither rebin nor
+   deposit (des_time_march.f:419-425).  Call it once per fluid step after
+   mfixdem_set_gas; a no-op (MFIXDEM_OK) without explicit coupling. */
+int32_t mfixdem_des_time_init(mfixdem_handle* h);
 /* The last mfixdem_drag_gs_gas results (NOT_READY before the first):
    f_gds[ncell], drag_am[ncell] (node (i,j,k) at the index of cell (i,j,k)),
    drag_bm[3*ncell] as DRAG_BM(IJK,1:3) (component planes); any may be

@@ -36,6 +36,7 @@ struct DevParams {
   int first_pass;   // Adams-Bashforth first step
   int store_forces; // debug: keep FC/TOW
   int fuse_dep;     // fused step: GARG_2012 deposition in the step kernel
+  int explicit_cpl; // DES_EXPLICITLY_COUPLED: FC += DRAG_FC, no rebinning
   double dt;
   double grav[3];
   double kn, kt, kn_w, kt_w, mew, mew_w;

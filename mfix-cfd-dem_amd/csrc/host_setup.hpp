@@ -440,6 +440,10 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
     return MFIXDEM_ERR_ARG;
   if (p.precision != MFIXDEM_FP64 && p.precision != MFIXDEM_MIXED) return MFIXDEM_ERR_ARG;
   if (p.coupled && p.model_b) return MFIXDEM_ERR_UNSUPPORTED;
+  // explicit coupling: DRAG_GS_DES1's F_GP path, for the filter schemes
+  if (p.des_explicitly_coupled &&
+      (!p.coupled || p.interp_scheme == MFIXDEM_INTERP_GARG_2012))
+    return MFIXDEM_ERR_UNSUPPORTED;
   if (p.des_neighbor_search != MFIXDEM_NSEARCH_GRID &&
       p.des_neighbor_search != MFIXDEM_NSEARCH_NSQUARE)
     return MFIXDEM_ERR_ARG;

@@ -53,6 +53,10 @@ struct StepPtrs {
   unsigned long long* dep_fix;
   int* rm_cnt;
   int* rm_list;
+  // explicit coupling, in global-id order: F_GP of the last gas-side drag
+  // and DRAG_FC [3*gid + d] of the last DES_TIME_INIT (null otherwise)
+  double* f_gp;
+  double* drag_fc;
 };
 
 enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
@@ -235,6 +239,11 @@ template <bool MIXED>
 __device__ __forceinline__ void add_drag_des1(const DevParams& P, const StepPtrs& a, int i,
                                               const double pos[3], const double vel[3],
                                               double fc[3]) {
+  if (P.explicit_cpl) {  // CALC_DRAG_DES, calc_drag_des.f:34-43
+    const double* d = a.drag_fc + 3 * (size_t)a.gid[i];
+    for (int c = 0; c < 3; c++) fc[c] = fc[c] + d[c];
+    return;
+  }
   const int pc = a.fcell[i];
   if (pc < 0) return;
   int ci, cj, ck;
@@ -761,7 +770,7 @@ __device__ __forceinline__ void step_tail(const DevParams& P0, const DevParams& 
 
   // ---- PARTICLES_IN_CELL locate, fused (particles_in_cell.f:87-154)
   int pc = 0;
-  if (MODE == kModeFused && P.coupled) {
+  if (MODE == kModeFused && P.coupled && !P.explicit_cpl) {
     const int I = locate_cell(a.xe, P.ni, 1.0 / (a.xe[1] - a.xe[0]), pos[0]);
     const int J = locate_cell(a.yn, P.nj, 1.0 / (a.yn[1] - a.yn[0]), pos[1]);
     const int K = locate_cell(a.zt, P.nk, 1.0 / (a.zt[1] - a.zt[0]), pos[2]);
@@ -2280,6 +2289,7 @@ __global__ __launch_bounds__(kBS) void k_gas_filter(DevParams P, StepPtrs a, con
   const double vrel = sqrt(dot3(vs, vs));
   const double f = drag_fgp_fp64(P.drag_type, lepg, P.mu_g0, P.ro_g0, vrel, 2.0 * a.radius[i],
                                  a.pvol[i]);
+  if (a.f_gp) a.f_gp[a.gid[i]] = f;  // F_GP(NP) for explicit coupling
   const double lbm[3] = {f * vel[0], f * vel[1], f * vel[2]};
   auto add = [&](int c, double wt) {
     for (int d = 0; d < 3; d++) atomicAdd(&drag_bm[(size_t)d * ncell + c], lbm[d] * wt);
@@ -2290,6 +2300,50 @@ __global__ __launch_bounds__(kBS) void k_gas_filter(DevParams P, StepPtrs a, con
                [&](int c, double w) { add(c, w / vol[c]); });
   else
     add(c0, 1.0 / vol[c0]);
+}
+
+// DES_TIME_INIT with explicit coupling, per normal particle in a fluid cell:
+// DRAG_FC = F_GP*(VELFP - v) (drag_gs_des1.f:126-137), VELFP from the
+// current cell-centre gas velocity with the particle's filter weights (or
+// its cell's value), then DRAG_FC += P_FORCE*PVOL with P_FORCE interpolated
+// the same way (calc_pg_grad.f:74-105); DRAG_FC = F_GP = 0 outside a fluid
+// cell.  Results in global-id order.
+__global__ __launch_bounds__(kBS) void k_drag_fc(DevParams P, StepPtrs a) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= P.n || a.state[i] != 1) return;  // IS_NORMAL
+  const int pc = a.fcell[i];
+  if (pc < 0) return;
+  int ci, cj, ck;
+  unpack_ijk(pc, ci, cj, ck);
+  const int c0 = lin_ijk(ci, cj, ck, P.ni, P.nj);
+  const int g = a.gid[i];
+  double* dfc = a.drag_fc + 3 * (size_t)g;
+  if (!a.fluid[c0]) {
+    for (int d = 0; d < 3; d++) dfc[d] = 0.0;
+    a.f_gp[g] = 0.0;
+    return;
+  }
+  const int S = P.stride;
+  double pos[3], vel[3];
+  load3(a.pos, S, i, pos);
+  load3(a.vel, S, i, vel);
+  double velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
+  if (P.interp_on && filter_scheme(P.interp)) {
+    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+      for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
+    });
+    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+      for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + a.p_force[3 * c + d] * w;
+    });
+  } else {
+    for (int d = 0; d < 3; d++) velfp[d] = a.ugc[3 * c0 + d];
+    for (int d = 0; d < 3; d++) lpf[d] = a.p_force[3 * c0 + d];
+  }
+  const double f = a.f_gp[g], pv = a.pvol[i];
+  double out[3];
+  for (int d = 0; d < 3; d++) out[d] = f * (velfp[d] - vel[d]);
+  for (int d = 0; d < 3; d++) out[d] = out[d] + lpf[d] * pv;
+  for (int d = 0; d < 3; d++) dfc[d] = out[d];
 }
 
 // k_gas_fgds: F_GDS of a fluid cell, 1/8 of the DRAG_AM of its 8 corner

@@ -120,6 +120,8 @@ struct mfixdem_handle {
   // the node-sum buffers) and the slot list
   int* rm_cnt = nullptr;
   int* rm_list = nullptr;
+  // explicit coupling (global-id order): F_GP of the gas side, DRAG_FC
+  double *f_gp = nullptr, *drag_fc = nullptr;
   int fb = 0;
   double dep_inv[MFIXDEM_MAX_PHASES]{};
   uint16_t* cell_facets = nullptr;
@@ -225,6 +227,8 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.dep_fix = h->rops_fix[h->fb];
   a.rm_cnt = h->rm_cnt ? h->rm_cnt + h->fb : nullptr;
   a.rm_list = h->rm_list;
+  a.f_gp = h->f_gp;
+  a.drag_fc = h->drag_fc;
   return a;
 }
 
@@ -271,7 +275,7 @@ bool ref_deposition(const mfixdem_handle* h) {
 // single system; the filter schemes and the reference-order deposition
 // keep their own kernels.
 bool fused_dep(const mfixdem_handle* h) {
-  return h->nsys == 1 && h->S.p.coupled && h->rm_cnt &&
+  return h->nsys == 1 && h->S.p.coupled && h->rm_cnt && !h->S.p.des_explicitly_coupled &&
          h->S.p.interp_scheme == MFIXDEM_INTERP_GARG_2012 && !ref_deposition(h);
 }
 
@@ -694,12 +698,18 @@ int32_t gas_tail(mfixdem_handle* h) {
   return deposit_current(h);
 }
 
+// PARTICLES_IN_CELL, weights and mean fields after every step
+// (des_time_march.f:419-425): coupled runs, unless explicitly coupled
+bool rebin_each_step(const mfixdem_handle* h) {
+  return h->S.p.coupled && !h->S.p.des_explicitly_coupled;
+}
+
 // the part of a step that is identical between rebuilds (graph-capturable)
 int32_t plain_step_body(mfixdem_handle* h) {
   CHECKRC(launch_step<kModeFused>(h));
   h->cur = 1 - h->cur;
   h->pcur = 1 - h->pcur;
-  if (h->S.p.coupled) CHECKRC(gas_tail(h));
+  if (rebin_each_step(h)) CHECKRC(gas_tail(h));
   return MFIXDEM_OK;
 }
 
@@ -744,7 +754,7 @@ int32_t one_step(mfixdem_handle* h, int32_t nn) {
       CHECKRC(do_neighbour(h));
       t_end(h, kTimeRebuild);
     }
-    if (h->S.p.coupled) {
+    if (rebin_each_step(h)) {
       t_begin(h);
       CHECKRC(gas_tail(h));
       t_end(h, kTimeGas);
@@ -860,7 +870,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
     f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]); f(h->pfl[b]);
     f(h->rops_fix[b]);
   }
-  f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->rm_cnt); f(h->rm_list);
+  f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->rm_cnt); f(h->rm_list); f(h->f_gp); f(h->drag_fc);
   f(h->xr_pack);
   f(h->fstart); f(h->fend);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
@@ -926,6 +936,12 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
     if (hipMemset(h->rops_fix[b], 0, sizeof(unsigned long long) * nc * h->S.p.nphases) != hipSuccess)
       return fail(MFIXDEM_ERR_HIP);
   if (hipMemset(h->rm_cnt, 0, 2 * sizeof(int)) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
+  if (h->S.p.des_explicitly_coupled) {
+    if ((rc = dalloc(&h->f_gp, n)) || (rc = dalloc(&h->drag_fc, 3 * n))) return fail(rc);
+    if (hipMemset(h->f_gp, 0, sizeof(double) * n) != hipSuccess ||
+        hipMemset(h->drag_fc, 0, sizeof(double) * 3 * n) != hipSuccess)
+      return fail(MFIXDEM_ERR_HIP);
+  }
   if (sort_pairs(nullptr, h->sort_tmp_bytes, h->keys[0], h->keys[1], h->vals[0], h->vals[1],
                  (int)n, h->key_bits, h->stream) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
@@ -1134,6 +1150,10 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
   HIPOK(hipMemsetAsync(h->fc_id, 0, sizeof(double) * 3 * S, h->stream));
   HIPOK(hipMemsetAsync(h->tow_id, 0, sizeof(double) * 3 * S, h->stream));
+  if (h->f_gp) {  // des_init_arrays.f:143
+    HIPOK(hipMemsetAsync(h->f_gp, 0, sizeof(double) * N, h->stream));
+    HIPOK(hipMemsetAsync(h->drag_fc, 0, sizeof(double) * 3 * N, h->stream));
+  }
   h->st = mfixdem_status{};
   h->n_pairs = 0;
   h->inited = false;
@@ -1403,6 +1423,16 @@ int32_t mfixdem_drag_gs_gas(mfixdem_handle* h) {
                      h->S.nk, h->fluid, h->gd_am, h->gd_fgds);
   HIPOK(hipGetLastError());
   h->gas_drag_ready = true;
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_des_time_init(mfixdem_handle* h) {
+  if (!h) return MFIXDEM_ERR_ARG;
+  if (!h->S.p.des_explicitly_coupled) return MFIXDEM_OK;
+  if (!h->inited) return MFIXDEM_ERR_NOT_READY;
+  if (h->n > 0)
+    hipLaunchKernelGGL(k_drag_fc, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h));
+  HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
 

@@ -18,6 +18,7 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
   P.hertz = p.coll_model == MFIXDEM_HERTZ;
   P.ab2 = p.integrator == MFIXDEM_ADAMS_BASHFORTH;
   P.coupled = p.coupled;
+  P.explicit_cpl = p.des_explicitly_coupled;
   P.drag_type = p.drag_type;
   P.nph = p.nphases;
   P.nf = (int)S.facets.size();

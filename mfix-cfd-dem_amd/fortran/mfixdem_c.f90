@@ -44,7 +44,7 @@ module mfixdem_c
      real(c_double) :: d_p0(MFIXDEM_MAX_PHASES), ro_s0(MFIXDEM_MAX_PHASES)
      real(c_double) :: max_radius, ro_g0, mu_g0, des_epg_clip
      real(c_double) :: des_interp_width
-     integer(c_int32_t) :: des_neighbor_search, reserved2
+     integer(c_int32_t) :: des_neighbor_search, des_explicitly_coupled
   end type mfixdem_params
 
   type, bind(C) :: mfixdem_status
@@ -158,6 +158,11 @@ module mfixdem_c
      end function
      ! DRAG_GS_GAS0 (drag_gs_des0.f:235): F_GDS, DRAG_AM, DRAG_BM for the gas solver
      integer(c_int32_t) function mfixdem_drag_gs_gas(h) bind(C, name="mfixdem_drag_gs_gas")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function
+     ! DES_TIME_INIT's coupled part with DES_EXPLICITLY_COUPLED (DRAG_FC)
+     integer(c_int32_t) function mfixdem_des_time_init(h) bind(C, name="mfixdem_des_time_init")
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function

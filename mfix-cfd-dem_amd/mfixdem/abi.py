@@ -61,7 +61,7 @@ class Params(C.Structure):
         ("ew_young", _D), ("vw_poisson", _D),
         ("d_p0", _D * MAX_PHASES), ("ro_s0", _D * MAX_PHASES),
         ("max_radius", _D), ("ro_g0", _D), ("mu_g0", _D), ("des_epg_clip", _D),
-        ("des_interp_width", _D), ("des_neighbor_search", _I), ("reserved2", _I),
+        ("des_interp_width", _D), ("des_neighbor_search", _I), ("des_explicitly_coupled", _I),
     ]
 
 
@@ -206,6 +206,7 @@ def _declare(lib):
         "mfixdem_set_pairs": ([_P, _IP, _IP, _DP, C.c_int64], _I),
         "mfixdem_get_gas_drag": ([_P, _DP, _DP, _DP], _I),
         "mfixdem_drag_gs_gas_noalloc": ([_P, _P, _P, _P], _I),
+        "mfixdem_des_time_init": ([_P], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -123,6 +123,10 @@ class DemSystem:
                   "get_gas_drag")
         return f, am, bm.reshape(3, self.ncell)
 
+    def des_time_init(self):
+        """DES_TIME_INIT's coupled part (explicit coupling: DRAG_FC)."""
+        abi.check(self.lib.mfixdem_des_time_init(self.h), "des_time_init")
+
     def calc_interp_weights(self):
         abi.check(self.lib.mfixdem_calc_interp_weights(self.h), "calc_interp_weights")
 

@@ -96,6 +96,8 @@ struct orc_sys {
   int* filter_cell;                   /* FILTER_CELL(27, MAX_PIP) -> [lc*ld + p], 0-based */
   double* filter_weight;              /* FILTER_WEIGHT(27, MAX_PIP) */
   double* solvolinc;                  /* SOLVOLINC(ijk, m) -> [m*ncell + ijk] */
+  double* f_gp;                       /* F_GP(MAX_PIP): the gas side's drag coefficient */
+  double* drag_fc;                    /* DRAG_FC(MAX_PIP,3) -> [d*ld + p] */
   int* pinc;
   int* pic_off;
   int* pic_list;
@@ -654,6 +656,9 @@ orc_sys* orc_create(const mfixdem_params* p) {
   if (p->des_neighbor_search != MFIXDEM_NSEARCH_GRID &&
       p->des_neighbor_search != MFIXDEM_NSEARCH_NSQUARE)
     return NULL;
+  /* explicit coupling: DRAG_GS_DES1's F_GP path, filter schemes only */
+  if (p->des_explicitly_coupled && (!p->coupled || p->interp_scheme == MFIXDEM_INTERP_GARG_2012))
+    return NULL;
   orc_sys* s = calloc(1, sizeof(orc_sys));
   s->p = *p;
   if (p->coll_model == MFIXDEM_HERTZ) derive_hertz(s);
@@ -703,6 +708,7 @@ orc_sys* orc_create(const mfixdem_params* p) {
   A(s->ugc, 3 * s->ncell, double); A(s->filter_cell, FILTER_N * ld, int);
   A(s->filter_weight, FILTER_N * ld, double);
   A(s->solvolinc, (size_t)s->ncell * p->nphases, double);
+  A(s->f_gp, ld, double); A(s->drag_fc, 3 * ld, double);
 #undef A
   for (int64_t q = 0; q < COLLISION_ARRAY_MAX * ld; q++) s->wall_fid[q] = -1;
   for (int c = 0; c < s->ncell; c++) s->ep_g[c] = 1.0;
@@ -721,7 +727,7 @@ void orc_destroy(orc_sys* s) {
                   s->neighbors, s->neighbors_old, s->pft, s->pft_old, s->wall_fid,
                   s->wall_pft, s->u_g, s->v_g, s->w_g, s->p_force, s->ep_g, s->rop_s,
                   s->rops_node, s->pinc, s->pic_off, s->pic_list, s->ugc, s->filter_cell,
-                  s->filter_weight, s->solvolinc};
+                  s->filter_weight, s->solvolinc, s->f_gp, s->drag_fc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(s);
 }
@@ -756,6 +762,8 @@ int orc_set_particles(orc_sys* s, int64_t n, const double* pos, const double* ve
     s->state[q] = NORMAL_PARTICLE;
   }
   for (int64_t q = n; q < s->max_pip; q++) s->state[q] = NONEXISTENT;
+  memset(s->f_gp, 0, sizeof(double) * (size_t)L);        /* des_init_arrays.f:143 */
+  memset(s->drag_fc, 0, sizeof(double) * 3 * (size_t)L);
   s->pip = n;
   s->first_pass = 1;
   s->inited = 0;
@@ -1731,6 +1739,14 @@ static void trilinear_weights(const double gst[2][2][2][3], const double pos[3],
 static void drag_des1(orc_sys* s) {
   int64_t L = s->ld;
   const mfixdem_params* p = &s->p;
+  if (p->des_explicitly_coupled) {
+    /* CALC_DRAG_DES, calc_drag_des.f:34-43: the drag formed at DES_TIME_INIT */
+    for (int64_t np = 0; np < s->max_pip; np++) {
+      if (s->state[np] != NORMAL_PARTICLE) continue;
+      for (int d = 0; d < 3; d++) s->fc[d * L + np] = s->fc[d * L + np] + s->drag_fc[d * L + np];
+    }
+    return;
+  }
   int nf = filter_size(s);
   for (int64_t np = 0; np < s->max_pip; np++) {
     if (s->state[np] != NORMAL_PARTICLE) continue;
@@ -2174,6 +2190,7 @@ static int drag_gs_gas1(orc_sys* s, double* f_gds, double* drag_bm) {
     double vrel = sqrt(dot3(vslp, vslp));
     double lforce = orc_drag_fgp(p->drag_type, lepg, p->mu_g0, p->ro_g0, vrel,
                                  2.0 * s->radius[np], s->pvol[np]);
+    s->f_gp[np] = lforce; /* F_GP(NP), set by DES_DRAG_GP */
     double lbm[3] = {lforce * vel[0], lforce * vel[1], lforce * vel[2]};
     int nl = p->des_interp_on ? nf : 1;
     for (int lc = 0; lc < nl; lc++) {
@@ -2195,6 +2212,51 @@ int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm)
     return drag_gs_gas0(s, f_gds, drag_am, drag_bm);
   memset(drag_am, 0, sizeof(double) * (size_t)s->ncell);
   return drag_gs_gas1(s, f_gds, drag_bm);
+}
+
+/* DES_TIME_INIT's coupled part with DES_EXPLICITLY_COUPLED
+   (des_time_march.f:155-161, 271): DRAG_GS_DES1's explicit branch
+   (drag_gs_des1.f:96-137) DRAG_FC = F_GP*(VELFP - DES_VEL_NEW) with the
+   F_GP of the last DRAG_GS_GAS1 and VELFP from the current gas field (the
+   filter weights of the last CALC_INTERP_WEIGHTS; the particles have not
+   moved since), then CALC_PG_GRAD (calc_pg_grad.f:74-105) adds
+   P_FORCE*PVOL, P_FORCE interpolated with the same weights. */
+int orc_des_time_init(orc_sys* s) {
+  if (!s->inited) return MFIXDEM_ERR_NOT_READY;
+  const mfixdem_params* p = &s->p;
+  if (!p->coupled || !p->des_explicitly_coupled) return 0;
+  int64_t L = s->ld;
+  int nf = filter_size(s);
+  for (int64_t np = 0; np < s->max_pip; np++) {
+    if (s->state[np] != NORMAL_PARTICLE) continue;
+    int ijk0 = s->pijk[3 * L + np] - 1;
+    if (!s->fluid[ijk0]) {
+      for (int d = 0; d < 3; d++) s->drag_fc[d * L + np] = 0.0;
+      s->f_gp[np] = 0.0;
+      continue;
+    }
+    double velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
+    if (p->des_interp_on) {
+      for (int lc = 0; lc < nf; lc++) {
+        int ijk = s->filter_cell[lc * L + np];
+        double w = s->filter_weight[lc * L + np];
+        for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + s->ugc[3 * ijk + d] * w;
+      }
+      for (int lc = 0; lc < nf; lc++) {
+        int ijk = s->filter_cell[lc * L + np];
+        double w = s->filter_weight[lc * L + np];
+        for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + s->p_force[3 * ijk + d] * w;
+      }
+    } else {
+      for (int d = 0; d < 3; d++) velfp[d] = s->ugc[3 * ijk0 + d];
+      for (int d = 0; d < 3; d++) lpf[d] = s->p_force[3 * ijk0 + d];
+    }
+    double vel[3] = {s->vel[np], s->vel[L + np], s->vel[2 * L + np]};
+    for (int d = 0; d < 3; d++) s->drag_fc[d * L + np] = s->f_gp[np] * (velfp[d] - vel[d]);
+    for (int d = 0; d < 3; d++)
+      s->drag_fc[d * L + np] = s->drag_fc[d * L + np] + lpf[d] * s->pvol[np];
+  }
+  return 0;
 }
 
 /* ------------------------------------------------------- driver */
@@ -2240,7 +2302,9 @@ int orc_time_step(orc_sys* s, int32_t nn) {
     rc = orc_neighbour(s);
     if (rc) return rc;
   }
-  if (s->p.coupled) {
+  /* explicit coupling rebins and deposits once per fluid step, in
+     CALC_DES_2FLUID, not here (des_time_march.f:419-425) */
+  if (s->p.coupled && !s->p.des_explicitly_coupled) {
     orc_particles_in_cell(s);
     orc_calc_interp_weights(s);
     rc = orc_comp_mean_fields(s);

@@ -61,6 +61,7 @@ int orc_comp_mean_fields(orc_sys* s);
    cells) for GARG_2012, DRAG_GS_GAS1 (drag_gs_des1.f:179-301; F_GDS and
    DRAG_BM at the cells, DRAG_AM 0) for the other schemes. */
 int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm);
+int orc_des_time_init(orc_sys* s);
 
 /* inspection */
 int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);

@@ -63,6 +63,10 @@ def load_oracle(omp: bool = False):
         "orc_set_gas_pressure": ([P, D], C.c_int),
         "orc_get_p_force": ([P, D], C.c_int),
         "orc_drag_gs_gas": ([P, D, D, D], C.c_int),
+        "orc_des_time_init": ([P], C.c_int),
+        "orc_particles_in_cell": ([P], C.c_int),
+        "orc_calc_interp_weights": ([P], C.c_int),
+        "orc_comp_mean_fields": ([P], C.c_int),
     }
     for n, (a, r) in sig.items():
         f = getattr(lib, n)
@@ -205,6 +209,18 @@ class OracleSystem:
         self._chk(self.lib.orc_drag_gs_gas(self.h, abi.dptr(f), abi.dptr(am), abi.dptr(bm)),
                   "drag_gs_gas")
         return f, am, bm.reshape(3, self.ncell)
+
+    def particles_in_cell(self):
+        self._chk(self.lib.orc_particles_in_cell(self.h), "particles_in_cell")
+
+    def calc_interp_weights(self):
+        self._chk(self.lib.orc_calc_interp_weights(self.h), "calc_interp_weights")
+
+    def comp_mean_fields(self):
+        self._chk(self.lib.orc_comp_mean_fields(self.h), "comp_mean_fields")
+
+    def des_time_init(self):
+        self._chk(self.lib.orc_des_time_init(self.h), "des_time_init")
 
     def set_gas_pressure(self, p_g):
         self._chk(self.lib.orc_set_gas_pressure(self.h, abi.dptr(np.ascontiguousarray(p_g))),

@@ -189,11 +189,19 @@ int32_t mfixdem_get_derived(const mfixdem_handle* h, mfixdem_derived* out);
 /* --- state in / out (the only host<->device traffic) --------------------- */
 /* Upload n particles (reference SoA layout, leading dimension ld).  phase is
    1-based PIJK(:,5).  Per-particle PVOL/PMASS/OMOI follow
-   make_arrays_des.f:288-291.  All particles start NORMAL_PARTICLE. */
+   make_arrays_des.f:288-291.  All particles start NORMAL_PARTICLE;
+   mfixdem_set_particle_state makes holes. */
 int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos,
                               const double* vel, const double* omega,
                               const double* radius, const double* ro_sol,
                               const int32_t* phase, int64_t ld);
+/* PARTICLE_STATE of slots 0..n-1 (global ids 1..n), after
+   mfixdem_set_particles and before mfixdem_init: 0 makes the slot a hole
+   (NONEXISTENT, discretelement_mod.f:68-78), skipped by every routine as
+   the reference skips it (IS_NONEXISTENT, calc_force_dem.f:185,211), 1
+   leaves it NORMAL.  Entering / exiting particles (2, 3) are
+   MFIXDEM_ERR_UNSUPPORTED; called after mfixdem_init it is MFIXDEM_ERR_ARG. */
+int32_t mfixdem_set_particle_state(mfixdem_handle* h, const int8_t* state, int64_t n);
 /* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal). */
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
                               double* omega, int8_t* state, int64_t ld);

@@ -2705,6 +2705,23 @@ __global__ void k_scatter_gid_i(int n, const int* gid, const int* src, int add, 
   if (i >= n) return;
   dst[gid[i]] = src[i] + add;
 }
+// Holes at set-up: slots q < n whose PARTICLE_STATE st[q] is 0 become
+// nonexistent with zero radius and state (slot order = global-id order
+// before the first sort).
+__global__ __launch_bounds__(kBS) void k_set_holes(int n, const int8_t* st, uint8_t* state,
+                                                   double* radius, int S, double* pos,
+                                                   double* vel, double* omega) {
+  const int q = blockIdx.x * kBS + threadIdx.x;
+  if (q >= n || st[q] != 0) return;
+  state[q] = 0;
+  radius[q] = 0.0;
+  for (int d = 0; d < 3; d++) {
+    pos[d * S + q] = 0.0;
+    vel[d * S + q] = 0.0;
+    omega[d * S + q] = 0.0;
+  }
+}
+
 __global__ void k_scatter_gid_u8(int n, const int* gid, const uint8_t* src, int8_t* dst) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;

@@ -1162,6 +1162,22 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   return MFIXDEM_OK;
 }
 
+int32_t mfixdem_set_particle_state(mfixdem_handle* h, const int8_t* state, int64_t n) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
+  if (!h || !state || n < 0 || n > h->n || h->inited) return MFIXDEM_ERR_ARG;
+  for (int64_t q = 0; q < n; q++)
+    if (state[q] != 0 && state[q] != 1) return MFIXDEM_ERR_UNSUPPORTED;
+  if (n == 0) return MFIXDEM_OK;
+  HIPOK(sync_stream(h));
+  HIPOK(hipMemcpy(h->stage, state, (size_t)n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_set_holes, dim3(nblk(n)), dim3(kBS), 0, h->stream, (int)n,
+                     (const int8_t*)h->stage, h->state[0], h->radius[0], h->stride, h->pos[0],
+                     h->vel[0], h->omega[0]);
+  HIPOK(hipGetLastError());
+  HIPOK(sync_stream(h));
+  return MFIXDEM_OK;
+}
+
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, double* omega,
                               int8_t* state, int64_t ld) {
   if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call

@@ -161,6 +161,14 @@ module mfixdem_c
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function
+     ! PARTICLE_STATE holes (0) before init
+     integer(c_int32_t) function mfixdem_set_particle_state(h, state, n) &
+          bind(C, name="mfixdem_set_particle_state")
+       import :: c_int32_t, c_int64_t, c_int8_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int8_t), intent(in) :: state(*)
+       integer(c_int64_t), value :: n
+     end function
      ! DES_TIME_INIT's coupled part with DES_EXPLICITLY_COUPLED (DRAG_FC)
      integer(c_int32_t) function mfixdem_des_time_init(h) bind(C, name="mfixdem_des_time_init")
        import :: c_int32_t, c_ptr

@@ -207,6 +207,7 @@ def _declare(lib):
         "mfixdem_get_gas_drag": ([_P, _DP, _DP, _DP], _I),
         "mfixdem_drag_gs_gas_noalloc": ([_P, _P, _P, _P], _I),
         "mfixdem_des_time_init": ([_P], _I),
+        "mfixdem_set_particle_state": ([_P, _BP, C.c_int64], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -55,6 +55,12 @@ class DemSystem:
             abi.dptr(parts.radius), abi.dptr(parts.ro_sol), abi.iptr(parts.phase), n),
             "set_particles")
 
+    def set_particle_state(self, state):
+        """PARTICLE_STATE of the first len(state) slots (0: hole), before init()."""
+        st = np.ascontiguousarray(state, dtype=np.int8)
+        abi.check(self.lib.mfixdem_set_particle_state(self.h, abi.bptr(st), st.size),
+                  "set_particle_state")
+
     def set_gas(self, gas: Gas):
         abi.check(self.lib.mfixdem_set_gas(
             self.h, abi.dptr(gas.u_g), abi.dptr(gas.v_g), abi.dptr(gas.w_g),

@@ -2259,6 +2259,19 @@ int orc_des_time_init(orc_sys* s) {
   return 0;
 }
 
+/* PARTICLE_STATE of slots 0..n-1 before set-up: 0 makes the slot a hole
+   (NONEXISTENT, discretelement_mod.f:68-78; its data cleared as
+   DELETE_PARTICLE clears it), 1 leaves it NORMAL; entering / exiting
+   particles (2, 3) and ghosts are not supported. */
+int orc_set_particle_state(orc_sys* s, const int8_t* state, int64_t n) {
+  if (!s || !state || n < 0 || n > s->max_pip || s->inited) return MFIXDEM_ERR_ARG;
+  for (int64_t q = 0; q < n; q++)
+    if (state[q] != NONEXISTENT && state[q] != NORMAL_PARTICLE) return MFIXDEM_ERR_UNSUPPORTED;
+  for (int64_t q = 0; q < n; q++)
+    if (state[q] == NONEXISTENT && s->state[q] != NONEXISTENT) delete_particle(s, q);
+  return 0;
+}
+
 /* ------------------------------------------------------- driver */
 int orc_desgrid_and_neighbour(orc_sys* s) {
   orc_desgrid_pic(s);

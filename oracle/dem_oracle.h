@@ -62,6 +62,7 @@ int orc_comp_mean_fields(orc_sys* s);
    DRAG_BM at the cells, DRAG_AM 0) for the other schemes. */
 int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm);
 int orc_des_time_init(orc_sys* s);
+int orc_set_particle_state(orc_sys* s, const int8_t* state, int64_t n);
 
 /* inspection */
 int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);

@@ -64,6 +64,7 @@ def load_oracle(omp: bool = False):
         "orc_get_p_force": ([P, D], C.c_int),
         "orc_drag_gs_gas": ([P, D, D, D], C.c_int),
         "orc_des_time_init": ([P], C.c_int),
+        "orc_set_particle_state": ([P, C.POINTER(C.c_int8), L], C.c_int),
         "orc_particles_in_cell": ([P], C.c_int),
         "orc_calc_interp_weights": ([P], C.c_int),
         "orc_comp_mean_fields": ([P], C.c_int),
@@ -218,6 +219,11 @@ class OracleSystem:
 
     def comp_mean_fields(self):
         self._chk(self.lib.orc_comp_mean_fields(self.h), "comp_mean_fields")
+
+    def set_particle_state(self, state):
+        st = np.ascontiguousarray(state, dtype=np.int8)
+        self._chk(self.lib.orc_set_particle_state(self.h, st.ctypes.data_as(C.POINTER(C.c_int8)),
+                                                  st.size), "set_particle_state")
 
     def des_time_init(self):
         self._chk(self.lib.orc_des_time_init(self.h), "des_time_init")

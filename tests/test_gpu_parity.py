@@ -738,3 +738,97 @@ def test_explicit_coupling_matches_oracle(scheme, interp_on, width):
     assert np.max(np.abs(gv - ov)) <= 1e-9 * max(1.0, np.max(np.abs(ov)))
     g.close()
     o.close()
+
+
+def _with_holes(n, frac, seed):
+    st = np.ones(n, dtype=np.int8)
+    st[np.random.default_rng(seed).random(n) < frac] = 0
+    return st
+
+
+def test_particle_state_holes_match_oracle():
+    """PARTICLE_STATE holes at set-up (mfixdem_set_particle_state): 15 % of
+    the slots nonexistent; cells, pair lists, PFT and the trajectory over
+    60 steps (two rebuilds) bit for bit against the oracle, holes kept."""
+    p, parts, _ = settling()
+    st = _with_holes(parts.n, 0.15, 9)
+    g, o = DemSystem(p), OracleSystem(p)
+    for s in (g, o):
+        s.set_particles(parts)
+        s.set_particle_state(st)
+        s.init()
+    gp_, op_ = assert_same_pairs(g, o)
+    assert np.array_equal(gp_, op_)
+    for s in (g, o):
+        s.run(1, 60)
+    a, b = g.get_particles(), o.get_particles()
+    assert np.array_equal(a[3], st) and np.array_equal(b[3], st)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    gp_, op_ = assert_same_pairs(g, o)
+    assert np.array_equal(gp_, op_)
+    with pytest.raises(abi.MfixDemError):   # after init
+        g.set_particle_state(st)
+    bad = st.copy()
+    bad[0] = 2                              # entering particles are not supported
+    h = DemSystem(p)
+    h.set_particles(parts)
+    with pytest.raises(abi.MfixDemError):
+        h.set_particle_state(bad)
+    for s in (g, o, h):
+        s.close()
+
+
+def test_des_res_restart_continues_the_run_bit_exact(tmp_path):
+    """A run with holes, saved at step 60 to <RUN_NAME>_DES.RES (positions,
+    velocities, omega, radius, density, phase, global ids, states and the
+    pair histories in the WRITE_RES0_DES layout), read back into a fresh
+    handle (particles at their global ids, holes, mfixdem_set_pairs) and
+    continued from NN = 61, ends bit for bit where the uninterrupted run
+    ends once that run rebuilds its list at the same moment (the restart
+    builds its list at set-up).  Spheres start 2 mm inside the walls and
+    move < 0.2 mm, so no wall history (which the restart file does not
+    hold) is lost; Euler, so no Adams-Bashforth history is either."""
+    from mfixdem import restart
+    p, parts, _ = workloads.settling_bed(n=600, seed=4, box=(0.014, 0.024, 0.014))
+    r = parts.radius
+    pos = workloads.rsa_positions(4, r, [0.002, 0.002, 0.002], [0.012, 0.022, 0.012])
+    parts = workloads.make_particles(pos, r, parts.ro_sol)
+    p, parts = workloads.jostled(p, parts, seed=5, shrink=0.93, v0=0.1, w0=100.0)
+    st = _with_holes(parts.n, 0.05, 6)
+    a = DemSystem(p)
+    a.set_particles(parts)
+    a.set_particle_state(st)
+    a.init()
+    a.run(1, 60)
+    res = restart.from_system(a, parts.radius, parts.ro_sol, parts.phase)
+    assert res.n == int(st.sum()) and np.any(res.pft != 0.0)
+    path = str(tmp_path / "RUN_DES.RES")
+    restart.write_des_res(path, res)
+    back = restart.read_des_res(path)
+    # rebuild the slots: particle with global id g in slot g-1, holes elsewhere
+    n = p.max_pip
+    full = lambda v, k: np.zeros((k, n)) if k else np.zeros(n)  # noqa: E731
+    P, V, W, R, RHO = full(0, 3), full(0, 3), full(0, 3), np.full(n, 1e-4), np.full(n, 1.0)
+    PH, ST = np.ones(n, dtype=np.int32), np.zeros(n, dtype=np.int8)
+    slot = back.gid - 1
+    P[:, slot], V[:, slot], W[:, slot] = back.pos, back.vel, back.omega
+    R[slot], RHO[slot], PH[slot], ST[slot] = back.radius, back.ro_sol, back.phase, back.state
+    rp = workloads.make_particles(P, R, RHO, vel=V, omega=W, phase=PH)
+    ends = 1 + np.cumsum(np.bincount(back.pair_owner - 1, minlength=n))
+    b = DemSystem(p)
+    b.set_particles(rp)
+    b.set_particle_state(ST)
+    b.set_pairs(ends.astype(np.int32), back.pair_partner, back.pft)
+    b.init()
+    a.desgrid_pic()
+    a.neighbour()
+    for s in (a, b):
+        s.run(61, 40)
+    for x, y in zip(a.get_particles(), b.get_particles()):
+        assert np.array_equal(x, y)
+    ai, an, ap = a.get_pairs()
+    bi, bn, bp = b.get_pairs()
+    assert np.array_equal(ai, bi) and np.array_equal(an, bn) and np.array_equal(ap, bp)
+    a.close()
+    b.close()

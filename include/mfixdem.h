@@ -54,7 +54,8 @@ enum {
                                     (calc_epg_des.f:146-203)               */
   MFIXDEM_ERR_ZERO_DISTANCE = 6, /* two particle centres coincide
                                     (calc_force_dem.f:267-271)             */
-  MFIXDEM_ERR_CAPACITY = 7,      /* caller buffer too small                */
+  MFIXDEM_ERR_CAPACITY = 7,      /* caller buffer too small, or more than 8
+                                    wall contacts of one particle at once  */
   MFIXDEM_ERR_NOT_READY = 8,     /* mfixdem_init not called / no gas field */
   MFIXDEM_ERR_UNSUPPORTED = 9,   /* option outside this build's scope      */
   MFIXDEM_ERR_SCRATCH = 10       /* input larger than the fixed device
@@ -164,7 +165,8 @@ typedef struct mfixdem_status {
   int64_t rebuilds;           /* neighbour-list builds                      */
   int64_t n_particles;        /* existing particles                          */
   int64_t n_pairs;            /* half-list pairs (reference NEIGHBORS size) */
-  int64_t n_contacts;         /* pairs in contact at the last step          */
+  int64_t n_contacts;         /* pair contacts summed over the steps run with
+                                 MFIXDEM_DEBUG_STORE_FORCES (0 without)     */
   int64_t rogue_deleted;      /* particles removed as rogue (cumulative)    */
   int64_t epg_errors;         /* fluid cells with EP_G out of (0,1]         */
   int64_t zero_distance;      /* coincident contact pairs seen              */
@@ -202,6 +204,29 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos,
    leaves it NORMAL.  Entering / exiting particles (2, 3) are
    MFIXDEM_ERR_UNSUPPORTED; called after mfixdem_init it is MFIXDEM_ERR_ARG. */
 int32_t mfixdem_set_particle_state(mfixdem_handle* h, const int8_t* state, int64_t n);
+/* Wall geometry: nf user facets (triangles of an STL file, as GET_STL_DATA
+   reads them into VERTEX / NORM_FACE, get_stl_data.f:767-1131), ids
+   0..nf-1, followed, when default_walls is non-zero, by the box faces W E
+   S N B T as facets nf..nf+5, one large triangle each
+   (CONVERT_DEFAULT_WALLS_TO_STL, stl_preproc_des_mod.f:973-1047).
+   vertex[(f*3 + axis)*3 + v] is VERTEX(v, axis, f) and normal[f*3 + axis]
+   NORM_FACE(axis, f), Fortran memory order.  Facets are binned to the DES
+   cells they overlap (BIN_FACETS_TO_DG, :135-203), and an edge shared with
+   a coplanar neighbour is flagged ignored
+   (FLAG_FACETS_WHERE_EDGE_COLLISION_IS_IGNORED, :362-491).  Replaces the
+   default set-up (box walls only); call before mfixdem_init; single-system
+   handles only. */
+int32_t mfixdem_set_facets(mfixdem_handle* h, int32_t nf, const double* vertex,
+                           const double* normal, int32_t default_walls);
+/* A moving BC_STL wall (get_stl_data.f:1484-1502): facets [first,
+   first+count) move at vel + omega x (facet centre - rot_center), omega in
+   rad/s; its component along the facet normal is STL_NORMAL_VEL, the rest
+   STL_TANGENTIAL_VEL, both taken off the contact's relative velocity
+   (CFRELVEL_WALL, calc_collision_wall_mod.f:836-837).  The facets keep
+   their place (a BC_STL wall moves only in its velocity).  Any time. */
+int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
+                                const double* vel, const double* omega,
+                                const double* rot_center);
 /* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal). */
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
                               double* omega, int8_t* state, int64_t ld);

@@ -15,7 +15,8 @@
 namespace mfixdem {
 
 constexpr int kMaxPh = 4;
-constexpr int kMaxF = 8;
+// wall-collision slots per particle: COLLISION_ARRAY_MAX (discretelement_mod.f:637)
+constexpr int kWallSlots = 8;
 constexpr double kSmall = 1.0e-15;   // SMALL_NUMBER, param1_mod.f:26
 constexpr double kLarge = 1.0e32;    // LARGE_NUMBER, param1_mod.f:25
 constexpr double kUndefD = 9.87654321e31;
@@ -24,6 +25,7 @@ struct FacetDev {
   double v[3][3];
   double n[3];
   double fmin, fmax;
+  double vt[3], vn[3];  // STL_TANGENTIAL_VEL, STL_NORMAL_VEL
   int dir;
   uint8_t ignore_edge[4];
 };
@@ -32,7 +34,7 @@ struct FacetDev {
 struct DevParams {
   int n;            // particle slots
   int stride;       // leading dimension of SoA arrays
-  int hertz, ab2, coupled, drag_type, nph, nf;
+  int hertz, ab2, coupled, drag_type, nph;
   int first_pass;   // Adams-Bashforth first step
   int store_forces; // debug: keep FC/TOW
   int fuse_dep;     // fused step: GARG_2012 deposition in the step kernel
@@ -51,7 +53,6 @@ struct DevParams {
   int interp, interp_on;     // MFIXDEM_INTERP_*, DES_INTERP_ON
   double fw, fw3, oofv;      // FILTER_WIDTH_INTERP, x3, OoFILTER_VOL (DPVM)
   double dxc[3];             // uniform fluid cell size per axis
-  FacetDev facets[kMaxF];
 };
 
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
@@ -336,9 +337,9 @@ __device__ __forceinline__ int wall_contact(const DevParams& P, const FacetDev& 
     double cr[3], vrot[3], vrel[3], vrt[3];
     cross3(om, nrm, cr);
     for (int c = 0; c < 3; c++) vrot[c] = distmod * cr[c];
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 3; c++) {  // CFRELVEL_WALL, calc_collision_wall_mod.f:836-837
       vrel[c] = vel[c] + vrot[c];
-      vrel[c] = vrel[c] - (0.0 + 0.0);
+      vrel[c] = vrel[c] - (F.vt[c] + F.vn[c]);
     }
     const double vrn = dot3(vrel, nrm);
     for (int c = 0; c < 3; c++) vrt[c] = vrel[c] - vrn * nrm[c];
@@ -384,7 +385,8 @@ __device__ __forceinline__ int wall_contact(const DevParams& P, const FacetDev& 
     const float omf[3] = {(float)om[0], (float)om[1], (float)om[2]};
     float cr[3], vrel[3], vrt[3];
     cross3f(omf, nf, cr);
-    for (int c = 0; c < 3; c++) vrel[c] = (float)vel[c] + dmf * cr[c];
+    for (int c = 0; c < 3; c++)
+      vrel[c] = (float)vel[c] + dmf * cr[c] - (float)(F.vt[c] + F.vn[c]);
     const float vrn = dot3f(vrel, nf);
     for (int c = 0; c < 3; c++) vrt[c] = vrel[c] - vrn * nf[c];
     float kn, kt, etan, etat;

@@ -27,7 +27,6 @@ namespace mfixdem {
 
 constexpr double kPi = 3.141592653589793;  // 4.D0*ATAN(1.D0)
 constexpr double kUndef = MFIXDEM_UNDEFINED;
-constexpr int kMaxFacets = 8;               // wall history kept per facet
 
 struct Facet {
   double v[3][3];   // v[vertex][axis]
@@ -35,6 +34,8 @@ struct Facet {
   int dir;          // axis of smallest extent (ADD_FACET)
   double fmin, fmax;
   uint8_t ignore_edge[4];
+  double vt[3];     // STL_TANGENTIAL_VEL
+  double vn[3];     // STL_NORMAL_VEL
 };
 
 struct Setup {
@@ -49,9 +50,12 @@ struct Setup {
   // DES grid (ghost layer included)
   int dg_ni = 0, dg_nj = 0, dg_nk = 0, dg_ncell = 0;
   int coarse[3] = {0, 0, 0};
-  // facets and per-DES-cell facet bitmask (bit f = facet f listed, ascending)
+  // facets (user STL first, then the default box walls) and the facets of
+  // each DES cell: ids dg_fids[dg_foff[c] .. dg_foff[c+1]), ascending
   std::vector<Facet> facets;
-  std::vector<uint16_t> cell_facets;
+  std::vector<int> dg_foff, dg_fids;
+  int user_facets = 0;
+  bool default_walls = true;
 };
 
 inline int fluid_ijk(const Setup& s, int i, int j, int k) {  // 1-based in, 0-based out
@@ -326,18 +330,26 @@ inline bool tri_box_overlap(const double c[3], const double h[3], const double v
   return dmax >= 0.0;
 }
 
-inline void setup_facets(Setup& s) {
+// DES_STL_PREPROCESSING (stl_preproc_des_mod.f:80-124): the user's STL
+// facets (vertex[(f*3 + axis)*3 + v], MFiX's VERTEX(3,3,N) memory order;
+// normal[f*3 + axis], NORM_FACE(3,N)) first, then with default walls
+// CONVERT_DEFAULT_WALLS_TO_STL (:973-1047), one large triangle per box face,
+// W E S N B T; BIN_FACETS_TO_DG (:135-203) with the +-1 cell halo and
+// TRI_BOX_OVERLAP, each cell's facets in ascending id;
+// FLAG_FACETS_WHERE_EDGE_COLLISION_IS_IGNORED (:362-480).  Wall velocities
+// start at zero.
+inline void setup_facets(Setup& s, int nu, const double* vertex, const double* normal,
+                         bool default_walls) {
   const mfixdem_params& p = s.p;
   const double xn = p.x_min, xx = s.x_max, yn = p.y_min, yx = s.y_max, zn = p.z_min,
                zx = s.z_max, XL = p.xlength, YL = p.ylength, ZL = p.zlength;
-  auto add = [&](double a0, double a1, double a2, double b0, double b1, double b2, double c0,
-                 double c1, double c2, double n0, double n1, double n2) {
+  s.facets.clear();
+  s.user_facets = nu;
+  s.default_walls = default_walls;
+  auto add = [&](const double v[3][3], const double n[3]) {
     Facet f{};
-    const double v[3][3] = {{a0, a1, a2}, {b0, b1, b2}, {c0, c1, c2}};
-    std::memcpy(f.v, v, sizeof(v));
-    f.n[0] = n0;
-    f.n[1] = n1;
-    f.n[2] = n2;
+    std::memcpy(f.v, v, sizeof(f.v));
+    for (int c = 0; c < 3; c++) f.n[c] = n[c];
     double smallest = 3.4028234663852886e38;  // HUGE(0.0)
     for (int ax = 0; ax < 3; ax++) {
       const double mn = std::min(v[0][ax], std::min(v[1][ax], v[2][ax]));
@@ -351,15 +363,29 @@ inline void setup_facets(Setup& s) {
     }
     s.facets.push_back(f);
   };
-  // one triangle per box face, order W E S N B T
-  add(xn, yn, zn, xn, yx + YL, zn, xn, yn, zx + ZL, 1.0, 0.0, 0.0);
-  add(xx, yn, zn, xx, yx + YL, zn, xx, yn, zx + ZL, -1.0, 0.0, 0.0);
-  add(xn, yn, zn, xx + XL, yn, zn, xn, yn, zx + ZL, 0.0, 1.0, 0.0);
-  add(xn, yx, zn, xx + XL, yx, zn, xn, yx, zx + ZL, 0.0, -1.0, 0.0);
-  add(xn, yn, zn, xx + XL, yn, zn, xn, yx + YL, zn, 0.0, 0.0, 1.0);
-  add(xn, yn, zx, xx + XL, yn, zx, xn, yx + YL, zx, 0.0, 0.0, -1.0);
+  for (int f = 0; f < nu; f++) {
+    double v[3][3], n[3];
+    for (int c = 0; c < 3; c++) {
+      for (int k = 0; k < 3; k++) v[k][c] = vertex[(f * 3 + c) * 3 + k];
+      n[c] = normal[f * 3 + c];
+    }
+    add(v, n);
+  }
+  auto box = [&](double a0, double a1, double a2, double b0, double b1, double b2, double c0,
+                 double c1, double c2, double n0, double n1, double n2) {
+    const double v[3][3] = {{a0, a1, a2}, {b0, b1, b2}, {c0, c1, c2}};
+    const double n[3] = {n0, n1, n2};
+    add(v, n);
+  };
+  if (default_walls) {  // one triangle per box face, order W E S N B T
+    box(xn, yn, zn, xn, yx + YL, zn, xn, yn, zx + ZL, 1.0, 0.0, 0.0);
+    box(xx, yn, zn, xx, yx + YL, zn, xx, yn, zx + ZL, -1.0, 0.0, 0.0);
+    box(xn, yn, zn, xx + XL, yn, zn, xn, yn, zx + ZL, 0.0, 1.0, 0.0);
+    box(xn, yx, zn, xx + XL, yx, zn, xn, yx, zx + ZL, 0.0, -1.0, 0.0);
+    box(xn, yn, zn, xx + XL, yn, zn, xn, yx + YL, zn, 0.0, 0.0, 1.0);
+    box(xn, yn, zx, xx + XL, yn, zx, xn, yx + YL, zx, 0.0, 0.0, -1.0);
+  }
   s.d.n_facets = (int32_t)s.facets.size();
-  s.cell_facets.assign(s.dg_ncell, 0);
   const double tol_stl = 1.0e-6;
   const double buffer = 1.1 * p.max_radius;
   const double ldx = 1.0 / s.d.dg_dxinv, ldy = 1.0 / s.d.dg_dyinv, ldz = 1.0 / s.d.dg_dzinv;
@@ -367,6 +393,7 @@ inline void setup_facets(Setup& s) {
   auto iof = [&](double x) { return floor_sat((x - s.d.dg_xstart) * s.d.dg_dxinv) + 2; };
   auto jof = [&](double y) { return floor_sat((y - s.d.dg_ystart) * s.d.dg_dyinv) + 2; };
   auto kof = [&](double z) { return floor_sat((z - s.d.dg_zstart) * s.d.dg_dzinv) + 2; };
+  std::vector<std::vector<int>> lists(s.dg_ncell);
   for (size_t nn = 0; nn < s.facets.size(); nn++) {
     const Facet& f = s.facets[nn];
     double lo[3], hi[3];
@@ -394,10 +421,16 @@ inline void setup_facets(Setup& s) {
                                s.d.dg_ystart + ((double)(jj - 2) + 0.5) * ldy,
                                s.d.dg_zstart + ((double)(kk - 2) + 0.5) * ldz};
           const double h[3] = {0.5 * ldx + buffer, 0.5 * ldy + buffer, 0.5 * ldz + buffer};
-          if (tri_box_overlap(c, h, f.v)) s.cell_facets[dg_ijk(s, ii, jj, kk)] |= (uint16_t)(1u << nn);
+          if (tri_box_overlap(c, h, f.v)) lists[dg_ijk(s, ii, jj, kk)].push_back((int)nn);
         }
   }
-  // facets sharing an edge at a small angle: edge hits ignored
+  s.dg_foff.assign(s.dg_ncell + 1, 0);
+  s.dg_fids.clear();
+  for (int c = 0; c < s.dg_ncell; c++) {
+    s.dg_foff[c + 1] = s.dg_foff[c] + (int)lists[c].size();
+    s.dg_fids.insert(s.dg_fids.end(), lists[c].begin(), lists[c].end());
+  }
+  // facets of one cell sharing an edge at a small angle: edge hits ignored
   const double cos_nb = std::cos(5.0 / 180.0 * kPi) + 1.0e-6;
   auto ec = [&](int f, int e, int ax) {
     const Facet& F = s.facets[f];
@@ -406,12 +439,11 @@ inline void setup_facets(Setup& s) {
     return 0.5 * (F.v[1][ax] + F.v[2][ax]);
   };
   for (int cell = 0; cell < s.dg_ncell; cell++) {
-    const uint16_t m = s.cell_facets[cell];
-    for (int f1 = 0; f1 < (int)s.facets.size(); f1++) {
-      if (!(m >> f1 & 1)) continue;
+    const std::vector<int>& L = lists[cell];
+    for (int f1 : L)
       for (int e = 0; e < 3; e++)
-        for (int f2 = 0; f2 < (int)s.facets.size(); f2++) {
-          if (f2 == f1 || !(m >> f2 & 1)) continue;
+        for (int f2 : L) {
+          if (f2 == f1) continue;
           for (int ne = 0; ne < 3; ne++) {
             const double d0 = ec(f2, ne, 0) - ec(f1, e, 0), d1 = ec(f2, ne, 1) - ec(f1, e, 1),
                          d2 = ec(f2, ne, 2) - ec(f1, e, 2);
@@ -425,7 +457,30 @@ inline void setup_facets(Setup& s) {
             }
           }
         }
-    }
+  }
+}
+
+// Moving walls of a BC_STL group (get_stl_data.f:1484-1502): facets
+// [first, first+count) move with vel + omega x (facet centre - rot_center);
+// STL_NORMAL_VEL is that velocity's component along the facet normal,
+// STL_TANGENTIAL_VEL the rest.
+inline void set_wall_motion(Setup& s, int first, int count, const double vel[3],
+                            const double omega[3], const double rot_center[3]) {
+  for (int f = first; f < first + count; f++) {
+    Facet& F = s.facets[f];
+    double center[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) center[c] = center[c] + F.v[v][c];
+    for (int c = 0; c < 3; c++) center[c] = center[c] / 3.0;
+    const double op[3] = {center[0] - rot_center[0], center[1] - rot_center[1],
+                          center[2] - rot_center[2]};
+    const double cr[3] = {omega[1] * op[2] - omega[2] * op[1], omega[2] * op[0] - omega[0] * op[2],
+                          omega[0] * op[1] - omega[1] * op[0]};
+    double avg[3];
+    for (int c = 0; c < 3; c++) avg[c] = vel[c] + cr[c];
+    const double an = avg[0] * F.n[0] + avg[1] * F.n[1] + avg[2] * F.n[2];
+    for (int c = 0; c < 3; c++) F.vn[c] = an * F.n[c];
+    for (int c = 0; c < 3; c++) F.vt[c] = avg[c] - F.vn[c];
   }
 }
 
@@ -484,8 +539,7 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
     if (0.5 * p.des_interp_width > dmin || 0.5 * p.des_interp_width > dgmin)
       return MFIXDEM_ERR_ARG;
   }
-  setup_facets(s);
-  if ((int)s.facets.size() > kMaxFacets) return MFIXDEM_ERR_UNSUPPORTED;
+  setup_facets(s, 0, nullptr, nullptr, true);
   return MFIXDEM_OK;
 }
 

@@ -30,9 +30,15 @@ struct StepPtrs {
   uint8_t* state;
   const uint8_t* phase;
   const int* dgc;
-  const uint16_t* cell_facets;
+  // walls: the facets, the facet ids of each DES cell
+  // (dg_fids[dg_foff[c] .. dg_foff[c+1]), ascending), and per particle
+  // kWallSlots wall-collision slots: bit k of wmask = slot k in use,
+  // wfid[k*stride + i] its facet, wpft[(k*3 + d)*stride + i] its history
+  const FacetDev* fac;
+  const int *dg_foff, *dg_fids;
   uint8_t* wmask;
-  double* wpft;                 // [(f*3 + d)*stride + i]
+  int* wfid;
+  double* wpft;
   const int *nbr_start, *nbr_split, *nbr, *nbr_pid;
   const double* pft_in;         // tangential history [3*pair + d], valid where pfl is 1
   double* pft_out;
@@ -60,7 +66,7 @@ struct StepPtrs {
 };
 
 enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
-       kFlagOutOfDomain, kFlagContacts, kFlagCount };
+       kFlagOutOfDomain, kFlagContacts, kFlagWallSlots, kFlagCount };
 
 enum { kModeFused = 0, kModeForce = 1, kModeDrag = 2, kModeIntegrate = 3 };
 
@@ -87,7 +93,7 @@ struct BatchDev {
   const DevParams* ptab;    // per-system DevParams
   const GridDev* gtab;      // per-system grids
   const int* cell_off;      // search cells of system s: [cell_off[s], cell_off[s+1])
-  const int* dg_off;        // first DES cell of system s in cell_facets
+  const int* dg_off;        // first DES cell of system s in the facet lists
   int* rogue;               // [2s]: smallest rogue id, [2s+1]: rogue count
   int nsys;
   int sys_shift;            // bit position of the system id in the sort key
@@ -401,6 +407,23 @@ __device__ __forceinline__ bool garg_drag_terms(const DevParams& P, const StepPt
   }
   for (int d = 0; d < 3; d++) q[d] = pf[d] * pvol;
   return true;
+}
+
+// GET_COLLISION's slot for a new wall contact (calc_collision_wall_mod.f:
+// 651-702): the last free slot, else the first slot whose facet is not
+// among the cell's facets [fl0, fl1); -1 when neither exists (the
+// reference then doubles its slot table).
+__device__ __forceinline__ int wall_slot(const StepPtrs& a, int S, int i, uint8_t wm, int fl0,
+                                         int fl1) {
+  for (int k = kWallSlots - 1; k >= 0; k--)
+    if (!((wm >> k) & 1)) return k;
+  for (int k = 0; k < kWallSlots; k++) {
+    const int id = a.wfid[k * S + i];
+    bool listed = false;
+    for (int q = fl0; q < fl1; q++) listed = listed || a.dg_fids[q] == id;
+    if (!listed) return k;
+  }
+  return -1;
 }
 
 // The pair's history carried unchanged into the next step.
@@ -851,8 +874,10 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
         pc = a.fcell[i];
         pv = a.pvol[i];
       }
-      // ---- walls (calc_collision_wall_mod.f:184-606), facet order ascending
-      const uint16_t fm = a.cell_facets[(BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i]];
+      // ---- walls (calc_collision_wall_mod.f:184-606), the cell's facets in
+      // ascending id
+      const int dc = (BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i];
+      const int fl0 = a.dg_foff[dc], fl1 = a.dg_foff[dc + 1];
       uint8_t wm = a.wmask[i];
       if (kEarlyDrag && P.coupled) {
         double t[3], q[3];
@@ -864,22 +889,30 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
           }
       }
       const uint8_t wm0 = wm;
-      if (fm == 0) {
-        wm = 0;
+      if (fl0 == fl1) {
+        wm = 0;  // no facets in the cell: every slot cleared (:189-193)
       } else {
-        for (int f = 0; f < P.nf; f++) {
-          if (!((fm >> f) & 1)) continue;
+        for (int q = fl0; q < fl1; q++) {
+          const int f = a.dg_fids[q];
+          int k = -1;  // the slot holding facet f
+          for (int s2 = 0; s2 < kWallSlots; s2++)
+            if (k < 0 && ((wm >> s2) & 1) && a.wfid[s2 * S + i] == f) k = s2;
           double hist[3];
-          const bool set = (wm >> f) & 1;
-          if (set)
-            for (int d = 0; d < 3; d++) hist[d] = a.wpft[(f * 3 + d) * S + i];
-          const int r = wall_contact<MIXED>(P, P.facets[f], pos, vel, om, ri, mi, phi, hist, set,
-                                            fc, tow);
-          if (r == 0) {
-            wm &= (uint8_t)~(1u << f);
-          } else if (r == 1) {
-            wm |= (uint8_t)(1u << f);
-            for (int d = 0; d < 3; d++) a.wpft[(f * 3 + d) * S + i] = hist[d];
+          if (k >= 0)
+            for (int d = 0; d < 3; d++) hist[d] = a.wpft[(k * 3 + d) * S + i];
+          const int r = wall_contact<MIXED>(P, a.fac[f], pos, vel, om, ri, mi, phi, hist,
+                                            k >= 0, fc, tow);
+          if (r == 0) {  // REMOVE_COLLISION
+            if (k >= 0) wm &= (uint8_t)~(1u << k);
+          } else if (r == 1) {  // GET_COLLISION / UPDATE_COLLISION
+            if (k < 0) k = wall_slot(a, S, i, wm, fl0, fl1);
+            if (k >= 0) {
+              a.wfid[k * S + i] = f;
+              for (int d = 0; d < 3; d++) a.wpft[(k * 3 + d) * S + i] = hist[d];
+              wm |= (uint8_t)(1u << k);
+            } else {
+              atomicOr(&a.flags[kFlagWallSlots], 1);
+            }
           }
         }
       }
@@ -1369,9 +1402,11 @@ struct PermPtrs {
   int *gid_o, *fcell_o, *dgc_o;
   const uint8_t *state, *phase, *wmask;
   uint8_t *state_o, *phase_o, *wmask_o;
+  const int* wfid;
+  int* wfid_o;
   const double* wpft;
   double* wpft_o;
-  int nf, ab2;
+  int ab2;
 };
 
 __global__ __launch_bounds__(kBS) void k_permute(int n, int S, PermPtrs p) {
@@ -1394,9 +1429,11 @@ __global__ __launch_bounds__(kBS) void k_permute(int n, int S, PermPtrs p) {
   p.phase_o[i] = p.phase[o];
   const uint8_t wm = p.wmask[o];
   p.wmask_o[i] = wm;
-  for (int f = 0; f < p.nf; f++)
-    if ((wm >> f) & 1)
-      for (int d = 0; d < 3; d++) p.wpft_o[(f * 3 + d) * S + i] = p.wpft[(f * 3 + d) * S + o];
+  for (int k = 0; k < kWallSlots; k++)
+    if ((wm >> k) & 1) {
+      p.wfid_o[k * S + i] = p.wfid[k * S + o];
+      for (int d = 0; d < 3; d++) p.wpft_o[(k * 3 + d) * S + i] = p.wpft[(k * 3 + d) * S + o];
+    }
   if (p.ab2)
     for (int c = 0; c < 3; c++) {
       p.acc_o[c * S + i] = p.acc[c * S + o];

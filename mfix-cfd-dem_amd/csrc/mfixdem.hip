@@ -77,7 +77,8 @@ struct mfixdem_handle {
   double *acc[2]{}, *racc[2]{};
   int *gid[2]{}, *fcell[2]{}, *dgc[2]{};
   uint8_t *state[2]{}, *phase[2]{}, *wmask[2]{};
-  double* wpft[2]{};
+  int* wfid[2]{};       // wall-collision slots: facet id per slot
+  double* wpft[2]{};    // and history per slot
   double *fc = nullptr, *tow = nullptr, *fc_id = nullptr, *tow_id = nullptr;
   // rebuild
   uint64_t* keys[2]{};
@@ -124,7 +125,9 @@ struct mfixdem_handle {
   double *f_gp = nullptr, *drag_fc = nullptr;
   int fb = 0;
   double dep_inv[MFIXDEM_MAX_PHASES]{};
-  uint16_t* cell_facets = nullptr;
+  // walls: facets and the facet lists of the DES cells (upload_facets)
+  FacetDev* fac = nullptr;
+  int *dg_foff = nullptr, *dg_fids = nullptr;
   double* stage = nullptr;  // export staging, 3*n doubles
   // status
   mfixdem_status st{};
@@ -211,8 +214,8 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.pos_o = h->pos[1 - c]; a.vel_o = h->vel[1 - c]; a.omega_o = h->omega[1 - c];
   a.radius = h->radius[t]; a.pmass = h->pmass[t]; a.omoi = h->omoi[t]; a.pvol = h->pvol[t];
   a.gid = h->gid[t]; a.state = h->state[t]; a.phase = h->phase[t]; a.dgc = h->dgc[t];
-  a.cell_facets = h->cell_facets;
-  a.wmask = h->wmask[t]; a.wpft = h->wpft[t];
+  a.fac = h->fac; a.dg_foff = h->dg_foff; a.dg_fids = h->dg_fids;
+  a.wmask = h->wmask[t]; a.wfid = h->wfid[t]; a.wpft = h->wpft[t];
   a.nbr_start = h->nbr_start[l]; a.nbr_split = h->nbr_split[l];
   a.nbr = h->nbr[l]; a.nbr_pid = h->nbr_pid[l];
   a.pft_in = h->pft[p]; a.pft_out = h->pft[1 - p];
@@ -408,7 +411,7 @@ int32_t do_bin(mfixdem_handle* h) {
   p.state = h->state[t]; p.phase = h->phase[t]; p.wmask = h->wmask[t];
   p.state_o = h->state[1 - t]; p.phase_o = h->phase[1 - t]; p.wmask_o = h->wmask[1 - t];
   p.wpft = h->wpft[t]; p.wpft_o = h->wpft[1 - t];
-  p.nf = h->P.nf;
+  p.wfid = h->wfid[t]; p.wfid_o = h->wfid[1 - t];
   p.ab2 = h->P.ab2;
   hipLaunchKernelGGL(k_permute, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, p);
   HIPOK(hipGetLastError());
@@ -459,6 +462,7 @@ void launch_rows_wave(mfixdem_handle* h, const BuildPtrs& b, int64_t threads) {
 int32_t fatal_flags(const int* fl) {
   if (fl[kFlagZeroDist] > 0) return MFIXDEM_ERR_ZERO_DISTANCE;
   if (fl[kFlagEpg] != 0) return MFIXDEM_ERR_EPG;
+  if (fl[kFlagWallSlots] != 0) return MFIXDEM_ERR_CAPACITY;  // > 8 live wall contacts
   return MFIXDEM_OK;
 }
 
@@ -787,6 +791,40 @@ int32_t download(mfixdem_handle* h, std::vector<T>& v, const T* d, int64_t count
   return MFIXDEM_OK;
 }
 
+// The walls of one or more set-ups (an ensemble's systems in order) to the
+// device: facets with global ids (system s's ids offset by the facets of
+// the systems before it) and one facet list per DES cell, the systems'
+// cells concatenated as the DES-cell offsets of the ensemble number them.
+int32_t upload_facets(mfixdem_handle* h, const std::vector<const Setup*>& sys) {
+  std::vector<FacetDev> fac, one;
+  std::vector<int> foff(1, 0), fids;
+  for (const Setup* S : sys) {
+    const int base = (int)fac.size();
+    facets_dev(*S, one);
+    fac.insert(fac.end(), one.begin(), one.end());
+    for (int c = 0; c < S->dg_ncell; c++) {
+      for (int q = S->dg_foff[c]; q < S->dg_foff[c + 1]; q++) fids.push_back(base + S->dg_fids[q]);
+      foff.push_back((int)fids.size());
+    }
+  }
+  dfree(h->fac);
+  dfree(h->dg_foff);
+  dfree(h->dg_fids);
+  int32_t rc;
+  if ((rc = dalloc(&h->fac, (int64_t)fac.size())) || (rc = dalloc(&h->dg_foff, (int64_t)foff.size())) ||
+      (rc = dalloc(&h->dg_fids, (int64_t)fids.size())))
+    return rc;
+  if (!fac.empty())
+    HIPOK(hipMemcpy(h->fac, fac.data(), sizeof(FacetDev) * fac.size(), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(h->dg_foff, foff.data(), sizeof(int) * foff.size(), hipMemcpyHostToDevice));
+  if (!fids.empty())
+    HIPOK(hipMemcpy(h->dg_fids, fids.data(), sizeof(int) * fids.size(), hipMemcpyHostToDevice));
+  return MFIXDEM_OK;
+}
+int32_t upload_facets(mfixdem_handle* h, const Setup& S) {
+  return upload_facets(h, std::vector<const Setup*>{&S});
+}
+
 int32_t upload_setup(mfixdem_handle* h) {
   Setup& S = h->S;
   HIPOK(hipMemcpy(h->xe, S.xe.data(), sizeof(double) * S.xe.size(), hipMemcpyHostToDevice));
@@ -795,8 +833,7 @@ int32_t upload_setup(mfixdem_handle* h) {
   HIPOK(hipMemcpy(h->vol, S.vol.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vol_surr, S.vol_surr.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->fluid, S.fluid.data(), S.ncell, hipMemcpyHostToDevice));
-  HIPOK(hipMemcpy(h->cell_facets, S.cell_facets.data(), sizeof(uint16_t) * S.dg_ncell,
-                  hipMemcpyHostToDevice));
+  CHECKRC(upload_facets(h, S));
   double ros[MFIXDEM_MAX_PHASES];
   for (int m = 0; m < MFIXDEM_MAX_PHASES; m++) ros[m] = S.p.ro_s0[m];
   HIPOK(hipMemcpy(h->ro_s, ros, sizeof(ros), hipMemcpyHostToDevice));
@@ -865,7 +902,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
     f(h->pos[b]); f(h->vel[b]); f(h->omega[b]); f(h->radius[b]);
     f(h->pmass[b]); f(h->omoi[b]); f(h->pvol[b]); f(h->acc[b]);
     f(h->racc[b]); f(h->gid[b]); f(h->fcell[b]); f(h->dgc[b]);
-    f(h->state[b]); f(h->phase[b]); f(h->wmask[b]); f(h->wpft[b]);
+    f(h->state[b]); f(h->phase[b]); f(h->wmask[b]); f(h->wfid[b]); f(h->wpft[b]);
     f(h->keys[b]); f(h->vals[b]); f(h->nbr_start[b]); f(h->nbr_split[b]);
     f(h->pair_base[b]); f(h->nbr[b]); f(h->nbr_pid[b]); f(h->pft[b]); f(h->pfl[b]);
     f(h->rops_fix[b]);
@@ -879,22 +916,21 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
   f(h->gd_fgp); f(h->gd_vol_node); f(h->gd_am); f(h->gd_bm); f(h->gd_fgds);
-  f(h->cell_facets); f(h->stage);
+  f(h->fac); f(h->dg_foff); f(h->dg_fids); f(h->stage);
   f(h->d_sys); f(h->d_ptab); f(h->d_gtab); f(h->d_cell_off); f(h->d_dg_off); f(h->d_rogue);
 }
 
 void fill_devparams(mfixdem_handle* h) { fill_params(h->S, h->n, h->stride, h->P, h->G); }
 
 // Device buffers, stream and events of a handle whose set-up (S, P, G,
-// n, stride, n_fine, key_bits) is filled in; facet_cells = DES cells of
-// the per-cell wall lists.  On failure the caller destroys the handle.
-int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
+// n, stride, n_fine, key_bits) is filled in.  On failure the caller
+// destroys the handle.
+int32_t alloc_buffers(mfixdem_handle* h) {
   const int64_t n = h->n, S = h->stride, nc = h->S.ncell;
   int32_t rc = MFIXDEM_OK;
   auto fail = [&](int32_t r) { return r; };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(MFIXDEM_ERR_HIP);
-  const int nf = std::max(1, h->P.nf);
   for (int b = 0; b < 2; b++) {
     if ((rc = dalloc(&h->pos[b], 3 * S)) || (rc = dalloc(&h->vel[b], 3 * S)) ||
         (rc = dalloc(&h->omega[b], 3 * S)) || (rc = dalloc(&h->radius[b], S)) ||
@@ -903,7 +939,8 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
         (rc = dalloc(&h->racc[b], 3 * S)) || (rc = dalloc(&h->gid[b], S)) ||
         (rc = dalloc(&h->fcell[b], S)) || (rc = dalloc(&h->dgc[b], S)) ||
         (rc = dalloc(&h->state[b], S)) || (rc = dalloc(&h->phase[b], S)) ||
-        (rc = dalloc(&h->wmask[b], S)) || (rc = dalloc(&h->wpft[b], 3 * nf * S)) ||
+        (rc = dalloc(&h->wmask[b], S)) || (rc = dalloc(&h->wfid[b], kWallSlots * S)) ||
+        (rc = dalloc(&h->wpft[b], 3 * kWallSlots * S)) ||
         (rc = dalloc(&h->keys[b], n)) || (rc = dalloc(&h->vals[b], n)) ||
         (rc = dalloc(&h->nbr_start[b], n + 1)) || (rc = dalloc(&h->nbr_split[b], n)) ||
         (rc = dalloc(&h->pair_base[b], n + 1)))
@@ -926,7 +963,7 @@ int32_t alloc_buffers(mfixdem_handle* h, int64_t facet_cells) {
       (rc = dalloc(&h->vol_surr, nc)) || (rc = dalloc(&h->ro_s, MFIXDEM_MAX_PHASES)) ||
       (rc = dalloc(&h->fluid, nc)) || (rc = dalloc(&h->pic_cnt, nc)) ||
       (rc = dalloc(&h->pic_off, nc + 1)) || (rc = dalloc(&h->pic_cur, nc)) ||
-      (rc = dalloc(&h->pic, n)) || (rc = dalloc(&h->cell_facets, facet_cells)) ||
+      (rc = dalloc(&h->pic, n)) ||
       (rc = dalloc(&h->stage, 3 * n)) ||
       (rc = dalloc(&h->rops_fix[0], nc * h->S.p.nphases)) ||
       (rc = dalloc(&h->rops_fix[1], nc * h->S.p.nphases)) || (rc = dalloc(&h->rm_cnt, 2)) ||
@@ -1051,7 +1088,7 @@ int32_t mfixdem_create(const mfixdem_params* p, mfixdem_handle** out) {
     return MFIXDEM_ERR_UNSUPPORTED;
   }
   h->key_bits = h->G.mbits;
-  if ((rc = alloc_buffers(h, h->S.dg_ncell)) || (rc = bind_scratch(h))) {
+  if ((rc = alloc_buffers(h)) || (rc = bind_scratch(h))) {
     mfixdem_destroy(h);
     return rc;
   }
@@ -1175,6 +1212,34 @@ int32_t mfixdem_set_particle_state(mfixdem_handle* h, const int8_t* state, int64
                      h->vel[0], h->omega[0]);
   HIPOK(hipGetLastError());
   HIPOK(sync_stream(h));
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_facets(mfixdem_handle* h, int32_t nf, const double* vertex,
+                           const double* normal, int32_t default_walls) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
+  if (!h || nf < 0 || (nf > 0 && (!vertex || !normal)) || h->inited) return MFIXDEM_ERR_ARG;
+  for (int64_t q = 0; q < 3 * (int64_t)nf; q++)
+    if (!std::isfinite(normal[q])) return MFIXDEM_ERR_ARG;
+  HIPOK(sync_stream(h));
+  setup_facets(h->S, nf, vertex, normal, default_walls != 0);
+  CHECKRC(upload_facets(h, h->S));
+  invalidate_graphs(h);
+  return MFIXDEM_OK;
+}
+
+int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
+                                const double* vel, const double* omega,
+                                const double* rot_center) {
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
+  if (!h || !vel || !omega || !rot_center || first < 0 || count < 0 ||
+      (int64_t)first + count > (int64_t)h->S.facets.size())
+    return MFIXDEM_ERR_ARG;
+  HIPOK(sync_stream(h));
+  set_wall_motion(h->S, first, count, vel, omega, rot_center);
+  std::vector<FacetDev> fac;
+  facets_dev(h->S, fac);
+  HIPOK(hipMemcpy(h->fac, fac.data(), sizeof(FacetDev) * fac.size(), hipMemcpyHostToDevice));
   return MFIXDEM_OK;
 }
 
@@ -1731,7 +1796,7 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
   std::vector<DevParams> ptab(nsys);
   std::vector<GridDev> gtab(nsys);
   std::vector<int> cell_off(nsys + 1, 0), dg_off(nsys + 1, 0);
-  int mbits = 3, nf = 0;
+  int mbits = 3;
   for (int s = 0; s < nsys; s++) {
     const Setup& S = h->sys_setup[s];
     fill_params(S, h->n, h->stride, ptab[s], gtab[s]);
@@ -1741,7 +1806,6 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
       return MFIXDEM_ERR_UNSUPPORTED;
     }
     mbits = std::max(mbits, gtab[s].mbits);
-    nf = std::max(nf, ptab[s].nf);
     cell_off[s + 1] = cell_off[s] + nfine;
     dg_off[s + 1] = dg_off[s] + S.dg_ncell;
   }
@@ -1753,20 +1817,15 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
   }
   for (auto& g : gtab) g.mbits = mbits;
   h->G = gtab[0];
-  // handle-level flags that rebuild stages apply to every slot: the largest
-  // facet count, and carry Adams-Bashforth history if any member uses it
-  h->P.nf = nf;
+  // a handle-level flag that rebuild stages apply to every slot: carry
+  // Adams-Bashforth history if any member uses it
   for (int s = 0; s < nsys; s++) h->P.ab2 = std::max(h->P.ab2, ptab[s].ab2);
   h->n_fine = cell_off[nsys];
   h->key_bits = mbits + sys_bits;
-  int32_t rc = alloc_buffers(h, dg_off[nsys]);
+  int32_t rc = alloc_buffers(h);
   std::vector<uint16_t> sys(h->stride, 0);
   for (int s = 0; s < nsys; s++)
     for (int q = h->sys_off[s]; q < h->sys_off[s + 1]; q++) sys[q] = (uint16_t)s;
-  std::vector<uint16_t> facets(dg_off[nsys]);
-  for (int s = 0; s < nsys; s++)
-    std::copy(h->sys_setup[s].cell_facets.begin(), h->sys_setup[s].cell_facets.end(),
-              facets.begin() + dg_off[s]);
   h->rogue_init.resize(2 * nsys);
   for (int s = 0; s < nsys; s++) {
     h->rogue_init[2 * s] = INT_MAX;
@@ -1783,10 +1842,13 @@ int32_t mfixdem_ensemble_create(int32_t nsys, const mfixdem_params* p, mfixdem_h
        hipMemcpy(h->d_gtab, gtab.data(), sizeof(GridDev) * nsys, hipMemcpyHostToDevice) ||
        hipMemcpy(h->d_cell_off, cell_off.data(), sizeof(int) * (nsys + 1),
                  hipMemcpyHostToDevice) ||
-       hipMemcpy(h->d_dg_off, dg_off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) ||
-       hipMemcpy(h->cell_facets, facets.data(), sizeof(uint16_t) * facets.size(),
-                 hipMemcpyHostToDevice)))
+       hipMemcpy(h->d_dg_off, dg_off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice)))
     rc = MFIXDEM_ERR_HIP;
+  if (rc == MFIXDEM_OK) {
+    std::vector<const Setup*> sp;
+    for (int s = 0; s < nsys; s++) sp.push_back(&h->sys_setup[s]);
+    rc = upload_facets(h, sp);
+  }
   if (rc != MFIXDEM_OK) {
     mfixdem_destroy(h);
     return rc;

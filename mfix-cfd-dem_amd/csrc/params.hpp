@@ -21,7 +21,6 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
   P.explicit_cpl = p.des_explicitly_coupled;
   P.drag_type = p.drag_type;
   P.nph = p.nphases;
-  P.nf = (int)S.facets.size();
   P.dt = S.d.dtsolid;
   for (int c = 0; c < 3; c++) P.grav[c] = p.grav[c];
   P.kn = p.kn; P.kt = S.d.kt; P.kn_w = p.kn_w; P.kt_w = S.d.kt_w;
@@ -50,15 +49,6 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
     P.oofv = 0.25 / (P.fw * P.fw * P.fw);
     P.fw3 = P.fw * 3;
   }
-  for (size_t f = 0; f < S.facets.size(); f++) {
-    FacetDev& F = P.facets[f];
-    std::memcpy(F.v, S.facets[f].v, sizeof(F.v));
-    std::memcpy(F.n, S.facets[f].n, sizeof(F.n));
-    F.fmin = S.facets[f].fmin;
-    F.fmax = S.facets[f].fmax;
-    F.dir = S.facets[f].dir;
-    std::memcpy(F.ignore_edge, S.facets[f].ignore_edge, 4);
-  }
   G.dg_ni = S.dg_ni; G.dg_nj = S.dg_nj; G.dg_nk = S.dg_nk;
   for (int d = 0; d < 3; d++) G.coarse[d] = S.coarse[d];
   G.dg_lo[0] = S.d.dg_xstart; G.dg_lo[1] = S.d.dg_ystart; G.dg_lo[2] = S.d.dg_zstart;
@@ -66,6 +56,23 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
   G.factor_rlm = p.factor_rlm;
 }
 
+
+// Device form of the facets of a set-up.
+inline void facets_dev(const Setup& S, std::vector<FacetDev>& out) {
+  out.clear();
+  for (const Facet& f : S.facets) {
+    FacetDev F{};
+    std::memcpy(F.v, f.v, sizeof(F.v));
+    std::memcpy(F.n, f.n, sizeof(F.n));
+    F.fmin = f.fmin;
+    F.fmax = f.fmax;
+    std::memcpy(F.vt, f.vt, sizeof(F.vt));
+    std::memcpy(F.vn, f.vn, sizeof(F.vn));
+    F.dir = f.dir;
+    std::memcpy(F.ignore_edge, f.ignore_edge, 4);
+    out.push_back(F);
+  }
+}
 
 // bits b per axis of a Morton code over cells 0..count-1 with 2^b > count
 inline int morton_axis_bits(int count) {

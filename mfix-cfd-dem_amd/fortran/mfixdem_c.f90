@@ -169,6 +169,23 @@ module mfixdem_c
        integer(c_int8_t), intent(in) :: state(*)
        integer(c_int64_t), value :: n
      end function
+     ! STL walls: VERTEX(3,3,nf) and NORM_FACE(3,nf) as GET_STL_DATA reads them
+     integer(c_int32_t) function mfixdem_set_facets(h, nf, vertex, normal, default_walls) &
+          bind(C, name="mfixdem_set_facets")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nf
+       real(c_double), intent(in) :: vertex(3, 3, *), normal(3, *)
+       integer(c_int32_t), value :: default_walls
+     end function
+     ! BC_STL wall motion: BC_WALL_VEL, BC_WALL_OMEGA (rad/s), BC_WALL_ROT_CENTER
+     integer(c_int32_t) function mfixdem_set_wall_motion(h, first, count, vel, omega, &
+          rot_center) bind(C, name="mfixdem_set_wall_motion")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: first, count
+       real(c_double), intent(in) :: vel(3), omega(3), rot_center(3)
+     end function
      ! DES_TIME_INIT's coupled part with DES_EXPLICITLY_COUPLED (DRAG_FC)
      integer(c_int32_t) function mfixdem_des_time_init(h) bind(C, name="mfixdem_des_time_init")
        import :: c_int32_t, c_ptr

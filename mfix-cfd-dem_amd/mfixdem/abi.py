@@ -208,6 +208,8 @@ def _declare(lib):
         "mfixdem_drag_gs_gas_noalloc": ([_P, _P, _P, _P], _I),
         "mfixdem_des_time_init": ([_P], _I),
         "mfixdem_set_particle_state": ([_P, _BP, C.c_int64], _I),
+        "mfixdem_set_facets": ([_P, _I, _DP, _DP, _I], _I),
+        "mfixdem_set_wall_motion": ([_P, _I, _I, _DP, _DP, _DP], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -61,6 +61,22 @@ class DemSystem:
         abi.check(self.lib.mfixdem_set_particle_state(self.h, abi.bptr(st), st.size),
                   "set_particle_state")
 
+    def set_facets(self, vertex, normal, default_walls=True):
+        """STL walls before init(): vertex[f, axis, v] (VERTEX(v, axis, f)),
+        normal[f, axis]; the box walls follow as facets nf..nf+5."""
+        v = np.ascontiguousarray(vertex, dtype=np.float64).reshape(-1)
+        nrm = np.ascontiguousarray(normal, dtype=np.float64).reshape(-1)
+        abi.check(self.lib.mfixdem_set_facets(self.h, nrm.size // 3, abi.dptr(v), abi.dptr(nrm),
+                                              int(bool(default_walls))), "set_facets")
+
+    def set_wall_motion(self, first, count, vel, omega=(0.0, 0.0, 0.0),
+                        rot_center=(0.0, 0.0, 0.0)):
+        """Facets [first, first+count) move at vel + omega x (centre - rot_center)."""
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (vel, omega, rot_center)]
+        abi.check(self.lib.mfixdem_set_wall_motion(self.h, int(first), int(count),
+                                                   *[abi.dptr(a) for a in arr]),
+                  "set_wall_motion")
+
     def set_gas(self, gas: Gas):
         abi.check(self.lib.mfixdem_set_gas(
             self.h, abi.dptr(gas.u_g), abi.dptr(gas.v_g), abi.dptr(gas.w_g),

@@ -465,3 +465,41 @@ def lattice_member(n, kn, en, mew, seed, phi=0.30, precision=abi.MIXED):
     p.ro_s0[0] = 2500.0
     p.max_radius = 0.5 * d
     return p, make_particles(pos, 0.5 * d, 2500.0)
+
+
+def funnel_facets(box, y_lo, y_hi, outlet):
+    """An STL funnel (a square hopper) inside the box (X, Y-up, Z): four
+    inclined faces from the box's side walls at y_hi down to a square outlet
+    of half-width `outlet` around the box's vertical axis at y_lo, two
+    triangles per face (the shared diagonal is a coplanar edge).  Returns
+    (vertex[f, axis, v], normal[f, axis], planes) with the normals pointing
+    into the funnel and planes = [(point, normal)] of the four faces."""
+    lx, _, lz = box
+    cx, cz = 0.5 * lx, 0.5 * lz
+    top = [(0.0, 0.0), (lx, 0.0), (lx, lz), (0.0, lz)]
+    bot = [(cx - outlet, cz - outlet), (cx + outlet, cz - outlet),
+           (cx + outlet, cz + outlet), (cx - outlet, cz + outlet)]
+    verts, norms, planes = [], [], []
+    for k in range(4):
+        a = np.array([top[k][0], y_hi, top[k][1]])
+        b = np.array([top[(k + 1) % 4][0], y_hi, top[(k + 1) % 4][1]])
+        c = np.array([bot[(k + 1) % 4][0], y_lo, bot[(k + 1) % 4][1]])
+        d = np.array([bot[k][0], y_lo, bot[k][1]])
+        n = np.cross(b - a, d - a)
+        n = n / np.linalg.norm(n)
+        if n[1] < 0.0:
+            n = -n
+        for tri in ((a, c, b), (a, d, c)):   # right-handed about n
+            verts.append(np.stack(tri, axis=1))   # [axis, v]
+            norms.append(n)
+        planes.append((a, n))
+    return np.array(verts), np.array(norms), planes
+
+
+def inside_funnel(pos, radius, planes, margin=1.0):
+    """Mask of the spheres on the inner side of every funnel face by more
+    than margin * radius."""
+    keep = np.ones(pos.shape[1], dtype=bool)
+    for a, n in planes:
+        keep &= (n @ (pos - a[:, None])) > margin * radius
+    return keep

@@ -33,7 +33,6 @@
 #define NORMAL_PARTICLE 1
 #define NONEXISTENT 0
 #define COLLISION_ARRAY_MAX 8  /* discretelement_mod.f:637 */
-#define MAXF 16                /* facets this restatement supports */
 #define MP MFIXDEM_MAX_PHASES
 #define FILTER_N 27            /* FILTER_SIZE = merge(27, 9, DO_K), 3-D */
 
@@ -59,14 +58,18 @@ struct orc_sys {
   int* dg_list;  /* particle indices (0-based), ascending within a cell */
   /* facets */
   int nf;
-  double vert[MAXF][3][3]; /* VERTEX(v, c, nf) -> vert[nf][v][c] */
-  double norm[MAXF][3];
-  unsigned char ignore_edge[MAXF][4];
+  double (*vert)[3][3];    /* VERTEX(v, c, nf) -> vert[nf][v][c] */
+  double (*norm)[3];
+  unsigned char (*ignore_edge)[4];
+  double (*wall_vt)[3];    /* STL_TANGENTIAL_VEL(:, nf) */
+  double (*wall_vn)[3];    /* STL_NORMAL_VEL(:, nf) */
   int* fc_count;           /* facets_at_dg count */
-  int (*fc_id)[MAXF];
-  int (*fc_dir)[MAXF];
-  double (*fc_min)[MAXF];
-  double (*fc_max)[MAXF];
+  int* fc_off;             /* list of DES cell c: [fc_off[c], fc_off[c] + fc_count[c]) */
+  int* fc_id;              /* facet ids, ascending within a cell */
+  int* fc_dir;
+  double* fc_min;
+  double* fc_max;
+  int wall_slot_overflow;  /* GET_COLLISION found no slot (the reference grows the table) */
   /* particles */
   int64_t max_pip, ld;
   int64_t pip;
@@ -500,105 +503,140 @@ static int tri_box_overlap(const double c[3], const double h[3], const double vt
   return 0;
 }
 
-/* CONVERT_DEFAULT_WALLS_TO_STL (stl_preproc_des_mod.f:973-1047): one large
-   triangle per box face; BIN_FACETS_TO_DG (:135-203) with ADD_FACET_FOR_DES
-   (:210-260) and ADD_FACET (:268-345); FLAG_FACETS_WHERE_EDGE_COLLISION_IS_
-   IGNORED (:362-480). */
-static void setup_facets(orc_sys* s) {
+/* DES_STL_PREPROCESSING (stl_preproc_des_mod.f:80-124): the user's STL
+   facets first (ids 0..nu-1), then, with default walls,
+   CONVERT_DEFAULT_WALLS_TO_STL (:973-1047): one large triangle per box face,
+   W E S N B T; BIN_FACETS_TO_DG (:135-203) with ADD_FACET_FOR_DES (:210-260)
+   and ADD_FACET (:268-345), each cell's facets in ascending id;
+   FLAG_FACETS_WHERE_EDGE_COLLISION_IS_IGNORED (:362-480).  user_vert is
+   VERTEX(3,3,nu) as MFiX stores it (vertex index fastest, then coordinate),
+   user_norm NORM_FACE(3,nu). */
+static void free_facets(orc_sys* s) {
+  void* ptrs[] = {s->vert, s->norm, s->ignore_edge, s->wall_vt, s->wall_vn, s->fc_count,
+                  s->fc_off, s->fc_id, s->fc_dir, s->fc_min, s->fc_max};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  s->vert = NULL; s->norm = NULL; s->ignore_edge = NULL; s->wall_vt = NULL; s->wall_vn = NULL;
+  s->fc_count = NULL; s->fc_off = NULL; s->fc_id = NULL; s->fc_dir = NULL;
+  s->fc_min = NULL; s->fc_max = NULL;
+}
+
+static void setup_facets(orc_sys* s, int nu, const double* user_vert, const double* user_norm,
+                         int default_walls) {
   const mfixdem_params* p = &s->p;
   double xn = p->x_min, xx = s->x_max, yn = p->y_min, yx = s->y_max, zn = p->z_min, zx = s->z_max;
   double XL = p->xlength, YL = p->ylength, ZL = p->zlength;
+  free_facets(s);
+  int nt = nu + (default_walls ? 6 : 0);
+  int na = nt > 0 ? nt : 1;
+  s->vert = calloc(na, sizeof(*s->vert));
+  s->norm = calloc(na, sizeof(*s->norm));
+  s->ignore_edge = calloc(na, sizeof(*s->ignore_edge));
+  s->wall_vt = calloc(na, sizeof(*s->wall_vt));
+  s->wall_vn = calloc(na, sizeof(*s->wall_vn));
   int n = 0;
+  for (; n < nu; n++)
+    for (int c = 0; c < 3; c++) {
+      for (int v = 0; v < 3; v++) s->vert[n][v][c] = user_vert[(n * 3 + c) * 3 + v];
+      s->norm[n][c] = user_norm[n * 3 + c];
+    }
 #define SETF(a0, a1, a2, b0, b1, b2, c0, c1, c2, n0, n1, n2) do { \
     s->vert[n][0][0] = a0; s->vert[n][0][1] = a1; s->vert[n][0][2] = a2; \
     s->vert[n][1][0] = b0; s->vert[n][1][1] = b1; s->vert[n][1][2] = b2; \
     s->vert[n][2][0] = c0; s->vert[n][2][1] = c1; s->vert[n][2][2] = c2; \
     s->norm[n][0] = n0; s->norm[n][1] = n1; s->norm[n][2] = n2; n++; } while (0)
-  SETF(xn, yn, zn, xn, yx + YL, zn, xn, yn, zx + ZL, 1.0, 0.0, 0.0);   /* west   */
-  SETF(xx, yn, zn, xx, yx + YL, zn, xx, yn, zx + ZL, -1.0, 0.0, 0.0);  /* east   */
-  SETF(xn, yn, zn, xx + XL, yn, zn, xn, yn, zx + ZL, 0.0, 1.0, 0.0);   /* south  */
-  SETF(xn, yx, zn, xx + XL, yx, zn, xn, yx, zx + ZL, 0.0, -1.0, 0.0);  /* north  */
-  SETF(xn, yn, zn, xx + XL, yn, zn, xn, yx + YL, zn, 0.0, 0.0, 1.0);   /* bottom */
-  SETF(xn, yn, zx, xx + XL, yn, zx, xn, yx + YL, zx, 0.0, 0.0, -1.0);  /* top    */
+  if (default_walls) {
+    SETF(xn, yn, zn, xn, yx + YL, zn, xn, yn, zx + ZL, 1.0, 0.0, 0.0);   /* west   */
+    SETF(xx, yn, zn, xx, yx + YL, zn, xx, yn, zx + ZL, -1.0, 0.0, 0.0);  /* east   */
+    SETF(xn, yn, zn, xx + XL, yn, zn, xn, yn, zx + ZL, 0.0, 1.0, 0.0);   /* south  */
+    SETF(xn, yx, zn, xx + XL, yx, zn, xn, yx, zx + ZL, 0.0, -1.0, 0.0);  /* north  */
+    SETF(xn, yn, zn, xx + XL, yn, zn, xn, yx + YL, zn, 0.0, 0.0, 1.0);   /* bottom */
+    SETF(xn, yn, zx, xx + XL, yn, zx, xn, yx + YL, zx, 0.0, 0.0, -1.0);  /* top    */
+  }
 #undef SETF
   s->nf = n;
   s->d.n_facets = n;
   s->fc_count = calloc(s->dg_ncell, sizeof(int));
-  s->fc_id = calloc(s->dg_ncell, sizeof(*s->fc_id));
-  s->fc_dir = calloc(s->dg_ncell, sizeof(*s->fc_dir));
-  s->fc_min = calloc(s->dg_ncell, sizeof(*s->fc_min));
-  s->fc_max = calloc(s->dg_ncell, sizeof(*s->fc_max));
+  s->fc_off = calloc(s->dg_ncell + 1, sizeof(int));
   const double tol_stl = 1.0e-6; /* cartesian_grid_init_namelist.f:748 */
   double buffer = 1.1 * p->max_radius;
   double ldx = 1.0 / s->dg_dxinv, ldy = 1.0 / s->dg_dyinv, ldz = 1.0 / s->dg_dzinv;
-  for (int nn = 0; nn < n; nn++) {
-    double X1 = s->vert[nn][0][0], X2 = X1, Y1 = s->vert[nn][0][1], Y2 = Y1,
-           Z1 = s->vert[nn][0][2], Z2 = Z1;
-    for (int v = 1; v < 3; v++) {
-      if (s->vert[nn][v][0] < X1) X1 = s->vert[nn][v][0];
-      if (s->vert[nn][v][0] > X2) X2 = s->vert[nn][v][0];
-      if (s->vert[nn][v][1] < Y1) Y1 = s->vert[nn][v][1];
-      if (s->vert[nn][v][1] > Y2) Y2 = s->vert[nn][v][1];
-      if (s->vert[nn][v][2] < Z1) Z1 = s->vert[nn][v][2];
-      if (s->vert[nn][v][2] > Z2) Z2 = s->vert[nn][v][2];
+  /* two passes over the (facet, cell) overlaps: count, then fill */
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      for (int c = 0; c < s->dg_ncell; c++) s->fc_off[c + 1] = s->fc_off[c] + s->fc_count[c];
+      int tot = s->fc_off[s->dg_ncell] > 0 ? s->fc_off[s->dg_ncell] : 1;
+      s->fc_id = calloc(tot, sizeof(int));
+      s->fc_dir = calloc(tot, sizeof(int));
+      s->fc_min = calloc(tot, sizeof(double));
+      s->fc_max = calloc(tot, sizeof(double));
+      memset(s->fc_count, 0, sizeof(int) * s->dg_ncell);
     }
-    int I1 = s->dg_iend2, I2 = s->dg_istart2;
-    if (X2 >= p->x_min - tol_stl && X1 <= s->x_max + tol_stl) {
-      I1 = imax_i(iofpos(s, X1) - 1, s->dg_istart2);
-      I2 = imin_i(iofpos(s, X2) + 1, s->dg_iend2);
-    }
-    int J1 = s->dg_jend2, J2 = s->dg_jstart2;
-    if (Y2 >= p->y_min - tol_stl && Y1 <= s->y_max + tol_stl) {
-      J1 = imax_i(jofpos(s, Y1) - 1, s->dg_jstart2);
-      J2 = imin_i(jofpos(s, Y2) + 1, s->dg_jend2);
-    }
-    int K1 = s->dg_kend2, K2 = s->dg_kstart2;
-    if (Z2 >= p->z_min - tol_stl && Z1 <= s->z_max + tol_stl) {
-      K1 = imax_i(kofpos(s, Z1) - 1, s->dg_kstart2);
-      K2 = imin_i(kofpos(s, Z2) + 1, s->dg_kend2);
-    }
-    for (int kk = K1; kk <= K2; kk++)
-      for (int jj = J1; jj <= J2; jj++)
-        for (int ii = I1; ii <= I2; ii++) {
-          int ijk = dg_funijk(s, ii, jj, kk) - 1;
-          double c[3], h[3];
-          c[0] = s->dg_xstart + ((double)(ii - s->dg_istart1) + 0.5) * ldx;
-          h[0] = 0.5 * ldx + buffer;
-          c[1] = s->dg_ystart + ((double)(jj - s->dg_jstart1) + 0.5) * ldy;
-          h[1] = 0.5 * ldy + buffer;
-          c[2] = s->dg_zstart + ((double)(kk - s->dg_kstart1) + 0.5) * ldz;
-          h[2] = 0.5 * ldz + buffer;
-          if (!tri_box_overlap(c, h, s->vert[nn])) continue;
-          int dup = 0;
-          for (int q = 0; q < s->fc_count[ijk]; q++)
-            if (s->fc_id[ijk][q] == nn) dup = 1;
-          if (dup) continue;
-          int q = s->fc_count[ijk]++;
-          s->fc_id[ijk][q] = nn;
-          double smallest = 3.4028234663852886e38; /* HUGE(0.0) */
-          for (int ax = 0; ax < 3; ax++) {
-            double mn = s->vert[nn][0][ax], mx = mn;
-            for (int v = 1; v < 3; v++) {
-              if (s->vert[nn][v][ax] < mn) mn = s->vert[nn][v][ax];
-              if (s->vert[nn][v][ax] > mx) mx = s->vert[nn][v][ax];
-            }
-            if (fabs(mx - mn) < smallest) {
-              s->fc_dir[ijk][q] = ax;
-              s->fc_min[ijk][q] = mn;
-              s->fc_max[ijk][q] = mx;
-              smallest = fabs(mx - mn);
+    for (int nn = 0; nn < n; nn++) {
+      double X1 = s->vert[nn][0][0], X2 = X1, Y1 = s->vert[nn][0][1], Y2 = Y1,
+             Z1 = s->vert[nn][0][2], Z2 = Z1;
+      for (int v = 1; v < 3; v++) {
+        if (s->vert[nn][v][0] < X1) X1 = s->vert[nn][v][0];
+        if (s->vert[nn][v][0] > X2) X2 = s->vert[nn][v][0];
+        if (s->vert[nn][v][1] < Y1) Y1 = s->vert[nn][v][1];
+        if (s->vert[nn][v][1] > Y2) Y2 = s->vert[nn][v][1];
+        if (s->vert[nn][v][2] < Z1) Z1 = s->vert[nn][v][2];
+        if (s->vert[nn][v][2] > Z2) Z2 = s->vert[nn][v][2];
+      }
+      int I1 = s->dg_iend2, I2 = s->dg_istart2;
+      if (X2 >= p->x_min - tol_stl && X1 <= s->x_max + tol_stl) {
+        I1 = imax_i(iofpos(s, X1) - 1, s->dg_istart2);
+        I2 = imin_i(iofpos(s, X2) + 1, s->dg_iend2);
+      }
+      int J1 = s->dg_jend2, J2 = s->dg_jstart2;
+      if (Y2 >= p->y_min - tol_stl && Y1 <= s->y_max + tol_stl) {
+        J1 = imax_i(jofpos(s, Y1) - 1, s->dg_jstart2);
+        J2 = imin_i(jofpos(s, Y2) + 1, s->dg_jend2);
+      }
+      int K1 = s->dg_kend2, K2 = s->dg_kstart2;
+      if (Z2 >= p->z_min - tol_stl && Z1 <= s->z_max + tol_stl) {
+        K1 = imax_i(kofpos(s, Z1) - 1, s->dg_kstart2);
+        K2 = imin_i(kofpos(s, Z2) + 1, s->dg_kend2);
+      }
+      for (int kk = K1; kk <= K2; kk++)
+        for (int jj = J1; jj <= J2; jj++)
+          for (int ii = I1; ii <= I2; ii++) {
+            int ijk = dg_funijk(s, ii, jj, kk) - 1;
+            double c[3], h[3];
+            c[0] = s->dg_xstart + ((double)(ii - s->dg_istart1) + 0.5) * ldx;
+            h[0] = 0.5 * ldx + buffer;
+            c[1] = s->dg_ystart + ((double)(jj - s->dg_jstart1) + 0.5) * ldy;
+            h[1] = 0.5 * ldy + buffer;
+            c[2] = s->dg_zstart + ((double)(kk - s->dg_kstart1) + 0.5) * ldz;
+            h[2] = 0.5 * ldz + buffer;
+            if (!tri_box_overlap(c, h, s->vert[nn])) continue;
+            int q = s->fc_count[ijk]++;
+            if (pass == 0) continue;
+            q += s->fc_off[ijk];
+            s->fc_id[q] = nn;
+            double smallest = 3.4028234663852886e38; /* HUGE(0.0) */
+            for (int ax = 0; ax < 3; ax++) {
+              double mn = s->vert[nn][0][ax], mx = mn;
+              for (int v = 1; v < 3; v++) {
+                if (s->vert[nn][v][ax] < mn) mn = s->vert[nn][v][ax];
+                if (s->vert[nn][v][ax] > mx) mx = s->vert[nn][v][ax];
+              }
+              if (fabs(mx - mn) < smallest) {
+                s->fc_dir[q] = ax;
+                s->fc_min[q] = mn;
+                s->fc_max[q] = mx;
+                smallest = fabs(mx - mn);
+              }
             }
           }
-        }
+    }
   }
   /* edge-collision flags */
-  memset(s->ignore_edge, 0, sizeof(s->ignore_edge));
   const double tol_edge = 1.0e-9;
   double cos_nb = cos(5.0 / 180.0 * PI_D) + 1.0e-6;
   for (int ijk = 0; ijk < s->dg_ncell; ijk++) {
-    int cnt = s->fc_count[ijk];
+    int cnt = s->fc_count[ijk], o = s->fc_off[ijk];
     for (int f1 = 0; f1 < cnt; f1++) {
-      int fid = s->fc_id[ijk][f1];
+      int fid = s->fc_id[o + f1];
       double ec[3][3];
       for (int c = 0; c < 3; c++) {
         ec[0][c] = 0.5 * (s->vert[fid][0][c] + s->vert[fid][1][c]);
@@ -608,7 +646,7 @@ static void setup_facets(orc_sys* s) {
       for (int e = 0; e < 3; e++)
         for (int f2 = 0; f2 < cnt; f2++) {
           if (f1 == f2) continue;
-          int nb = s->fc_id[ijk][f2];
+          int nb = s->fc_id[o + f2];
           double nec[3][3];
           for (int c = 0; c < 3; c++) {
             nec[0][c] = 0.5 * (s->vert[nb][0][c] + s->vert[nb][1][c]);
@@ -627,6 +665,40 @@ static void setup_facets(orc_sys* s) {
         }
     }
   }
+}
+
+/* User STL facets (replacing any given before), before set-up; the box
+   walls follow them unless default_walls is 0. */
+int orc_set_facets(orc_sys* s, int32_t nu, const double* vertex, const double* normal,
+                   int32_t default_walls) {
+  if (!s || nu < 0 || (nu > 0 && (!vertex || !normal)) || s->inited) return MFIXDEM_ERR_ARG;
+  setup_facets(s, nu, vertex, normal, default_walls);
+  return 0;
+}
+
+/* Moving walls of a BC_STL group (get_stl_data.f:1484-1502): facets
+   [first, first+count) move with the wall velocity vel plus omega x (facet
+   centre - rot_center); STL_NORMAL_VEL is that velocity's component along
+   the facet normal and STL_TANGENTIAL_VEL the rest. */
+int orc_set_wall_motion(orc_sys* s, int32_t first, int32_t count, const double* vel,
+                        const double* omega, const double* rot_center) {
+  if (!s || first < 0 || count < 0 || first + count > s->nf || !vel || !omega || !rot_center)
+    return MFIXDEM_ERR_ARG;
+  for (int n = first; n < first + count; n++) {
+    double center[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) center[c] = center[c] + s->vert[n][v][c];
+    for (int c = 0; c < 3; c++) center[c] = center[c] / 3.0;
+    double op[3] = {center[0] - rot_center[0], center[1] - rot_center[1],
+                    center[2] - rot_center[2]};
+    double cr[3], avg[3];
+    cross3(omega, op, cr);
+    for (int c = 0; c < 3; c++) avg[c] = vel[c] + cr[c];
+    double an = dot3(avg, s->norm[n]);
+    for (int c = 0; c < 3; c++) s->wall_vn[n][c] = an * s->norm[n][c];
+    for (int c = 0; c < 3; c++) s->wall_vt[n][c] = avg[c] - s->wall_vn[n][c];
+  }
+  return 0;
 }
 
 /* --------------------------------------------------------------- create */
@@ -668,7 +740,7 @@ orc_sys* orc_create(const mfixdem_params* p) {
   s->d.dtsolid = (p->dtsolid == UNDEF) ? s->d.tcoll / p->dtsolid_factor : p->dtsolid;
   setup_fluid_grid(s);
   setup_des_grid(s);
-  setup_facets(s);
+  setup_facets(s, 0, NULL, NULL, 1);
   if (check_interp(s)) {
     orc_destroy(s);
     return NULL;
@@ -719,7 +791,7 @@ orc_sys* orc_create(const mfixdem_params* p) {
 void orc_destroy(orc_sys* s) {
   if (!s) return;
   void* ptrs[] = {s->xe, s->yn, s->zt, s->vol, s->vol_surr, s->fluid, s->dg_cnt, s->dg_off,
-                  s->dg_list, s->fc_count, s->fc_id, s->fc_dir, s->fc_min, s->fc_max,
+                  s->dg_list,
                   s->pos, s->vel, s->omega, s->radius, s->ro_sol, s->pvol, s->pmass,
                   s->omoi, s->phase, s->state, s->fc, s->tow, s->fc_last, s->tow_last,
                   s->pos_old, s->vel_old, s->omega_old, s->acc_old, s->rot_acc_old,
@@ -729,6 +801,7 @@ void orc_destroy(orc_sys* s) {
                   s->rops_node, s->pinc, s->pic_off, s->pic_list, s->ugc, s->filter_cell,
                   s->filter_weight, s->solvolinc, s->f_gp, s->drag_fc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  free_facets(s);
   free(s);
 }
 
@@ -1408,11 +1481,17 @@ static void get_collision(orc_sys* s, int64_t q, int nf, double out[3]) {
     for (int cc = 0; cc < COLLISION_ARRAY_MAX && free_index == -1; cc++) {
       int listed = 0;
       for (int lc = 0; lc < s->fc_count[cell]; lc++)
-        if (s->wall_fid[cc * L + q] == s->fc_id[cell][lc]) listed = 1;
+        if (s->wall_fid[cc * L + q] == s->fc_id[s->fc_off[cell] + lc]) listed = 1;
       if (!listed) free_index = cc;
     }
   }
-  if (free_index == -1) free_index = COLLISION_ARRAY_MAX - 1; /* facets <= 8: unreachable */
+  if (free_index == -1) {
+    /* the reference doubles COLLISION_ARRAY_MAX here (:691-696); this
+       restatement keeps 8 slots and reports the run */
+    s->wall_slot_overflow = 1;
+    for (int d = 0; d < 3; d++) out[d] = 0.0;
+    return;
+  }
   s->wall_fid[free_index * L + q] = nf;
   for (int d = 0; d < 3; d++) {
     s->wall_pft[(free_index * 3 + d) * L + q] = 0.0;
@@ -1452,10 +1531,11 @@ static void wall_force_particle(orc_sys* s, int64_t q) {
   }
   int ph = s->phase[q];
   for (int cnt = 0; cnt < s->fc_count[cell]; cnt++) {
-    int axis = s->fc_dir[cell][cnt];
-    int nf = s->fc_id[cell][cnt];
-    if (s->fc_min[cell][cnt] > pmax[axis]) { remove_collision(s, q, nf); continue; }
-    if (s->fc_max[cell][cnt] < pmin[axis]) { remove_collision(s, q, nf); continue; }
+    int lq = s->fc_off[cell] + cnt;
+    int axis = s->fc_dir[lq];
+    int nf = s->fc_id[lq];
+    if (s->fc_min[lq] > pmax[axis]) { remove_collision(s, q, nf); continue; }
+    if (s->fc_max[lq] < pmin[axis]) { remove_collision(s, q, nf); continue; }
     double vv[3] = {s->vert[nf][0][0] - pos[0], s->vert[nf][0][1] - pos[1], s->vert[nf][0][2] - pos[2]};
     double line_t = dot3(vv, s->norm[nf]);
     if (line_t <= -1.0001 * r) { remove_collision(s, q, nf); continue; }
@@ -1477,11 +1557,11 @@ static void wall_force_particle(orc_sys* s, int64_t q) {
     double cr[3], vrot[3];
     cross3(om, normal, cr);
     for (int c = 0; c < 3; c++) vrot[c] = distmod * cr[c];
-    /* CFRELVEL_WALL (:806-847), static wall */
+    /* CFRELVEL_WALL (:806-847): the wall's tangential and normal velocity */
     double vrel[3];
     for (int c = 0; c < 3; c++) {
       vrel[c] = s->vel[c * L + q] + vrot[c];
-      vrel[c] = vrel[c] - (0.0 + 0.0);
+      vrel[c] = vrel[c] - (s->wall_vt[nf][c] + s->wall_vn[nf][c]);
     }
     double vrn = dot3(vrel, normal);
     double vrt[3];
@@ -2304,6 +2384,7 @@ int orc_time_step(orc_sys* s, int32_t nn) {
   int64_t L = s->ld;
   int rc = orc_calc_force_dem(s);
   if (rc) return rc;
+  if (s->wall_slot_overflow) return MFIXDEM_ERR_CAPACITY;
   if (s->p.coupled) orc_calc_drag_des(s);
   if (s->p.integrator == MFIXDEM_ADAMS_BASHFORTH) cfupdateold(s);
   memcpy(s->fc_last, s->fc, sizeof(double) * 3 * L);

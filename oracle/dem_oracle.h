@@ -63,6 +63,10 @@ int orc_comp_mean_fields(orc_sys* s);
 int orc_drag_gs_gas(orc_sys* s, double* f_gds, double* drag_am, double* drag_bm);
 int orc_des_time_init(orc_sys* s);
 int orc_set_particle_state(orc_sys* s, const int8_t* state, int64_t n);
+int orc_set_facets(orc_sys* s, int32_t nu, const double* vertex, const double* normal,
+                   int32_t default_walls);
+int orc_set_wall_motion(orc_sys* s, int32_t first, int32_t count, const double* vel,
+                        const double* omega, const double* rot_center);
 
 /* inspection */
 int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);

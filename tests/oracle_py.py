@@ -65,6 +65,8 @@ def load_oracle(omp: bool = False):
         "orc_drag_gs_gas": ([P, D, D, D], C.c_int),
         "orc_des_time_init": ([P], C.c_int),
         "orc_set_particle_state": ([P, C.POINTER(C.c_int8), L], C.c_int),
+        "orc_set_facets": ([P, C.c_int32, D, D, C.c_int32], C.c_int),
+        "orc_set_wall_motion": ([P, C.c_int32, C.c_int32, D, D, D], C.c_int),
         "orc_particles_in_cell": ([P], C.c_int),
         "orc_calc_interp_weights": ([P], C.c_int),
         "orc_comp_mean_fields": ([P], C.c_int),
@@ -224,6 +226,20 @@ class OracleSystem:
         st = np.ascontiguousarray(state, dtype=np.int8)
         self._chk(self.lib.orc_set_particle_state(self.h, st.ctypes.data_as(C.POINTER(C.c_int8)),
                                                   st.size), "set_particle_state")
+
+    def set_facets(self, vertex, normal, default_walls=True):
+        """STL facets before init: vertex (nf, 3 coords, 3 vertices) = MFiX's
+        VERTEX(3,3,nf) memory order, normal (nf, 3)."""
+        v = np.ascontiguousarray(vertex, dtype=np.float64)
+        nv = np.ascontiguousarray(normal, dtype=np.float64)
+        self._chk(self.lib.orc_set_facets(self.h, v.shape[0], abi.dptr(v), abi.dptr(nv),
+                                          1 if default_walls else 0), "set_facets")
+
+    def set_wall_motion(self, first, count, vel, omega=(0.0, 0.0, 0.0),
+                        rot_center=(0.0, 0.0, 0.0)):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vel, omega, rot_center)]
+        self._chk(self.lib.orc_set_wall_motion(self.h, first, count, *[abi.dptr(x) for x in a]),
+                  "set_wall_motion")
 
     def des_time_init(self):
         self._chk(self.lib.orc_des_time_init(self.h), "des_time_init")

@@ -27,7 +27,7 @@ struct FacetDev {
   double fmin, fmax;
   double vt[3], vn[3];  // STL_TANGENTIAL_VEL, STL_NORMAL_VEL
   int dir;
-  uint8_t ignore_edge[4];
+  int ignore_edges;  // bit e set: contacts on edge e are ignored
 };
 
 // Everything a step kernel needs besides array pointers; passed by value.
@@ -53,6 +53,11 @@ struct DevParams {
   int interp, interp_on;     // MFIXDEM_INTERP_*, DES_INTERP_ON
   double fw, fw3, oofv;      // FILTER_WIDTH_INTERP, x3, OoFILTER_VOL (DPVM)
   double dxc[3];             // uniform fluid cell size per axis
+  // walls of at most kWallSlots facets: the facets themselves (slot k of a
+  // particle holds facet k); nfs = 0 when there are more (facet table and
+  // per-cell lists in device memory)
+  int nfs;
+  FacetDev facets[kWallSlots];
 };
 
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
@@ -251,7 +256,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
   if (d1 < 0.0 && d2 < 0.0) {
     for (int i = 0; i < 3; i++) cp[i] = F.v[0][i];
     *pe = 1; *edge = 1;
-    if (F.ignore_edge[1]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 1) & 1) cp[0] = kUndefD;
     return;
   }
   double bp[3];
@@ -260,7 +265,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
   if (d3 > 0.0 && d4 < d3) {
     for (int i = 0; i < 3; i++) cp[i] = F.v[1][i];
     *pe = 1; *edge = 1;
-    if (F.ignore_edge[1]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 1) & 1) cp[0] = kUndefD;
     return;
   }
   const double vc = d1 * d4 - d3 * d2;
@@ -268,7 +273,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
     const double v = d1 / (d1 - d3);
     for (int i = 0; i < 3; i++) cp[i] = F.v[0][i] + v * ab[i];
     *pe = 1; *edge = 1;
-    if (F.ignore_edge[1]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 1) & 1) cp[0] = kUndefD;
     return;
   }
   double cpv[3];
@@ -277,7 +282,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
   if (d6 > 0.0 && d5 < d6) {
     for (int i = 0; i < 3; i++) cp[i] = F.v[2][i];
     *pe = 1; *edge = 2;
-    if (F.ignore_edge[2]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 2) & 1) cp[0] = kUndefD;
     return;
   }
   const double vb = d5 * d2 - d1 * d6;
@@ -285,7 +290,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
     const double w = d2 / (d2 - d6);
     for (int i = 0; i < 3; i++) cp[i] = F.v[0][i] + w * ac[i];
     *pe = 1; *edge = 2;
-    if (F.ignore_edge[2]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 2) & 1) cp[0] = kUndefD;
     return;
   }
   const double va = d3 * d6 - d5 * d4;
@@ -293,7 +298,7 @@ __device__ __forceinline__ void closest_pt_tri(const FacetDev& F, const double p
     const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
     for (int i = 0; i < 3; i++) cp[i] = F.v[1][i] + w * (F.v[2][i] - F.v[1][i]);
     *pe = 1; *edge = 3;
-    if (F.ignore_edge[3]) cp[0] = kUndefD;
+    if ((F.ignore_edges >> 3) & 1) cp[0] = kUndefD;
     return;
   }
   const double denom = 1.0 / (va + vb + vc);
@@ -311,7 +316,9 @@ __device__ __forceinline__ int wall_contact(const DevParams& P, const FacetDev& 
                                             const double om[3], double r, double pmass,
                                             int ph, double hist[3], bool hist_set,
                                             double fc[3], double tow[3]) {
-  const double pmax = pos[F.dir] + r, pmin = pos[F.dir] - r;
+  // pos[F.dir] by selects: no dynamic index into an array in registers
+  const double pd = F.dir == 0 ? pos[0] : (F.dir == 1 ? pos[1] : pos[2]);
+  const double pmax = pd + r, pmin = pd - r;
   if (F.fmin > pmax) return 0;
   if (F.fmax < pmin) return 0;
   const double vv[3] = {F.v[0][0] - pos[0], F.v[0][1] - pos[1], F.v[0][2] - pos[2]};
@@ -323,7 +330,7 @@ __device__ __forceinline__ int wall_contact(const DevParams& P, const FacetDev& 
   const double dist[3] = {cp[0] - pos[0], cp[1] - pos[1], cp[2] - pos[2]};
   const double distsq = dot3(dist, dist);
   if (distsq >= r * r - kSmall) return 0;
-  if (pe && F.ignore_edge[edge]) return 2;
+  if (pe && ((F.ignore_edges >> edge) & 1)) return 2;
   const double distmod = sqrt(distsq);
   double nrm[3] = {dist[0], dist[1], dist[2]};
   if (distsq > 0.0) {

@@ -30,10 +30,14 @@ struct StepPtrs {
   uint8_t* state;
   const uint8_t* phase;
   const int* dgc;
-  // walls: the facets, the facet ids of each DES cell
-  // (dg_fids[dg_foff[c] .. dg_foff[c+1]), ascending), and per particle
-  // kWallSlots wall-collision slots: bit k of wmask = slot k in use,
-  // wfid[k*stride + i] its facet, wpft[(k*3 + d)*stride + i] its history
+  // walls: per particle kWallSlots wall-collision slots, bit k of wmask =
+  // slot k in use, wpft[(k*3 + d)*stride + i] its history.  At most
+  // kWallSlots facets (DevParams::nfs > 0): slot k holds facet k and
+  // cell_fmask[c] has bit f set for each facet f of DES cell c.  More
+  // facets: the table fac, the facet ids of each DES cell
+  // (dg_fids[dg_foff[c] .. dg_foff[c+1]), ascending) and wfid[k*stride + i]
+  // the facet of slot k.
+  const uint8_t* cell_fmask;
   const FacetDev* fac;
   const int *dg_foff, *dg_fids;
   uint8_t* wmask;
@@ -424,6 +428,27 @@ __device__ __forceinline__ int wall_slot(const StepPtrs& a, int S, int i, uint8_
     if (!listed) return k;
   }
   return -1;
+}
+
+// Facet f of the table read through the scalar cache: f must be the same
+// in every lane of the wavefront, and the table is not written while a
+// kernel runs (constant address space).
+__device__ __forceinline__ FacetDev facet_uniform(const FacetDev* fac, int f) {
+  typedef const __attribute__((address_space(4))) FacetDev* ConstFacets;
+  const ConstFacets c = (ConstFacets)fac + f;
+  FacetDev F;
+  for (int v = 0; v < 3; v++)
+    for (int d = 0; d < 3; d++) F.v[v][d] = c->v[v][d];
+  for (int d = 0; d < 3; d++) {
+    F.n[d] = c->n[d];
+    F.vt[d] = c->vt[d];
+    F.vn[d] = c->vn[d];
+  }
+  F.fmin = c->fmin;
+  F.fmax = c->fmax;
+  F.dir = c->dir;
+  F.ignore_edges = c->ignore_edges;
+  return F;
 }
 
 // The pair's history carried unchanged into the next step.
@@ -838,7 +863,7 @@ __device__ __forceinline__ void step_tail(const DevParams& P0, const DevParams& 
 // after the pair forces, where CALC_DRAG_DES adds them.  With
 // P0.fuse_dep it also deposits the particles' new positions
 // (COMP_MEAN_FIELDS0 node sums) before it ends.
-template <bool MIXED, int MODE, bool FILTER, bool BATCH = false>
+template <bool MIXED, int MODE, bool FILTER, bool BATCH = false, bool STL = false>
 __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev B) {
   constexpr bool kPairs = MODE == kModeFused || MODE == kModeForce;
   constexpr bool kEarlyDrag = MODE == kModeFused && !FILTER && !BATCH;
@@ -877,7 +902,8 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
       // ---- walls (calc_collision_wall_mod.f:184-606), the cell's facets in
       // ascending id
       const int dc = (BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i];
-      const int fl0 = a.dg_foff[dc], fl1 = a.dg_foff[dc + 1];
+      const int fm = STL ? 0 : a.cell_fmask[dc];
+      const int fl0 = STL ? a.dg_foff[dc] : 0, fl1 = STL ? a.dg_foff[dc + 1] : 0;
       uint8_t wm = a.wmask[i];
       if (kEarlyDrag && P.coupled) {
         double t[3], q[3];
@@ -889,30 +915,57 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
           }
       }
       const uint8_t wm0 = wm;
-      if (fl0 == fl1) {
-        wm = 0;  // no facets in the cell: every slot cleared (:189-193)
-      } else {
-        for (int q = fl0; q < fl1; q++) {
-          const int f = a.dg_fids[q];
-          int k = -1;  // the slot holding facet f
-          for (int s2 = 0; s2 < kWallSlots; s2++)
-            if (k < 0 && ((wm >> s2) & 1) && a.wfid[s2 * S + i] == f) k = s2;
+      if (!STL) {
+        if (fm == 0) wm = 0;  // no facets in the cell: every slot cleared (:189-193)
+        for (int f = 0; f < P.nfs; f++) {
+          if (!((fm >> f) & 1)) continue;
           double hist[3];
-          if (k >= 0)
-            for (int d = 0; d < 3; d++) hist[d] = a.wpft[(k * 3 + d) * S + i];
-          const int r = wall_contact<MIXED>(P, a.fac[f], pos, vel, om, ri, mi, phi, hist,
-                                            k >= 0, fc, tow);
+          const bool set = (wm >> f) & 1;
+          if (set)
+            for (int d = 0; d < 3; d++) hist[d] = a.wpft[(f * 3 + d) * S + i];
+          const int r = wall_contact<MIXED>(P, P.facets[f], pos, vel, om, ri, mi, phi, hist, set,
+                                            fc, tow);
           if (r == 0) {  // REMOVE_COLLISION
-            if (k >= 0) wm &= (uint8_t)~(1u << k);
+            wm &= (uint8_t)~(1u << f);
           } else if (r == 1) {  // GET_COLLISION / UPDATE_COLLISION
-            if (k < 0) k = wall_slot(a, S, i, wm, fl0, fl1);
-            if (k >= 0) {
-              a.wfid[k * S + i] = f;
-              for (int d = 0; d < 3; d++) a.wpft[(k * 3 + d) * S + i] = hist[d];
-              wm |= (uint8_t)(1u << k);
-            } else {
-              atomicOr(&a.flags[kFlagWallSlots], 1);
-            }
+            wm |= (uint8_t)(1u << f);
+            for (int d = 0; d < 3; d++) a.wpft[(f * 3 + d) * S + i] = hist[d];
+          }
+        }
+      }
+      if (STL && fl0 == fl1) wm = 0;
+      // The facets are walked in wave-uniform passes: a pass takes the next
+      // facet of the first lane that has one left, reads it once for the
+      // wavefront through the scalar cache, and the lanes whose next facet
+      // it is evaluate it; every lane still meets its own facets in
+      // ascending id.
+      for (int q = fl0; STL;) {
+        const int mine = q < fl1 ? a.dg_fids[q] : -1;
+        const unsigned long long left = __ballot(mine >= 0);
+        if (left == 0ull) break;
+        const int f = __builtin_amdgcn_readlane(mine, __builtin_ctzll(left));
+        const FacetDev F = facet_uniform(a.fac, f);
+        // lanes taking this pass (a ballot, so that the facet index stays
+        // the wave-uniform f rather than the lane's own copy of it)
+        if (!((__ballot(mine == f) >> __lane_id()) & 1ull)) continue;
+        q++;
+        int k = -1;  // the slot holding facet f
+        for (int s2 = 0; s2 < kWallSlots; s2++)
+          if (k < 0 && ((wm >> s2) & 1) && a.wfid[s2 * S + i] == f) k = s2;
+        double hist[3];
+        if (k >= 0)
+          for (int d = 0; d < 3; d++) hist[d] = a.wpft[(k * 3 + d) * S + i];
+        const int r = wall_contact<MIXED>(P, F, pos, vel, om, ri, mi, phi, hist, k >= 0, fc, tow);
+        if (r == 0) {  // REMOVE_COLLISION
+          if (k >= 0) wm &= (uint8_t)~(1u << k);
+        } else if (r == 1) {  // GET_COLLISION / UPDATE_COLLISION
+          if (k < 0) k = wall_slot(a, S, i, wm, fl0, fl1);
+          if (k >= 0) {
+            a.wfid[k * S + i] = f;
+            for (int d = 0; d < 3; d++) a.wpft[(k * 3 + d) * S + i] = hist[d];
+            wm |= (uint8_t)(1u << k);
+          } else {
+            atomicOr(&a.flags[kFlagWallSlots], 1);
           }
         }
       }

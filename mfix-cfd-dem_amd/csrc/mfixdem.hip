@@ -128,6 +128,7 @@ struct mfixdem_handle {
   // walls: facets and the facet lists of the DES cells (upload_facets)
   FacetDev* fac = nullptr;
   int *dg_foff = nullptr, *dg_fids = nullptr;
+  uint8_t* cell_fmask = nullptr;
   double* stage = nullptr;  // export staging, 3*n doubles
   // status
   mfixdem_status st{};
@@ -214,6 +215,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.pos_o = h->pos[1 - c]; a.vel_o = h->vel[1 - c]; a.omega_o = h->omega[1 - c];
   a.radius = h->radius[t]; a.pmass = h->pmass[t]; a.omoi = h->omoi[t]; a.pvol = h->pvol[t];
   a.gid = h->gid[t]; a.state = h->state[t]; a.phase = h->phase[t]; a.dgc = h->dgc[t];
+  a.cell_fmask = h->cell_fmask;
   a.fac = h->fac; a.dg_foff = h->dg_foff; a.dg_fids = h->dg_fids;
   a.wmask = h->wmask[t]; a.wfid = h->wfid[t]; a.wpft = h->wpft[t];
   a.nbr_start = h->nbr_start[l]; a.nbr_split = h->nbr_split[l];
@@ -337,6 +339,16 @@ int32_t launch_step(mfixdem_handle* h) {
   if (h->nsys > 1) {  // ensembles: pure DEM, per-system parameter tables
     if (mixed) hipLaunchKernelGGL((k_step<true, MODE, false, true>), grid, blk, 0, h->stream, P, a, B);
     else hipLaunchKernelGGL((k_step<false, MODE, false, true>), grid, blk, 0, h->stream, P, a, B);
+  } else if (P.nfs == 0) {  // more facets than wall slots: facet table and slot search
+    const bool filt = h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012;
+    if (filt && mixed)
+      hipLaunchKernelGGL((k_step<true, MODE, true, false, true>), grid, blk, 0, h->stream, P, a, B);
+    else if (filt)
+      hipLaunchKernelGGL((k_step<false, MODE, true, false, true>), grid, blk, 0, h->stream, P, a, B);
+    else if (mixed)
+      hipLaunchKernelGGL((k_step<true, MODE, false, false, true>), grid, blk, 0, h->stream, P, a, B);
+    else
+      hipLaunchKernelGGL((k_step<false, MODE, false, false, true>), grid, blk, 0, h->stream, P, a, B);
   } else if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
     if (mixed) hipLaunchKernelGGL((k_step<true, MODE, true>), grid, blk, 0, h->stream, P, a, B);
     else hipLaunchKernelGGL((k_step<false, MODE, true>), grid, blk, 0, h->stream, P, a, B);
@@ -792,28 +804,41 @@ int32_t download(mfixdem_handle* h, std::vector<T>& v, const T* d, int64_t count
 }
 
 // The walls of one or more set-ups (an ensemble's systems in order) to the
-// device: facets with global ids (system s's ids offset by the facets of
-// the systems before it) and one facet list per DES cell, the systems'
-// cells concatenated as the DES-cell offsets of the ensemble number them.
+// device, the systems' DES cells concatenated as the ensemble's DES-cell
+// offsets number them: for a system of at most kWallSlots facets a facet
+// mask per DES cell (its own facet ids; the facets travel in the kernel
+// arguments), otherwise the facet table (system s's ids offset by the
+// facets of the systems before it) and one facet list per DES cell.
 int32_t upload_facets(mfixdem_handle* h, const std::vector<const Setup*>& sys) {
   std::vector<FacetDev> fac, one;
   std::vector<int> foff(1, 0), fids;
+  std::vector<uint8_t> fmask;
   for (const Setup* S : sys) {
     const int base = (int)fac.size();
     facets_dev(*S, one);
     fac.insert(fac.end(), one.begin(), one.end());
+    const bool small = walls_small(*S);
     for (int c = 0; c < S->dg_ncell; c++) {
-      for (int q = S->dg_foff[c]; q < S->dg_foff[c + 1]; q++) fids.push_back(base + S->dg_fids[q]);
+      uint8_t m = 0;
+      for (int q = S->dg_foff[c]; q < S->dg_foff[c + 1]; q++) {
+        fids.push_back(base + S->dg_fids[q]);
+        if (small) m |= (uint8_t)(1u << S->dg_fids[q]);
+      }
       foff.push_back((int)fids.size());
+      fmask.push_back(m);
     }
   }
   dfree(h->fac);
   dfree(h->dg_foff);
   dfree(h->dg_fids);
+  dfree(h->cell_fmask);
   int32_t rc;
   if ((rc = dalloc(&h->fac, (int64_t)fac.size())) || (rc = dalloc(&h->dg_foff, (int64_t)foff.size())) ||
-      (rc = dalloc(&h->dg_fids, (int64_t)fids.size())))
+      (rc = dalloc(&h->dg_fids, (int64_t)fids.size())) ||
+      (rc = dalloc(&h->cell_fmask, (int64_t)fmask.size())))
     return rc;
+  if (!fmask.empty())
+    HIPOK(hipMemcpy(h->cell_fmask, fmask.data(), fmask.size(), hipMemcpyHostToDevice));
   if (!fac.empty())
     HIPOK(hipMemcpy(h->fac, fac.data(), sizeof(FacetDev) * fac.size(), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->dg_foff, foff.data(), sizeof(int) * foff.size(), hipMemcpyHostToDevice));
@@ -916,7 +941,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
   f(h->gd_fgp); f(h->gd_vol_node); f(h->gd_am); f(h->gd_bm); f(h->gd_fgds);
-  f(h->fac); f(h->dg_foff); f(h->dg_fids); f(h->stage);
+  f(h->fac); f(h->dg_foff); f(h->dg_fids); f(h->cell_fmask); f(h->stage);
   f(h->d_sys); f(h->d_ptab); f(h->d_gtab); f(h->d_cell_off); f(h->d_dg_off); f(h->d_rogue);
 }
 
@@ -1223,6 +1248,7 @@ int32_t mfixdem_set_facets(mfixdem_handle* h, int32_t nf, const double* vertex,
     if (!std::isfinite(normal[q])) return MFIXDEM_ERR_ARG;
   HIPOK(sync_stream(h));
   setup_facets(h->S, nf, vertex, normal, default_walls != 0);
+  small_facets(h->S, h->P);
   CHECKRC(upload_facets(h, h->S));
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -1237,9 +1263,12 @@ int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
     return MFIXDEM_ERR_ARG;
   HIPOK(sync_stream(h));
   set_wall_motion(h->S, first, count, vel, omega, rot_center);
+  small_facets(h->S, h->P);  // facets in the kernel arguments: graphs recorded again
+  invalidate_graphs(h);
   std::vector<FacetDev> fac;
   facets_dev(h->S, fac);
-  HIPOK(hipMemcpy(h->fac, fac.data(), sizeof(FacetDev) * fac.size(), hipMemcpyHostToDevice));
+  if (!fac.empty())
+    HIPOK(hipMemcpy(h->fac, fac.data(), sizeof(FacetDev) * fac.size(), hipMemcpyHostToDevice));
   return MFIXDEM_OK;
 }
 

@@ -10,6 +10,37 @@
 
 namespace mfixdem {
 
+// Device form of the facets of a set-up.
+inline FacetDev facet_dev(const Facet& f) {
+  FacetDev F{};
+  std::memcpy(F.v, f.v, sizeof(F.v));
+  std::memcpy(F.n, f.n, sizeof(F.n));
+  F.fmin = f.fmin;
+  F.fmax = f.fmax;
+  std::memcpy(F.vt, f.vt, sizeof(F.vt));
+  std::memcpy(F.vn, f.vn, sizeof(F.vn));
+  F.dir = f.dir;
+  F.ignore_edges = 0;
+  for (int e = 0; e < 4; e++) F.ignore_edges |= f.ignore_edge[e] ? 1 << e : 0;
+  return F;
+}
+inline void facets_dev(const Setup& S, std::vector<FacetDev>& out) {
+  out.clear();
+  for (const Facet& f : S.facets) out.push_back(facet_dev(f));
+}
+// Walls of at most kWallSlots facets travel in the kernel arguments, and
+// slot k of a particle then holds facet k: with no more facets than slots a
+// free slot always exists, so GET_COLLISION never evicts and the history
+// each facet sees is the same as with the reference's slot search.
+inline bool walls_small(const Setup& S) { return S.facets.size() <= (size_t)kWallSlots; }
+inline void small_facets(const Setup& S, DevParams& P) {
+  P.nfs = 0;
+  for (int f = 0; f < kWallSlots; f++) P.facets[f] = FacetDev{};
+  if (!walls_small(S)) return;
+  P.nfs = (int)S.facets.size();
+  for (int f = 0; f < P.nfs; f++) P.facets[f] = facet_dev(S.facets[f]);
+}
+
 inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev& G) {
   const mfixdem_params& p = S.p;
   P = DevParams{};
@@ -54,24 +85,7 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
   G.dg_lo[0] = S.d.dg_xstart; G.dg_lo[1] = S.d.dg_ystart; G.dg_lo[2] = S.d.dg_zstart;
   G.dg_inv[0] = S.d.dg_dxinv; G.dg_inv[1] = S.d.dg_dyinv; G.dg_inv[2] = S.d.dg_dzinv;
   G.factor_rlm = p.factor_rlm;
-}
-
-
-// Device form of the facets of a set-up.
-inline void facets_dev(const Setup& S, std::vector<FacetDev>& out) {
-  out.clear();
-  for (const Facet& f : S.facets) {
-    FacetDev F{};
-    std::memcpy(F.v, f.v, sizeof(F.v));
-    std::memcpy(F.n, f.n, sizeof(F.n));
-    F.fmin = f.fmin;
-    F.fmax = f.fmax;
-    std::memcpy(F.vt, f.vt, sizeof(F.vt));
-    std::memcpy(F.vn, f.vn, sizeof(F.vn));
-    F.dir = f.dir;
-    std::memcpy(F.ignore_edge, f.ignore_edge, 4);
-    out.push_back(F);
-  }
+  small_facets(S, P);
 }
 
 // bits b per axis of a Morton code over cells 0..count-1 with 2^b > count

@@ -187,3 +187,28 @@ def test_wall_contact_capacity_matches_oracle():
         g.synchronize()
     assert e.value.code == abi.ERRORS_BY_NAME["CAPACITY"]
     g.close()
+
+
+def test_both_wall_paths_agree():
+    """Walls of at most eight facets run with slot k = facet k and the
+    facets in the kernel arguments; more facets use the facet table and the
+    reference's slot search.  The box walls alone, and the box walls after
+    three extra facets far outside the domain (nine facets: the general
+    path, same facets per cell), give the same run bit for bit, equal to
+    the oracle's."""
+    p, parts, _ = workloads.settling_bed(n=2000, seed=3, box=BOX)
+    p, parts = workloads.jostled(p, parts, seed=4)
+    # triangles in z = 5 m planes, at y = 5, 6, 7 m
+    far = np.array([[[5.0, 6.0, 5.0], [5.0, 5.0, 6.0], [5.0, 5.0, 5.0]]] * 3)
+    far[:, 1, :] += np.arange(3)[:, None]
+    n = np.array([[0.0, 0.0, 1.0]] * 3)
+    small = start(DemSystem, p, parts)
+    general = start(DemSystem, p, parts, far, n, default_walls=True)
+    o = start(OracleSystem, p, parts, far, n, default_walls=True)
+    for s in (small, general, o):
+        s.set_wall_motion(2 if s is small else 5, 1, (0.1, 0.0, 0.0))   # the floor slides
+        s.run(1, 300)
+    same_state(general, o)
+    same_state(small, o)
+    for s in (small, general, o):
+        s.close()

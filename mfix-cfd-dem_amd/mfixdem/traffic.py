@@ -22,7 +22,7 @@ import sys
 import time
 
 COPY_BYTES = 1 << 30   # 1 GiB read + 1 GiB written by the calibration copy
-STEP_KERNEL = "k_step<false,0,false,false>"   # FP64, fused, GARG, single system
+STEP_KERNEL = "k_step<false,0,false,false,false>"   # FP64, fused, GARG, single system, <= 8 facets
 
 
 KERNEL_SOURCES = ("csrc/kernels.hpp", "csrc/device_physics.hpp", "csrc/params.hpp")

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MFIXDEM_ABI_VERSION 2
+#define MFIXDEM_ABI_VERSION 3
 #define MFIXDEM_MAX_PHASES 4
 #define MFIXDEM_MAX_PAIRS_IN (MFIXDEM_MAX_PHASES * (MFIXDEM_MAX_PHASES + 1) / 2)
 /* MFiX's UNDEFINED sentinel (model/param1_mod.f:20). */
@@ -141,6 +141,14 @@ typedef struct mfixdem_params {
   int32_t des_neighbor_search;/* MFIXDEM_NSEARCH_* (4)                      */
   int32_t des_explicitly_coupled; /* DES_EXPLICITLY_COUPLED (0): drag formed once
                                   per fluid step, mfixdem_des_time_init     */
+  /* Fluid cell sizes of the interior cells, DX(I) for I = 1..IMAX (the
+     namelist's DX(0:IMAX-1); set_geometry.f, shift_dxyz.f:36-44), DY(1..JMAX),
+     DZ(1..KMAX); NULL = uniform XLENGTH/IMAX ...  The ghost cells take the
+     size of their interior neighbour, as SHIFT_DXYZ does.  Each sum must
+     equal XLENGTH ... to 1e-12 relative.  Read by mfixdem_create only. */
+  const double* dx;
+  const double* dy;
+  const double* dz;
 } mfixdem_params;
 
 /* Quantities derived once at set-up (check_solids_dem.f:519-1122,

@@ -52,7 +52,6 @@ struct DevParams {
   // filter schemes (DES_INTERP_SCHEME other than GARG_2012)
   int interp, interp_on;     // MFIXDEM_INTERP_*, DES_INTERP_ON
   double fw, fw3, oofv;      // FILTER_WIDTH_INTERP, x3, OoFILTER_VOL (DPVM)
-  double dxc[3];             // uniform fluid cell size per axis
   // walls of at most kWallSlots facets: the facets themselves (slot k of a
   // particle holds facet k); nfs = 0 when there are more (facet table and
   // per-cell lists in device memory)

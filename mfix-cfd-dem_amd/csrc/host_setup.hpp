@@ -43,7 +43,10 @@ struct Setup {
   mfixdem_derived d{};
   // fluid grid: cells 1..ni (ghosts included), faces 0..ni
   int ni = 0, nj = 0, nk = 0, ncell = 0;
-  double dx = 0, dy = 0, dz = 0;
+  // cell sizes DX(I), I = 1..ni, at cdx[I-1] (the ghost cells take their
+  // interior neighbour's size); smallest interior sizes; largest cell volume
+  std::vector<double> cdx, cdy, cdz;
+  double dx_min = 0, dy_min = 0, dz_min = 0, vol_max = 0;
   double x_max = 0, y_max = 0, z_max = 0;
   std::vector<double> xe, yn, zt, vol, vol_surr;
   std::vector<uint8_t> fluid;
@@ -206,20 +209,33 @@ inline void setup_grids(Setup& s) {
   s.nj = p.jmax + 2;
   s.nk = p.kmax + 2;
   s.ncell = s.ni * s.nj * s.nk;
-  s.dx = p.xlength / (double)p.imax;
-  s.dy = p.ylength / (double)p.jmax;
-  s.dz = p.zlength / (double)p.kmax;
   s.x_max = p.x_min + p.xlength;
   s.y_max = p.y_min + p.ylength;
   s.z_max = p.z_min + p.zlength;
-  auto faces = [](std::vector<double>& f, int n, double lo, double d) {
-    f.assign(n + 1, 0.0);
-    f[0] = lo - d;
-    for (int i = 1; i <= n; i++) f[i] = f[i - 1] + d;
+  // SHIFT_DXYZ (shift_dxyz.f:36-44): interior sizes given or uniform, the
+  // ghost cells copy their neighbour
+  auto sizes = [](std::vector<double>& c, int n, const double* in, double len, double& mn) {
+    c.assign(n + 2, 0.0);
+    for (int i = 1; i <= n; i++) c[i] = in ? in[i - 1] : len / (double)n;
+    c[0] = c[1];
+    c[n + 1] = c[n];
+    mn = c[1];
+    for (int i = 1; i <= n; i++) mn = std::min(mn, c[i]);
   };
-  faces(s.xe, s.ni, p.x_min, s.dx);
-  faces(s.yn, s.nj, p.y_min, s.dy);
-  faces(s.zt, s.nk, p.z_min, s.dz);
+  sizes(s.cdx, p.imax, p.dx, p.xlength, s.dx_min);
+  sizes(s.cdy, p.jmax, p.dy, p.ylength, s.dy_min);
+  sizes(s.cdz, p.kmax, p.dz, p.zlength, s.dz_min);
+  // SET_GEOMETRY_DES (set_geometry_des.f:72-74): XE(0) = X_MIN - DX(1),
+  // XE(I) = XE(I-1) + DX(I)
+  auto faces = [](std::vector<double>& f, int n, double lo, const std::vector<double>& c) {
+    f.assign(n + 1, 0.0);
+    f[0] = lo - c[0];
+    for (int i = 1; i <= n; i++) f[i] = f[i - 1] + c[i - 1];
+  };
+  faces(s.xe, s.ni, p.x_min, s.cdx);
+  faces(s.yn, s.nj, p.y_min, s.cdy);
+  faces(s.zt, s.nk, p.z_min, s.cdz);
+  s.vol_max = 0.0;
   s.vol.assign(s.ncell, 0.0);
   s.vol_surr.assign(s.ncell, 0.0);
   s.fluid.assign(s.ncell, 0);
@@ -227,7 +243,9 @@ inline void setup_grids(Setup& s) {
     for (int i = 1; i <= s.ni; i++)
       for (int j = 1; j <= s.nj; j++) {
         const int c = fluid_ijk(s, i, j, k);
-        s.vol[c] = s.dx * s.dy * (1.0 * s.dz);
+        // set_geometry1.f:125, X(I) = 1 (Cartesian)
+        s.vol[c] = s.cdx[i - 1] * s.cdy[j - 1] * (1.0 * s.cdz[k - 1]);
+        s.vol_max = std::max(s.vol_max, s.vol[c]);
         s.fluid[c] = (i >= 2 && i <= p.imax + 1 && j >= 2 && j <= p.jmax + 1 && k >= 2 &&
                       k <= p.kmax + 1);
       }
@@ -245,9 +263,10 @@ inline void setup_grids(Setup& s) {
   // DES grid
   const double width = 3.0 * (2.0 * p.max_radius);
   int gi = p.desgridsearch_imax, gj = p.desgridsearch_jmax, gk = p.desgridsearch_kmax;
-  if (gi <= 0) gi = std::max(1, (int)(p.xlength / std::min(width, s.dx)));
-  if (gj <= 0) gj = std::max(1, (int)(p.ylength / std::min(width, s.dy)));
-  if (gk <= 0) gk = std::max(1, (int)(p.zlength / std::min(width, s.dz)));
+  // check_geometry.f:134-148: WIDTH_X = min(WIDTH, minval(DX))
+  if (gi <= 0) gi = std::max(1, (int)(p.xlength / std::min(width, s.dx_min)));
+  if (gj <= 0) gj = std::max(1, (int)(p.ylength / std::min(width, s.dy_min)));
+  if (gk <= 0) gk = std::max(1, (int)(p.zlength / std::min(width, s.dz_min)));
   const int isz = (int)std::lround((s.xe[p.imax + 1] - s.xe[1]) / (p.xlength / gi));
   const int jsz = (int)std::lround((s.yn[p.jmax + 1] - s.yn[1]) / (p.ylength / gj));
   const int ksz = (int)std::lround((s.zt[p.kmax + 1] - s.zt[1]) / (p.zlength / gk));
@@ -520,12 +539,26 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
   }
   if (p.drag_type != MFIXDEM_DRAG_GIDASPOW && p.drag_type != MFIXDEM_DRAG_WEN_YU)
     return MFIXDEM_ERR_ARG;
+  // non-uniform cell sizes: positive, and summing to the domain length
+  auto sizes_ok = [](const double* c, int n, double len) {
+    if (!c) return true;
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) {
+      if (!(c[i] > 0.0) || !std::isfinite(c[i])) return false;
+      sum += c[i];
+    }
+    return std::fabs(sum - len) <= 1e-12 * std::fabs(len);
+  };
+  if (!sizes_ok(p.dx, p.imax, p.xlength) || !sizes_ok(p.dy, p.jmax, p.ylength) ||
+      !sizes_ok(p.dz, p.kmax, p.zlength))
+    return MFIXDEM_ERR_ARG;
   s = Setup{};
   s.p = p;
   if (p.coll_model == MFIXDEM_HERTZ) derive_hertz(s);
   else derive_lsd(s);
   s.d.dtsolid = (p.dtsolid == kUndef) ? s.d.tcoll / p.dtsolid_factor : p.dtsolid;
   setup_grids(s);
+  s.p.dx = s.p.dy = s.p.dz = nullptr;  // the sizes now live in cdx, cdy, cdz
   if ((int64_t)s.dg_ni * s.dg_nj * s.dg_nk > (int64_t)1 << 30) return MFIXDEM_ERR_ARG;
   if (s.dg_ni > 1023 || s.dg_nj > 1023 || s.dg_nk > 1023) return MFIXDEM_ERR_UNSUPPORTED;
   // fluid cells are packed in 10 bits per axis on the device
@@ -533,7 +566,7 @@ inline int32_t build_setup(const mfixdem_params& p, Setup& s) {
   if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
       p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
     // set_filter_des.f:70-87: half-width within a fluid cell and a DES cell
-    const double dmin = std::min(std::min(s.dx, s.dy), s.dz);
+    const double dmin = std::min(std::min(s.dx_min, s.dy_min), s.dz_min);
     const double dgmin = std::min(std::min(1.0 / s.d.dg_dxinv, 1.0 / s.d.dg_dyinv),
                                   1.0 / s.d.dg_dzinv);
     if (0.5 * p.des_interp_width > dmin || 0.5 * p.des_interp_width > dgmin)

@@ -53,6 +53,7 @@ struct StepPtrs {
   double *acc_old, *rot_acc_old;
   int* fcell;
   const double *xe, *yn, *zt;
+  const double* csz;  // cell sizes DX(1..ni), DY(1..nj), DZ(1..nk), concatenated
   const double *vnode, *p_force, *ep_g;
   const double* ugc;            // cell-centre gas velocity [3*ijk + d] (filter schemes)
   const uint8_t* fluid;
@@ -167,21 +168,22 @@ __device__ __forceinline__ bool fluid_at(const DevParams& P, const uint8_t* flui
                                          int K) {
   return on_grid(P, I, J, K) && fluid[lin_ijk(I, J, K, P.ni, P.nj)];
 }
-// one axis of CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:365-460)
-__device__ __forceinline__ void lhat_axis(double xpos, double face_lo, double d, bool fm, bool fp,
-                                          double sw[3]) {
-  const double xc = face_lo + 0.5 * d;
-  const double oo = 1.0 / d;  // oDX_E = ONE/(HALF*(DX+DX)) on a uniform grid
+// one axis of CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:365-460) in cell
+// I: face_lo = XE(I-1), c = &DX(I) (its neighbours at c[-1], c[1]);
+// oDX_E(L) = ONE/(HALF*(DX(L+1)+DX(L))) (set_geometry.f:341-343)
+__device__ __forceinline__ void lhat_axis(double xpos, double face_lo, const double* c, bool fm,
+                                          bool fp, double sw[3]) {
+  const double xc = face_lo + 0.5 * c[0];
   sw[0] = 0.0;
   sw[1] = 1.0;
   sw[2] = 0.0;
   if (xpos < xc) {
     if (fm) {
-      sw[0] = (xc - xpos) * oo;
+      sw[0] = (xc - xpos) * (1.0 / (0.5 * (c[0] + c[-1])));
       sw[1] = 1.0 - sw[0];
     }
   } else if (fp) {
-    sw[2] = (xpos - xc) * oo;
+    sw[2] = (xpos - xc) * (1.0 / (0.5 * (c[1] + c[0])));
     sw[1] = 1.0 - sw[2];
   }
 }
@@ -193,17 +195,19 @@ __device__ __forceinline__ void lhat_axis(double xpos, double face_lo, double d,
 // target gets weight 0).  fn(cell, weight) is called for each, cell 0-based.
 template <class Fn>
 __device__ __forceinline__ void for_filter(const DevParams& P, const double* xe, const double* yn,
-                                           const double* zt, const uint8_t* fluid, int ci, int cj,
-                                           int ck, const double pos[3], Fn fn) {
+                                           const double* zt, const double* csz,
+                                           const uint8_t* fluid, int ci, int cj, int ck,
+                                           const double pos[3], Fn fn) {
   const int own = lin_ijk(ci, cj, ck, P.ni, P.nj);
   if (P.interp == MFIXDEM_INTERP_LINEAR_HAT) {
     double sx[3], sy[3], sz[3];
-    lhat_axis(pos[0], xe[ci - 1], P.dxc[0], fluid_at(P, fluid, ci - 1, cj, ck),
+    // c[-1] / c[1] are read only where that neighbour is a fluid cell
+    lhat_axis(pos[0], xe[ci - 1], csz + (ci - 1), fluid_at(P, fluid, ci - 1, cj, ck),
               fluid_at(P, fluid, ci + 1, cj, ck), sx);
-    lhat_axis(pos[1], yn[cj - 1], P.dxc[1], fluid_at(P, fluid, ci, cj - 1, ck),
+    lhat_axis(pos[1], yn[cj - 1], csz + P.ni + (cj - 1), fluid_at(P, fluid, ci, cj - 1, ck),
               fluid_at(P, fluid, ci, cj + 1, ck), sy);
-    lhat_axis(pos[2], zt[ck - 1], P.dxc[2], fluid_at(P, fluid, ci, cj, ck - 1),
-              fluid_at(P, fluid, ci, cj, ck + 1), sz);
+    lhat_axis(pos[2], zt[ck - 1], csz + P.ni + P.nj + (ck - 1),
+              fluid_at(P, fluid, ci, cj, ck - 1), fluid_at(P, fluid, ci, cj, ck + 1), sz);
 #pragma unroll 1
     for (int kc = 0; kc < 3; kc++)
 #pragma unroll 1
@@ -261,7 +265,7 @@ __device__ __forceinline__ void add_drag_des1(const DevParams& P, const StepPtrs
   const int c0 = lin_ijk(ci, cj, ck, P.ni, P.nj);
   double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
   if (P.interp_on && filter_scheme(P.interp)) {
-    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+    for_filter(P, a.xe, a.yn, a.zt, a.csz, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
       lepg = lepg + a.ep_g[c] * w;
       for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
       for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + a.p_force[3 * c + d] * w;
@@ -1184,7 +1188,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
                                                         const uint8_t* phase, const uint8_t* state,
                                                         const int* fcell, const double* xe,
                                                         const double* yn, const double* zt,
-                                                        const uint8_t* fluid,
+                                                        const double* csz, const uint8_t* fluid,
                                                         unsigned long long* fix, Removal rm) {
   __shared__ unsigned long long tile[kDepCap];
   __shared__ int box[6];
@@ -1250,7 +1254,7 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
     };
     if (filter_scheme(P.interp)) {
       const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
-      for_filter(P, xe, yn, zt, fluid, ci, cj, ck, x, add);
+      for_filter(P, xe, yn, zt, csz, fluid, ci, cj, ck, x, add);
     } else {
       add(lin_ijk(ci, cj, ck, P.ni, P.nj), 1.0);
     }
@@ -2366,7 +2370,7 @@ __global__ __launch_bounds__(kBS) void k_gas_filter(DevParams P, StepPtrs a, con
   const bool filt = P.interp_on && filter_scheme(P.interp);
   double lepg = 0.0, velfp[3] = {0.0, 0.0, 0.0};
   if (filt) {
-    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+    for_filter(P, a.xe, a.yn, a.zt, a.csz, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
       lepg = lepg + a.ep_g[c] * w;
       for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
     });
@@ -2386,7 +2390,7 @@ __global__ __launch_bounds__(kBS) void k_gas_filter(DevParams P, StepPtrs a, con
     atomicAdd(&f_gds[c], f * wt);
   };
   if (filt)
-    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos,
+    for_filter(P, a.xe, a.yn, a.zt, a.csz, a.fluid, ci, cj, ck, pos,
                [&](int c, double w) { add(c, w / vol[c]); });
   else
     add(c0, 1.0 / vol[c0]);
@@ -2419,10 +2423,10 @@ __global__ __launch_bounds__(kBS) void k_drag_fc(DevParams P, StepPtrs a) {
   load3(a.vel, S, i, vel);
   double velfp[3] = {0.0, 0.0, 0.0}, lpf[3] = {0.0, 0.0, 0.0};
   if (P.interp_on && filter_scheme(P.interp)) {
-    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+    for_filter(P, a.xe, a.yn, a.zt, a.csz, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
       for (int d = 0; d < 3; d++) velfp[d] = velfp[d] + a.ugc[3 * c + d] * w;
     });
-    for_filter(P, a.xe, a.yn, a.zt, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
+    for_filter(P, a.xe, a.yn, a.zt, a.csz, a.fluid, ci, cj, ck, pos, [&](int c, double w) {
       for (int d = 0; d < 3; d++) lpf[d] = lpf[d] + a.p_force[3 * c + d] * w;
     });
   } else {
@@ -2458,7 +2462,7 @@ __global__ __launch_bounds__(kBS) void k_gas_fgds(int ni, int nj, int nk, const 
 // (I, J, K) for I <= imax+1, J <= jmax+1, K <= kmax+1, summed in the
 // reference's order; 0 at every other index.
 __global__ __launch_bounds__(kBS) void k_vol_node(int imax, int jmax, int kmax, int ni, int nj,
-                                                  int nk, const uint8_t* fluid, double v,
+                                                  int nk, const uint8_t* fluid, const double* csz,
                                                   double* vol_node) {
   const int node = blockIdx.x * kBS + threadIdx.x;
   if (node >= ni * nj * nk) return;
@@ -2467,8 +2471,11 @@ __global__ __launch_bounds__(kBS) void k_vol_node(int imax, int jmax, int kmax, 
   const int K = node / (nj * ni) + 1;
   double a = 0.0;
   if (I <= imax + 1 && J <= jmax + 1 && K <= kmax + 1) {
+    // dx(I)*dy(J)*dz(K) of each cell (cfassign.f:154-157)
     auto volf = [&](int i, int j, int k) {
-      return fluid[(j - 1) + (i - 1) * nj + (k - 1) * nj * ni] ? v : 0.0;
+      return fluid[(j - 1) + (i - 1) * nj + (k - 1) * nj * ni]
+                 ? csz[i - 1] * csz[ni + j - 1] * csz[ni + nj + k - 1]
+                 : 0.0;
     };
     const double af = 0.125;
     a = af * (volf(I, J, K) + volf(I + 1, J, K) + volf(I, J + 1, K) + volf(I + 1, J + 1, K));
@@ -2740,9 +2747,9 @@ __device__ __forceinline__ double calc_del_phi(bool valid_lo, bool valid_hi, dou
 // CALC_PG_GRAD (calc_pg_grad.f:20-110) / CALC_GRAD_DES (calc_grad_des.f:21-102):
 // P_FORCE = -grad P_G in fluid cells, 0 elsewhere (box walls: the valid
 // pressure cells are the fluid cells; no cyclic pressure drop).
-__global__ __launch_bounds__(kBS) void k_pg_grad(int ni, int nj, int nk, double dx, double dy,
-                                                 double dz, const uint8_t* fluid,
-                                                 const double* p_g, double* p_force) {
+__global__ __launch_bounds__(kBS) void k_pg_grad(int ni, int nj, int nk, const double* csz,
+                                                 const uint8_t* fluid, const double* p_g,
+                                                 double* p_force) {
   const int c = blockIdx.x * kBS + threadIdx.x;
   if (c >= ni * nj * nk) return;
   double* f = p_force + 3 * c;
@@ -2750,14 +2757,17 @@ __global__ __launch_bounds__(kBS) void k_pg_grad(int ni, int nj, int nk, double 
     f[0] = f[1] = f[2] = 0.0;
     return;
   }
-  // fluid cells are interior, so every neighbour index is on the grid
+  // fluid cells are interior, so every neighbour index is on the grid;
+  // DX(I), DX of the west and east neighbours (calc_grad_des.f:85-87)
   const int sj = 1, si = nj, sk = nj * ni;
-  const double g0 = calc_del_phi(fluid[c - si], fluid[c + si], p_g[c], dx, p_g[c - si], dx,
-                                 p_g[c + si], dx);
-  const double g1 = calc_del_phi(fluid[c - sj], fluid[c + sj], p_g[c], dy, p_g[c - sj], dy,
-                                 p_g[c + sj], dy);
-  const double g2 = calc_del_phi(fluid[c - sk], fluid[c + sk], p_g[c], dz, p_g[c - sk], dz,
-                                 p_g[c + sk], dz);
+  const int J = c % nj, I = (c / nj) % ni, K = c / sk;  // 0-based
+  const double *dx = csz + I, *dy = csz + ni + J, *dz = csz + ni + nj + K;
+  const double g0 = calc_del_phi(fluid[c - si], fluid[c + si], p_g[c], dx[0], p_g[c - si], dx[-1],
+                                 p_g[c + si], dx[1]);
+  const double g1 = calc_del_phi(fluid[c - sj], fluid[c + sj], p_g[c], dy[0], p_g[c - sj], dy[-1],
+                                 p_g[c + sj], dy[1]);
+  const double g2 = calc_del_phi(fluid[c - sk], fluid[c + sk], p_g[c], dz[0], p_g[c - sk], dz[-1],
+                                 p_g[c + sk], dz[1]);
   f[0] = -g0;
   f[1] = -g1;
   f[2] = -g2;

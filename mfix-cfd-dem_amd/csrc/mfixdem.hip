@@ -106,6 +106,7 @@ struct mfixdem_handle {
   // gas
   double* ugc = nullptr;  // cell-centre gas velocity (filter schemes)
   double* gas_stage = nullptr;  // 3*ncell: staged U_G/V_G/W_G, P_G or P_FORCE uploads
+  double* csz = nullptr;  // cell sizes DX(1..ni), DY(1..nj), DZ(1..nk)
   double *xe = nullptr, *yn = nullptr, *zt = nullptr, *vnode = nullptr, *p_force = nullptr,
          *ep_g = nullptr, *rop_s = nullptr, *rops_node = nullptr, *vol = nullptr,
          *vol_surr = nullptr, *ro_s = nullptr;
@@ -225,7 +226,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.fc = h->fc; a.tow = h->tow; a.fc_id = h->fc_id; a.tow_id = h->tow_id;
   a.acc_old = h->acc[t]; a.rot_acc_old = h->racc[t];
   a.fcell = h->fcell[t];
-  a.xe = h->xe; a.yn = h->yn; a.zt = h->zt;
+  a.xe = h->xe; a.yn = h->yn; a.zt = h->zt; a.csz = h->csz;
   a.vnode = h->vnode; a.p_force = h->p_force; a.ep_g = h->ep_g; a.fluid = h->fluid;
   a.ugc = h->ugc;
   a.flags = h->flags;
@@ -692,8 +693,8 @@ int32_t deposit_current(mfixdem_handle* h) {
   if (h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012) {
     hipLaunchKernelGGL(k_deposit_filter, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, P,
                        nullptr, h->pos[h->cur], h->pvol[h->tw], h->phase[h->tw],
-                       h->state[h->tw], h->fcell[h->tw], h->xe, h->yn, h->zt, h->fluid,
-                       h->rops_fix[h->fb], rm);
+                       h->state[h->tw], h->fcell[h->tw], h->xe, h->yn, h->zt, h->csz,
+                       h->fluid, h->rops_fix[h->fb], rm);
     HIPOK(hipGetLastError());
     return do_cell_fields(h, true);
   }
@@ -855,6 +856,10 @@ int32_t upload_setup(mfixdem_handle* h) {
   HIPOK(hipMemcpy(h->xe, S.xe.data(), sizeof(double) * S.xe.size(), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->yn, S.yn.data(), sizeof(double) * S.yn.size(), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->zt, S.zt.data(), sizeof(double) * S.zt.size(), hipMemcpyHostToDevice));
+  std::vector<double> csz(S.cdx);
+  csz.insert(csz.end(), S.cdy.begin(), S.cdy.end());
+  csz.insert(csz.end(), S.cdz.begin(), S.cdz.end());
+  HIPOK(hipMemcpy(h->csz, csz.data(), sizeof(double) * csz.size(), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vol, S.vol.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->vol_surr, S.vol_surr.data(), sizeof(double) * S.ncell, hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(h->fluid, S.fluid.data(), S.ncell, hipMemcpyHostToDevice));
@@ -936,7 +941,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->xr_pack);
   f(h->fstart); f(h->fend);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
-  f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->xe); f(h->yn);
+  f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->csz); f(h->xe); f(h->yn);
   f(h->zt); f(h->vnode); f(h->ugc); f(h->gas_stage); f(h->p_force); f(h->ep_g); f(h->rop_s);
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
@@ -980,6 +985,7 @@ int32_t alloc_buffers(mfixdem_handle* h) {
       (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
       (rc = dalloc(&h->scan_bsum, std::max<int64_t>(n, nc) / kScanTile + 2)) ||
       (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
+      (rc = dalloc(&h->csz, h->S.ni + h->S.nj + h->S.nk)) ||
       (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
       (rc = dalloc(&h->vnode, 3 * nc)) || (rc = dalloc(&h->p_force, 3 * nc)) ||
       (rc = dalloc(&h->ugc, 3 * nc)) || (rc = dalloc(&h->gas_stage, 3 * nc)) ||
@@ -1174,7 +1180,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   {
     double rho_max[MFIXDEM_MAX_PHASES] = {0, 0, 0, 0};
     for (int64_t q = 0; q < n; q++) rho_max[PH[q]] = std::max(rho_max[PH[q]], ro_sol[q]);
-    const double vcell = h->S.dx * h->S.dy * h->S.dz;
+    const double vcell = h->S.vol_max;
     // the filter schemes sum solids volume into cells: bounded by the
     // volume of the 27 cells a weight can come from
     const bool vols = h->S.p.interp_scheme != MFIXDEM_INTERP_GARG_2012;
@@ -1330,8 +1336,8 @@ int32_t mfixdem_set_gas_pressure(mfixdem_handle* h, const double* p_g) {
   HIPOK(sync_stream(h));  // queued steps may still read P_FORCE
   HIPOK(hipMemcpyAsync(h->gas_stage, p_g, sizeof(double) * nc, hipMemcpyHostToDevice,
                        h->stream));
-  hipLaunchKernelGGL(k_pg_grad, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk, S.dx,
-                     S.dy, S.dz, h->fluid, h->gas_stage, h->p_force);
+  hipLaunchKernelGGL(k_pg_grad, dim3(nblk(nc)), dim3(kBS), 0, h->stream, S.ni, S.nj, S.nk, h->csz,
+                     h->fluid, h->gas_stage, h->p_force);
   HIPOK(hipGetLastError());
   HIPOK(sync_stream(h));
   invalidate_graphs(h);
@@ -1472,17 +1478,19 @@ int32_t mfixdem_comp_mean_fields(mfixdem_handle* h) {
   return deposit_current(h);
 }
 
-// DES_VOL_NODE (compute_volume_of_nodes, cfassign.f:100-230) of the uniform
-// grid, in the reference's sum order, and the buffers of the gas-side drag.
+// DES_VOL_NODE (compute_volume_of_nodes, cfassign.f:100-230) with each
+// cell's dx(I)*dy(J)*dz(K), in the reference's sum order, and the buffers
+// of the gas-side drag.
 int32_t ensure_gas_drag(mfixdem_handle* h) {
   if (h->gd_am) return MFIXDEM_OK;
-  const mfixdem_params& p = h->S.p;
-  const int ni = h->S.ni, nj = h->S.nj, nc = h->S.ncell;
-  const double dx = p.xlength / (double)p.imax, dy = p.ylength / (double)p.jmax,
-               dz = p.zlength / (double)p.kmax;
-  const double v = dx * dy * dz, af = 0.125;
+  const Setup& S = h->S;
+  const mfixdem_params& p = S.p;
+  const int ni = S.ni, nj = S.nj, nc = S.ncell;
+  const double af = 0.125;
   auto cell = [&](int i, int j, int k) { return (j - 1) + (i - 1) * nj + (k - 1) * nj * ni; };
-  auto volf = [&](int i, int j, int k) { return h->S.fluid[cell(i, j, k)] ? v : 0.0; };
+  auto volf = [&](int i, int j, int k) {
+    return S.fluid[cell(i, j, k)] ? S.cdx[i - 1] * S.cdy[j - 1] * S.cdz[k - 1] : 0.0;
+  };
   std::vector<double> vn((size_t)nc, 0.0);
   for (int k = 1; k <= p.kmax + 1; k++)
     for (int j = 1; j <= p.jmax + 1; j++)
@@ -1590,10 +1598,8 @@ int32_t mfixdem_drag_gs_gas_noalloc(mfixdem_handle* h, double* f_gds, double* dr
   }
   double* rec = h->scratch_ptr;                          // kGasRec * n, all written
   double* vol_node = h->scratch_ptr + (int64_t)kGasRec * h->n;  // nc, all written
-  const double v = (p.xlength / (double)p.imax) * (p.ylength / (double)p.jmax) *
-                   (p.zlength / (double)p.kmax);
   hipLaunchKernelGGL(k_vol_node, dim3(nblk(nc)), dim3(kBS), 0, h->stream, p.imax, p.jmax, p.kmax,
-                     h->S.ni, h->S.nj, h->S.nk, h->fluid, v, vol_node);
+                     h->S.ni, h->S.nj, h->S.nk, h->fluid, h->csz, vol_node);
   CHECKRC(do_pic_lists(h));
   if (h->n > 0)
     hipLaunchKernelGGL(k_gas_fgp, dim3(nblk(h->n)), dim3(kBS), 0, h->stream, h->P, step_ptrs(h),

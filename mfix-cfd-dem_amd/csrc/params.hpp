@@ -72,7 +72,6 @@ inline void fill_params(const Setup& S, int n, int stride, DevParams& P, GridDev
   P.ro_g0 = p.ro_g0; P.mu_g0 = p.mu_g0;
   P.interp = p.interp_scheme;
   P.interp_on = p.des_interp_on;
-  P.dxc[0] = S.dx; P.dxc[1] = S.dy; P.dxc[2] = S.dz;
   if (p.interp_scheme == MFIXDEM_INTERP_SQUARE_DPVM ||
       p.interp_scheme == MFIXDEM_INTERP_GAUSS_DPVM) {
     // set_filter_des.f:85,99-100

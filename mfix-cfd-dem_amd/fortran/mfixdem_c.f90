@@ -8,7 +8,7 @@ module mfixdem_c
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: MFIXDEM_ABI_VERSION = 2
+  integer(c_int32_t), parameter :: MFIXDEM_ABI_VERSION = 3
   integer, parameter :: MFIXDEM_MAX_PHASES = 4
   integer, parameter :: MFIXDEM_MAX_PAIRS_IN = 10
   real(c_double), parameter :: MFIXDEM_UNDEFINED = 9.87654321d31
@@ -45,6 +45,7 @@ module mfixdem_c
      real(c_double) :: max_radius, ro_g0, mu_g0, des_epg_clip
      real(c_double) :: des_interp_width
      integer(c_int32_t) :: des_neighbor_search, des_explicitly_coupled
+     type(c_ptr) :: dx, dy, dz      ! c_loc(DX(IMIN1)) ... or c_null_ptr (uniform)
   end type mfixdem_params
 
   type, bind(C) :: mfixdem_status

@@ -15,7 +15,7 @@ import numpy as np
 MAX_PHASES = 4
 MAX_PAIRS_IN = MAX_PHASES * (MAX_PHASES + 1) // 2
 UNDEFINED = 9.87654321e31
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 LSD, HERTZ = 0, 1
 EULER, ADAMS_BASHFORTH = 0, 1
@@ -62,7 +62,28 @@ class Params(C.Structure):
         ("d_p0", _D * MAX_PHASES), ("ro_s0", _D * MAX_PHASES),
         ("max_radius", _D), ("ro_g0", _D), ("mu_g0", _D), ("des_epg_clip", _D),
         ("des_interp_width", _D), ("des_neighbor_search", _I), ("des_explicitly_coupled", _I),
+        ("dx", C.POINTER(C.c_double)), ("dy", C.POINTER(C.c_double)),
+        ("dz", C.POINTER(C.c_double)),
     ]
+
+    def set_cell_sizes(self, dx=None, dy=None, dz=None):
+        """Non-uniform fluid grid: DX(1..IMAX), DY(1..JMAX), DZ(1..KMAX) (None
+        keeps an axis uniform).  The arrays are kept with the parameters, and
+        XLENGTH ... are set to their sums."""
+        keep = []
+        for name, arr, n, lname in (("dx", dx, self.imax, "xlength"),
+                                    ("dy", dy, self.jmax, "ylength"),
+                                    ("dz", dz, self.kmax, "zlength")):
+            if arr is None:
+                setattr(self, name, C.POINTER(C.c_double)())
+                continue
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            if a.shape != (n,):
+                raise ValueError(f"{name} needs {n} cell sizes, got {a.shape}")
+            keep.append(a)
+            setattr(self, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+            setattr(self, lname, float(a.sum()))
+        self._cell_sizes = keep
 
 
 class Derived(C.Structure):

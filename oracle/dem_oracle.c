@@ -42,7 +42,10 @@ struct orc_sys {
   double kt, kt_w;
   /* fluid grid (MFiX serial: cells 1..IMAX+2, faces XE(0:IMAX+2)) */
   int ni, nj, nk, ncell;
-  double dx, dy, dz;
+  /* DX(I) of cell I = 1..ni at cdx[I-1] (ghosts included); smallest
+     interior sizes */
+  double *cdx, *cdy, *cdz;
+  double dx_min, dy_min, dz_min;
   double *xe, *yn, *zt;
   double *vol, *vol_surr;
   unsigned char* fluid;
@@ -324,30 +327,45 @@ static void derive_hertz(orc_sys* s) {
 }
 
 /* -------------------------------------------------------- geometry set-up */
-/* Fluid faces: CHECK_AXIS uniform DA = ALENGTH/DBLE(NA) (check_axis.f:152-156),
-   SET_GEOMETRY_DES cumulative faces (set_geometry_des.f:72-87); VOL = DX*DY*DZ
-   (set_geometry1.f:125 with X=1); VOL_SURR (make_arrays_des.f:78-101). */
+/* Cell sizes of one axis: the interior sizes given (DX(0:IMAX-1) of the
+   namelist, shifted to DX(2..IMAX+1), shift_dxyz.f:36-44) or CHECK_AXIS's
+   uniform DA = ALENGTH/DBLE(NA) (check_axis.f:152-156); the ghost cells
+   take their neighbour's size.  Returns the array (n + 2 entries). */
+static double* cell_sizes(int n, const double* in, double len, double* mn) {
+  double* c = calloc((size_t)n + 2, sizeof(double));
+  for (int i = 1; i <= n; i++) c[i] = in ? in[i - 1] : len / (double)n;
+  c[0] = c[1];
+  c[n + 1] = c[n];
+  *mn = c[1];
+  for (int i = 1; i <= n; i++) *mn = c[i] < *mn ? c[i] : *mn;
+  return c;
+}
+
+/* SET_GEOMETRY_DES cumulative faces XE(0) = X_MIN - DX(1), XE(I) = XE(I-1)
+   + DX(I) (set_geometry_des.f:72-87); VOL = DX*DY*DZ (set_geometry1.f:125
+   with X=1); VOL_SURR (make_arrays_des.f:78-101). */
 static void setup_fluid_grid(orc_sys* s) {
   const mfixdem_params* p = &s->p;
   s->ni = p->imax + 2;
   s->nj = p->jmax + 2;
   s->nk = p->kmax + 2;
   s->ncell = s->ni * s->nj * s->nk;
-  s->dx = p->xlength / (double)p->imax;
-  s->dy = p->ylength / (double)p->jmax;
-  s->dz = p->zlength / (double)p->kmax;
+  s->cdx = cell_sizes(p->imax, p->dx, p->xlength, &s->dx_min);
+  s->cdy = cell_sizes(p->jmax, p->dy, p->ylength, &s->dy_min);
+  s->cdz = cell_sizes(p->kmax, p->dz, p->zlength, &s->dz_min);
+  s->p.dx = s->p.dy = s->p.dz = NULL;  /* copied */
   s->x_max = p->x_min + p->xlength;
   s->y_max = p->y_min + p->ylength;
   s->z_max = p->z_min + p->zlength;
   s->xe = calloc(s->ni + 1, sizeof(double));
   s->yn = calloc(s->nj + 1, sizeof(double));
   s->zt = calloc(s->nk + 1, sizeof(double));
-  s->xe[0] = p->x_min - s->dx;
-  for (int i = 1; i <= s->ni; i++) s->xe[i] = s->xe[i - 1] + s->dx;
-  s->yn[0] = p->y_min - s->dy;
-  for (int j = 1; j <= s->nj; j++) s->yn[j] = s->yn[j - 1] + s->dy;
-  s->zt[0] = p->z_min - s->dz;
-  for (int k = 1; k <= s->nk; k++) s->zt[k] = s->zt[k - 1] + s->dz;
+  s->xe[0] = p->x_min - s->cdx[0];
+  for (int i = 1; i <= s->ni; i++) s->xe[i] = s->xe[i - 1] + s->cdx[i - 1];
+  s->yn[0] = p->y_min - s->cdy[0];
+  for (int j = 1; j <= s->nj; j++) s->yn[j] = s->yn[j - 1] + s->cdy[j - 1];
+  s->zt[0] = p->z_min - s->cdz[0];
+  for (int k = 1; k <= s->nk; k++) s->zt[k] = s->zt[k - 1] + s->cdz[k - 1];
   s->vol = calloc(s->ncell, sizeof(double));
   s->vol_surr = calloc(s->ncell, sizeof(double));
   s->fluid = calloc(s->ncell, 1);
@@ -355,7 +373,7 @@ static void setup_fluid_grid(orc_sys* s) {
     for (int i = 1; i <= s->ni; i++)
       for (int j = 1; j <= s->nj; j++) {
         int ijk = fluid_ijk(s, i, j, k) - 1;
-        s->vol[ijk] = s->dx * s->dy * (1.0 * s->dz);
+        s->vol[ijk] = s->cdx[i - 1] * s->cdy[j - 1] * (1.0 * s->cdz[k - 1]);
         s->fluid[ijk] = (i >= 2 && i <= p->imax + 1 && j >= 2 && j <= p->jmax + 1 &&
                          k >= 2 && k <= p->kmax + 1);
       }
@@ -379,9 +397,10 @@ static void setup_des_grid(orc_sys* s) {
   double max_diam = 2.0 * p->max_radius;
   double width = 3.0 * max_diam;
   int gi = p->desgridsearch_imax, gj = p->desgridsearch_jmax, gk = p->desgridsearch_kmax;
-  double wx = width < s->dx ? width : s->dx;
-  double wy = width < s->dy ? width : s->dy;
-  double wz = width < s->dz ? width : s->dz;
+  /* WIDTH_X = min(WIDTH, minval(DX)) (check_geometry.f:137-144) */
+  double wx = width < s->dx_min ? width : s->dx_min;
+  double wy = width < s->dy_min ? width : s->dy_min;
+  double wz = width < s->dz_min ? width : s->dz_min;
   if (gi <= 0) { gi = (int)(p->xlength / wx); if (gi < 1) gi = 1; }
   if (gj <= 0) { gj = (int)(p->ylength / wy); if (gj < 1) gj = 1; }
   if (gk <= 0) { gk = (int)(p->zlength / wz); if (gk < 1) gk = 1; }
@@ -714,7 +733,7 @@ static int check_interp(const orc_sys* s) {
     case MFIXDEM_INTERP_SQUARE_DPVM:
     case MFIXDEM_INTERP_GAUSS_DPVM: {
       if (w == UNDEF) return 1;
-      double dmin = fmin(fmin(s->dx, s->dy), s->dz);
+      double dmin = fmin(fmin(s->dx_min, s->dy_min), s->dz_min);
       double dgmin = fmin(fmin(1.0 / s->dg_dxinv, 1.0 / s->dg_dyinv), 1.0 / s->dg_dzinv);
       return 0.5 * w > dmin || 0.5 * w > dgmin;
     }
@@ -731,6 +750,19 @@ orc_sys* orc_create(const mfixdem_params* p) {
   /* explicit coupling: DRAG_GS_DES1's F_GP path, filter schemes only */
   if (p->des_explicitly_coupled && (!p->coupled || p->interp_scheme == MFIXDEM_INTERP_GARG_2012))
     return NULL;
+  /* non-uniform cell sizes: positive, summing to the domain length */
+  const double* in[3] = {p->dx, p->dy, p->dz};
+  const int na[3] = {p->imax, p->jmax, p->kmax};
+  const double len[3] = {p->xlength, p->ylength, p->zlength};
+  for (int a = 0; a < 3; a++) {
+    if (!in[a]) continue;
+    double sum = 0.0;
+    for (int i = 0; i < na[a]; i++) {
+      if (!(in[a][i] > 0.0) || !isfinite(in[a][i])) return NULL;
+      sum += in[a][i];
+    }
+    if (fabs(sum - len[a]) > 1e-12 * fabs(len[a])) return NULL;
+  }
   orc_sys* s = calloc(1, sizeof(orc_sys));
   s->p = *p;
   if (p->coll_model == MFIXDEM_HERTZ) derive_hertz(s);
@@ -790,7 +822,7 @@ orc_sys* orc_create(const mfixdem_params* p) {
 
 void orc_destroy(orc_sys* s) {
   if (!s) return;
-  void* ptrs[] = {s->xe, s->yn, s->zt, s->vol, s->vol_surr, s->fluid, s->dg_cnt, s->dg_off,
+  void* ptrs[] = {s->cdx, s->cdy, s->cdz, s->xe, s->yn, s->zt, s->vol, s->vol_surr, s->fluid, s->dg_cnt, s->dg_off,
                   s->dg_list,
                   s->pos, s->vel, s->omega, s->radius, s->ro_sol, s->pvol, s->pmass,
                   s->omoi, s->phase, s->state, s->fc, s->tow, s->fc_last, s->tow_last,
@@ -933,12 +965,15 @@ static void interp_weights1(orc_sys* s, int64_t q) {
 
 /* one axis of CALC_INTERP_WEIGHTS2 (calc_interp_weights.f:365-460): hat
    weights about the cell centre, own cell if the neighbour is not fluid */
-static void lhat_axis(double xpos, double face_lo, double d, int fluid_m, int fluid_p,
+/* c = &DX(I) of the particle's cell I, c[-1] and c[1] its neighbours';
+   lOoDX = oDX_E(I-1) or oDX_E(I), oDX_E(L) = ONE/(HALF*(DX(L+1)+DX(L)))
+   (set_geometry.f:341-343) */
+static void lhat_axis(double xpos, double face_lo, const double* c, int fluid_m, int fluid_p,
                       double sw[3]) {
-  double xc = face_lo + 0.5 * d;
-  double oo = 1.0 / d; /* oDX_E = ONE/(HALF*(DX+DX)) on a uniform grid */
+  double xc = face_lo + 0.5 * c[0];
   if (xpos < xc) {
     if (fluid_m) {
+      double oo = 1.0 / (0.5 * (c[0] + c[-1]));
       sw[0] = (xc - xpos) * oo;
       sw[1] = 1.0 - sw[0];
       sw[2] = 0.0;
@@ -947,6 +982,7 @@ static void lhat_axis(double xpos, double face_lo, double d, int fluid_m, int fl
     }
   } else {
     if (fluid_p) {
+      double oo = 1.0 / (0.5 * (c[1] + c[0]));
       sw[0] = 0.0;
       sw[2] = (xpos - xc) * oo;
       sw[1] = 1.0 - sw[2];
@@ -968,11 +1004,11 @@ static void interp_weights2(orc_sys* s, int64_t q) {
   int i = s->pijk[q], j = s->pijk[L + q], k = s->pijk[2 * L + q];
   int ijk = s->pijk[3 * L + q] - 1;
   double sx[3], sy[3], sz[3];
-  lhat_axis(s->pos[q], s->xe[i - 1], s->dx, fluid_at(s, i - 1, j, k), fluid_at(s, i + 1, j, k),
-            sx);
-  lhat_axis(s->pos[L + q], s->yn[j - 1], s->dy, fluid_at(s, i, j - 1, k),
+  lhat_axis(s->pos[q], s->xe[i - 1], s->cdx + (i - 1), fluid_at(s, i - 1, j, k),
+            fluid_at(s, i + 1, j, k), sx);
+  lhat_axis(s->pos[L + q], s->yn[j - 1], s->cdy + (j - 1), fluid_at(s, i, j - 1, k),
             fluid_at(s, i, j + 1, k), sy);
-  lhat_axis(s->pos[2 * L + q], s->zt[k - 1], s->dz, fluid_at(s, i, j, k - 1),
+  lhat_axis(s->pos[2 * L + q], s->zt[k - 1], s->cdz + (k - 1), fluid_at(s, i, j, k - 1),
             fluid_at(s, i, j, k + 1), sz);
   int idx = 0;
   for (int kc = 0; kc <= 2; kc++)
@@ -1055,16 +1091,17 @@ int orc_set_gas_pressure(orc_sys* s, const double* p_g) {
           continue;
         }
         int lo = fluid_ijk(s, i - 1, j, k) - 1, hi = fluid_ijk(s, i + 1, j, k) - 1;
-        f[0] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dx, p_g[lo], s->dx,
-                            p_g[hi], s->dx);
+        /* dx(i), dx of the west and east cells (calc_grad_des.f:85-87) */
+        f[0] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->cdx[i - 1], p_g[lo],
+                            s->cdx[i - 2], p_g[hi], s->cdx[i]);
         lo = fluid_ijk(s, i, j - 1, k) - 1;
         hi = fluid_ijk(s, i, j + 1, k) - 1;
-        f[1] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dy, p_g[lo], s->dy,
-                            p_g[hi], s->dy);
+        f[1] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->cdy[j - 1], p_g[lo],
+                            s->cdy[j - 2], p_g[hi], s->cdy[j]);
         lo = fluid_ijk(s, i, j, k - 1) - 1;
         hi = fluid_ijk(s, i, j, k + 1) - 1;
-        f[2] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->dz, p_g[lo], s->dz,
-                            p_g[hi], s->dz);
+        f[2] = calc_del_phi(s->fluid[lo], s->fluid[hi], p_g[ijk], s->cdz[k - 1], p_g[lo],
+                            s->cdz[k - 2], p_g[hi], s->cdz[k]);
         for (int d = 0; d < 3; d++) f[d] = -f[d];
       }
   return 0;
@@ -2138,8 +2175,8 @@ int orc_comp_mean_fields(orc_sys* s) {
 /* DES_VOL_NODE, compute_volume_of_nodes (cfassign.f:100-230): the volume
    around each grid node (i, j, k) in [1, imax+1] x [1, jmax+1] x [1, kmax+1]
    from the fluid cells of its 2x2x2 block, in the reference's sum order
-   (the upper K layer is added as ijkp, ipjpkp, ijpkp, ipjkp).  Uniform grid:
-   every cell has the volume dx*dy*dz. */
+   (the upper K layer is added as ijkp, ipjpkp, ijpkp, ipjkp), each cell's
+   volume dx(I)*dy(J)*dz(K) (:154-157). */
 static void des_vol_node(const orc_sys* s, double* vn) {
   const mfixdem_params* p = &s->p;
   const double af = 0.125;
@@ -2147,8 +2184,8 @@ static void des_vol_node(const orc_sys* s, double* vn) {
   for (int k = 1; k <= p->kmax + 1; k++)
     for (int j = 1; j <= p->jmax + 1; j++)
       for (int i = 1; i <= p->imax + 1; i++) {
-        const double v = s->dx * s->dy * s->dz;
-#define VOLF(I, J, K) (fluid_at(s, (I), (J), (K)) ? v : 0.0)
+#define VOLF(I, J, K) \
+  (fluid_at(s, (I), (J), (K)) ? s->cdx[(I)-1] * s->cdy[(J)-1] * s->cdz[(K)-1] : 0.0)
         double a = af * (VOLF(i, j, k) + VOLF(i + 1, j, k) + VOLF(i, j + 1, k) +
                          VOLF(i + 1, j + 1, k));
         a = a + af * (VOLF(i, j, k + 1) + VOLF(i + 1, j + 1, k + 1) + VOLF(i, j + 1, k + 1) +

@@ -44,26 +44,48 @@ class LinearField:
         return self.c0 + self.grad @ x
 
 
+def cell_sizes(p):
+    """Per axis, the sizes of cells 1..N+2 (index cell - 1): the given
+    interior sizes (p.dx ...) or the uniform ALENGTH/N, ghosts copying
+    their neighbour."""
+    out = []
+    for ptr, n, length in ((p.dx, p.imax, p.xlength), (p.dy, p.jmax, p.ylength),
+                           (p.dz, p.kmax, p.zlength)):
+        inner = (np.array([ptr[i] for i in range(n)]) if ptr else
+                 np.full(n, length / n))
+        out.append(np.concatenate([[inner[0]], inner, [inner[-1]]]))
+    return out
+
+
 def grid(p):
     """Face arrays XE(0..IMAX+2), YN, ZT (0-based index = face index) and
-    the cell size of a uniform grid starting at X_MIN etc."""
-    d = np.array([p.xlength / p.imax, p.ylength / p.jmax, p.zlength / p.kmax])
+    the cell sizes of each axis (cell_sizes)."""
+    dc = cell_sizes(p)
     lo = np.array([p.x_min, p.y_min, p.z_min])
-    n = [p.imax, p.jmax, p.kmax]
-    f = [lo[a] - d[a] + d[a] * np.arange(n[a] + 3) for a in range(3)]
-    return f, d
+    f = []
+    for a in range(3):
+        fa = [lo[a] - dc[a][0]]
+        for c in dc[a]:
+            fa.append(fa[-1] + c)
+        f.append(np.array(fa))
+    return f, dc
+
+
+def centre(f, dc, a, cell):
+    """Centre of (1-based) cell `cell` along axis a."""
+    return f[a][cell - 1] + 0.5 * dc[a][cell - 1]
 
 
 def gas_fields(p, vel: LinearField, pforce: LinearField):
     """Gas with U_G/V_G/W_G sampled on their staggered faces and P_FORCE at
     the cell centres (MFiX serial layout, J fastest, then I, then K)."""
-    f, d = grid(p)
+    f, dc = grid(p)
     ni, nj, nk = p.imax + 2, p.jmax + 2, p.kmax + 2
     K, I, J = np.meshgrid(np.arange(1, nk + 1), np.arange(1, ni + 1), np.arange(1, nj + 1),
                           indexing="ij")
-    xc = f[0][I - 1] + 0.5 * d[0]
-    yc = f[1][J - 1] + 0.5 * d[1]
-    zc = f[2][K - 1] + 0.5 * d[2]
+    xc = centre(f, dc, 0, I)
+    yc = centre(f, dc, 1, J)
+    zc = centre(f, dc, 2, K)
     u = vel.c0[0] + vel.grad[0, 0] * f[0][I] + vel.grad[0, 1] * yc + vel.grad[0, 2] * zc
     v = vel.c0[1] + vel.grad[1, 0] * xc + vel.grad[1, 1] * f[1][J] + vel.grad[1, 2] * zc
     w = vel.c0[2] + vel.grad[2, 0] * xc + vel.grad[2, 1] * yc + vel.grad[2, 2] * f[2][K]
@@ -88,21 +110,25 @@ def sphere_fraction_beyond(dist, r):
 def expected_at(p, x, cell, field: LinearField):
     """Closed-form interpolated value of `field` at particle position x in
     fluid cell `cell` = (I, J, K) (1-based) for the configured scheme."""
-    f, d = grid(p)
+    f, dc = grid(p)
     x = np.asarray(x, dtype=np.float64)
-    centre = np.array([f[a][cell[a] - 1] + 0.5 * d[a] for a in range(3)])
+    c = np.array([centre(f, dc, a, cell[a]) for a in range(3)])
     scheme = p.interp_scheme
     if not p.des_interp_on or scheme == abi.INTERP_NONE:
-        return field(centre)
+        return field(c)
     if scheme == abi.INTERP_LINEAR_HAT:
         return field(x)
-    # SQUARE_DPVM / GAUSS_DPVM: sphere of radius DES_INTERP_WIDTH/2
+    # SQUARE_DPVM / GAUSS_DPVM: sphere of radius DES_INTERP_WIDTH/2; the
+    # weights of an axis sum to 1, so a linear field is read at the
+    # weighted mean of the three cell centres
     r = 0.5 * p.des_interp_width
-    out = field(centre).copy()
+    out = field(c).copy()
     for a in range(3):
         w_lo = sphere_fraction_beyond(x[a] - f[a][cell[a] - 1], r)
         w_hi = sphere_fraction_beyond(f[a][cell[a]] - x[a], r)
-        out = out + field.grad[:, a] * d[a] * (w_hi - w_lo)
+        c_lo = centre(f, dc, a, cell[a] - 1)
+        c_hi = centre(f, dc, a, cell[a] + 1)
+        out = out + field.grad[:, a] * (w_hi * (c_hi - c[a]) - w_lo * (c[a] - c_lo))
     return out
 
 
@@ -110,7 +136,7 @@ def sparse_particles(p, n, seed, v_scale=0.3):
     """n spheres of the bed's d_p0 in cells whose 26 neighbours are fluid,
     far enough apart that no pair is listed and away from every wall."""
     rng = np.random.default_rng(seed)
-    f, d = grid(p)
+    f, _ = grid(p)
     r = 0.5 * p.d_p0[0]
     lo = np.array([f[a][2] for a in range(3)]) + r
     hi = np.array([f[a][[p.imax, p.jmax, p.kmax][a]] for a in range(3)]) - r
@@ -125,7 +151,7 @@ def sparse_particles(p, n, seed, v_scale=0.3):
 
 
 def cell_of(p, x):
-    f, d = grid(p)
+    f, _ = grid(p)
     return tuple(int(np.searchsorted(f[a], x[a], side="right")) for a in range(3))
 
 
@@ -198,10 +224,24 @@ def _lin_fields():
 LIN_VEL, LIN_PF = _lin_fields()
 
 
-def linear_case(scheme, interp_on, width, drag=abi.DRAG_GIDASPOW, n=24, seed=9):
-    """A sparse bed (no pairs, no wall contacts) in the linear gas field."""
+def stretched_sizes(n, length, ratio):
+    """n cell sizes growing geometrically by `ratio`, summing to length."""
+    g = ratio ** np.arange(n)
+    d = length * g / g.sum()
+    d[-1] = length - d[:-1].sum()
+    return d
+
+
+def linear_case(scheme, interp_on, width, drag=abi.DRAG_GIDASPOW, n=24, seed=9,
+                stretch=None):
+    """A sparse bed (no pairs, no wall contacts) in the linear gas field;
+    stretch = growth ratio of a non-uniform grid (None: uniform)."""
     p, _, _ = workloads.fluidised_bed(n=8, seed=seed, lx=LX, ly=LY, lz=LZ, cells=CELLS,
                                       dg=(0, 0, 0))
+    if stretch is not None:
+        p.set_cell_sizes(stretched_sizes(p.imax, p.xlength, stretch),
+                         stretched_sizes(p.jmax, p.ylength, 1.0 / stretch),
+                         stretched_sizes(p.kmax, p.zlength, stretch ** 0.5))
     p.max_pip = n
     p.max_radius = 5.0e-4
     p.drag_type = drag

@@ -36,10 +36,12 @@ def test_params_layout_and_defaults_match_c():
     assert lib.mfixdem_params_default(C.byref(c)) == 0
     py = abi.default_params()
     assert C.sizeof(abi.Params) == 20 * 4 + 8 * (8 + 3 + 9 + 2 * 10 + 2 * 4 + 2 * 4 + 2 + 2 * 4 + 4) \
-        + 8 + 2 * 4
+        + 8 + 2 * 4 + 3 * 8
     for name, _t in abi.Params._fields_:
         a, b = getattr(c, name), getattr(py, name)
-        if hasattr(a, "__len__"):
+        if name in ("dx", "dy", "dz"):
+            assert not a and not b, name      # NULL: uniform cells
+        elif hasattr(a, "__len__"):
             assert list(a) == list(b), name
         else:
             assert a == b, name

@@ -295,6 +295,26 @@ def test_fortran_driver_through_c_abi():
     assert np.array_equal(fy, np.array(ys))
 
 
+def test_udf_overrides_run_dem01():
+    """The MFiX UDF overrides (bridge, CALC_FORCE_DEM, CFNEWVALUES,
+    NEIGHBOUR, ...; built against stand-ins of MFiX's modules by
+    tests/fortran/Makefile) run DEM01 through MFiX's variables and routine
+    calls to the oracle's trajectory bit for bit."""
+    exe = os.path.join(_paths.ROOT, "tests", "fortran", "_build", "udf_dem01")
+    out = subprocess.run([exe, "10000", "0.9", "300"], capture_output=True, text=True,
+                         timeout=120, check=True).stdout.split("\n")
+    fy = np.array([float(l.split()[1]) for l in out if l.strip()])
+    p, parts, _ = workloads.dem01(kn=1e4, en=0.9)
+    o = OracleSystem(p)
+    o.set_particles(parts)
+    o.init()
+    ys = []
+    for nn in range(1, 301):
+        o.step(nn)
+        ys.append(o.get_particles()[0][1, 0])
+    assert np.array_equal(fy, np.array(ys))
+
+
 def test_mixed_precision_force_error():
     """FP32 contact and drag arithmetic: max relative force error against
     the FP64 path on the same state <= 1e-4 (error / max |F|)."""

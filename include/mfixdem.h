@@ -235,6 +235,38 @@ int32_t mfixdem_set_facets(mfixdem_handle* h, int32_t nf, const double* vertex,
 int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
                                 const double* vel, const double* omega,
                                 const double* rot_center);
+/* DEM mass inlets and outlets on faces of the box: BC_TYPE MASS_INFLOW,
+   MASS_OUTFLOW, or another outflow (P_OUTFLOW, OUTFLOW: exiting particles
+   keep only their velocity normal to the boundary) for the DEM solids
+   (des_bc_mod.f, set_bc_dem_mi.f:18-377, layout_mi_dem.f:88-444,
+   set_bc_dem_mo.f:32-177).  plane is BC_PLANE: the side of the boundary
+   the domain lies on (a south-wall inlet is MFIXDEM_PLANE_N). */
+enum { MFIXDEM_BC_MASS_INFLOW = 1, MFIXDEM_BC_MASS_OUTFLOW = 2, MFIXDEM_BC_OUTFLOW = 3 };
+enum { MFIXDEM_PLANE_W = 0, MFIXDEM_PLANE_E = 1, MFIXDEM_PLANE_S = 2, MFIXDEM_PLANE_N = 3,
+       MFIXDEM_PLANE_B = 4, MFIXDEM_PLANE_T = 5 };
+typedef struct mfixdem_dem_bc {
+  int32_t type;                 /* MFIXDEM_BC_*                                 */
+  int32_t plane;                /* MFIXDEM_PLANE_* (BC_PLANE)                   */
+  double x_w, x_e, y_s, y_n, z_b, z_t;  /* BC_X_w ... BC_Z_t: a rectangle in a
+                                   face of the box (one pair equal)           */
+  double u_s[MFIXDEM_MAX_PHASES];       /* BC_U_s, BC_V_s, BC_W_s per DEM phase */
+  double v_s[MFIXDEM_MAX_PHASES];
+  double w_s[MFIXDEM_MAX_PHASES];
+  double ep_s[MFIXDEM_MAX_PHASES];      /* BC_EP_s (inlets; 0 = phase absent)   */
+  double massflow_s[MFIXDEM_MAX_PHASES];/* BC_MASSFLOW_s; UNDEFINED: from the
+                                   velocity, BC_EP_s and the area             */
+  double mi_start_time, mi_end_time;    /* BC_MI_START_TIME / _END_TIME (inlets) */
+} mfixdem_dem_bc;
+/* The DEM boundaries of a run, before mfixdem_init; seed starts the random
+   stream that stands in for RANDOM_NUMBER (inlet layout permutation, phase
+   and position in the window).  Each step then runs MASS_INFLOW_DEM and
+   MASS_OUTFLOW_DEM (des_time_march.f:404-405): particles ENTERING (2) and
+   EXITING (3) move without forces or walls (cfnewvalues.f:700-720), new
+   particles take the lowest NONEXISTENT slots (mass_inflow_dem.f:206-222),
+   a deleted particle forces the neighbour search.  Euler integration,
+   single-system handles.  Steps with boundaries wait for the host once. */
+int32_t mfixdem_set_dem_bcs(mfixdem_handle* h, int32_t n, const mfixdem_dem_bc* bcs,
+                            uint64_t seed);
 /* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal). */
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
                               double* omega, int8_t* state, int64_t ld);

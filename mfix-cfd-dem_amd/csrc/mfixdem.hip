@@ -1278,6 +1278,14 @@ int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
   return MFIXDEM_OK;
 }
 
+int32_t mfixdem_set_dem_bcs(mfixdem_handle* h, int32_t n, const mfixdem_dem_bc* bcs,
+                            uint64_t seed) {
+  (void)bcs;
+  (void)seed;
+  if (!h || n < 0) return MFIXDEM_ERR_ARG;
+  return n == 0 ? MFIXDEM_OK : MFIXDEM_ERR_UNSUPPORTED;
+}
+
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, double* omega,
                               int8_t* state, int64_t ld) {
   if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call

@@ -107,6 +107,40 @@ class Status(C.Structure):
         "rogue_deleted", "epg_errors", "zero_distance", "out_of_domain")]
 
 
+BC_MASS_INFLOW, BC_MASS_OUTFLOW, BC_OUTFLOW = 1, 2, 3
+PLANE_W, PLANE_E, PLANE_S, PLANE_N, PLANE_B, PLANE_T = range(6)
+
+
+class DemBc(C.Structure):
+    """mfixdem_dem_bc: a DEM mass inlet or outlet on a face of the box."""
+    _fields_ = [
+        ("type", _I), ("plane", _I),
+        ("x_w", _D), ("x_e", _D), ("y_s", _D), ("y_n", _D), ("z_b", _D), ("z_t", _D),
+        ("u_s", _D * MAX_PHASES), ("v_s", _D * MAX_PHASES), ("w_s", _D * MAX_PHASES),
+        ("ep_s", _D * MAX_PHASES), ("massflow_s", _D * MAX_PHASES),
+        ("mi_start_time", _D), ("mi_end_time", _D),
+    ]
+
+
+def dem_bc(kind, plane, lo, hi, vel=None, ep_s=None, massflow=None, t0=0.0, t1=1.0e30):
+    """A DemBc from a region lo..hi (x, y, z), per-phase velocities
+    [(u, v, w), ...], BC_EP_s and BC_MASSFLOW_s (None: UNDEFINED)."""
+    b = DemBc()
+    b.type, b.plane = kind, plane
+    b.x_w, b.y_s, b.z_b = lo
+    b.x_e, b.y_n, b.z_t = hi
+    for m in range(MAX_PHASES):
+        b.massflow_s[m] = UNDEFINED
+    for m, v in enumerate(vel or []):
+        b.u_s[m], b.v_s[m], b.w_s[m] = v
+    for m, e in enumerate(ep_s or []):
+        b.ep_s[m] = e
+    for m, f in enumerate(massflow or []):
+        b.massflow_s[m] = UNDEFINED if f is None else f
+    b.mi_start_time, b.mi_end_time = t0, t1
+    return b
+
+
 def default_params() -> Params:
     """Reference defaults; identical to mfixdem_params_default() in C."""
     p = Params()
@@ -231,6 +265,7 @@ def _declare(lib):
         "mfixdem_set_particle_state": ([_P, _BP, C.c_int64], _I),
         "mfixdem_set_facets": ([_P, _I, _DP, _DP, _I], _I),
         "mfixdem_set_wall_motion": ([_P, _I, _I, _DP, _DP, _DP], _I),
+        "mfixdem_set_dem_bcs": ([_P, _I, C.POINTER(DemBc), C.c_uint64], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

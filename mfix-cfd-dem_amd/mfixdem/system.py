@@ -77,6 +77,11 @@ class DemSystem:
                                                    *[abi.dptr(a) for a in arr]),
                   "set_wall_motion")
 
+    def set_dem_bcs(self, bcs, seed=1):
+        """DEM mass inlets and outlets (abi.dem_bc) before init()."""
+        arr = (abi.DemBc * max(len(bcs), 1))(*bcs)
+        abi.check(self.lib.mfixdem_set_dem_bcs(self.h, len(bcs), arr, seed), "set_dem_bcs")
+
     def set_gas(self, gas: Gas):
         abi.check(self.lib.mfixdem_set_gas(
             self.h, abi.dptr(gas.u_g), abi.dptr(gas.v_g), abi.dptr(gas.w_g),

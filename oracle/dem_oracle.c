@@ -32,6 +32,9 @@
 #define UNDEF MFIXDEM_UNDEFINED
 #define NORMAL_PARTICLE 1
 #define NONEXISTENT 0
+#define ENTERING_PARTICLE 2
+#define EXITING_PARTICLE 3
+#define NUMFRAC_LIMIT 10000
 #define COLLISION_ARRAY_MAX 8  /* discretelement_mod.f:637 */
 #define MP MFIXDEM_MAX_PHASES
 #define FILTER_N 27            /* FILTER_SIZE = merge(27, 9, DO_K), 3-D */
@@ -109,7 +112,31 @@ struct orc_sys {
   int* pic_list;
   mfixdem_status st;
   int inited;
+  /* DEM mass inlets and outlets (des_bc_mod.f) */
+  int nbc;
+  struct orc_bc* bcs;
+  uint64_t rng;                       /* stands in for RANDOM_NUMBER */
+  double s_time;                      /* S_TIME of the step being taken */
 };
+
+/* One DEM boundary: the input, its axes, the DES cells whose particles it
+   looks at (DEM_BCMI_IJK / DEM_BCMO_IJK as an index box), and for an inlet
+   the layout of LAYOUT_MI_DEM and the seeding schedule of SET_BC_DEM_MI. */
+typedef struct orc_bc {
+  mfixdem_dem_bc bc;
+  int ax, pa, qa;                     /* normal axis; in-plane axes P, Q */
+  int low;                            /* boundary at the low end of ax */
+  double face;                        /* XE/YN/ZT of the boundary plane */
+  int clo[3], chi[3];
+  int occupants, vacancy, pi_count, polydisperse, L;
+  double pi_factor, mi_time, window, offset;
+  double *P, *Q;
+  int *W, *H;
+  int* poly_layout;                   /* DEM_BC_POLY_LAYOUT, NUMFRAC_LIMIT entries, 0-based phase */
+  double vel[MP][3];                  /* BC_U_s, BC_V_s, BC_W_s after SET_BC_DEM_MI */
+} orc_bc;
+
+static void free_bcs(orc_sys* s);
 
 /* ---------------------------------------------------------------- helpers */
 static inline double dot3(const double a[3], const double b[3]) {
@@ -834,6 +861,7 @@ void orc_destroy(orc_sys* s) {
                   s->filter_weight, s->solvolinc, s->f_gp, s->drag_fc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free_facets(s);
+  free_bcs(s);
   free(s);
 }
 
@@ -2389,6 +2417,432 @@ int orc_set_particle_state(orc_sys* s, const int8_t* state, int64_t n) {
   return 0;
 }
 
+/* ------------------------------------------- DEM mass inlets / outlets */
+/* The seeded stream standing in for RANDOM_NUMBER: splitmix64 (Steele,
+   Lea and Flood, 2014), 53 bits to [0, 1). */
+static double orc_random(orc_sys* s) {
+  uint64_t z = (s->rng += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* CALC_CELL_INTERSECT (calc_cell.f:129-180): the interior cell 2..n+1
+   whose faces bracket loc (both ends inclusive, the lower cell first);
+   c[i-1] = size of cell i. */
+static int cell_intersect(double rmin, double loc, const double* c, int n) {
+  double start = rmin, end = rmin;
+  for (int lc = 2; lc <= n + 1; lc++) {
+    end = start + c[lc - 1];
+    if (start <= loc && loc <= end) return lc;
+    start = end;
+  }
+  if (loc < rmin) return 2;
+  if (loc > end) return n + 1;
+  return -1;
+}
+
+static void bc_axes(const mfixdem_dem_bc* b, int* ax, int* pa, int* qa, int* low) {
+  switch (b->plane) {
+    case MFIXDEM_PLANE_N: case MFIXDEM_PLANE_S: *ax = 1; *pa = 0; *qa = 2; break;
+    case MFIXDEM_PLANE_E: case MFIXDEM_PLANE_W: *ax = 0; *pa = 1; *qa = 2; break;
+    default: *ax = 2; *pa = 0; *qa = 1; break;
+  }
+  *low = b->plane == MFIXDEM_PLANE_N || b->plane == MFIXDEM_PLANE_E || b->plane == MFIXDEM_PLANE_T;
+}
+
+static int ofpos_ax(const orc_sys* s, int a, double x) {
+  return a == 0 ? iofpos(s, x) : (a == 1 ? jofpos(s, x) : kofpos(s, x));
+}
+/* IofBCPOS (desgrid_mod.f:428-432): NINT((x - start) dxinv) + start1 - 1 */
+static int ofbcpos_ax(const orc_sys* s, int a, double x) {
+  const double st = a == 0 ? s->dg_xstart : (a == 1 ? s->dg_ystart : s->dg_zstart);
+  const double inv = a == 0 ? s->dg_dxinv : (a == 1 ? s->dg_dyinv : s->dg_dzinv);
+  const int s1 = a == 0 ? s->dg_istart1 : (a == 1 ? s->dg_jstart1 : s->dg_kstart1);
+  return (int)lround((x - st) * inv) + s1 - 1;
+}
+
+static void free_bcs(orc_sys* s) {
+  for (int b = 0; b < s->nbc; b++) {
+    free(s->bcs[b].P); free(s->bcs[b].Q); free(s->bcs[b].W); free(s->bcs[b].H);
+    free(s->bcs[b].poly_layout);
+  }
+  free(s->bcs);
+  s->bcs = NULL;
+  s->nbc = 0;
+}
+
+/* SET_BC_DEM_MI (set_bc_dem_mi.f:18-377) with LAYOUT_MI_DEM
+   (layout_mi_dem.f:88-444 and its E/W, T/B twins), SET_DEM_BCMI_IJK
+   (set_bc_dem_mi.f:391-553) and SET_BC_DEM_MO (set_bc_dem_mo.f:32-177) for
+   boundaries on faces of the box (no STL facets across an inlet). */
+static int setup_bcs(orc_sys* s) {
+  const mfixdem_params* p = &s->p;
+  const double dt = s->d.dtsolid;
+  const double* csz[3] = {s->cdx, s->cdy, s->cdz};
+  const double* faces[3] = {s->xe, s->yn, s->zt};
+  const int nax[3] = {p->imax, p->jmax, p->kmax};
+  const double amin[3] = {p->x_min, p->y_min, p->z_min};
+  const int lo1[3] = {s->dg_istart2, s->dg_jstart2, s->dg_kstart2};
+  const int hi1[3] = {s->dg_iend2, s->dg_jend2, s->dg_kend2};
+  const int dmin1[3] = {s->dg_istart1, s->dg_jstart1, s->dg_kstart1};
+  const int dmax1[3] = {s->dg_iend1, s->dg_jend1, s->dg_kend1};
+  double min_radius = 1e300;
+  for (int m = 0; m < p->nphases; m++)
+    if (p->d_p0[m] > 0.0 && 0.5 * p->d_p0[m] < min_radius) min_radius = 0.5 * p->d_p0[m];
+  for (int bi = 0; bi < s->nbc; bi++) {
+    orc_bc* B = &s->bcs[bi];
+    const mfixdem_dem_bc* b = &B->bc;
+    bc_axes(b, &B->ax, &B->pa, &B->qa, &B->low);
+    const int ax = B->ax, pa = B->pa, qa = B->qa;
+    const double blo[3] = {b->x_w, b->y_s, b->z_b}, bhi[3] = {b->x_e, b->y_n, b->z_t};
+    const double coord = blo[ax];
+    B->face = B->low ? faces[ax][1] : faces[ax][nax[ax] + 1];
+    for (int c = 0; c < 3; c++) {
+      B->vel[0][c] = 0.0;
+    }
+    if (b->type == MFIXDEM_BC_MASS_INFLOW) {
+      /* DEM_BCMI_IJK */
+      for (int a = 0; a < 3; a++) {
+        B->clo[a] = imax_i(ofbcpos_ax(s, a, blo[a]) - 1, dmin1[a]);
+        B->chi[a] = imin_i(ofbcpos_ax(s, a, bhi[a]) + 1, dmax1[a]);
+      }
+      if (B->low) {
+        B->clo[ax] = ofpos_ax(s, ax, coord + min_radius);
+        B->chi[ax] = ofpos_ax(s, ax, coord + p->max_radius);
+      } else {
+        B->clo[ax] = ofpos_ax(s, ax, coord - p->max_radius);
+        B->chi[ax] = ofpos_ax(s, ax, coord - min_radius);
+      }
+      /* phases at the inlet, the largest diameter */
+      int plist[MP], pcnt = 0;
+      double max_dia = 0.0;
+      for (int m = 0; m < p->nphases; m++) {
+        if (!(b->ep_s[m] > 0.0)) continue;
+        plist[pcnt++] = m;
+        if (p->d_p0[m] > max_dia) max_dia = p->d_p0[m];
+      }
+      if (pcnt == 0) return MFIXDEM_ERR_ARG;
+      B->polydisperse = pcnt > 1;
+      /* LAYOUT_MI_DEM: windows of the inlet rectangle */
+      const double plen = bhi[pa] - blo[pa], qlen = bhi[qa] - blo[qa];
+      const int wmax = (int)floor((float)(plen / max_dia));
+      const int hmax = (int)floor((float)(qlen / max_dia));
+      if (wmax < 1 || hmax < 1) return MFIXDEM_ERR_ARG;
+      const int J = cell_intersect(amin[ax], coord, csz[ax], nax[ax]);
+      const double shift_side = B->low ? -1.0 : 1.0;
+      B->offset = coord + max_dia * shift_side;
+      B->L = J + (int)shift_side;
+      B->window = fmin(plen / wmax, qlen / hmax);
+      const double window = B->window;
+      double* mesh_p = malloc(sizeof(double) * wmax);
+      double* mesh_q = malloc(sizeof(double) * hmax);
+      int* mesh_w = malloc(sizeof(int) * wmax);
+      int* mesh_h = malloc(sizeof(int) * hmax);
+      double shift = 0.5 * (plen - wmax * window);
+      const double p0 = blo[pa] + shift;
+      for (int lc = 1; lc <= wmax; lc++) {
+        mesh_p[lc - 1] = p0 + (double)(lc - 1) * window;
+        mesh_w[lc - 1] = cell_intersect(amin[pa], mesh_p[lc - 1] + 0.5 * window, csz[pa], nax[pa]);
+      }
+      shift = 0.5 * (qlen - hmax * window);
+      const double q0 = blo[qa] + shift;
+      for (int lc = 1; lc <= hmax; lc++) {
+        mesh_q[lc - 1] = q0 + (double)(lc - 1) * window;
+        mesh_h[lc - 1] = cell_intersect(amin[qa], mesh_q[lc - 1] + 0.5 * window, csz[qa], nax[qa]);
+      }
+      /* FULL_MAP: windows whose corner cells next to the inlet are fluid */
+      unsigned char* full = calloc((size_t)wmax * hmax, 1);
+      int occupants = 0;
+      for (int h = 0; h < hmax; h++)
+        for (int w = 0; w < wmax; w++) {
+          int ok = 1;
+          const double cp[2] = {mesh_p[w], mesh_p[w] + window};
+          const double cq[2] = {mesh_q[h], mesh_q[h] + window};
+          for (int a = 0; a < 2 && ok; a++)
+            for (int c = 0; c < 2 && ok; c++) {
+              int idx[3];
+              idx[ax] = J;
+              idx[pa] = cell_intersect(amin[pa], cp[a], csz[pa], nax[pa]);
+              idx[qa] = cell_intersect(amin[qa], cq[c], csz[qa], nax[qa]);
+              if (idx[0] < 1 || idx[1] < 1 || idx[2] < 1 ||
+                  !s->fluid[fluid_ijk(s, idx[0], idx[1], idx[2]) - 1])
+                ok = 0;
+            }
+          full[h * wmax + w] = (unsigned char)ok;
+          occupants += ok;
+        }
+      if (occupants == 0) {
+        free(mesh_p); free(mesh_q); free(mesh_w); free(mesh_h); free(full);
+        return MFIXDEM_ERR_ARG;
+      }
+      B->occupants = occupants;
+      /* RAND_MAP: a random order of the windows, by rejection */
+      int* rand_map = calloc((size_t)occupants + 1, sizeof(int));
+      int ll = 1;
+      while (rand_map[occupants] == 0) {
+        const double tmp = orc_random(s);
+        const int ti = (int)ceil((float)(tmp * (double)occupants));
+        for (int lc = 1; lc <= ll; lc++) {
+          if (ti == rand_map[lc]) break;
+          if (lc == ll) {
+            rand_map[lc] = ti;
+            ll++;
+            break;
+          }
+        }
+      }
+      B->vacancy = 1;
+      B->P = malloc(sizeof(double) * (occupants + 1));
+      B->Q = malloc(sizeof(double) * (occupants + 1));
+      B->W = malloc(sizeof(int) * (occupants + 1));
+      B->H = malloc(sizeof(int) * (occupants + 1));
+      int lc = 0;
+      for (int h = 0; h < hmax; h++)
+        for (int w = 0; w < wmax; w++) {
+          if (!full[h * wmax + w]) continue;
+          const int L2 = rand_map[++lc];
+          B->W[L2] = mesh_w[w];
+          B->H[L2] = mesh_h[h];
+          B->P[L2] = mesh_p[w];
+          B->Q[L2] = mesh_q[h];
+        }
+      free(rand_map); free(mesh_p); free(mesh_q); free(mesh_w); free(mesh_h); free(full);
+      /* seeding rate (set_bc_dem_mi.f:104-175) */
+      const double area = plen * qlen;
+      double vel_tmp[MP] = {0}, npm[MP] = {0}, max_vel = 0.0;
+      const double* bcv[3] = {b->u_s, b->v_s, b->w_s};
+      for (int k = 0; k < pcnt; k++) {
+        const int m = plist[k];
+        vel_tmp[m] = fabs(bcv[ax][m]);
+        if (fabs(vel_tmp[m]) > max_vel) max_vel = fabs(vel_tmp[m]);
+        const double d = p->d_p0[m];
+        const double p_vol = (PI_D * (d * d * d)) / 6.0;
+        if (b->massflow_s[m] == 0.0) npm[m] = 0.0;
+        else if (b->massflow_s[m] != UNDEF) npm[m] = b->massflow_s[m] / (p->ro_s0[m] * p_vol);
+        else npm[m] = vel_tmp[m] * area * b->ep_s[m] / p_vol;
+      }
+      double nppsec = 0.0;
+      for (int m = 0; m < MP; m++) nppsec += npm[m];
+      const double nppdt = nppsec * dt;
+      if (nppdt == 0.0) {
+        B->pi_count = 0;
+        B->pi_factor = 987654321.0;   /* UNDEFINED_I */
+      } else if (nppdt < 1.0) {
+        B->pi_count = 1;
+        B->pi_factor = (double)(float)(1.0 / nppdt);
+      } else {
+        B->pi_count = (int)ceil((float)nppdt);
+        B->pi_factor = 1.0;
+      }
+      if (B->pi_count > 0) {
+        const double minipv = max_dia / (dt * B->pi_factor *
+                                         floor((float)occupants / (float)B->pi_count));
+        if (max_vel < minipv) return MFIXDEM_ERR_ARG;   /* Error 1110 */
+      }
+      /* one speed, normal to the inlet, for every phase (:184-205) */
+      for (int m = 0; m < MP; m++) {
+        B->vel[m][0] = b->u_s[m];
+        B->vel[m][1] = b->v_s[m];
+        B->vel[m][2] = b->w_s[m];
+      }
+      for (int k = 0; k < pcnt; k++) {
+        const int m = plist[k];
+        if (b->massflow_s[m] == 0.0) B->vel[m][ax] = 0.0;
+        else B->vel[m][ax] = B->low ? fabs(max_vel) : -fabs(max_vel);
+      }
+      /* DEM_BC_POLY_LAYOUT (:240-255) */
+      B->poly_layout = malloc(sizeof(int) * NUMFRAC_LIMIT);
+      if (B->polydisperse) {
+        int bot = 1;
+        for (int k = 0; k < pcnt - 1; k++) {
+          const int m = plist[k];
+          const double scaled = (double)NUMFRAC_LIMIT * (npm[m] / nppsec);
+          const int top = (int)floor(scaled) + (bot - 1);
+          for (int r = bot; r <= top; r++) B->poly_layout[r - 1] = m;
+          bot = top + 1;
+        }
+        for (int r = bot; r <= NUMFRAC_LIMIT; r++) B->poly_layout[r - 1] = plist[pcnt - 1];
+      } else {
+        for (int r = 0; r < NUMFRAC_LIMIT; r++) B->poly_layout[r] = plist[0];
+      }
+      B->mi_time = 0.0 + B->pi_factor * dt;
+    } else {
+      /* DEM_BCMO_IJK */
+      for (int a = 0; a < 3; a++) {
+        B->clo[a] = ofbcpos_ax(s, a, blo[a]);
+        B->chi[a] = ofbcpos_ax(s, a, bhi[a]) + 1;
+      }
+    }
+    for (int a = 0; a < 3; a++) {   /* dg_is_ON_myPE_plus1layers */
+      B->clo[a] = imax_i(B->clo[a], lo1[a]);
+      B->chi[a] = imin_i(B->chi[a], hi1[a]);
+    }
+  }
+  return 0;
+}
+
+/* distance of the particle centre from the boundary plane, into the domain */
+static double bc_dist(const orc_sys* s, const orc_bc* B, int64_t q) {
+  const double x = s->pos[B->ax * s->ld + q];
+  return B->low ? x - B->face : B->face - x;
+}
+
+#define FOR_BC_PARTICLES(s, B, q)                                               \
+  for (int bk = (B)->clo[2]; bk <= (B)->chi[2]; bk++)                           \
+    for (int bj = (B)->clo[1]; bj <= (B)->chi[1]; bj++)                         \
+      for (int bi_ = (B)->clo[0]; bi_ <= (B)->chi[0]; bi_++)                    \
+        for (int lp = (s)->dg_off[dg_funijk(s, bi_, bj, bk) - 1],               \
+                 le = (s)->dg_off[dg_funijk(s, bi_, bj, bk)], q = 0;            \
+             lp < le && ((q = (s)->dg_list[lp]), 1); lp++)
+
+/* SET_NEW_PARTICLE_PROPS (mass_inflow_dem.f:419-642) for a sphere. */
+static void new_particle(orc_sys* s, const orc_bc* B, int m, int64_t q, const double pos[3],
+                         const int ijkp[3]) {
+  const int64_t L = s->ld;
+  const double r = s->p.d_p0[m] * 0.5;
+  s->state[q] = ENTERING_PARTICLE;
+  for (int c = 0; c < 3; c++) {
+    s->pos[c * L + q] = pos[c];
+    s->vel[c * L + q] = B->vel[m][c];
+    s->omega[c * L + q] = 0.0;
+    s->pijk[c * L + q] = ijkp[c];
+  }
+  s->radius[q] = r;
+  s->ro_sol[q] = s->p.ro_s0[m];
+  s->pijk[3 * L + q] = fluid_ijk(s, ijkp[0], ijkp[1], ijkp[2]);
+  s->phase[q] = m;
+  const int li = imin_i(s->dg_iend2, imax_i(s->dg_istart2, iofpos(s, pos[0])));
+  const int lj = imin_i(s->dg_jend2, imax_i(s->dg_jstart2, jofpos(s, pos[1])));
+  const int lk = imin_i(s->dg_kend2, imax_i(s->dg_kstart2, kofpos(s, pos[2])));
+  s->dg_pijk[q] = dg_funijk(s, li, lj, lk);
+  s->pvol[q] = (4.0 / 3.0) * PI_D * (r * r * r);
+  s->pmass[q] = s->pvol[q] * s->ro_sol[q];
+  s->omoi[q] = 2.5 / (s->pmass[q] * (r * r));
+  for (int k = 0; k < COLLISION_ARRAY_MAX; k++) s->wall_fid[k * L + q] = -1;
+  if (s->p.des_explicitly_coupled) {
+    s->f_gp[q] = 0.0;
+    for (int c = 0; c < 3; c++) s->drag_fc[c * L + q] = 0.0;
+  }
+  s->pip++;
+}
+
+/* MASS_INFLOW_DEM (mass_inflow_dem.f:44-293) with SEED_NEW_PARTICLE
+   (:306-408) */
+static int mass_inflow(orc_sys* s) {
+  const double dt = s->d.dtsolid;
+  for (int bi = 0; bi < s->nbc; bi++) {
+    orc_bc* B = &s->bcs[bi];
+    if (B->bc.type != MFIXDEM_BC_MASS_INFLOW) continue;
+    FOR_BC_PARTICLES(s, B, q) {
+      if (s->state[q] == EXITING_PARTICLE) continue;
+      if (bc_dist(s, B, q) > s->radius[q] && s->state[q] == ENTERING_PARTICLE)
+        s->state[q] = NORMAL_PARTICLE;
+    }
+    double rnd[3];
+    for (int c = 0; c < 3; c++) rnd[c] = orc_random(s);
+    if (B->mi_time > s->s_time) continue;
+    if (s->s_time < B->bc.mi_start_time || s->s_time > B->bc.mi_end_time) continue;
+    int64_t ls = 0;
+    for (int ip = 0; ip < B->pi_count; ip++) {
+      const int vac = B->vacancy;
+      B->vacancy = vac % B->occupants + 1;
+      int m = B->poly_layout[0];
+      if (B->polydisperse) {
+        const int ri = (int)ceil((double)NUMFRAC_LIMIT * rnd[0]);
+        m = B->poly_layout[(ri < 1 ? 1 : ri) - 1];
+      }
+      const double d = s->p.d_p0[m];
+      const double r1 = 0.5 * d + (B->window - d) * rnd[0];
+      const double r2 = 0.5 * d + (B->window - d) * rnd[1];
+      double pos[3];
+      int ijkp[3];
+      pos[B->pa] = B->P[vac] + r1;
+      pos[B->qa] = B->Q[vac] + r2;
+      pos[B->ax] = B->offset;
+      ijkp[B->pa] = B->W[vac];
+      ijkp[B->qa] = B->H[vac];
+      ijkp[B->ax] = B->L;
+      while (ls < s->max_pip && s->state[ls] != NONEXISTENT) ls++;
+      if (ls >= s->max_pip) return MFIXDEM_ERR_CAPACITY;   /* the reference grows MAX_PIP */
+      new_particle(s, B, m, ls, pos, ijkp);
+    }
+    B->mi_time = B->mi_time + B->pi_factor * dt;
+  }
+  return 0;
+}
+
+/* MASS_OUTFLOW_DEM (mass_outflow_dem.f:13-188) */
+static void mass_outflow(orc_sys* s, int* force_nsearch) {
+  const int64_t L = s->ld;
+  for (int bi = 0; bi < s->nbc; bi++) {
+    orc_bc* B = &s->bcs[bi];
+    if (B->bc.type == MFIXDEM_BC_MASS_INFLOW) continue;
+    const int freeze_vel = B->bc.type != MFIXDEM_BC_MASS_OUTFLOW;
+    const double sgn = B->low ? -1.0 : 1.0;
+    FOR_BC_PARTICLES(s, B, q) {
+      if (s->state[q] == NONEXISTENT || s->state[q] == ENTERING_PARTICLE) continue;
+      const double dist = bc_dist(s, B, q);
+      if (dist > s->radius[q]) {
+        s->state[q] = NORMAL_PARTICLE;
+      } else if (dist > 0.0) {
+        if (freeze_vel) {
+          if (s->vel[B->ax * L + q] * sgn > 0.0) {
+            for (int c = 0; c < 3; c++)
+              s->vel[c * L + q] = s->vel[c * L + q] * (c == B->ax ? 1.0 : 0.0);
+            s->state[q] = EXITING_PARTICLE;
+          }
+        } else {
+          const int m = s->phase[q];
+          s->vel[q] = B->bc.u_s[m];
+          s->vel[L + q] = B->bc.v_s[m];
+          s->vel[2 * L + q] = B->bc.w_s[m];
+          s->state[q] = EXITING_PARTICLE;
+        }
+      } else {
+        delete_particle(s, q);
+        *force_nsearch = 1;
+      }
+    }
+  }
+}
+
+/* UPDATE_EXIT_DES_VEL (mass_outflow_dem.f:444-514), Euler */
+static void update_exit_des_vel(orc_sys* s) {
+  const int64_t L = s->ld;
+  const double dt = s->d.dtsolid;
+  const double* g = s->p.grav;
+  for (int bi = 0; bi < s->nbc; bi++) {
+    orc_bc* B = &s->bcs[bi];
+    if (B->bc.type != MFIXDEM_BC_OUTFLOW) continue;
+    FOR_BC_PARTICLES(s, B, q) {
+      if (s->state[q] != EXITING_PARTICLE) continue;
+      for (int c = 0; c < 3; c++) {
+        const int64_t k = c * L + q;
+        s->vel[k] = s->vel[k] + dt * (s->fc[k] / s->pmass[q] + g[c]);
+        s->vel[k] = s->vel[k] * (c == B->ax ? 1.0 : 0.0);
+      }
+    }
+  }
+}
+
+int orc_set_dem_bcs(orc_sys* s, int32_t n, const mfixdem_dem_bc* bcs, uint64_t seed) {
+  if (!s || n < 0 || (n > 0 && !bcs) || s->inited) return MFIXDEM_ERR_ARG;
+  if (n > 0 && s->p.integrator != MFIXDEM_EULER) return MFIXDEM_ERR_UNSUPPORTED;
+  for (int b = 0; b < n; b++) {
+    if (bcs[b].type < MFIXDEM_BC_MASS_INFLOW || bcs[b].type > MFIXDEM_BC_OUTFLOW)
+      return MFIXDEM_ERR_ARG;
+    if (bcs[b].plane < MFIXDEM_PLANE_W || bcs[b].plane > MFIXDEM_PLANE_T) return MFIXDEM_ERR_ARG;
+  }
+  free_bcs(s);
+  s->nbc = n;
+  s->bcs = calloc((size_t)(n > 0 ? n : 1), sizeof(orc_bc));
+  for (int b = 0; b < n; b++) s->bcs[b].bc = bcs[b];
+  s->rng = seed;
+  return setup_bcs(s);
+}
+
 /* ------------------------------------------------------- driver */
 int orc_desgrid_and_neighbour(orc_sys* s) {
   orc_desgrid_pic(s);
@@ -2426,8 +2880,14 @@ int orc_time_step(orc_sys* s, int32_t nn) {
   if (s->p.integrator == MFIXDEM_ADAMS_BASHFORTH) cfupdateold(s);
   memcpy(s->fc_last, s->fc, sizeof(double) * 3 * L);
   memcpy(s->tow_last, s->tow, sizeof(double) * 3 * L);
+  if (s->nbc) update_exit_des_vel(s);
   orc_cfnewvalues(s);
   int do_nsearch = (nn == 1) || (nn % s->p.neighbor_search_n == 0);
+  if (s->nbc) {
+    rc = mass_inflow(s);
+    if (rc) return rc;
+    mass_outflow(s, &do_nsearch);
+  }
   if (do_nsearch) {
     orc_desgrid_pic(s);
     rc = orc_neighbour(s);
@@ -2440,6 +2900,7 @@ int orc_time_step(orc_sys* s, int32_t nn) {
     orc_calc_interp_weights(s);
     rc = orc_comp_mean_fields(s);
   }
+  s->s_time = s->s_time + s->d.dtsolid;
   s->st.steps++;
   return rc;
 }

@@ -67,6 +67,8 @@ int orc_set_facets(orc_sys* s, int32_t nu, const double* vertex, const double* n
                    int32_t default_walls);
 int orc_set_wall_motion(orc_sys* s, int32_t first, int32_t count, const double* vel,
                         const double* omega, const double* rot_center);
+/* DEM mass inlets and outlets (mfixdem_set_dem_bcs), before orc_init */
+int orc_set_dem_bcs(orc_sys* s, int32_t n, const mfixdem_dem_bc* bcs, uint64_t seed);
 
 /* inspection */
 int orc_get_forces(const orc_sys* s, double* fc, double* tow, int64_t ld);

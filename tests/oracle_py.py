@@ -67,6 +67,7 @@ def load_oracle(omp: bool = False):
         "orc_set_particle_state": ([P, C.POINTER(C.c_int8), L], C.c_int),
         "orc_set_facets": ([P, C.c_int32, D, D, C.c_int32], C.c_int),
         "orc_set_wall_motion": ([P, C.c_int32, C.c_int32, D, D, D], C.c_int),
+        "orc_set_dem_bcs": ([P, C.c_int32, C.POINTER(abi.DemBc), C.c_uint64], C.c_int),
         "orc_particles_in_cell": ([P], C.c_int),
         "orc_calc_interp_weights": ([P], C.c_int),
         "orc_comp_mean_fields": ([P], C.c_int),
@@ -240,6 +241,10 @@ class OracleSystem:
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vel, omega, rot_center)]
         self._chk(self.lib.orc_set_wall_motion(self.h, first, count, *[abi.dptr(x) for x in a]),
                   "set_wall_motion")
+
+    def set_dem_bcs(self, bcs, seed=1):
+        arr = (abi.DemBc * max(len(bcs), 1))(*bcs)
+        self._chk(self.lib.orc_set_dem_bcs(self.h, len(bcs), arr, seed), "set_dem_bcs")
 
     def des_time_init(self):
         self._chk(self.lib.orc_des_time_init(self.h), "des_time_init")

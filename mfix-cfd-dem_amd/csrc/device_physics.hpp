@@ -57,6 +57,8 @@ struct DevParams {
   // per-cell lists in device memory)
   int nfs;
   FacetDev facets[kWallSlots];
+  // DEM mass inlets / outlets (StepPtrs::bcs): 0 when the run has none
+  int nbc;
 };
 
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {

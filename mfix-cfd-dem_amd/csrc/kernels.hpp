@@ -22,6 +22,44 @@ namespace mfixdem {
 
 constexpr int kBS = 256;
 
+// PARTICLE_STATE values (discretelement_mod.f:68-78).  A particle seeded by
+// an inlet since the last neighbour search is ENTERING but in no DES cell
+// list yet: kSeeded marks it until the next binning turns it into kEntering,
+// so that the boundary checks, which visit the particles of the lists, pass
+// it over.
+enum : uint8_t { kNonexistent = 0, kNormal = 1, kEntering = 2, kExiting = 3, kSeeded = 4 };
+
+// A DEM boundary as the particle kernels see it (des_bc_mod.f): the DES
+// cells whose particles it checks (DEM_BCMI_IJK / DEM_BCMO_IJK as an index
+// box, 1-based, ghost layer included), its normal axis and side, the
+// position of its plane and, per phase, the velocity BC_U_s/V_s/W_s.
+struct BcDev {
+  int type;              // MFIXDEM_BC_*
+  int ax, low;           // normal axis; 1 when the boundary is at the low end
+  int lo[3], hi[3];      // DES cell box
+  int dg_ni, dg_nj;      // DES grid extents (to decode a cell index)
+  double face;
+  double vel[kMaxPh][3];
+};
+__device__ __forceinline__ bool bc_holds(const BcDev& b, int dc) {
+  const int j = dc % b.dg_nj + 1, t = dc / b.dg_nj;
+  const int i = t % b.dg_ni + 1, k = t / b.dg_ni + 1;
+  return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] &&
+         k <= b.hi[2];
+}
+// distance of a centre from the boundary plane, into the domain
+__device__ __forceinline__ double bc_dist(const BcDev& b, const double x[3]) {
+  return b.low ? x[b.ax] - b.face : b.face - x[b.ax];
+}
+
+// One particle an inlet seeds (SET_NEW_PARTICLE_PROPS, mass_inflow_dem.f:
+// 419-642), formed on the host; slot_of_gid places it.
+struct SeedDev {
+  double pos[3], vel[3];
+  double radius, pvol, pmass, omoi;
+  int gid, dgc, fcell, phase;
+};
+
 struct StepPtrs {
   const double *pos, *vel, *omega;
   double *pos_o, *vel_o, *omega_o;
@@ -68,6 +106,7 @@ struct StepPtrs {
   // and DRAG_FC [3*gid + d] of the last DES_TIME_INIT (null otherwise)
   double* f_gp;
   double* drag_fc;
+  const BcDev* bcs;             // DEM boundaries (DevParams::nbc of them)
 };
 
 enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
@@ -124,7 +163,9 @@ __device__ __forceinline__ void store3(double* a, int S, int i, const double v[3
 // Fluid cell of a particle, kept packed as I | J << 10 | K << 20 (1-based,
 // ghost layer included; negative = outside the grid) so that kernels get
 // I, J, K with shifts instead of divisions by the grid size.
-__device__ __forceinline__ int pack_ijk(int I, int J, int K) { return I | (J << 10) | (K << 20); }
+__host__ __device__ __forceinline__ int pack_ijk(int I, int J, int K) {
+  return I | (J << 10) | (K << 20);
+}
 __device__ __forceinline__ void unpack_ijk(int c, int& I, int& J, int& K) {
   I = c & 1023;
   J = (c >> 10) & 1023;
@@ -737,6 +778,22 @@ __device__ __forceinline__ void fused_deposit(const DevParams& P, const StepPtrs
 // The per-particle part of a step after the pair pass (drag, integrator,
 // fluid-cell locate) for a live particle i; fills dep for the fused
 // deposition when P0.fuse_dep.
+// UPDATE_EXIT_DES_VEL (mass_outflow_dem.f:444-514), Euler: an exiting
+// particle in the cells of an outlet other than MASS_OUTFLOW takes this
+// step's acceleration and keeps only the component normal to the outlet,
+// once per such outlet.
+__device__ __forceinline__ void exit_velocity(const DevParams& P, const BcDev* bcs, int dc,
+                                           const double fc[3], double mi, double vel[3]) {
+  for (int k = 0; k < P.nbc; k++) {
+    const BcDev& b = bcs[k];
+    if (b.type != MFIXDEM_BC_OUTFLOW || !bc_holds(b, dc)) continue;
+    for (int c = 0; c < 3; c++) {
+      vel[c] = vel[c] + P.dt * (fc[c] / mi + P.grav[c]);
+      vel[c] = vel[c] * (c == b.ax ? 1.0 : 0.0);
+    }
+  }
+}
+
 template <bool MIXED, int MODE, bool FILTER, bool BATCH, bool kEarlyDrag>
 __device__ __forceinline__ void step_tail(const DevParams& P0, const DevParams& P,
                                           const StepPtrs& a, int i, int st, double fc[3],
@@ -791,11 +848,15 @@ __device__ __forceinline__ void step_tail(const DevParams& P0, const DevParams& 
   const double dt = P.dt;
   const double omoi = a.omoi[i];
   if (!P.ab2) {
+    // entering and exiting particles move at their velocity (:492, :593)
+    const bool normal = st == kNormal;
+    if (st == kExiting && P.nbc) exit_velocity(P, a.bcs, a.dgc[i], fc, mi, vel);
     for (int c = 0; c < 3; c++) {
-      vel[c] = vel[c] + dt * (fc[c] / mi + P.grav[c]);
+      if (normal) vel[c] = vel[c] + dt * (fc[c] / mi + P.grav[c]);
       pos[c] = pos[c] + vel[c] * dt;
     }
-    for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
+    if (normal)
+      for (int c = 0; c < 3; c++) om[c] = om[c] + tow[c] * omoi * dt;
   } else {
     double acc[3], racc[3];
     if (P0.first_pass) {
@@ -905,9 +966,13 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
       }
       // ---- walls (calc_collision_wall_mod.f:184-606), the cell's facets in
       // ascending id
+      // only normal particles meet the walls (calc_collision_wall_mod.f:
+      // 105); entering and exiting ones keep their wall slots as they are
+      const bool walls = st == kNormal;
       const int dc = (BATCH ? B.dg_off[B.sys[i]] : 0) + a.dgc[i];
-      const int fm = STL ? 0 : a.cell_fmask[dc];
-      const int fl0 = STL ? a.dg_foff[dc] : 0, fl1 = STL ? a.dg_foff[dc + 1] : 0;
+      const int fm = STL || !walls ? 0 : a.cell_fmask[dc];
+      const int fl0 = STL && walls ? a.dg_foff[dc] : 0;
+      const int fl1 = STL && walls ? a.dg_foff[dc + 1] : 0;
       uint8_t wm = a.wmask[i];
       if (kEarlyDrag && P.coupled) {
         double t[3], q[3];
@@ -919,7 +984,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
           }
       }
       const uint8_t wm0 = wm;
-      if (!STL) {
+      if (!STL && walls) {
         if (fm == 0) wm = 0;  // no facets in the cell: every slot cleared (:189-193)
         for (int f = 0; f < P.nfs; f++) {
           if (!((fm >> f) & 1)) continue;
@@ -937,7 +1002,7 @@ __global__ __launch_bounds__(kBS) void k_step(DevParams P0, StepPtrs a, BatchDev
           }
         }
       }
-      if (STL && fl0 == fl1) wm = 0;
+      if (STL && walls && fl0 == fl1) wm = 0;
       // The facets are walked in wave-uniform passes: a pass takes the next
       // facet of the first lane that has one left, reads it once for the
       // wavefront through the scalar cache, and the lanes whose next facet
@@ -1435,7 +1500,9 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const dou
     f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
   }
   dgc[i] = (c3[1] - 1) + (c3[0] - 1) * G.dg_nj + (c3[2] - 1) * G.dg_nj * G.dg_ni;
-  if (dg_ghost(G, c3[0], c3[1], c3[2])) {
+  // rogue: a normal particle in the ghost layer (only normal particles have
+  // rows, desgrid_mod.f:1096-1104); entering ones start outside the box
+  if (state[i] == kNormal && dg_ghost(G, c3[0], c3[1], c3[2])) {
     if (BATCH) {
       atomicMin(&B.rogue[2 * s], gid[i]);
       atomicAdd(&B.rogue[2 * s + 1], 1);
@@ -1482,7 +1549,8 @@ __global__ __launch_bounds__(kBS) void k_permute(int n, int S, PermPtrs p) {
   p.gid_o[i] = p.gid[o];
   p.fcell_o[i] = p.fcell[o];
   p.dgc_o[i] = p.dgc[o];
-  p.state_o[i] = p.state[o];
+  const uint8_t st = p.state[o];
+  p.state_o[i] = st == kSeeded ? kEntering : st;  // binned now: in a DES cell list
   p.phase_o[i] = p.phase[o];
   const uint8_t wm = p.wmask[o];
   p.wmask_o[i] = wm;
@@ -1566,16 +1634,22 @@ constexpr int kHalfCells = 14;
 constexpr int kHitCap = 8;
 
 // The owner and stencil checks of a pair within the cut-off (h, t) and the
-// append to the owner's row.
+// append to the owner's row.  Only normal particles have rows; a row takes
+// every existing partner except normal ones of lower id (desgrid_mod.f:
+// 1160-1175), so a pair of two normal particles belongs to the lower id and
+// a pair of a normal particle with an entering or exiting one to the normal
+// particle.
 template <bool BATCH>
 __device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b, int S, int g_r,
                                             int h, int gh, int t) {
-  if (b.state[t] == 0) return;
+  const uint8_t sh = b.state[h], stt = b.state[t];
+  if (stt == kNonexistent) return;
   const int gt = b.gid[t];
-  // owner = lower id
-  const int own = gh < gt ? h : t, oth = gh < gt ? t : h;
-  const int go = gh < gt ? gh : gt, gp = gh < gt ? gt : gh;
-  if (b.state[own] != 1 || go >= g_r) return;
+  if (sh != kNormal && stt != kNormal) return;
+  const bool h_owns = sh == kNormal && (stt != kNormal || gh < gt);
+  const int own = h_owns ? h : t, oth = h_owns ? t : h;
+  const int go = h_owns ? gh : gt, gp = h_owns ? gt : gh;
+  if (go >= g_r) return;
   int ci, cj, ck, li, lj, lk;
   dg_decode(G, b.dgc[own], ci, cj, ck);
   if (dg_ghost(G, ci, cj, ck)) return;
@@ -2123,6 +2197,124 @@ __global__ __launch_bounds__(kBS) void k_delete_rogue(int n, int S, GridDev G0, 
     omega[c * S + i] = 0.0;
   }
   radius[i] = 0.0;
+}
+
+// ------------------------------------------------- DEM inlets / outlets
+struct BcPtrs {
+  double *pos, *vel, *omega, *radius, *pmass, *omoi, *pvol;
+  const int *gid, *dgc;
+  uint8_t *state, *wmask;
+  const uint8_t* phase;
+  const BcDev* bcs;
+  int nbc, n, S;
+  int* del_cnt;   // deleted particles: count and global ids
+  int* del_list;
+};
+
+// The checks MASS_INFLOW_DEM (mass_inflow_dem.f:68-139) and MASS_OUTFLOW_DEM
+// (mass_outflow_dem.f:13-188) make of the particles in each boundary's DES
+// cells, one particle per thread.  The reference walks the boundaries in
+// turn and the cell lists inside each; every check reads and writes only
+// its own particle, so a thread taking its particle through the boundaries
+// in the same order (inlets first, as MASS_INFLOW_DEM runs first) ends in
+// the same state.  A particle is in a cell list once binned: seeded ones
+// (kSeeded) are not.
+//   inlet:  ENTERING -> NORMAL once the centre is a radius inside;
+//   outlet: NORMAL beyond a radius inside; EXITING between (MASS_OUTFLOW
+//           gives it the outlet's velocity, other outlets keep only its
+//           outward normal velocity, and only while it moves outwards);
+//           deleted (DELETE_PARTICLE, :201-280) once the centre is out.
+__global__ __launch_bounds__(kBS) void k_dem_bcs(BcPtrs b) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= b.n) return;
+  const int st0 = b.state[i];
+  if (st0 == kNonexistent || st0 == kSeeded) return;
+  const int S = b.S, dc = b.dgc[i];
+  int st = st0;
+  double x[3], v[3];
+  load3(b.pos, S, i, x);
+  load3(b.vel, S, i, v);
+  const double r = b.radius[i];
+  bool vset = false;
+  for (int k = 0; k < b.nbc; k++) {
+    const BcDev& B = b.bcs[k];
+    if (B.type != MFIXDEM_BC_MASS_INFLOW || !bc_holds(B, dc)) continue;
+    if (st == kExiting) continue;
+    if (bc_dist(B, x) > r && st == kEntering) st = kNormal;
+  }
+  for (int k = 0; k < b.nbc; k++) {
+    const BcDev& B = b.bcs[k];
+    if (B.type == MFIXDEM_BC_MASS_INFLOW || !bc_holds(B, dc)) continue;
+    if (st == kNonexistent || st == kEntering) continue;
+    const double dist = bc_dist(B, x);
+    if (dist > r) {
+      st = kNormal;
+    } else if (dist > 0.0) {
+      if (B.type == MFIXDEM_BC_MASS_OUTFLOW) {
+        const int m = b.phase[i];
+        for (int c = 0; c < 3; c++) v[c] = B.vel[m][c];
+        st = kExiting;
+        vset = true;
+      } else if (v[B.ax] * (B.low ? -1.0 : 1.0) > 0.0) {
+        for (int c = 0; c < 3; c++) v[c] = v[c] * (c == B.ax ? 1.0 : 0.0);
+        st = kExiting;
+        vset = true;
+      }
+    } else {
+      st = kNonexistent;
+    }
+  }
+  if (st == kNonexistent) {
+    const double z[3] = {0.0, 0.0, 0.0};
+    store3(b.pos, S, i, z);
+    store3(b.vel, S, i, z);
+    store3(b.omega, S, i, z);
+    b.radius[i] = 0.0;
+    b.pmass[i] = 3.4028234663852886e38;  // HUGE(0.0) as DELETE_PARTICLE sets it
+    b.pvol[i] = 3.4028234663852886e38;
+    b.omoi[i] = 0.0;
+    b.wmask[i] = 0;
+    b.state[i] = kNonexistent;
+    b.del_list[atomicAdd(b.del_cnt, 1)] = b.gid[i];
+    return;
+  }
+  if (st != st0) b.state[i] = (uint8_t)st;
+  if (vset) store3(b.vel, S, i, v);
+}
+
+struct SeedPtrs {
+  double *pos, *vel, *omega, *radius, *pmass, *omoi, *pvol;
+  int *dgc, *fcell;
+  uint8_t *state, *phase, *wmask;
+  const int* slot_of_gid;
+  int S;
+};
+// SET_NEW_PARTICLE_PROPS (mass_inflow_dem.f:419-642): the seeded particles
+// written into the slots of their global ids, ENTERING and not yet binned.
+__global__ __launch_bounds__(kBS) void k_seed(int ns, const SeedDev* sd, SeedPtrs p) {
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  if (k >= ns) return;
+  const SeedDev d = sd[k];
+  const int i = p.slot_of_gid[d.gid], S = p.S;
+  const double z[3] = {0.0, 0.0, 0.0};
+  store3(p.pos, S, i, d.pos);
+  store3(p.vel, S, i, d.vel);
+  store3(p.omega, S, i, z);
+  p.radius[i] = d.radius;
+  p.pvol[i] = d.pvol;
+  p.pmass[i] = d.pmass;
+  p.omoi[i] = d.omoi;
+  p.dgc[i] = d.dgc;
+  p.fcell[i] = d.fcell;
+  p.phase[i] = (uint8_t)d.phase;
+  p.wmask[i] = 0;
+  p.state[i] = kSeeded;
+}
+
+// slot of each global id, after a binning has reordered the slots
+__global__ __launch_bounds__(kBS) void k_slot_of_gid(int n, const int* gid, int* slot_of_gid) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) slot_of_gid[gid[i]] = i;
 }
 
 // ---------------------------------------------------------- gas coupling
@@ -2822,10 +3014,12 @@ __global__ __launch_bounds__(kBS) void k_set_holes(int n, const int8_t* st, uint
   }
 }
 
+// PARTICLE_STATE in global-id order (a seeded particle is ENTERING)
 __global__ void k_scatter_gid_u8(int n, const int* gid, const uint8_t* src, int8_t* dst) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
-  dst[gid[i]] = (int8_t)src[i];
+  const uint8_t v = src[i];
+  dst[gid[i]] = (int8_t)(v == kSeeded ? kEntering : v);
 }
 
 }  // namespace mfixdem

@@ -22,6 +22,7 @@
 #include "device_physics.hpp"
 #include "host_setup.hpp"
 #include "kernels.hpp"
+#include "dem_bcs.hpp"
 #include "params.hpp"
 
 using namespace mfixdem;
@@ -131,6 +132,13 @@ struct mfixdem_handle {
   int *dg_foff = nullptr, *dg_fids = nullptr;
   uint8_t* cell_fmask = nullptr;
   double* stage = nullptr;  // export staging, 3*n doubles
+  // DEM mass inlets / outlets (mfixdem_set_dem_bcs): host schedule and
+  // layout, the device table, this step's seeds, the slot of each global id
+  // (rewritten at every binning) and the particles deleted in a step
+  DemBoundaries dbc;
+  BcDev* d_bcs = nullptr;
+  SeedDev* d_seeds = nullptr;
+  int *slot_of_gid = nullptr, *del_cnt = nullptr, *del_list = nullptr;
   // status
   mfixdem_status st{};
   bool inited = false, gas_set = false, first_pass = true, binned = false;
@@ -235,6 +243,7 @@ StepPtrs step_ptrs(mfixdem_handle* h) {
   a.rm_list = h->rm_list;
   a.f_gp = h->f_gp;
   a.drag_fc = h->drag_fc;
+  a.bcs = h->d_bcs;
   return a;
 }
 
@@ -439,6 +448,11 @@ int32_t do_bin(mfixdem_handle* h) {
     hipLaunchKernelGGL((k_cell_ranges<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
                        h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
   HIPOK(hipGetLastError());
+  if (h->slot_of_gid) {  // inlets place their seeds by global id
+    hipLaunchKernelGGL(k_slot_of_gid, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[h->tw],
+                       h->slot_of_gid);
+    HIPOK(hipGetLastError());
+  }
   h->binned = true;
   invalidate_graphs(h);
   return MFIXDEM_OK;
@@ -488,7 +502,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
   b.own_cnt = h->own_cnt; b.flags = h->flags;
   int fl[kFlagCount];
-  bool wave_rows = h->row_kernel != MFIXDEM_ROWS_THREAD;
+  // (the wavefront row kernels take normal particles only: boundaries use
+  // the thread kernel, whose owner rule knows entering and exiting ones)
+  bool wave_rows = h->row_kernel != MFIXDEM_ROWS_THREAD && !h->P.nbc;
   // neighbour.f:60-66: NSQUARE for DES_NEIGHBOR_SEARCH = 1 (no rogue handling
   // there: that lives in DESGRID_NEIGH_BUILD)
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
@@ -730,7 +746,88 @@ int32_t plain_step_body(mfixdem_handle* h) {
   return MFIXDEM_OK;
 }
 
+// The free global ids, from the particle states (at init, and after rogue
+// particles have been removed).
+int32_t refresh_free_gids(mfixdem_handle* h) {
+  const int n = h->n;
+  hipLaunchKernelGGL(k_scatter_gid_u8, dim3(nblk(n)), dim3(kBS), 0, h->stream, n,
+                     h->gid[h->tw], h->state[h->tw], (int8_t*)h->stage);
+  HIPOK(hipGetLastError());
+  std::vector<int8_t> st(n);
+  HIPOK(hipMemcpyAsync(st.data(), h->stage, n, hipMemcpyDeviceToHost, h->stream));
+  HIPOK(sync_stream(h));
+  h->dbc.free_gids.clear();
+  for (int g = 0; g < n; g++)
+    if (st[g] == kNonexistent) h->dbc.free_gids.insert(g);
+  return MFIXDEM_OK;
+}
+
+// DES_TIME_STEP with DEM boundaries (des_time_march.f:354-380): the step
+// kernel (with UPDATE_EXIT_DES_VEL), then MASS_INFLOW_DEM and
+// MASS_OUTFLOW_DEM, and the neighbour search, which a deleted particle
+// forces.  Launched directly (the seeds differ from step to step), with one
+// wait for the count of deleted particles.
+int32_t bc_step(mfixdem_handle* h, int32_t nn) {
+  DemBoundaries& D = h->dbc;
+  bool rebuild = (nn == 1) || (nn % h->S.p.neighbor_search_n == 0);
+  const int n = h->n;
+  t_begin(h);
+  CHECKRC(launch_step<kModeFused>(h));
+  t_end(h, kTimeStep);
+  h->cur = 1 - h->cur;
+  h->pcur = 1 - h->pcur;
+  const int c = h->cur, t = h->tw;
+  CHECKRC(fill_i32(h->stream, h->del_cnt, 1, 0));
+  BcPtrs b{};
+  b.pos = h->pos[c]; b.vel = h->vel[c]; b.omega = h->omega[c];
+  b.radius = h->radius[t]; b.pmass = h->pmass[t]; b.omoi = h->omoi[t]; b.pvol = h->pvol[t];
+  b.gid = h->gid[t]; b.dgc = h->dgc[t]; b.state = h->state[t]; b.wmask = h->wmask[t];
+  b.phase = h->phase[t];
+  b.bcs = h->d_bcs; b.nbc = (int)D.b.size(); b.n = n; b.S = h->stride;
+  b.del_cnt = h->del_cnt; b.del_list = h->del_list;
+  hipLaunchKernelGGL(k_dem_bcs, dim3(nblk(n)), dim3(kBS), 0, h->stream, b);
+  HIPOK(hipGetLastError());
+  CHECKRC(plan_seeds(h->S, D));
+  const int ns = (int)D.seeds.size();
+  if (ns > 0) {
+    HIPOK(hipMemcpyAsync(h->d_seeds, D.seeds.data(), sizeof(SeedDev) * ns,
+                         hipMemcpyHostToDevice, h->stream));
+    SeedPtrs sp{};
+    sp.pos = h->pos[c]; sp.vel = h->vel[c]; sp.omega = h->omega[c];
+    sp.radius = h->radius[t]; sp.pmass = h->pmass[t]; sp.omoi = h->omoi[t];
+    sp.pvol = h->pvol[t]; sp.dgc = h->dgc[t]; sp.fcell = h->fcell[t];
+    sp.state = h->state[t]; sp.phase = h->phase[t]; sp.wmask = h->wmask[t];
+    sp.slot_of_gid = h->slot_of_gid; sp.S = h->stride;
+    hipLaunchKernelGGL(k_seed, dim3(nblk(ns)), dim3(kBS), 0, h->stream, ns, h->d_seeds, sp);
+    HIPOK(hipGetLastError());
+  }
+  int nd = 0;
+  HIPOK(hipMemcpyAsync(&nd, h->del_cnt, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPOK(sync_stream(h));  // also frees the seed records for the next step
+  if (nd > 0) {
+    std::vector<int> gone(nd);
+    HIPOK(hipMemcpyAsync(gone.data(), h->del_list, sizeof(int) * nd, hipMemcpyDeviceToHost,
+                         h->stream));
+    HIPOK(sync_stream(h));
+    for (int g : gone) D.free_gids.insert(g);
+    rebuild = true;  // FORCE_NSEARCH (mass_outflow_dem.f:178)
+  }
+  if (rebuild) {
+    const int64_t rogue0 = h->st.rogue_deleted;
+    t_begin(h);
+    CHECKRC(do_bin(h));
+    CHECKRC(do_neighbour(h));
+    t_end(h, kTimeRebuild);
+    if (h->st.rogue_deleted != rogue0) CHECKRC(refresh_free_gids(h));
+  }
+  D.s_time = D.s_time + h->S.d.dtsolid;
+  h->first_pass = false;
+  h->st.steps++;
+  return MFIXDEM_OK;
+}
+
 int32_t one_step(mfixdem_handle* h, int32_t nn) {
+  if (h->P.nbc) return bc_step(h, nn);
   const bool rebuild = (nn == 1) || (nn % h->S.p.neighbor_search_n == 0);
   const bool use_graph = !rebuild && !h->first_pass && !timing(h) &&
                          !(h->debug & (MFIXDEM_DEBUG_STORE_FORCES | MFIXDEM_DEBUG_NO_GRAPH));
@@ -948,9 +1045,13 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->gd_fgp); f(h->gd_vol_node); f(h->gd_am); f(h->gd_bm); f(h->gd_fgds);
   f(h->fac); f(h->dg_foff); f(h->dg_fids); f(h->cell_fmask); f(h->stage);
   f(h->d_sys); f(h->d_ptab); f(h->d_gtab); f(h->d_cell_off); f(h->d_dg_off); f(h->d_rogue);
+  f(h->d_bcs); f(h->d_seeds); f(h->slot_of_gid); f(h->del_cnt); f(h->del_list);
 }
 
-void fill_devparams(mfixdem_handle* h) { fill_params(h->S, h->n, h->stride, h->P, h->G); }
+void fill_devparams(mfixdem_handle* h) {
+  fill_params(h->S, h->n, h->stride, h->P, h->G);
+  h->P.nbc = (int)h->dbc.b.size();
+}
 
 // Device buffers, stream and events of a handle whose set-up (S, P, G,
 // n, stride, n_fine, key_bits) is filled in.  On failure the caller
@@ -1280,10 +1381,42 @@ int32_t mfixdem_set_wall_motion(mfixdem_handle* h, int32_t first, int32_t count,
 
 int32_t mfixdem_set_dem_bcs(mfixdem_handle* h, int32_t n, const mfixdem_dem_bc* bcs,
                             uint64_t seed) {
-  (void)bcs;
-  (void)seed;
-  if (!h || n < 0) return MFIXDEM_ERR_ARG;
-  return n == 0 ? MFIXDEM_OK : MFIXDEM_ERR_UNSUPPORTED;
+  if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
+  if (!h || n < 0 || (n > 0 && !bcs) || h->inited) return MFIXDEM_ERR_ARG;
+  const mfixdem_params& p = h->S.p;
+  // Euler steps of an uncoupled system on the DES grid search
+  if (n > 0 && (p.integrator != MFIXDEM_EULER || p.coupled ||
+                p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE))
+    return MFIXDEM_ERR_UNSUPPORTED;
+  for (int k = 0; k < n; k++) {
+    if (bcs[k].type < MFIXDEM_BC_MASS_INFLOW || bcs[k].type > MFIXDEM_BC_OUTFLOW)
+      return MFIXDEM_ERR_ARG;
+    if (bcs[k].plane < MFIXDEM_PLANE_W || bcs[k].plane > MFIXDEM_PLANE_T) return MFIXDEM_ERR_ARG;
+  }
+  DemBoundaries D;
+  D.b.resize(n);
+  for (int k = 0; k < n; k++) D.b[k].in = bcs[k];
+  D.rng = seed;
+  CHECKRC(setup_dem_bcs(h->S, D));
+  HIPOK(sync_stream(h));
+  for (auto** q : {&h->slot_of_gid, &h->del_cnt, &h->del_list}) dfree(*q);
+  dfree(h->d_bcs);
+  dfree(h->d_seeds);
+  h->dbc = DemBoundaries{};
+  h->P.nbc = 0;
+  invalidate_graphs(h);
+  if (n == 0) return MFIXDEM_OK;
+  const int64_t N = h->n;
+  CHECKRC(dalloc(&h->d_bcs, n));
+  CHECKRC(dalloc(&h->d_seeds, std::max(D.max_seeds, 1)));
+  CHECKRC(dalloc(&h->slot_of_gid, N));
+  CHECKRC(dalloc(&h->del_cnt, 1));
+  CHECKRC(dalloc(&h->del_list, N));
+  const std::vector<BcDev> bd = bcs_dev(h->S, D);
+  HIPOK(hipMemcpy(h->d_bcs, bd.data(), sizeof(BcDev) * n, hipMemcpyHostToDevice));
+  h->dbc = std::move(D);
+  h->P.nbc = n;
+  return MFIXDEM_OK;
 }
 
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel, double* omega,
@@ -1386,6 +1519,7 @@ int32_t mfixdem_init(mfixdem_handle* h) {
   }
   CHECKRC(do_bin(h));
   CHECKRC(do_neighbour(h));
+  if (h->P.nbc) CHECKRC(refresh_free_gids(h));
   if (h->S.p.coupled) {
     if (ref_deposition(h)) {
       CHECKRC(ref_mean_fields(h));

@@ -4,7 +4,9 @@
 ! (particles_in_cell.f:20) on the GPU.  INIT_PARTICLES_IN_CELL, which  !
 ! MAKE_ARRAYS_DES calls once the particles are set up                  !
 ! (make_arrays_des.f:343), is where the case's handle is opened: the   !
-! particles go to the GPU there and PIJK comes back.                   !
+! particles go to the GPU there, are located in the fluid grid (PIJK   !
+! of every particle, coupled or not, as INIT_PARTICLES_IN_CELL sets    !
+! it) and PIJK comes back.                                             !
 !----------------------------------------------------------------------!
 MODULE PARTICLES_IN_CELL_MOD
   use mfixdem_c, only: mfixdem_particles_in_cell
@@ -17,6 +19,7 @@ CONTAINS
 
   SUBROUTINE INIT_PARTICLES_IN_CELL
     call mfixdem_open()
+    call mfixdem_fail(mfixdem_particles_in_cell(h), 'particles_in_cell')
     call mfixdem_pull()
   END SUBROUTINE INIT_PARTICLES_IN_CELL
 END MODULE PARTICLES_IN_CELL_MOD

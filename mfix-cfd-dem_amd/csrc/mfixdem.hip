@@ -1574,6 +1574,8 @@ int32_t mfixdem_calc_drag_des(mfixdem_handle* h) {
 }
 int32_t mfixdem_cfnewvalues(mfixdem_handle* h) {
   if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  // inlets and outlets run inside the whole step (bc_step) only
+  if (h->P.nbc) return MFIXDEM_ERR_UNSUPPORTED;
   CHECKRC(launch_step<kModeIntegrate>(h));
   h->cur = 1 - h->cur;
   h->first_pass = false;

@@ -1516,6 +1516,64 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const dou
   keys[i] = sys_key | m;
 }
 
+// DESGRID_PIC locate as k_bin, for one system, with a counting sort in
+// place of the key sort: each particle counts itself into its search cell
+// (cells numbered in Morton order by cell_rank; nonexistent particles into
+// a last cell after them) and keeps the place it drew.  A cell's particles
+// end up in no particular order among themselves (no result depends on it).
+__global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, const double* pos,
+                                                   const uint8_t* state, const int* gid,
+                                                   int* dgc, const int* cell_rank,
+                                                   int* cell_cnt, int nfine, uint64_t* place,
+                                                   int* flags) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  int r = nfine;
+  if (state[i] == kNonexistent) {
+    dgc[i] = 0;
+  } else {
+    const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+    const int nn[3] = {G.dg_ni, G.dg_nj, G.dg_nk};
+    int c3[3], f3[3];
+    for (int d = 0; d < 3; d++) {
+      const int v = floor_sat_d((x[d] - G.dg_lo[d]) * G.dg_inv[d]) + 2;
+      c3[d] = v < 1 ? 1 : (v > nn[d] ? nn[d] : v);
+      const int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
+      f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
+    }
+    dgc[i] = (c3[1] - 1) + (c3[0] - 1) * G.dg_nj + (c3[2] - 1) * G.dg_nj * G.dg_ni;
+    if (state[i] == kNormal && dg_ghost(G, c3[0], c3[1], c3[2])) {  // rogue, as in k_bin
+      atomicMin(&flags[kFlagRogueGid], gid[i]);
+      atomicAdd(&flags[kFlagRogueCount], 1);
+    }
+    r = cell_rank[f3[0] + G.fn[0] * (f3[1] + G.fn[1] * f3[2])];
+  }
+  const int w = atomicAdd(&cell_cnt[r], 1);
+  place[i] = ((uint64_t)r << 32) | (uint32_t)w;
+}
+
+// The new order (perm[new slot] = old slot) from the counted places and the
+// cells' first slots (cell_off: exclusive scan of the counts, in rank
+// order); and each search cell's slot range, indexed by its grid position.
+// The counts go back to zero for the next binning.
+__global__ __launch_bounds__(kBS) void k_bin_place(int n, int nfine, const uint64_t* place,
+                                                   const int* cell_rank, const int* cell_off,
+                                                   int* cell_cnt, int* perm, int* cstart,
+                                                   int* cend) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) {
+    const uint64_t p = place[i];
+    perm[cell_off[(int)(p >> 32)] + (int)(uint32_t)p] = i;
+  }
+  if (i < nfine) {
+    const int r = cell_rank[i];
+    cstart[i] = cell_off[r];
+    cend[i] = cell_off[r + 1];
+    cell_cnt[r] = 0;
+  }
+  if (i == 0) cell_cnt[nfine] = 0;
+}
+
 struct PermPtrs {
   const int* perm;
   const double *pos, *vel, *omega;
@@ -1607,6 +1665,7 @@ struct BuildPtrs {
   int ld;
   int maxrow;
   int* flags;
+  int* in_cnt;                  // incoming entries per particle (thread-row kernel)
 };
 
 // Own rows of DESGRID_NEIGH_BUILD (desgrid_mod.f:1057-1214): j in the
@@ -1614,12 +1673,12 @@ struct BuildPtrs {
 // Half-stencil search: the thread of particle h tests each unordered pair
 // once -- partners in its own search cell after it (storage order) and in
 // the 13 neighbouring cells with a positive offset (z, then y, then x).  A
-// pair within the cut-off belongs to the row of its lower-id particle (the
-// owner), which must be normal, outside the ghost DES cells and before the
-// first rogue id; the partner must exist and lie in the owner's reference
-// stencil.  Entries are appended to the owner's row through an atomic slot
-// (own_cnt zeroed beforehand); k_sort_rows then puts each row in the
-// reference's order (stencil lk, lj, li, then ascending id).
+// pair within the cut-off belongs to the row of its owner (own_row_hit),
+// which must be outside the ghost DES cells and before the first rogue id;
+// the partner must lie in the owner's reference stencil.  Entries are
+// appended to the owner's row through an atomic slot (own_cnt and in_cnt
+// zeroed beforehand); k_place then puts each row in the reference's order
+// (stencil lk, lj, li, then ascending id).
 //
 // Search cells hold about one particle, so a loop per cell would cost a
 // wavefront the slowest lane's count in every one of the 14 cells.  The
@@ -1665,17 +1724,30 @@ __device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b
     b.own_tmp[(size_t)slot * b.ld + own] = oth;
     b.own_key[(size_t)slot * b.ld + own] =
         ((uint64_t)((lk * 1024 + lj) * 1024 + li) << 32) | (uint32_t)gp;
+    atomicAdd(&b.in_cnt[oth], 1);
   } else {
     atomicMax(&b.flags[kFlagOwnOverflow], slot + 1);
   }
 }
 
+// kRowLanes lanes share one particle's half stencil (lane p takes the cells
+// p, p + kRowLanes, ...): shorter candidate walks per lane, and the range
+// table each lane keeps in LDS shrinks with them, so more wavefronts fit on a
+// CU to hide the walk's dependent loads.
+#ifndef MFIXDEM_ROW_LANES
+#define MFIXDEM_ROW_LANES 2
+#endif
+constexpr int kRowLanes = MFIXDEM_ROW_LANES;
+
 template <bool BATCH = false>
 __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, BuildPtrs b,
                                                   BatchDev B) {
-  __shared__ int s_lo[kHalfCells][kBS], s_hi[kHalfCells][kBS];
-  __shared__ int s_hit[kHitCap][kBS];
-  const int h = blockIdx.x * kBS + threadIdx.x;
+  constexpr int kCells = (kHalfCells + kRowLanes - 1) / kRowLanes;
+  constexpr int kHits = kRowLanes == 1 ? kHitCap : kHitCap / 2;
+  __shared__ int s_lo[kCells][kBS], s_hi[kCells][kBS];
+  __shared__ int s_hit[kHits][kBS];
+  const int id = blockIdx.x * kBS + threadIdx.x;
+  const int h = id / kRowLanes, part = id % kRowLanes;
   if (h >= n || b.state[h] == 0) return;
   const int s = BATCH ? B.sys[h] : 0;
   const GridDev& G = sys_grid<BATCH>(G0, B, s);
@@ -1689,24 +1761,25 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
     const int f = floor_sat_d((x[d] - G.f_lo[d]) * G.f_inv);
     f3[d] = f < 0 ? 0 : (f >= G.fn[d] ? G.fn[d] - 1 : f);
   }
-  int lo[kHalfCells], hi[kHalfCells];
+  int lo[kCells], hi[kCells];
 #pragma unroll
-  for (int q = 0; q < kHalfCells; q++) {
+  for (int c = 0; c < kCells; c++) {
     // q = 0: own cell; q = 1..13: offsets (dx, dy, dz) after (0, 0, 0)
+    const int q = part + c * kRowLanes;
     const int o = q == 0 ? 13 : 13 + q;  // index in the 3x3x3 block, x fastest
     const int fx = f3[0] + o % 3 - 1, fy = f3[1] + (o / 3) % 3 - 1, fz = f3[2] + o / 9 - 1;
-    const bool in = fx >= 0 && fx < G.fn[0] && fy >= 0 && fy < G.fn[1] && fz >= 0 &&
-                    fz < G.fn[2];
+    const bool in = q < kHalfCells && fx >= 0 && fx < G.fn[0] && fy >= 0 && fy < G.fn[1] &&
+                    fz >= 0 && fz < G.fn[2];
     const int fc = in ? cbase + fx + G.fn[0] * (fy + G.fn[1] * fz) : cbase;
-    lo[q] = q == 0 ? h + 1 : b.fstart[fc];
-    hi[q] = in ? b.fend[fc] : 0;
+    lo[c] = q == 0 ? h + 1 : b.fstart[fc];
+    hi[c] = in ? b.fend[fc] : 0;
   }
   int m = 0;
 #pragma unroll
-  for (int q = 0; q < kHalfCells; q++)
-    if (hi[q] > lo[q]) {
-      s_lo[m][threadIdx.x] = lo[q];
-      s_hi[m][threadIdx.x] = hi[q];
+  for (int c = 0; c < kCells; c++)
+    if (hi[c] > lo[c]) {
+      s_lo[m][threadIdx.x] = lo[c];
+      s_hi[m][threadIdx.x] = hi[c];
       m++;
     }
   if (m == 0) return;
@@ -1729,7 +1802,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
     const double d0 = x[0] - pc[0], d1 = x[1] - pc[1], d2 = x[2] - pc[2];
     const double ds = d0 * d0 + d1 * d1 + d2 * d2;
     if (ds > sr * sr) continue;
-    if (nh < kHitCap) s_hit[nh++][threadIdx.x] = tc;
+    if (nh < kHits) s_hit[nh++][threadIdx.x] = tc;
     else own_row_hit<BATCH>(G, b, S, g_r, h, gh, tc);  // a crowded stencil: at once
   }
 #pragma unroll 1
@@ -1756,18 +1829,11 @@ __global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, const double* pos
   }
 }
 
-// Each row of k_own_rows in the reference's order (keys are distinct: the
-// partner id is part of the key), and each entry counted in its partner's
-// incoming row (in_cnt zeroed beforehand).  Rows are short: up to kRegRow
+// A row of k_own_rows put in the reference's order (keys are distinct: the
+// partner id is part of the key), in place.  Rows are short: up to kRegRow
 // entries are sorted in registers, longer ones by insertion in place.
 constexpr int kRegRow = 8;
-__global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b, int* in_cnt) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  const int cnt = min(b.own_cnt[i], b.maxrow);
-  int* row = b.own_tmp + i;
-  uint64_t* key = b.own_key + i;
-  const size_t ld = b.ld;
+__device__ __forceinline__ void sort_own_row(int* row, uint64_t* key, size_t ld, int cnt) {
   if (cnt <= kRegRow) {
     uint64_t k[kRegRow];
     int v[kRegRow];
@@ -1792,11 +1858,7 @@ __global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b, int* in_c
         }
 #pragma unroll
     for (int a = 0; a < kRegRow; a++)
-      if (a < cnt) {
-        key[a * ld] = k[a];
-        row[a * ld] = v[a];
-        atomicAdd(&in_cnt[v[a]], 1);
-      }
+      if (a < cnt) row[a * ld] = v[a];
     return;
   }
   for (int a = 1; a < cnt; a++) {
@@ -1811,7 +1873,6 @@ __global__ __launch_bounds__(kBS) void k_sort_rows(int n, BuildPtrs b, int* in_c
     key[q * ld] = k;
     row[q * ld] = v;
   }
-  for (int a = 0; a < cnt; a++) atomicAdd(&in_cnt[row[a * ld]], 1);
 }
 
 // Own rows, one wavefront per search-grid cell.  The candidates of the 27
@@ -2094,16 +2155,20 @@ struct CarryPtrs {
 // with the particles (gid_new[j] = gid_old[perm[j]]), so "old partner id ==
 // new partner id" is tested as "old partner slot == perm[j]", one gather
 // fewer per comparison.
-__global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, const int* own_tmp,
-                                               const int* pair_base, const int* nbr_start,
-                                               const int* in_cnt, int* nbr_split, int* in_cur,
-                                               int* nbr, int* nbr_pid, CarryPtrs c) {
+// Rows appended through atomic slots (own_key given) are first put in the
+// reference's order.
+__global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* own_tmp,
+                                               uint64_t* own_key, const int* pair_base,
+                                               const int* nbr_start, const int* in_cnt,
+                                               int* nbr_split, int* in_cur, int* nbr,
+                                               int* nbr_pid, CarryPtrs c) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int split = nbr_start[i] + in_cnt[i];
   nbr_split[i] = split;
   const int cnt = own_cnt[i];
   if (cnt == 0) return;
+  if (own_key && cnt > 1) sort_own_row(own_tmp + i, own_key + i, (size_t)n, cnt);
   const int pb = pair_base[i];
   const int o = c.perm[i];
   const int os = c.old_split[o], oe = c.old_start[o + 1], opb = c.old_pair_base[o];

@@ -88,6 +88,9 @@ struct mfixdem_handle {
   size_t sort_tmp_bytes = 0;
   int *fstart = nullptr, *fend = nullptr;
   int n_fine = 0;
+  // one system: counting sort over the search cells (Morton rank of each
+  // cell, counts with one more cell for nonexistent particles, their scan)
+  int *fine_rank = nullptr, *fine_cnt = nullptr, *fine_off = nullptr;
   int *own_cnt = nullptr, *own_tmp = nullptr, *in_cnt = nullptr, *in_cur = nullptr,
       *tot_cnt = nullptr;
   uint64_t* own_key = nullptr;
@@ -408,18 +411,26 @@ int32_t do_bin(mfixdem_handle* h) {
     HIPOK(hipMemcpyAsync(h->d_rogue, h->rogue_init.data(), sizeof(int) * 2 * h->nsys,
                          hipMemcpyHostToDevice, h->stream));
   const int t = h->tw, c = h->cur;
-  if (batch)
+  if (batch) {  // ensembles: key sort (system id above the Morton code)
     hipLaunchKernelGGL((k_bin<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
                        h->pos[c], h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0],
                        h->flags, h->B);
-  else
-    hipLaunchKernelGGL((k_bin<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
-                       h->pos[c], h->state[t], h->gid[t], h->dgc[t], h->keys[0], h->vals[0],
-                       h->flags, h->B);
-  HIPOK(hipGetLastError());
-  size_t tb = h->sort_tmp_bytes;
-  HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->vals[1], n,
-                   h->key_bits, h->stream));
+    HIPOK(hipGetLastError());
+    size_t tb = h->sort_tmp_bytes;
+    HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->vals[1], n,
+                     h->key_bits, h->stream));
+  } else {  // one system: counting sort over the search cells
+    const int nf = h->n_fine;
+    hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
+                       h->pos[c], h->state[t], h->gid[t], h->dgc[t], h->fine_rank, h->fine_cnt,
+                       nf, h->keys[0], h->flags);
+    HIPOK(hipGetLastError());
+    CHECKRC(scan_i32(h, h->fine_cnt, h->fine_off, (int64_t)nf + 1));
+    hipLaunchKernelGGL(k_bin_place, dim3(nblk(std::max(n, nf))), dim3(kBS), 0, h->stream, n, nf,
+                       h->keys[0], h->fine_rank, h->fine_off, h->fine_cnt, h->vals[1],
+                       h->fstart, h->fend);
+    HIPOK(hipGetLastError());
+  }
   PermPtrs p{};
   p.perm = h->vals[1];
   p.pos = h->pos[c]; p.vel = h->vel[c]; p.omega = h->omega[c];
@@ -439,15 +450,13 @@ int32_t do_bin(mfixdem_handle* h) {
   HIPOK(hipGetLastError());
   h->cur = 1 - c;
   h->tw = 1 - t;
-  CHECKRC(fill_i32(h->stream, h->fstart, h->n_fine, 0));
-  CHECKRC(fill_i32(h->stream, h->fend, h->n_fine, 0));
-  if (batch)
+  if (batch) {
+    CHECKRC(fill_i32(h->stream, h->fstart, h->n_fine, 0));
+    CHECKRC(fill_i32(h->stream, h->fend, h->n_fine, 0));
     hipLaunchKernelGGL((k_cell_ranges<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
                        h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
-  else
-    hipLaunchKernelGGL((k_cell_ranges<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->G,
-                       h->keys[1], h->state[h->tw], h->fstart, h->fend, h->B);
-  HIPOK(hipGetLastError());
+    HIPOK(hipGetLastError());
+  }
   if (h->slot_of_gid) {  // inlets place their seeds by global id
     hipLaunchKernelGGL(k_slot_of_gid, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[h->tw],
                        h->slot_of_gid);
@@ -500,7 +509,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   b.pos = h->pos[c]; b.radius = h->radius[t]; b.gid = h->gid[t]; b.dgc = h->dgc[t];
   b.xr = h->xr_pack;
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
-  b.own_cnt = h->own_cnt; b.flags = h->flags;
+  b.own_cnt = h->own_cnt; b.flags = h->flags; b.in_cnt = h->in_cnt;
   int fl[kFlagCount];
   // (the wavefront row kernels take normal particles only: boundaries use
   // the thread kernel, whose owner rule knows entering and exiting ones)
@@ -521,19 +530,17 @@ int32_t do_neighbour(mfixdem_handle* h) {
       const int64_t threads = (int64_t)h->n_fine * 64;
       launch_rows_wave(h, b, threads);
     } else {
-      // half-stencil rows appended through atomic slots, then sorted (the
-      // sort also counts the incoming entries)
+      // half-stencil rows appended through atomic slots (each entry also
+      // counted in its partner's incoming row), put in order by k_place
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
       hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, b.pos,
                          b.radius, b.state, h->xr_pack);
+      const dim3 rg(nblk((int64_t)n * kRowLanes));  // kRowLanes threads per particle
       if (h->nsys > 1)
-        hipLaunchKernelGGL((k_own_rows<true>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
-                           h->G, b, h->B);
+        hipLaunchKernelGGL((k_own_rows<true>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);
       else
-        hipLaunchKernelGGL((k_own_rows<false>), dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S,
-                           h->G, b, h->B);
-      hipLaunchKernelGGL(k_sort_rows, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, b, h->in_cnt);
+        hipLaunchKernelGGL((k_own_rows<false>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);
     }
     HIPOK(hipGetLastError());
     // own rows -> pair ids, read back with the flags in the same wait (an
@@ -582,8 +589,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   cp.pfl_old = h->pfl[h->pcur];
   cp.pfl_new = h->pfl[1 - h->pcur];
   hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
-                     h->pair_base[nl], h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur,
-                     h->nbr[nl], h->nbr_pid[nl], cp);
+                     nsq || wave_rows ? nullptr : h->own_key, h->pair_base[nl],
+                     h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur, h->nbr[nl],
+                     h->nbr_pid[nl], cp);
   hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
                      h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
   HIPOK(hipGetLastError());
@@ -1036,7 +1044,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   }
   f(h->fc); f(h->tow); f(h->fc_id); f(h->tow_id); f(h->sort_tmp); f(h->rm_cnt); f(h->rm_list); f(h->f_gp); f(h->drag_fc);
   f(h->xr_pack);
-  f(h->fstart); f(h->fend);
+  f(h->fstart); f(h->fend); f(h->fine_rank); f(h->fine_cnt); f(h->fine_off);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
   f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->csz); f(h->xe); f(h->yn);
   f(h->zt); f(h->vnode); f(h->ugc); f(h->gas_stage); f(h->p_force); f(h->ep_g); f(h->rop_s);
@@ -1051,6 +1059,38 @@ void for_each_buffer(mfixdem_handle* h, F f) {
 void fill_devparams(mfixdem_handle* h) {
   fill_params(h->S, h->n, h->stride, h->P, h->G);
   h->P.nbc = (int)h->dbc.b.size();
+}
+
+// Search cells of one system for the counting sort: the rank of each cell
+// (x fastest) in the Morton order of its (fx, fy, fz), so that the binned
+// particles lie in Morton order; zeroed counts.
+int32_t setup_fine_cells(mfixdem_handle* h) {
+  dfree(h->fine_rank);
+  dfree(h->fine_cnt);
+  dfree(h->fine_off);
+  if (h->nsys > 1) return MFIXDEM_OK;
+  const GridDev& G = h->G;
+  const int nf = h->n_fine;
+  if ((int64_t)G.fn[0] * G.fn[1] * G.fn[2] != nf) return MFIXDEM_ERR_ARG;
+  auto spread = [](uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 10; b++) r |= ((v >> b) & 1ull) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, int>> code((size_t)nf);
+  for (int z = 0, c = 0; z < G.fn[2]; z++)
+    for (int y = 0; y < G.fn[1]; y++)
+      for (int x = 0; x < G.fn[0]; x++, c++)
+        code[c] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), c};
+  std::sort(code.begin(), code.end());
+  std::vector<int> rank((size_t)nf);
+  for (int k = 0; k < nf; k++) rank[code[k].second] = k;
+  CHECKRC(dalloc(&h->fine_rank, nf));
+  CHECKRC(dalloc(&h->fine_cnt, (int64_t)nf + 1));
+  CHECKRC(dalloc(&h->fine_off, (int64_t)nf + 2));
+  HIPOK(hipMemcpy(h->fine_rank, rank.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+  HIPOK(hipMemset(h->fine_cnt, 0, sizeof(int) * ((size_t)nf + 1)));
+  return MFIXDEM_OK;
 }
 
 // Device buffers, stream and events of a handle whose set-up (S, P, G,
@@ -1084,7 +1124,9 @@ int32_t alloc_buffers(mfixdem_handle* h) {
       (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
       (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
       (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
-      (rc = dalloc(&h->scan_bsum, std::max<int64_t>(n, nc) / kScanTile + 2)) ||
+      (rc = dalloc(&h->scan_bsum,
+                   std::max<int64_t>(std::max<int64_t>(n, nc), (int64_t)h->n_fine + 1) /
+                           kScanTile + 2)) ||
       (rc = dalloc(&h->flags, kFlagCount)) || (rc = dalloc(&h->xe, h->S.ni + 1)) ||
       (rc = dalloc(&h->csz, h->S.ni + h->S.nj + h->S.nk)) ||
       (rc = dalloc(&h->yn, h->S.nj + 1)) || (rc = dalloc(&h->zt, h->S.nk + 1)) ||
@@ -1116,7 +1158,7 @@ int32_t alloc_buffers(mfixdem_handle* h) {
     return fail(MFIXDEM_ERR_HIP);
   if (hipMalloc(&h->sort_tmp, std::max<size_t>(h->sort_tmp_bytes, 16)) != hipSuccess)
     return fail(MFIXDEM_ERR_OOM);
-  if ((rc = upload_setup(h))) return fail(rc);
+  if ((rc = upload_setup(h)) || (rc = setup_fine_cells(h))) return fail(rc);
   if (hipMemset(h->flags, 0, sizeof(int) * kFlagCount) != hipSuccess) return fail(MFIXDEM_ERR_HIP);
   return MFIXDEM_OK;
 }
@@ -2142,6 +2184,7 @@ int32_t mfixdem_set_tuning(mfixdem_handle* h, double cell_scale, int32_t row_ker
   CHECKRC(dalloc(&h->fstart, n_fine));
   CHECKRC(dalloc(&h->fend, n_fine));
   h->n_fine = n_fine;
+  CHECKRC(setup_fine_cells(h));
   h->cell_scale = cell_scale;
   h->row_kernel = row_kernel;
   h->binned = false;

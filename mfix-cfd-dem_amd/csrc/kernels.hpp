@@ -1426,6 +1426,11 @@ struct GridDev {
   int fn[3];
   double f_lo[3], f_inv;
   int mbits;  // bits of the Morton code in the sort key
+  // single-precision prefilter of the row walk: coordinates relative to
+  // pf_c; |x - pf_c| <= pf_xmax on every axis for particles it covers;
+  // pf_slack bounds the rounding of a distance so computed
+  double pf_c[3];
+  float pf_xmax, pf_slack;
 };
 
 __device__ __forceinline__ int floor_sat_d(double v) {
@@ -1656,6 +1661,8 @@ __global__ __launch_bounds__(kBS) void k_cell_ranges(int n, GridDev G0, const ui
 struct BuildPtrs {
   const double *pos, *radius;
   const double* xr;             // [4*i + c]: x, y, z, radius (k_pack_xr), read per candidate
+  const float4* xf;             // the same in single precision about GridDev::pf_c (radius
+                                // < 0: outside the prefilter's range)
   const int *gid, *dgc;
   const uint8_t* state;
   const int *fstart, *fend;     // search-grid cell ranges
@@ -1730,6 +1737,23 @@ __device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b
   }
 }
 
+// A candidate that passed the walk's prefilter: the cut-off test of
+// DESGRID_NEIGH_BUILD in double precision (desgrid_mod.f:1177-1183), then
+// the owner and stencil checks.  A nonexistent candidate sits far away and
+// fails the test.
+template <bool BATCH>
+__device__ __forceinline__ void own_row_exact(const GridDev& G, const BuildPtrs& b, int S,
+                                              int g_r, int h, int gh, const double x[3],
+                                              double r, int t) {
+  const double2* xr2 = reinterpret_cast<const double2*>(b.xr);
+  const double2 u = xr2[2 * t], v = xr2[2 * t + 1];  // x, y | z, radius: one 32-byte line
+  const double sr = G.factor_rlm * (r + v.y);
+  const double d0 = x[0] - u.x, d1 = x[1] - u.y, d2 = x[2] - v.x;
+  const double ds = d0 * d0 + d1 * d1 + d2 * d2;
+  if (ds > sr * sr) return;
+  own_row_hit<BATCH>(G, b, S, g_r, h, gh, t);
+}
+
 // kRowLanes lanes share one particle's half stencil (lane p takes the cells
 // p, p + kRowLanes, ...): shorter candidate walks per lane, and the range
 // table each lane keeps in LDS shrinks with them, so more wavefronts fit on a
@@ -1783,49 +1807,66 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
       m++;
     }
   if (m == 0) return;
+  // The walk tests each candidate in single precision, 16 bytes a candidate,
+  // with a margin no rounding can cross; the few that pass are tested again
+  // in double precision (the reference's criterion) before they count.
+  const float4 xh = b.xf[h];
+  const bool coarse = xh.w >= 0.0f;  // h inside the prefilter's range
+  const float frlm = (float)G.factor_rlm * (1.0f + 1.0f / 65536.0f);
   int k = 0, t = s_lo[0][threadIdx.x], t1 = s_hi[0][threadIdx.x], nh = 0;
-  // candidate t's data, loaded one step ahead
-  const double2* xr2 = reinterpret_cast<const double2*>(b.xr);
-  double2 u_t = xr2[2 * t], v_t = xr2[2 * t + 1];  // x, y | z, radius: one 32-byte line
+  float4 u = b.xf[t];  // candidate t, loaded one step ahead
 #pragma unroll 1
   while (k < m) {
     const int tc = t;
-    const double rc = v_t.y, pc[3] = {u_t.x, u_t.y, v_t.x};
+    const float4 uc = u;
     if (++t == t1 && ++k < m) {
       t = s_lo[k][threadIdx.x];
       t1 = s_hi[k][threadIdx.x];
     }
     const int tn = k < m ? t : tc;  // in range either way
-    u_t = xr2[2 * tn];
-    v_t = xr2[2 * tn + 1];
-    const double sr = G.factor_rlm * (r + rc);  // a removed candidate fails the test below
-    const double d0 = x[0] - pc[0], d1 = x[1] - pc[1], d2 = x[2] - pc[2];
-    const double ds = d0 * d0 + d1 * d1 + d2 * d2;
-    if (ds > sr * sr) continue;
+    u = b.xf[tn];
+    const float lim = frlm * (xh.w + uc.w) + G0.pf_slack;  // the grid k_pack_xr used
+    const float e0 = xh.x - uc.x, e1 = xh.y - uc.y, e2 = xh.z - uc.z;
+    if (coarse && uc.w >= 0.0f && e0 * e0 + e1 * e1 + e2 * e2 > lim * lim) continue;
     if (nh < kHits) s_hit[nh++][threadIdx.x] = tc;
-    else own_row_hit<BATCH>(G, b, S, g_r, h, gh, tc);  // a crowded stencil: at once
+    else own_row_exact<BATCH>(G, b, S, g_r, h, gh, x, r, tc);  // a crowded stencil: at once
   }
 #pragma unroll 1
-  for (int e = 0; e < nh; e++) own_row_hit<BATCH>(G, b, S, g_r, h, gh, s_hit[e][threadIdx.x]);
+  for (int e = 0; e < nh; e++)
+    own_row_exact<BATCH>(G, b, S, g_r, h, gh, x, r, s_hit[e][threadIdx.x]);
 }
 
 // Position and radius of particles [0, n) packed per particle for the
 // candidate walk of k_own_rows (one cache line per candidate instead of five).
 // A nonexistent particle gets a position no cut-off reaches, so the walk
 // needs no state load; the hit path checks the state itself.
+// The single-precision copy is relative to G.pf_c; a particle beyond
+// pf_xmax on some axis gets radius -1 there (every test with it is then
+// made in double precision), a nonexistent one a position no cut-off
+// reaches.
 constexpr double kFarAway = 1.0e300;
-__global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, const double* pos,
+__global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, GridDev G, const double* pos,
                                                  const double* radius, const uint8_t* state,
-                                                 double* xr) {
+                                                 double* xr, float4* xf) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   double2* o = reinterpret_cast<double2*>(xr) + 2 * i;
   if (state[i] == 0) {
     o[0] = make_double2(kFarAway, kFarAway);
     o[1] = make_double2(kFarAway, radius[i]);
+    xf[i] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   } else {
-    o[0] = make_double2(pos[i], pos[S + i]);
-    o[1] = make_double2(pos[2 * S + i], radius[i]);
+    const double p[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+    o[0] = make_double2(p[0], p[1]);
+    o[1] = make_double2(p[2], radius[i]);
+    float f[3];
+    bool in = true;
+    for (int d = 0; d < 3; d++) {
+      const double q = p[d] - G.pf_c[d];
+      in = in && fabs(q) <= (double)G.pf_xmax;
+      f[d] = (float)q;
+    }
+    xf[i] = make_float4(f[0], f[1], f[2], in ? (float)radius[i] : -1.0f);
   }
 }
 
@@ -2172,6 +2213,42 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
   const int pb = pair_base[i];
   const int o = c.perm[i];
   const int os = c.old_split[o], oe = c.old_start[o + 1], opb = c.old_pair_base[o];
+  if (cnt <= kRegRow && oe - os <= kRegRow) {
+    // a short row with a short old row: every load, atomic and store of
+    // the row issued side by side rather than one entry after another
+    int jv[kRegRow], nj[kRegRow], pj[kRegRow], sl[kRegRow], old[kRegRow];
+#pragma unroll
+    for (int k = 0; k < kRegRow; k++)
+      if (k < cnt) jv[k] = own_tmp[(size_t)k * n + i];
+#pragma unroll
+    for (int r = 0; r < kRegRow; r++)
+      if (r < oe - os) old[r] = c.old_nbr[os + r];
+#pragma unroll
+    for (int k = 0; k < kRegRow; k++)
+      if (k < cnt) {
+        nj[k] = nbr_start[jv[k]];
+        pj[k] = c.perm[jv[k]];
+        sl[k] = atomicAdd(&in_cur[jv[k]], 1);
+      }
+#pragma unroll
+    for (int k = 0; k < kRegRow; k++)
+      if (k < cnt) {
+        nbr[split + k] = jv[k];
+        nbr_pid[split + k] = pb + k;
+        nbr[nj[k] + sl[k]] = i;
+        nbr_pid[nj[k] + sl[k]] = pb + k;
+        int op = -1;
+#pragma unroll
+        for (int r = kRegRow - 1; r >= 0; r--)  // the first match wins
+          if (r < oe - os && old[r] == pj[k]) op = opb + r;
+        const int p = pb + k;
+        const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
+        c.pfl_new[p] = f;
+        if (f)
+          for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
+      }
+    return;
+  }
   for (int k = 0; k < cnt; k++) {
     const int j = own_tmp[(size_t)k * n + i];
     nbr[split + k] = j;

@@ -508,6 +508,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   BuildPtrs b{};
   b.pos = h->pos[c]; b.radius = h->radius[t]; b.gid = h->gid[t]; b.dgc = h->dgc[t];
   b.xr = h->xr_pack;
+  // the single-precision copy for the walk's prefilter lives in the export
+  // stage (3 doubles per slot), which no call uses during a rebuild
+  b.xf = reinterpret_cast<float4*>(h->stage);
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
   b.own_cnt = h->own_cnt; b.flags = h->flags; b.in_cnt = h->in_cnt;
   int fl[kFlagCount];
@@ -534,8 +537,8 @@ int32_t do_neighbour(mfixdem_handle* h) {
       // counted in its partner's incoming row), put in order by k_place
       CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
       CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
-      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, b.pos,
-                         b.radius, b.state, h->xr_pack);
+      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b.pos,
+                         b.radius, b.state, h->xr_pack, reinterpret_cast<float4*>(h->stage));
       const dim3 rg(nblk((int64_t)n * kRowLanes));  // kRowLanes threads per particle
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);

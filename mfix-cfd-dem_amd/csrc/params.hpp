@@ -111,6 +111,16 @@ inline int32_t search_grid(const mfixdem_params& p, int64_t n, GridDev& G, int& 
   }
   G.f_inv = 1.0 / hcut;
   n_fine = G.fn[0] * G.fn[1] * G.fn[2];
+  // prefilter of the row walk: single-precision coordinates about the
+  // grid's centre, good to 2^-24 of the half extent each; a computed
+  // distance is then within 2^-20 of it (generously) of the true one
+  double half = 0.0;
+  for (int d = 0; d < 3; d++) {
+    G.pf_c[d] = lo[d] + 0.5 * G.fn[d] * hcut;
+    half = std::max(half, 0.5 * G.fn[d] * hcut);
+  }
+  G.pf_xmax = (float)half;
+  G.pf_slack = (float)std::ldexp((double)G.pf_xmax, -20);
   (void)n;
   G.mbits = 3 * morton_axis_bits(std::max(G.fn[0], std::max(G.fn[1], G.fn[2])));
   return MFIXDEM_OK;

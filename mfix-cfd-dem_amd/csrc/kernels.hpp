@@ -1335,27 +1335,28 @@ __global__ __launch_bounds__(kBS) void k_deposit_filter(DevParams P, const int* 
 }
 
 // ---------------------------------------------------------------- scans
-__device__ __forceinline__ int block_excl_scan(int v, int* total, int* lds) {
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* total, T* lds) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int x = v;
+  T x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
+    const T y = __shfl_up(x, o, 64);
     if (lane >= o) x += y;
   }
   if (lane == 63) lds[wid] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int s = 0;
+    T s = 0;
     for (int w = 0; w < kBS / 64; w++) {
-      const int t = lds[w];
+      const T t = lds[w];
       lds[w] = s;
       s += t;
     }
     lds[kBS / 64] = s;
   }
   __syncthreads();
-  const int res = x - v + lds[wid];
+  const T res = x - v + lds[wid];
   *total = lds[kBS / 64];
   __syncthreads();
   return res;
@@ -1404,6 +1405,65 @@ __global__ __launch_bounds__(kBS) void k_scan_add(int* out, int n, const int* bs
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i < n) out[i] += bsum[i / kScanTile];
   if (i == 0) out[n] = bsum[nb];
+}
+
+// The offsets of the neighbour rows in one scan: each particle's own and
+// incoming counts packed as (own << 32) + in, so one exclusive scan gives
+// the pair ids (high half) and the row starts (the sum of the halves).
+// The low halves never carry: their total is the pair count, below 2^31.
+__global__ __launch_bounds__(kBS) void k_scan_rows_tiles(const int* own, const int* in,
+                                                         int* pair_base, int* row_start, int n,
+                                                         long long* bsum) {
+  __shared__ long long lds[kBS / 64 + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  long long v[kScanPer], s = 0;
+  for (int q = 0; q < kScanPer; q++) {
+    v[q] = (base + q < n) ? ((long long)own[base + q] << 32) + in[base + q] : 0;
+    s += v[q];
+  }
+  long long tot;
+  long long pre = block_excl_scan(s, &tot, lds);
+  for (int q = 0; q < kScanPer; q++) {
+    if (base + q < n) {
+      pair_base[base + q] = (int)(pre >> 32);
+      row_start[base + q] = (int)(pre >> 32) + (int)(pre & 0xffffffffll);
+    }
+    pre += v[q];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_sums64(long long* bsum, int nb) {
+  __shared__ long long lds[kBS / 64 + 1];
+  const int per = (nb + kBS - 1) / kBS;
+  const int b0 = threadIdx.x * per;
+  long long s = 0;
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) s += bsum[b0 + q];
+  long long tot;
+  long long pre = block_excl_scan(s, &tot, lds);
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) {
+      const long long t = bsum[b0 + q];
+      bsum[b0 + q] = pre;
+      pre += t;
+    }
+  if (threadIdx.x == 0) bsum[nb] = tot;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_rows_add(int* pair_base, int* row_start, int n,
+                                                       const long long* bsum, int nb) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) {
+    const long long o = bsum[i / kScanTile];
+    pair_base[i] += (int)(o >> 32);
+    row_start[i] += (int)(o >> 32) + (int)(o & 0xffffffffll);
+  }
+  if (i == 0) {
+    const long long t = bsum[nb];
+    pair_base[n] = (int)(t >> 32);
+    row_start[n] = (int)(t >> 32) + (int)(t & 0xffffffffll);
+  }
 }
 
 __global__ void k_fill_i32(int* a, int n, int v) {
@@ -1553,8 +1613,23 @@ __global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, cons
     }
     r = cell_rank[f3[0] + G.fn[0] * (f3[1] + G.fn[1] * f3[2])];
   }
-  const int w = atomicAdd(&cell_cnt[r], 1);
-  place[i] = ((uint64_t)r << 32) | (uint32_t)w;
+  // The particles are still in the last binning's order, so neighbouring
+  // lanes mostly share a cell: each run of equal cells takes its places
+  // with one atomic, by its first lane.  The active lanes are a prefix of
+  // the wavefront (lanes past n returned above), so a run ends at the next
+  // run's first lane or at the last active lane.
+  const int lane = __lane_id();
+  const int prev = __shfl_up(r, 1, 64);
+  const unsigned long long act = __ballot(1);
+  const unsigned long long head = __ballot(lane == 0 || prev != r);
+  const unsigned long long below = head & ((2ull << lane) - 1ull);   // heads at or below lane
+  const int first = 63 - __clzll(below);
+  const unsigned long long above = head & ~((2ull << lane) - 1ull);   // heads after lane
+  const int last = above ? __builtin_ctzll(above) - 1 : 63 - __clzll(act);
+  int w0 = 0;
+  if (lane == first) w0 = atomicAdd(&cell_cnt[r], last - first + 1);
+  w0 = __shfl(w0, first, 64);
+  place[i] = ((uint64_t)r << 32) | (uint32_t)(w0 + lane - first);
 }
 
 // The new order (perm[new slot] = old slot) from the counted places and the
@@ -1564,11 +1639,15 @@ __global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, cons
 __global__ __launch_bounds__(kBS) void k_bin_place(int n, int nfine, const uint64_t* place,
                                                    const int* cell_rank, const int* cell_off,
                                                    int* cell_cnt, int* perm, int* cstart,
-                                                   int* cend) {
+                                                   int* cend, int* zero0, int* zero1,
+                                                   int* zero2) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i < n) {
     const uint64_t p = place[i];
     perm[cell_off[(int)(p >> 32)] + (int)(uint32_t)p] = i;
+    zero0[i] = 0;  // the row counters of the neighbour build that follows
+    zero1[i] = 0;
+    zero2[i] = 0;
   }
   if (i < nfine) {
     const int r = cell_rank[i];

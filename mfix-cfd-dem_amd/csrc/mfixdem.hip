@@ -106,6 +106,8 @@ struct mfixdem_handle {
   int64_t pft_cap[2]{0, 0};
   int64_t n_pairs = 0;
   int* scan_bsum = nullptr;
+  long long* scan_bsum64 = nullptr;
+  bool counts_zeroed = false;  // own_cnt, in_cnt, in_cur cleared by the last binning
   int* flags = nullptr;
   // gas
   double* ugc = nullptr;  // cell-centre gas velocity (filter schemes)
@@ -216,6 +218,21 @@ int32_t scan_i32(mfixdem_handle* h, const int* in, int* out, int64_t n) {
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBS), 0, h->stream, h->scan_bsum, std::max(nb, 1));
   hipLaunchKernelGGL(k_scan_add, dim3(nblk(n + 1)), dim3(kBS), 0, h->stream, out, (int)n,
                      h->scan_bsum, std::max(nb, 1));
+  HIPOK(hipGetLastError());
+  return MFIXDEM_OK;
+}
+
+// Pair ids and row starts of the neighbour lists in one scan of the own and
+// incoming counts (k_scan_rows_tiles).
+int32_t scan_rows(mfixdem_handle* h, const int* own, const int* in, int* pair_base,
+                  int* row_start, int64_t n) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_rows_tiles, dim3(std::max(nb, 1)), dim3(kBS), 0, h->stream, own, in,
+                     pair_base, row_start, (int)n, h->scan_bsum64);
+  hipLaunchKernelGGL(k_scan_sums64, dim3(1), dim3(kBS), 0, h->stream, h->scan_bsum64,
+                     std::max(nb, 1));
+  hipLaunchKernelGGL(k_scan_rows_add, dim3(nblk(n + 1)), dim3(kBS), 0, h->stream, pair_base,
+                     row_start, (int)n, h->scan_bsum64, std::max(nb, 1));
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
@@ -428,8 +445,9 @@ int32_t do_bin(mfixdem_handle* h) {
     CHECKRC(scan_i32(h, h->fine_cnt, h->fine_off, (int64_t)nf + 1));
     hipLaunchKernelGGL(k_bin_place, dim3(nblk(std::max(n, nf))), dim3(kBS), 0, h->stream, n, nf,
                        h->keys[0], h->fine_rank, h->fine_off, h->fine_cnt, h->vals[1],
-                       h->fstart, h->fend);
+                       h->fstart, h->fend, h->own_cnt, h->in_cnt, h->in_cur);
     HIPOK(hipGetLastError());
+    h->counts_zeroed = true;
   }
   PermPtrs p{};
   p.perm = h->vals[1];
@@ -522,6 +540,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
   const int nl = 1 - h->lt;
   int npairs = 0;
+  bool zeroed = h->counts_zeroed;       // row counters cleared by the binning
+  const bool cur_zeroed = zeroed;
+  h->counts_zeroed = false;
   for (;;) {
     b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow; b.ld = n;
     if (nsq) {
@@ -535,8 +556,10 @@ int32_t do_neighbour(mfixdem_handle* h) {
     } else {
       // half-stencil rows appended through atomic slots (each entry also
       // counted in its partner's incoming row), put in order by k_place
-      CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
-      CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
+      if (!zeroed) {
+        CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
+        CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
+      }
       hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b.pos,
                          b.radius, b.state, h->xr_pack, reinterpret_cast<float4*>(h->stage));
       const dim3 rg(nblk((int64_t)n * kRowLanes));  // kRowLanes threads per particle
@@ -546,9 +569,12 @@ int32_t do_neighbour(mfixdem_handle* h) {
         hipLaunchKernelGGL((k_own_rows<false>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);
     }
     HIPOK(hipGetLastError());
-    // own rows -> pair ids, read back with the flags in the same wait (an
-    // overflowing pass is built again and its pair ids with it)
-    CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
+    zeroed = false;
+    // own rows -> pair ids (and, with the thread rows, whose incoming counts
+    // are known already, the row starts), read back with the flags in the
+    // same wait (an overflowing pass is built again and its pair ids with it)
+    if (nsq || wave_rows) CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
+    else CHECKRC(scan_rows(h, h->own_cnt, h->in_cnt, h->pair_base[nl], h->nbr_start[nl], n));
     HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
                          h->stream));
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
@@ -577,12 +603,12 @@ int32_t do_neighbour(mfixdem_handle* h) {
     CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
     hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
                        h->own_tmp, h->maxrow, h->in_cnt);
+    hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
+                       h->own_cnt, h->tot_cnt);
+    HIPOK(hipGetLastError());
+    CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
   }
-  hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
-                     h->own_cnt, h->tot_cnt);
-  HIPOK(hipGetLastError());
-  CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
-  CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
+  if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
   CarryPtrs cp{};
   cp.perm = h->vals[1];
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
@@ -1049,7 +1075,7 @@ void for_each_buffer(mfixdem_handle* h, F f) {
   f(h->xr_pack);
   f(h->fstart); f(h->fend); f(h->fine_rank); f(h->fine_cnt); f(h->fine_off);
   f(h->own_cnt); f(h->own_tmp); f(h->own_key); f(h->in_cnt); f(h->in_cur);
-  f(h->tot_cnt); f(h->scan_bsum); f(h->flags); f(h->csz); f(h->xe); f(h->yn);
+  f(h->tot_cnt); f(h->scan_bsum); f(h->scan_bsum64); f(h->flags); f(h->csz); f(h->xe); f(h->yn);
   f(h->zt); f(h->vnode); f(h->ugc); f(h->gas_stage); f(h->p_force); f(h->ep_g); f(h->rop_s);
   f(h->rops_node); f(h->vol); f(h->vol_surr); f(h->ro_s); f(h->fluid);
   f(h->pic_cnt); f(h->pic_off); f(h->pic_cur); f(h->pic);
@@ -1127,6 +1153,7 @@ int32_t alloc_buffers(mfixdem_handle* h) {
       (rc = dalloc(&h->own_cnt, n)) || (rc = dalloc(&h->own_tmp, n * h->maxrow)) ||
       (rc = dalloc(&h->own_key, n * h->maxrow)) || (rc = dalloc(&h->in_cnt, n)) ||
       (rc = dalloc(&h->in_cur, n)) || (rc = dalloc(&h->tot_cnt, n)) ||
+      (rc = dalloc(&h->scan_bsum64, n / kScanTile + 2)) ||
       (rc = dalloc(&h->scan_bsum,
                    std::max<int64_t>(std::max<int64_t>(n, nc), (int64_t)h->n_fine + 1) /
                            kScanTile + 2)) ||

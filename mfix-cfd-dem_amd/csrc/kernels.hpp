@@ -1910,9 +1910,75 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
     if (nh < kHits) s_hit[nh++][threadIdx.x] = tc;
     else own_row_exact<BATCH>(G, b, S, g_r, h, gh, x, r, tc);  // a crowded stencil: at once
   }
-#pragma unroll 1
-  for (int e = 0; e < nh; e++)
-    own_row_exact<BATCH>(G, b, S, g_r, h, gh, x, r, s_hit[e][threadIdx.x]);
+  if (nh == 0) return;
+  // The waiting candidates, all at once: their loads issued together, the
+  // owner and stencil checks with h's own cell and stencil formed once, the
+  // row slots taken with atomics issued side by side.  Same rules as
+  // own_row_exact and own_row_hit.
+  const uint8_t sh = b.state[h];
+  int hc[3], hlo[3], hhi[3];
+  dg_decode(G, b.dgc[h], hc[0], hc[1], hc[2]);
+  stencil_off(G, x, hlo, hhi);
+  const double2* xr2 = reinterpret_cast<const double2*>(b.xr);
+  int tv[kHits], own[kHits], oth[kHits], slot[kHits];
+  uint64_t key[kHits];
+  bool ok[kHits];
+#pragma unroll
+  for (int e = 0; e < kHits; e++) {
+    ok[e] = false;
+    if (e < nh) tv[e] = s_hit[e][threadIdx.x];
+  }
+#pragma unroll
+  for (int e = 0; e < kHits; e++) {
+    if (e >= nh) continue;
+    const int t = tv[e];
+    const double2 u = xr2[2 * t], v = xr2[2 * t + 1];
+    const uint8_t stt = b.state[t];
+    const int gt = b.gid[t], dt = b.dgc[t];
+    const double sr = G.factor_rlm * (r + v.y);
+    const double d0 = x[0] - u.x, d1 = x[1] - u.y, d2 = x[2] - v.x;
+    if (d0 * d0 + d1 * d1 + d2 * d2 > sr * sr) continue;  // DESGRID_NEIGH_BUILD cut-off
+    if (stt == kNonexistent || (sh != kNormal && stt != kNormal)) continue;
+    const bool h_owns = sh == kNormal && (stt != kNormal || gh < gt);
+    if ((h_owns ? gh : gt) >= g_r) continue;
+    int oc[3], lo[3], hi[3], pc[3];
+    if (h_owns) {
+      for (int d = 0; d < 3; d++) {
+        oc[d] = hc[d];
+        lo[d] = hlo[d];
+        hi[d] = hhi[d];
+      }
+      dg_decode(G, dt, pc[0], pc[1], pc[2]);
+    } else {
+      dg_decode(G, dt, oc[0], oc[1], oc[2]);
+      const double xt[3] = {u.x, u.y, v.x};
+      stencil_off(G, xt, lo, hi);
+      for (int d = 0; d < 3; d++) pc[d] = hc[d];
+    }
+    if (dg_ghost(G, oc[0], oc[1], oc[2])) continue;
+    bool in = true;
+    for (int d = 0; d < 3; d++) in = in && pc[d] >= oc[d] - lo[d] && pc[d] <= oc[d] + hi[d];
+    if (!in) continue;
+    ok[e] = true;
+    own[e] = h_owns ? h : t;
+    oth[e] = h_owns ? t : h;
+    key[e] = ((uint64_t)((pc[2] * 1024 + pc[1]) * 1024 + pc[0]) << 32) |
+             (uint32_t)(h_owns ? gt : gh);
+  }
+#pragma unroll
+  for (int e = 0; e < kHits; e++)
+    if (ok[e]) slot[e] = atomicAdd(&b.own_cnt[own[e]], 1);
+#pragma unroll
+  for (int e = 0; e < kHits; e++) {
+    if (!ok[e]) continue;
+    if (slot[e] < b.maxrow) {
+      b.own_tmp[(size_t)slot[e] * b.ld + own[e]] = oth[e];
+      b.own_key[(size_t)slot[e] * b.ld + own[e]] = key[e];
+      atomicAdd(&b.in_cnt[oth[e]], 1);
+    } else {
+      atomicMax(&b.flags[kFlagOwnOverflow], slot[e] + 1);
+    }
+  }
 }
 
 // Position and radius of particles [0, n) packed per particle for the

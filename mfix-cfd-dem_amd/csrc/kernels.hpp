@@ -1587,7 +1587,7 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const dou
 // a last cell after them) and keeps the place it drew.  A cell's particles
 // end up in no particular order among themselves (no result depends on it).
 __global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, const double* pos,
-                                                   const uint8_t* state, const int* gid,
+                                                   uint8_t* state, const int* gid,
                                                    int* dgc, const int* cell_rank,
                                                    int* cell_cnt, int nfine, uint64_t* place,
                                                    int* flags) {
@@ -1597,6 +1597,7 @@ __global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, cons
   if (state[i] == kNonexistent) {
     dgc[i] = 0;
   } else {
+    if (state[i] == kSeeded) state[i] = kEntering;  // binned now: in a DES cell list
     const double x[3] = {pos[i], pos[S + i], pos[2 * S + i]};
     const int nn[3] = {G.dg_ni, G.dg_nj, G.dg_nk};
     int c3[3], f3[3];
@@ -1752,6 +1753,8 @@ struct BuildPtrs {
   int maxrow;
   int* flags;
   int* in_cnt;                  // incoming entries per particle (thread-row kernel)
+  const int* idx;               // slot at each position of the search-cell order, when the
+                                // binning left the slots as they were (null: the same)
 };
 
 // Own rows of DESGRID_NEIGH_BUILD (desgrid_mod.f:1057-1214): j in the
@@ -1816,10 +1819,10 @@ __device__ __forceinline__ void own_row_hit(const GridDev& G, const BuildPtrs& b
   }
 }
 
-// A candidate that passed the walk's prefilter: the cut-off test of
-// DESGRID_NEIGH_BUILD in double precision (desgrid_mod.f:1177-1183), then
-// the owner and stencil checks.  A nonexistent candidate sits far away and
-// fails the test.
+// A candidate that passed the walk's prefilter (t: its position in the
+// search-cell order): the cut-off test of DESGRID_NEIGH_BUILD in double
+// precision (desgrid_mod.f:1177-1183), then the owner and stencil checks.
+// A nonexistent candidate sits far away and fails the test.
 template <bool BATCH>
 __device__ __forceinline__ void own_row_exact(const GridDev& G, const BuildPtrs& b, int S,
                                               int g_r, int h, int gh, const double x[3],
@@ -1830,7 +1833,7 @@ __device__ __forceinline__ void own_row_exact(const GridDev& G, const BuildPtrs&
   const double d0 = x[0] - u.x, d1 = x[1] - u.y, d2 = x[2] - v.x;
   const double ds = d0 * d0 + d1 * d1 + d2 * d2;
   if (ds > sr * sr) return;
-  own_row_hit<BATCH>(G, b, S, g_r, h, gh, t);
+  own_row_hit<BATCH>(G, b, S, g_r, h, gh, b.idx ? b.idx[t] : t);
 }
 
 // kRowLanes lanes share one particle's half stencil (lane p takes the cells
@@ -1850,8 +1853,10 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   __shared__ int s_lo[kCells][kBS], s_hi[kCells][kBS];
   __shared__ int s_hit[kHits][kBS];
   const int id = blockIdx.x * kBS + threadIdx.x;
-  const int h = id / kRowLanes, part = id % kRowLanes;
-  if (h >= n || b.state[h] == 0) return;
+  const int hp = id / kRowLanes, part = id % kRowLanes;  // position in the search-cell order
+  if (hp >= n) return;
+  const int h = b.idx ? b.idx[hp] : hp;                   // and slot
+  if (b.state[h] == 0) return;
   const int s = BATCH ? B.sys[h] : 0;
   const GridDev& G = sys_grid<BATCH>(G0, B, s);
   const int cbase = BATCH ? B.cell_off[s] : 0;
@@ -1874,7 +1879,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
     const bool in = q < kHalfCells && fx >= 0 && fx < G.fn[0] && fy >= 0 && fy < G.fn[1] &&
                     fz >= 0 && fz < G.fn[2];
     const int fc = in ? cbase + fx + G.fn[0] * (fy + G.fn[1] * fz) : cbase;
-    lo[c] = q == 0 ? h + 1 : b.fstart[fc];
+    lo[c] = q == 0 ? hp + 1 : b.fstart[fc];
     hi[c] = in ? b.fend[fc] : 0;
   }
   int m = 0;
@@ -1889,7 +1894,7 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   // The walk tests each candidate in single precision, 16 bytes a candidate,
   // with a margin no rounding can cross; the few that pass are tested again
   // in double precision (the reference's criterion) before they count.
-  const float4 xh = b.xf[h];
+  const float4 xh = b.xf[hp];
   const bool coarse = xh.w >= 0.0f;  // h inside the prefilter's range
   const float frlm = (float)G.factor_rlm * (1.0f + 1.0f / 65536.0f);
   int k = 0, t = s_lo[0][threadIdx.x], t1 = s_hi[0][threadIdx.x], nh = 0;
@@ -1931,8 +1936,8 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
 #pragma unroll
   for (int e = 0; e < kHits; e++) {
     if (e >= nh) continue;
-    const int t = tv[e];
-    const double2 u = xr2[2 * t], v = xr2[2 * t + 1];
+    const int tp = tv[e], t = b.idx ? b.idx[tp] : tp;  // position, slot
+    const double2 u = xr2[2 * tp], v = xr2[2 * tp + 1];
     const uint8_t stt = b.state[t];
     const int gt = b.gid[t], dt = b.dgc[t];
     const double sr = G.factor_rlm * (r + v.y);
@@ -1990,16 +1995,17 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
 // made in double precision), a nonexistent one a position no cut-off
 // reaches.
 constexpr double kFarAway = 1.0e300;
-__global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, GridDev G, const double* pos,
-                                                 const double* radius, const uint8_t* state,
-                                                 double* xr, float4* xf) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  double2* o = reinterpret_cast<double2*>(xr) + 2 * i;
+__global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, GridDev G, const int* idx,
+                                                 const double* pos, const double* radius,
+                                                 const uint8_t* state, double* xr, float4* xf) {
+  const int k = blockIdx.x * kBS + threadIdx.x;  // position in the search-cell order
+  if (k >= n) return;
+  const int i = idx ? idx[k] : k;
+  double2* o = reinterpret_cast<double2*>(xr) + 2 * k;
   if (state[i] == 0) {
     o[0] = make_double2(kFarAway, kFarAway);
     o[1] = make_double2(kFarAway, radius[i]);
-    xf[i] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+    xf[k] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   } else {
     const double p[3] = {pos[i], pos[S + i], pos[2 * S + i]};
     o[0] = make_double2(p[0], p[1]);
@@ -2011,7 +2017,7 @@ __global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, GridDev G, const 
       in = in && fabs(q) <= (double)G.pf_xmax;
       f[d] = (float)q;
     }
-    xf[i] = make_float4(f[0], f[1], f[2], in ? (float)radius[i] : -1.0f);
+    xf[k] = make_float4(f[0], f[1], f[2], in ? (float)radius[i] : -1.0f);
   }
 }
 
@@ -2327,7 +2333,7 @@ __global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const i
 
 // history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
 struct CarryPtrs {
-  const int* perm;              // new slot -> old slot
+  const int* perm;              // new slot -> old slot (null: the slots stayed)
   const int *old_start, *old_split, *old_nbr, *old_pair_base;
   const double* pft_old;
   double* pft_new;
@@ -2356,7 +2362,7 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
   if (cnt == 0) return;
   if (own_key && cnt > 1) sort_own_row(own_tmp + i, own_key + i, (size_t)n, cnt);
   const int pb = pair_base[i];
-  const int o = c.perm[i];
+  const int o = c.perm ? c.perm[i] : i;
   const int os = c.old_split[o], oe = c.old_start[o + 1], opb = c.old_pair_base[o];
   if (cnt <= kRegRow && oe - os <= kRegRow) {
     // a short row with a short old row: every load, atomic and store of
@@ -2372,7 +2378,7 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
     for (int k = 0; k < kRegRow; k++)
       if (k < cnt) {
         nj[k] = nbr_start[jv[k]];
-        pj[k] = c.perm[jv[k]];
+        pj[k] = c.perm ? c.perm[jv[k]] : jv[k];
         sl[k] = atomicAdd(&in_cur[jv[k]], 1);
       }
 #pragma unroll
@@ -2401,7 +2407,7 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
     const int slot = atomicAdd(&in_cur[j], 1);
     nbr[nbr_start[j] + slot] = i;
     nbr_pid[nbr_start[j] + slot] = pb + k;
-    const int oj = c.perm[j];
+    const int oj = c.perm ? c.perm[j] : j;
     int op = -1;
     for (int r = os; r < oe; r++)
       if (c.old_nbr[r] == oj) {

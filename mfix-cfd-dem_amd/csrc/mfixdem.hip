@@ -108,6 +108,14 @@ struct mfixdem_handle {
   int* scan_bsum = nullptr;
   long long* scan_bsum64 = nullptr;
   bool counts_zeroed = false;  // own_cnt, in_cnt, in_cur cleared by the last binning
+  // A binning of one system with the thread-row search moves the particle
+  // data into the search-cell order only every kPermuteEvery-th time (and
+  // the first time); otherwise the slots stay and bin_idx lists them in that
+  // order.  bin_perm: new slot -> old slot of the last binning (null: none).
+  const int* bin_idx = nullptr;
+  const int* bin_perm = nullptr;
+  int bins_since_permute = 0;
+  bool order_set = false;
   int* flags = nullptr;
   // gas
   double* ugc = nullptr;  // cell-centre gas velocity (filter schemes)
@@ -413,12 +421,25 @@ int32_t ensure_pft_capacity(mfixdem_handle* h, int b, int64_t pairs) {
   return MFIXDEM_OK;
 }
 
+// The particle data moves into the search-cell order every kPermuteEvery-th
+// binning: particles move little between binnings (at most a fraction of a
+// search cell in NEIGHBOR_SEARCH_N steps), so the slots stay close to that
+// order for many binnings.
+#ifndef MFIXDEM_PERMUTE_EVERY
+#define MFIXDEM_PERMUTE_EVERY 16
+#endif
+constexpr int kPermuteEvery = MFIXDEM_PERMUTE_EVERY;
+
 // DESGRID_PIC(.TRUE.) (desgrid_mod.f:877): DES cells, search-grid sort,
 // permutation of every per-particle array into the new order.
 int32_t do_bin(mfixdem_handle* h) {
   const int n = h->n, S = h->stride;
   int init[kFlagCount] = {INT_MAX, 0, 0, 0, 0, 0, 0};
   // only the rebuild flags are reset here
+  const bool thread_rows = h->S.p.des_neighbor_search != MFIXDEM_NSEARCH_NSQUARE &&
+                           (h->row_kernel == MFIXDEM_ROWS_THREAD || h->P.nbc);
+  const bool permute = h->nsys > 1 || !thread_rows || !h->order_set ||
+                       h->bins_since_permute + 1 >= kPermuteEvery;
   HIPOK(hipMemcpyAsync(h->flags + kFlagRogueGid, init, 2 * sizeof(int), hipMemcpyHostToDevice,
                        h->stream));
   HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, init + kFlagOwnOverflow, sizeof(int),
@@ -464,10 +485,20 @@ int32_t do_bin(mfixdem_handle* h) {
   p.wpft = h->wpft[t]; p.wpft_o = h->wpft[1 - t];
   p.wfid = h->wfid[t]; p.wfid_o = h->wfid[1 - t];
   p.ab2 = h->P.ab2;
-  hipLaunchKernelGGL(k_permute, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, p);
-  HIPOK(hipGetLastError());
-  h->cur = 1 - c;
-  h->tw = 1 - t;
+  if (permute) {
+    hipLaunchKernelGGL(k_permute, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, p);
+    HIPOK(hipGetLastError());
+    h->cur = 1 - c;
+    h->tw = 1 - t;
+    h->bin_idx = nullptr;
+    h->bin_perm = h->vals[1];
+    h->bins_since_permute = 0;
+    h->order_set = true;
+  } else {
+    h->bin_idx = h->vals[1];
+    h->bin_perm = nullptr;
+    h->bins_since_permute++;
+  }
   if (batch) {
     CHECKRC(fill_i32(h->stream, h->fstart, h->n_fine, 0));
     CHECKRC(fill_i32(h->stream, h->fend, h->n_fine, 0));
@@ -530,6 +561,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   // stage (3 doubles per slot), which no call uses during a rebuild
   b.xf = reinterpret_cast<float4*>(h->stage);
   b.state = h->state[t]; b.fstart = h->fstart; b.fend = h->fend;
+  b.idx = h->bin_idx;
   b.own_cnt = h->own_cnt; b.flags = h->flags; b.in_cnt = h->in_cnt;
   int fl[kFlagCount];
   // (the wavefront row kernels take normal particles only: boundaries use
@@ -560,8 +592,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
         CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
         CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
       }
-      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b.pos,
-                         b.radius, b.state, h->xr_pack, reinterpret_cast<float4*>(h->stage));
+      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b.idx,
+                         b.pos, b.radius, b.state, h->xr_pack,
+                         reinterpret_cast<float4*>(h->stage));
       const dim3 rg(nblk((int64_t)n * kRowLanes));  // kRowLanes threads per particle
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);
@@ -610,7 +643,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   }
   if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
   CarryPtrs cp{};
-  cp.perm = h->vals[1];
+  cp.perm = h->bin_perm;
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
   cp.pft_old = h->pft[h->pcur];
@@ -1400,6 +1433,9 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   h->inited = false;
   h->first_pass = true;
   h->binned = false;
+  h->order_set = false;  // the first binning puts the new particles in order
+  h->bin_idx = h->bin_perm = nullptr;
+  h->bins_since_permute = 0;
   return MFIXDEM_OK;
 }
 

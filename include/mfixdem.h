@@ -261,13 +261,20 @@ typedef struct mfixdem_dem_bc {
    stream that stands in for RANDOM_NUMBER (inlet layout permutation, phase
    and position in the window).  Each step then runs MASS_INFLOW_DEM and
    MASS_OUTFLOW_DEM (des_time_march.f:404-405): particles ENTERING (2) and
-   EXITING (3) move without forces or walls (cfnewvalues.f:700-720), new
-   particles take the lowest NONEXISTENT slots (mass_inflow_dem.f:206-222),
-   a deleted particle forces the neighbour search.  Euler integration,
-   single-system handles.  Steps with boundaries wait for the host once. */
+   EXITING (3) push on the normal particles they touch but move at their own
+   velocity, meeting no walls (cfnewvalues.f:700-720; an exiting particle at
+   an OUTFLOW outlet takes the step's acceleration along the normal,
+   UPDATE_EXIT_DES_VEL), new particles take the lowest NONEXISTENT slots
+   (mass_inflow_dem.f:206-222), a deleted particle forces the neighbour
+   search.  Euler integration, uncoupled, grid search, single-system
+   handles (else MFIXDEM_ERR_UNSUPPORTED); the whole step (mfixdem_step,
+   mfixdem_run) runs them, and mfixdem_cfnewvalues refuses such a handle.
+   Steps with boundaries wait for the host once.  No counterpart in the
+   reference for the seed. */
 int32_t mfixdem_set_dem_bcs(mfixdem_handle* h, int32_t n, const mfixdem_dem_bc* bcs,
                             uint64_t seed);
-/* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal). */
+/* Download in global-id order.  state: PARTICLE_STATE (0 nonexistent, 1 normal,
+   2 entering, 3 exiting). */
 int32_t mfixdem_get_particles(mfixdem_handle* h, double* pos, double* vel,
                               double* omega, int8_t* state, int64_t ld);
 /* Frozen gas field: staggered U_G/V_G/W_G and P_FORCE(3,IJK) (= -grad p,

@@ -25,6 +25,11 @@ module mfixdem_c
   integer(c_int32_t), parameter :: MFIXDEM_ROWS_WAVE192 = 2, MFIXDEM_ROWS_WAVE256 = 3
   integer(c_int32_t), parameter :: MFIXDEM_NSEARCH_NSQUARE = 1, MFIXDEM_NSEARCH_GRID = 4
   integer(c_int32_t), parameter :: MFIXDEM_DEBUG_STORE_FORCES = 1
+  integer(c_int32_t), parameter :: MFIXDEM_BC_MASS_INFLOW = 1, MFIXDEM_BC_MASS_OUTFLOW = 2
+  integer(c_int32_t), parameter :: MFIXDEM_BC_OUTFLOW = 3
+  integer(c_int32_t), parameter :: MFIXDEM_PLANE_W = 0, MFIXDEM_PLANE_E = 1
+  integer(c_int32_t), parameter :: MFIXDEM_PLANE_S = 2, MFIXDEM_PLANE_N = 3
+  integer(c_int32_t), parameter :: MFIXDEM_PLANE_B = 4, MFIXDEM_PLANE_T = 5
 
   type, bind(C) :: mfixdem_params
      integer(c_int32_t) :: abi_version, coll_model, integrator, precision
@@ -47,6 +52,16 @@ module mfixdem_c
      integer(c_int32_t) :: des_neighbor_search, des_explicitly_coupled
      type(c_ptr) :: dx, dy, dz      ! c_loc(DX(IMIN1)) ... or c_null_ptr (uniform)
   end type mfixdem_params
+
+  ! one DEM boundary of mfix.dat: BC_TYPE, BC_PLANE, BC_X_w ... BC_Z_t,
+  ! BC_U_s/V_s/W_s, BC_EP_s, BC_MASSFLOW_s, BC_MI_START_TIME/END_TIME
+  type, bind(C) :: mfixdem_dem_bc
+     integer(c_int32_t) :: type, plane
+     real(c_double) :: x_w, x_e, y_s, y_n, z_b, z_t
+     real(c_double) :: u_s(MFIXDEM_MAX_PHASES), v_s(MFIXDEM_MAX_PHASES), w_s(MFIXDEM_MAX_PHASES)
+     real(c_double) :: ep_s(MFIXDEM_MAX_PHASES), massflow_s(MFIXDEM_MAX_PHASES)
+     real(c_double) :: mi_start_time, mi_end_time
+  end type mfixdem_dem_bc
 
   type, bind(C) :: mfixdem_status
      integer(c_int64_t) :: steps, rebuilds, n_particles, n_pairs, n_contacts
@@ -186,6 +201,15 @@ module mfixdem_c
        type(c_ptr), value :: h
        integer(c_int32_t), value :: first, count
        real(c_double), intent(in) :: vel(3), omega(3), rot_center(3)
+     end function
+     ! DEM mass inlets and outlets, before mfixdem_init
+     integer(c_int32_t) function mfixdem_set_dem_bcs(h, n, bcs, seed) &
+          bind(C, name="mfixdem_set_dem_bcs")
+       import :: c_int32_t, c_int64_t, c_ptr, mfixdem_dem_bc
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: n
+       type(mfixdem_dem_bc), intent(in) :: bcs(*)
+       integer(c_int64_t), value :: seed
      end function
      ! DES_TIME_INIT's coupled part with DES_EXPLICITLY_COUPLED (DRAG_FC)
      integer(c_int32_t) function mfixdem_des_time_init(h) bind(C, name="mfixdem_des_time_init")

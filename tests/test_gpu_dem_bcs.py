@@ -161,3 +161,29 @@ def test_refused_combinations():
     with pytest.raises(abi.MfixDemError):
         g.set_dem_bcs([inlet])
     g.close()
+
+
+def test_hopper_inlet_funnel_outlet_bit_exact():
+    """A hopper: an inlet in the top face feeds spheres that fall into an
+    STL funnel (eight triangles inside the box walls), pile on its faces,
+    drain through its throat and leave through a MASS_OUTFLOW outlet in
+    the bottom face; only normal spheres meet the walls."""
+    box = (0.01, 0.035, 0.01)
+    p = params(box, (2, 7, 2), nsearch=20, grav=(0.0, -9.81, 0.0), max_pip=1200)
+    v, n, _ = workloads.funnel_facets(box, 0.004, 0.012, 0.0015)
+    inlet = abi.dem_bc(abi.BC_MASS_INFLOW, abi.PLANE_S, (0.002, box[1], 0.002),
+                       (0.008, box[1], 0.008), vel=[(0.0, -0.3, 0.0)], ep_s=[0.2])
+    outlet = abi.dem_bc(abi.BC_MASS_OUTFLOW, abi.PLANE_N, (0.0, 0.0, 0.0), (box[0], 0.0, box[2]),
+                        vel=[(0.0, -0.5, 0.0)])
+    empty = workloads.make_particles(np.zeros((3, 0)), 0.5 * D, RHO)
+    g, o = DemSystem(p), OracleSystem(p)
+    for s in (g, o):
+        s.set_particles(empty)
+        s.set_facets(v, n, True)
+        s.set_dem_bcs([inlet, outlet], seed=5)
+        s.init()
+    run_compare(g, o, 12000, 1500)
+    assert o.status().rebuilds > 12000 // p.neighbor_search_n + 20   # deletions at the outlet
+    assert_same_pairs(g, o)
+    g.close()
+    o.close()

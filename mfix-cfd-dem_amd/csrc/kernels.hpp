@@ -1581,6 +1581,32 @@ __global__ __launch_bounds__(kBS) void k_bin(int n, int S, GridDev G0, const dou
   keys[i] = sys_key | m;
 }
 
+// The single-precision copy is relative to G.pf_c; a particle beyond
+// pf_xmax on some axis gets radius -1 there (every test with it is then
+// made in double precision), a nonexistent one a position no cut-off
+// reaches.
+constexpr double kFarAway = 1.0e300;
+__device__ __forceinline__ void pack_candidate(const GridDev& G, const double p[3], double r,
+                                               bool exists, double* xr, float4* xf, int k) {
+  double2* o = reinterpret_cast<double2*>(xr) + 2 * k;
+  if (!exists) {
+    o[0] = make_double2(kFarAway, kFarAway);
+    o[1] = make_double2(kFarAway, r);
+    xf[k] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+    return;
+  }
+  o[0] = make_double2(p[0], p[1]);
+  o[1] = make_double2(p[2], r);
+  float f[3];
+  bool in = true;
+  for (int d = 0; d < 3; d++) {
+    const double q = p[d] - G.pf_c[d];
+    in = in && fabs(q) <= (double)G.pf_xmax;
+    f[d] = (float)q;
+  }
+  xf[k] = make_float4(f[0], f[1], f[2], in ? (float)r : -1.0f);
+}
+
 // DESGRID_PIC locate as k_bin, for one system, with a counting sort in
 // place of the key sort: each particle counts itself into its search cell
 // (cells numbered in Morton order by cell_rank; nonexistent particles into
@@ -1637,15 +1663,28 @@ __global__ __launch_bounds__(kBS) void k_bin_count(int n, int S, GridDev G, cons
 // cells' first slots (cell_off: exclusive scan of the counts, in rank
 // order); and each search cell's slot range, indexed by its grid position.
 // The counts go back to zero for the next binning.
+struct BinPack {               // k_bin_place's packing of the row walk's candidates
+  GridDev G;
+  int S;
+  const double *pos, *radius;
+  const uint8_t* state;
+  double* xr;
+  float4* xf;
+};
 __global__ __launch_bounds__(kBS) void k_bin_place(int n, int nfine, const uint64_t* place,
                                                    const int* cell_rank, const int* cell_off,
-                                                   int* cell_cnt, int* perm, int* cstart,
-                                                   int* cend, int* zero0, int* zero1,
-                                                   int* zero2) {
+                                                   int* cell_cnt, int* perm, int* inv,
+                                                   int* cstart, int* cend, int* zero0,
+                                                   int* zero1, int* zero2, BinPack bp) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i < n) {
     const uint64_t p = place[i];
-    perm[cell_off[(int)(p >> 32)] + (int)(uint32_t)p] = i;
+    const int np = cell_off[(int)(p >> 32)] + (int)(uint32_t)p;
+    perm[np] = i;
+    inv[i] = np;
+    // the candidate data of the row walk, at the particle's new place
+    const double x[3] = {bp.pos[i], bp.pos[bp.S + i], bp.pos[2 * bp.S + i]};
+    pack_candidate(bp.G, x, bp.radius[i], bp.state[i] != 0, bp.xr, bp.xf, np);
     zero0[i] = 0;  // the row counters of the neighbour build that follows
     zero1[i] = 0;
     zero2[i] = 0;
@@ -1986,39 +2025,19 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   }
 }
 
-// Position and radius of particles [0, n) packed per particle for the
-// candidate walk of k_own_rows (one cache line per candidate instead of five).
-// A nonexistent particle gets a position no cut-off reaches, so the walk
+// Position and radius of the particles packed in the search-cell order for
+// the candidate walk of k_own_rows (one cache line per candidate instead of
+// five), after a binning that did not pack them (k_bin_place does).  A
+// nonexistent particle gets a position no cut-off reaches, so the walk
 // needs no state load; the hit path checks the state itself.
-// The single-precision copy is relative to G.pf_c; a particle beyond
-// pf_xmax on some axis gets radius -1 there (every test with it is then
-// made in double precision), a nonexistent one a position no cut-off
-// reaches.
-constexpr double kFarAway = 1.0e300;
 __global__ __launch_bounds__(kBS) void k_pack_xr(int n, int S, GridDev G, const int* idx,
                                                  const double* pos, const double* radius,
                                                  const uint8_t* state, double* xr, float4* xf) {
   const int k = blockIdx.x * kBS + threadIdx.x;  // position in the search-cell order
   if (k >= n) return;
   const int i = idx ? idx[k] : k;
-  double2* o = reinterpret_cast<double2*>(xr) + 2 * k;
-  if (state[i] == 0) {
-    o[0] = make_double2(kFarAway, kFarAway);
-    o[1] = make_double2(kFarAway, radius[i]);
-    xf[k] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-  } else {
-    const double p[3] = {pos[i], pos[S + i], pos[2 * S + i]};
-    o[0] = make_double2(p[0], p[1]);
-    o[1] = make_double2(p[2], radius[i]);
-    float f[3];
-    bool in = true;
-    for (int d = 0; d < 3; d++) {
-      const double q = p[d] - G.pf_c[d];
-      in = in && fabs(q) <= (double)G.pf_xmax;
-      f[d] = (float)q;
-    }
-    xf[k] = make_float4(f[0], f[1], f[2], in ? (float)radius[i] : -1.0f);
-  }
+  const double p[3] = {pos[i], pos[S + i], pos[2 * S + i]};
+  pack_candidate(G, p, radius[i], state[i] != 0, xr, xf, k);
 }
 
 // A row of k_own_rows put in the reference's order (keys are distinct: the
@@ -2333,7 +2352,6 @@ __global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const i
 
 // history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
 struct CarryPtrs {
-  const int* perm;              // new slot -> old slot (null: the slots stayed)
   const int *old_start, *old_split, *old_nbr, *old_pair_base;
   const double* pft_old;
   double* pft_new;
@@ -2341,19 +2359,16 @@ struct CarryPtrs {
   uint8_t* pfl_new;
 };
 
-// Places the own entries of each row and scatters the transposed
-// (incoming) entries; carries each own pair's history from the particle's
-// old row (the same owner and partner ids), or starts it at zero.  Ids move
-// with the particles (gid_new[j] = gid_old[perm[j]]), so "old partner id ==
-// new partner id" is tested as "old partner slot == perm[j]", one gather
-// fewer per comparison.
-// Rows appended through atomic slots (own_key given) are first put in the
-// reference's order.
+// Places the own entries of each row (put in the reference's order first
+// when they were appended through atomic slots: own_key given) and
+// scatters the transposed (incoming) entries through atomic slots.  Every
+// new pair starts with no history; k_carry then brings over the histories
+// there are.
 __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* own_tmp,
                                                uint64_t* own_key, const int* pair_base,
                                                const int* nbr_start, const int* in_cnt,
                                                int* nbr_split, int* in_cur, int* nbr,
-                                               int* nbr_pid, CarryPtrs c) {
+                                               int* nbr_pid, uint8_t* pfl_new) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i >= n) return;
   const int split = nbr_start[i] + in_cnt[i];
@@ -2362,23 +2377,17 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
   if (cnt == 0) return;
   if (own_key && cnt > 1) sort_own_row(own_tmp + i, own_key + i, (size_t)n, cnt);
   const int pb = pair_base[i];
-  const int o = c.perm ? c.perm[i] : i;
-  const int os = c.old_split[o], oe = c.old_start[o + 1], opb = c.old_pair_base[o];
-  if (cnt <= kRegRow && oe - os <= kRegRow) {
-    // a short row with a short old row: every load, atomic and store of
-    // the row issued side by side rather than one entry after another
-    int jv[kRegRow], nj[kRegRow], pj[kRegRow], sl[kRegRow], old[kRegRow];
+  if (cnt <= kRegRow) {
+    // a short row: every load, atomic and store of the row issued side by
+    // side rather than one entry after another
+    int jv[kRegRow], nj[kRegRow], sl[kRegRow];
 #pragma unroll
     for (int k = 0; k < kRegRow; k++)
       if (k < cnt) jv[k] = own_tmp[(size_t)k * n + i];
 #pragma unroll
-    for (int r = 0; r < kRegRow; r++)
-      if (r < oe - os) old[r] = c.old_nbr[os + r];
-#pragma unroll
     for (int k = 0; k < kRegRow; k++)
       if (k < cnt) {
         nj[k] = nbr_start[jv[k]];
-        pj[k] = c.perm ? c.perm[jv[k]] : jv[k];
         sl[k] = atomicAdd(&in_cur[jv[k]], 1);
       }
 #pragma unroll
@@ -2388,15 +2397,7 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
         nbr_pid[split + k] = pb + k;
         nbr[nj[k] + sl[k]] = i;
         nbr_pid[nj[k] + sl[k]] = pb + k;
-        int op = -1;
-#pragma unroll
-        for (int r = kRegRow - 1; r >= 0; r--)  // the first match wins
-          if (r < oe - os && old[r] == pj[k]) op = opb + r;
-        const int p = pb + k;
-        const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
-        c.pfl_new[p] = f;
-        if (f)
-          for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
+        pfl_new[pb + k] = 0;
       }
     return;
   }
@@ -2407,18 +2408,46 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
     const int slot = atomicAdd(&in_cur[j], 1);
     nbr[nbr_start[j] + slot] = i;
     nbr_pid[nbr_start[j] + slot] = pb + k;
-    const int oj = c.perm ? c.perm[j] : j;
-    int op = -1;
-    for (int r = os; r < oe; r++)
-      if (c.old_nbr[r] == oj) {
-        op = opb + (r - os);
+    pfl_new[pb + k] = 0;
+  }
+}
+
+// NEIGHBOUR's history carry-over (neighbour.f:67-126): a new pair takes the
+// history of the same pair (same owner, same partner) in the owner's old
+// row, or starts at zero.  Only old pairs that hold a history (flag set:
+// the pairs in contact, a small share of the list) have anything to give,
+// so the carry starts from them: the thread of each old row finds each of
+// its flagged pairs in the owner's new row and copies the history there.
+// Slots map through inv (old slot -> new slot; null: the slots stayed).
+// A pair whose owner changed is not in the owner's new row and, as in the
+// reference, starts at zero.
+__global__ __launch_bounds__(kBS) void k_carry(int n, const int* inv, CarryPtrs c,
+                                               const int* pair_base, const int* nbr_split,
+                                               const int* nbr_start, const int* nbr) {
+  const int o = blockIdx.x * kBS + threadIdx.x;  // the owner's old slot
+  if (o >= n) return;
+  const int os = c.old_split[o], oe = c.old_start[o + 1];
+  if (os == oe) return;
+  const int opb = c.old_pair_base[o];
+  int i = -1, s0 = 0, s1 = 0, pb = 0;
+  for (int r = os; r < oe; r++) {
+    const int op = opb + (r - os);
+    if (!c.pfl_old[op]) continue;
+    if (i < 0) {
+      i = inv ? inv[o] : o;
+      s0 = nbr_split[i];
+      s1 = nbr_start[i + 1];
+      pb = pair_base[i];
+    }
+    const int jo = c.old_nbr[r];
+    const int j = inv ? inv[jo] : jo;
+    for (int q = s0; q < s1; q++)
+      if (nbr[q] == j) {
+        const int p = pb + (q - s0);
+        c.pfl_new[p] = 1;
+        for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
         break;
       }
-    const int p = pb + k;
-    const uint8_t f = op >= 0 ? c.pfl_old[op] : 0;  // no old pair: history zero
-    c.pfl_new[p] = f;
-    if (f)
-      for (int d = 0; d < 3; d++) c.pft_new[3 * p + d] = c.pft_old[3 * op + d];
   }
 }
 
@@ -2602,6 +2631,13 @@ __global__ __launch_bounds__(kBS) void k_seed(int ns, const SeedDev* sd, SeedPtr
   p.phase[i] = (uint8_t)d.phase;
   p.wmask[i] = 0;
   p.state[i] = kSeeded;
+}
+
+// the inverse of a binning's order (ensembles, whose key sort gives the
+// order only)
+__global__ __launch_bounds__(kBS) void k_invert(int n, const int* perm, int* inv) {
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  if (k < n) inv[perm[k]] = k;
 }
 
 // slot of each global id, after a binning has reordered the slots

@@ -108,12 +108,14 @@ struct mfixdem_handle {
   int* scan_bsum = nullptr;
   long long* scan_bsum64 = nullptr;
   bool counts_zeroed = false;  // own_cnt, in_cnt, in_cur cleared by the last binning
+  bool xr_packed = false;      // the row walk's candidate data written by the last binning
   // A binning of one system with the thread-row search moves the particle
   // data into the search-cell order only every kPermuteEvery-th time (and
   // the first time); otherwise the slots stay and bin_idx lists them in that
-  // order.  bin_perm: new slot -> old slot of the last binning (null: none).
+  // order.  bin_inv: old slot -> new slot of the last binning (null: the
+  // slots stayed), for the history carry.
   const int* bin_idx = nullptr;
-  const int* bin_perm = nullptr;
+  const int* bin_inv = nullptr;
   int bins_since_permute = 0;
   bool order_set = false;
   int* flags = nullptr;
@@ -457,6 +459,9 @@ int32_t do_bin(mfixdem_handle* h) {
     size_t tb = h->sort_tmp_bytes;
     HIPOK(sort_pairs(h->sort_tmp, tb, h->keys[0], h->keys[1], h->vals[0], h->vals[1], n,
                      h->key_bits, h->stream));
+    hipLaunchKernelGGL(k_invert, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->vals[1],
+                       h->vals[0]);
+    HIPOK(hipGetLastError());
   } else {  // one system: counting sort over the search cells
     const int nf = h->n_fine;
     hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
@@ -464,11 +469,20 @@ int32_t do_bin(mfixdem_handle* h) {
                        nf, h->keys[0], h->flags);
     HIPOK(hipGetLastError());
     CHECKRC(scan_i32(h, h->fine_cnt, h->fine_off, (int64_t)nf + 1));
+    BinPack bp{};
+    bp.G = h->G;
+    bp.S = S;
+    bp.pos = h->pos[c];
+    bp.radius = h->radius[t];
+    bp.state = h->state[t];
+    bp.xr = h->xr_pack;
+    bp.xf = reinterpret_cast<float4*>(h->stage);
     hipLaunchKernelGGL(k_bin_place, dim3(nblk(std::max(n, nf))), dim3(kBS), 0, h->stream, n, nf,
                        h->keys[0], h->fine_rank, h->fine_off, h->fine_cnt, h->vals[1],
-                       h->fstart, h->fend, h->own_cnt, h->in_cnt, h->in_cur);
+                       h->vals[0], h->fstart, h->fend, h->own_cnt, h->in_cnt, h->in_cur, bp);
     HIPOK(hipGetLastError());
     h->counts_zeroed = true;
+    h->xr_packed = true;
   }
   PermPtrs p{};
   p.perm = h->vals[1];
@@ -491,12 +505,12 @@ int32_t do_bin(mfixdem_handle* h) {
     h->cur = 1 - c;
     h->tw = 1 - t;
     h->bin_idx = nullptr;
-    h->bin_perm = h->vals[1];
+    h->bin_inv = h->vals[0];
     h->bins_since_permute = 0;
     h->order_set = true;
   } else {
     h->bin_idx = h->vals[1];
-    h->bin_perm = nullptr;
+    h->bin_inv = nullptr;
     h->bins_since_permute++;
   }
   if (batch) {
@@ -574,7 +588,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   int npairs = 0;
   bool zeroed = h->counts_zeroed;       // row counters cleared by the binning
   const bool cur_zeroed = zeroed;
+  const bool packed = h->xr_packed;     // and the candidates packed by it
   h->counts_zeroed = false;
+  h->xr_packed = false;
   for (;;) {
     b.own_tmp = h->own_tmp; b.own_key = h->own_key; b.maxrow = h->maxrow; b.ld = n;
     if (nsq) {
@@ -592,9 +608,10 @@ int32_t do_neighbour(mfixdem_handle* h) {
         CHECKRC(fill_i32(h->stream, h->own_cnt, n, 0));
         CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
       }
-      hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G, b.idx,
-                         b.pos, b.radius, b.state, h->xr_pack,
-                         reinterpret_cast<float4*>(h->stage));
+      if (!packed)
+        hipLaunchKernelGGL(k_pack_xr, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, S, h->G,
+                           b.idx, b.pos, b.radius, b.state, h->xr_pack,
+                           reinterpret_cast<float4*>(h->stage));
       const dim3 rg(nblk((int64_t)n * kRowLanes));  // kRowLanes threads per particle
       if (h->nsys > 1)
         hipLaunchKernelGGL((k_own_rows<true>), rg, dim3(kBS), 0, h->stream, n, S, h->G, b, h->B);
@@ -643,7 +660,6 @@ int32_t do_neighbour(mfixdem_handle* h) {
   }
   if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
   CarryPtrs cp{};
-  cp.perm = h->bin_perm;
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
   cp.pft_old = h->pft[h->pcur];
@@ -653,7 +669,9 @@ int32_t do_neighbour(mfixdem_handle* h) {
   hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
                      nsq || wave_rows ? nullptr : h->own_key, h->pair_base[nl],
                      h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur, h->nbr[nl],
-                     h->nbr_pid[nl], cp);
+                     h->nbr_pid[nl], cp.pfl_new);
+  hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->bin_inv, cp,
+                     h->pair_base[nl], h->nbr_split[nl], h->nbr_start[nl], h->nbr[nl]);
   hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
                      h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
   HIPOK(hipGetLastError());
@@ -1434,7 +1452,8 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   h->first_pass = true;
   h->binned = false;
   h->order_set = false;  // the first binning puts the new particles in order
-  h->bin_idx = h->bin_perm = nullptr;
+  h->xr_packed = h->counts_zeroed = false;
+  h->bin_idx = h->bin_inv = nullptr;
   h->bins_since_permute = 0;
   return MFIXDEM_OK;
 }
@@ -1442,6 +1461,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
 int32_t mfixdem_set_particle_state(mfixdem_handle* h, const int8_t* state, int64_t n) {
   if (h && h->nsys > 1) return MFIXDEM_ERR_ARG;  // single-system call
   if (!h || !state || n < 0 || n > h->n || h->inited) return MFIXDEM_ERR_ARG;
+  h->xr_packed = false;
   for (int64_t q = 0; q < n; q++)
     if (state[q] != 0 && state[q] != 1) return MFIXDEM_ERR_UNSUPPORTED;
   if (n == 0) return MFIXDEM_OK;
@@ -1699,6 +1719,7 @@ int32_t mfixdem_neighbour(mfixdem_handle* h) {
 }
 int32_t mfixdem_particles_in_cell(mfixdem_handle* h) {
   if (!h || !h->inited) return h ? MFIXDEM_ERR_NOT_READY : MFIXDEM_ERR_ARG;
+  h->xr_packed = false;  // particles may be removed: the walk's candidates are packed again
   const int n = h->n;
   hipLaunchKernelGGL(k_locate, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->stride, h->S.ni,
                      h->S.nj, h->S.nk, h->xe, h->yn, h->zt, h->pos[h->cur], h->state[h->tw],

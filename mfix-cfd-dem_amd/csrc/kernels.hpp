@@ -110,7 +110,7 @@ struct StepPtrs {
 };
 
 enum { kFlagRogueGid = 0, kFlagRogueCount, kFlagZeroDist, kFlagOwnOverflow, kFlagEpg,
-       kFlagOutOfDomain, kFlagContacts, kFlagWallSlots, kFlagCount };
+       kFlagOutOfDomain, kFlagContacts, kFlagWallSlots, kFlagPairs, kFlagCount };
 
 enum { kModeFused = 0, kModeForce = 1, kModeDrag = 2, kModeIntegrate = 3 };
 
@@ -1452,7 +1452,8 @@ __global__ __launch_bounds__(kBS) void k_scan_sums64(long long* bsum, int nb) {
 }
 
 __global__ __launch_bounds__(kBS) void k_scan_rows_add(int* pair_base, int* row_start, int n,
-                                                       const long long* bsum, int nb) {
+                                                       const long long* bsum, int nb,
+                                                       int* flags) {
   const int i = blockIdx.x * kBS + threadIdx.x;
   if (i < n) {
     const long long o = bsum[i / kScanTile];
@@ -1463,6 +1464,7 @@ __global__ __launch_bounds__(kBS) void k_scan_rows_add(int* pair_base, int* row_
     const long long t = bsum[nb];
     pair_base[n] = (int)(t >> 32);
     row_start[n] = (int)(t >> 32) + (int)(t & 0xffffffffll);
+    flags[kFlagPairs] = (int)(t >> 32);  // read back with the other flags
   }
 }
 
@@ -2352,6 +2354,7 @@ __global__ __launch_bounds__(kBS) void k_add_counts(int n, const int* a, const i
 
 // history carry-over (neighbour.f:77-117), matched by (owner id, neighbour id)
 struct CarryPtrs {
+  int* flags;
   const int *old_start, *old_split, *old_nbr, *old_pair_base;
   const double* pft_old;
   double* pft_new;
@@ -2425,6 +2428,11 @@ __global__ __launch_bounds__(kBS) void k_carry(int n, const int* inv, CarryPtrs 
                                                const int* pair_base, const int* nbr_split,
                                                const int* nbr_start, const int* nbr) {
   const int o = blockIdx.x * kBS + threadIdx.x;  // the owner's old slot
+  if (o == 0) {  // the host has read this build's flags: the next build's start values
+    c.flags[kFlagRogueGid] = INT_MAX;
+    c.flags[kFlagRogueCount] = 0;
+    c.flags[kFlagOwnOverflow] = 0;
+  }
   if (o >= n) return;
   const int os = c.old_split[o], oe = c.old_start[o + 1];
   if (os == oe) return;

@@ -109,6 +109,7 @@ struct mfixdem_handle {
   long long* scan_bsum64 = nullptr;
   bool counts_zeroed = false;  // own_cnt, in_cnt, in_cur cleared by the last binning
   bool xr_packed = false;      // the row walk's candidate data written by the last binning
+  bool flags_set = false;      // the rebuild flags hold their start values (k_carry sets them)
   // A binning of one system with the thread-row search moves the particle
   // data into the search-cell order only every kPermuteEvery-th time (and
   // the first time); otherwise the slots stay and bin_idx lists them in that
@@ -242,7 +243,7 @@ int32_t scan_rows(mfixdem_handle* h, const int* own, const int* in, int* pair_ba
   hipLaunchKernelGGL(k_scan_sums64, dim3(1), dim3(kBS), 0, h->stream, h->scan_bsum64,
                      std::max(nb, 1));
   hipLaunchKernelGGL(k_scan_rows_add, dim3(nblk(n + 1)), dim3(kBS), 0, h->stream, pair_base,
-                     row_start, (int)n, h->scan_bsum64, std::max(nb, 1));
+                     row_start, (int)n, h->scan_bsum64, std::max(nb, 1), h->flags);
   HIPOK(hipGetLastError());
   return MFIXDEM_OK;
 }
@@ -442,10 +443,13 @@ int32_t do_bin(mfixdem_handle* h) {
                            (h->row_kernel == MFIXDEM_ROWS_THREAD || h->P.nbc);
   const bool permute = h->nsys > 1 || !thread_rows || !h->order_set ||
                        h->bins_since_permute + 1 >= kPermuteEvery;
-  HIPOK(hipMemcpyAsync(h->flags + kFlagRogueGid, init, 2 * sizeof(int), hipMemcpyHostToDevice,
-                       h->stream));
-  HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, init + kFlagOwnOverflow, sizeof(int),
-                       hipMemcpyHostToDevice, h->stream));
+  if (!h->flags_set) {
+    HIPOK(hipMemcpyAsync(h->flags + kFlagRogueGid, init, 2 * sizeof(int),
+                         hipMemcpyHostToDevice, h->stream));
+    HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, init + kFlagOwnOverflow, sizeof(int),
+                         hipMemcpyHostToDevice, h->stream));
+  }
+  h->flags_set = false;
   const bool batch = h->nsys > 1;
   if (batch)
     HIPOK(hipMemcpyAsync(h->d_rogue, h->rogue_init.data(), sizeof(int) * 2 * h->nsys,
@@ -623,12 +627,16 @@ int32_t do_neighbour(mfixdem_handle* h) {
     // own rows -> pair ids (and, with the thread rows, whose incoming counts
     // are known already, the row starts), read back with the flags in the
     // same wait (an overflowing pass is built again and its pair ids with it)
-    if (nsq || wave_rows) CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
-    else CHECKRC(scan_rows(h, h->own_cnt, h->in_cnt, h->pair_base[nl], h->nbr_start[nl], n));
-    HIPOK(hipMemcpyAsync(&npairs, h->pair_base[nl] + n, sizeof(int), hipMemcpyDeviceToHost,
-                         h->stream));
+    if (nsq || wave_rows) {
+      CHECKRC(scan_i32(h, h->own_cnt, h->pair_base[nl], n));
+      HIPOK(hipMemcpyAsync(h->flags + kFlagPairs, h->pair_base[nl] + n, sizeof(int),
+                           hipMemcpyDeviceToDevice, h->stream));
+    } else {
+      CHECKRC(scan_rows(h, h->own_cnt, h->in_cnt, h->pair_base[nl], h->nbr_start[nl], n));
+    }
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
     HIPOK(sync_stream(h));
+    npairs = fl[kFlagPairs];
     if (fl[kFlagOwnOverflow] <= h->maxrow) break;
     int z = 0;
     HIPOK(hipMemcpyAsync(h->flags + kFlagOwnOverflow, &z, sizeof(int), hipMemcpyHostToDevice,
@@ -660,6 +668,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
   }
   if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
   CarryPtrs cp{};
+  cp.flags = h->flags;
   cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
   cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
   cp.pft_old = h->pft[h->pcur];
@@ -672,6 +681,7 @@ int32_t do_neighbour(mfixdem_handle* h) {
                      h->nbr_pid[nl], cp.pfl_new);
   hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->bin_inv, cp,
                      h->pair_base[nl], h->nbr_split[nl], h->nbr_start[nl], h->nbr[nl]);
+  h->flags_set = true;
   hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
                      h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
   HIPOK(hipGetLastError());
@@ -1102,6 +1112,7 @@ int32_t upload_ensemble(mfixdem_handle* h) {
     HIPOK(hipMemcpy(h->pair_base[b], Z.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   }
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
+  h->flags_set = false;
   h->st = mfixdem_status{};
   h->n_pairs = 0;
   h->first_pass = true;
@@ -1440,6 +1451,7 @@ int32_t mfixdem_set_particles(mfixdem_handle* h, int64_t n, const double* pos, c
   }
   // on the handle's stream, which every later call on the handle waits for
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
+  h->flags_set = false;
   HIPOK(hipMemsetAsync(h->fc_id, 0, sizeof(double) * 3 * S, h->stream));
   HIPOK(hipMemsetAsync(h->tow_id, 0, sizeof(double) * 3 * S, h->stream));
   if (h->f_gp) {  // des_init_arrays.f:143
@@ -1639,6 +1651,7 @@ int32_t mfixdem_init(mfixdem_handle* h) {
   if (h->nsys > 1) CHECKRC(upload_ensemble(h));
   const int n = h->n;
   HIPOK(hipMemsetAsync(h->flags, 0, sizeof(int) * kFlagCount, h->stream));
+  h->flags_set = false;
   if (h->S.p.coupled) {
     hipLaunchKernelGGL(k_locate, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->stride, h->S.ni,
                        h->S.nj, h->S.nk, h->xe, h->yn, h->zt, h->pos[h->cur], h->state[h->tw],

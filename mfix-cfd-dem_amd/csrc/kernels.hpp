@@ -2362,6 +2362,21 @@ struct CarryPtrs {
   uint8_t* pfl_new;
 };
 
+// Placement launched before the host has read the build's flags: it runs
+// only if the rows did not overflow and the lists and histories fit the
+// buffers (otherwise the host sees the same flags and places again).
+struct PlaceGuard {
+  const int* flags;  // null: no guard
+  int maxrow;
+  int64_t cap_entries, cap_pairs;
+};
+__device__ __forceinline__ bool place_ok(const PlaceGuard& g) {
+  if (!g.flags) return true;
+  const int64_t pairs = g.flags[kFlagPairs];
+  return g.flags[kFlagOwnOverflow] <= g.maxrow && 2 * pairs <= g.cap_entries &&
+         pairs <= g.cap_pairs;
+}
+
 // Places the own entries of each row (put in the reference's order first
 // when they were appended through atomic slots: own_key given) and
 // scatters the transposed (incoming) entries through atomic slots.  Every
@@ -2371,9 +2386,10 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
                                                uint64_t* own_key, const int* pair_base,
                                                const int* nbr_start, const int* in_cnt,
                                                int* nbr_split, int* in_cur, int* nbr,
-                                               int* nbr_pid, uint8_t* pfl_new) {
+                                               int* nbr_pid, uint8_t* pfl_new,
+                                               PlaceGuard pg) {
   const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || !place_ok(pg)) return;
   const int split = nbr_start[i] + in_cnt[i];
   nbr_split[i] = split;
   const int cnt = own_cnt[i];
@@ -2426,8 +2442,10 @@ __global__ __launch_bounds__(kBS) void k_place(int n, const int* own_cnt, int* o
 // reference, starts at zero.
 __global__ __launch_bounds__(kBS) void k_carry(int n, const int* inv, CarryPtrs c,
                                                const int* pair_base, const int* nbr_split,
-                                               const int* nbr_start, const int* nbr) {
+                                               const int* nbr_start, const int* nbr,
+                                               PlaceGuard pg) {
   const int o = blockIdx.x * kBS + threadIdx.x;  // the owner's old slot
+  if (!place_ok(pg)) return;
   if (o == 0) {  // the host has read this build's flags: the next build's start values
     c.flags[kFlagRogueGid] = INT_MAX;
     c.flags[kFlagRogueCount] = 0;
@@ -2461,9 +2479,10 @@ __global__ __launch_bounds__(kBS) void k_carry(int n, const int* inv, CarryPtrs 
 
 // incoming entries in ascending owner id (short rows in registers)
 __global__ __launch_bounds__(kBS) void k_sort_in(int n, const int* gid, const int* nbr_start,
-                                                 const int* nbr_split, int* nbr, int* nbr_pid) {
+                                                 const int* nbr_split, int* nbr, int* nbr_pid,
+                                                 PlaceGuard pg) {
   const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || !place_ok(pg)) return;
   const int s = nbr_start[i], e = nbr_split[i], cnt = e - s;
   if (cnt < 2) return;
   if (cnt <= kRegRow) {

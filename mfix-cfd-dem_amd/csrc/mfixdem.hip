@@ -590,6 +590,30 @@ int32_t do_neighbour(mfixdem_handle* h) {
   const bool nsq = h->S.p.des_neighbor_search == MFIXDEM_NSEARCH_NSQUARE;
   const int nl = 1 - h->lt;
   int npairs = 0;
+  // placement of the built rows (k_place, the history carry, the incoming
+  // order); guarded when launched before the host has read the flags
+  auto place = [&](const PlaceGuard& pg) -> int32_t {
+    CarryPtrs cp{};
+    cp.flags = h->flags;
+    cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
+    cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
+    cp.pft_old = h->pft[h->pcur];
+    cp.pft_new = h->pft[1 - h->pcur];
+    cp.pfl_old = h->pfl[h->pcur];
+    cp.pfl_new = h->pfl[1 - h->pcur];
+    hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
+                       h->own_tmp, nsq || wave_rows ? nullptr : h->own_key, h->pair_base[nl],
+                       h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur, h->nbr[nl],
+                       h->nbr_pid[nl], cp.pfl_new, pg);
+    hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->bin_inv, cp,
+                       h->pair_base[nl], h->nbr_split[nl], h->nbr_start[nl], h->nbr[nl], pg);
+    hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
+                       h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl], pg);
+    HIPOK(hipGetLastError());
+    return MFIXDEM_OK;
+  };
+  PlaceGuard spec{};  // the guard of a placement launched ahead of the wait
+  bool placed = false;
   bool zeroed = h->counts_zeroed;       // row counters cleared by the binning
   const bool cur_zeroed = zeroed;
   const bool packed = h->xr_packed;     // and the candidates packed by it
@@ -635,6 +659,14 @@ int32_t do_neighbour(mfixdem_handle* h) {
       CHECKRC(scan_rows(h, h->own_cnt, h->in_cnt, h->pair_base[nl], h->nbr_start[nl], n));
     }
     HIPOK(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
+    // thread rows have their row starts already: place them ahead of the
+    // wait, guarded, while the host waits for the flags
+    placed = false;
+    if (!nsq && !wave_rows && cur_zeroed && h->nbr[nl] && h->pft[1 - h->pcur]) {
+      spec = PlaceGuard{h->flags, h->maxrow, h->nbr_cap[nl], h->pft_cap[1 - h->pcur]};
+      CHECKRC(place(spec));
+      placed = true;
+    }
     HIPOK(sync_stream(h));
     npairs = fl[kFlagPairs];
     if (fl[kFlagOwnOverflow] <= h->maxrow) break;
@@ -655,35 +687,24 @@ int32_t do_neighbour(mfixdem_handle* h) {
   }
   CHECKRC(fatal_flags(fl));
   if (!nsq && fl[kFlagRogueCount] > 0 && !h->S.p.remove_rogue) return MFIXDEM_ERR_ROGUE;
-  CHECKRC(ensure_list_capacity(h, nl, 2 * (int64_t)npairs));
-  CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
-  if (nsq || wave_rows) {  // rows built sorted: count the incoming entries here
-    CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
-    hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
-                       h->own_tmp, h->maxrow, h->in_cnt);
-    hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
-                       h->own_cnt, h->tot_cnt);
-    HIPOK(hipGetLastError());
-    CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
+  const bool fits = placed && 2 * (int64_t)npairs <= spec.cap_entries &&
+                    (int64_t)npairs <= spec.cap_pairs;
+  if (!fits) {
+    CHECKRC(ensure_list_capacity(h, nl, 2 * (int64_t)npairs));
+    CHECKRC(ensure_pft_capacity(h, 1 - h->pcur, npairs));
+    if (nsq || wave_rows) {  // rows built sorted: count the incoming entries here
+      CHECKRC(fill_i32(h->stream, h->in_cnt, n, 0));
+      hipLaunchKernelGGL(k_count_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt,
+                         h->own_tmp, h->maxrow, h->in_cnt);
+      hipLaunchKernelGGL(k_add_counts, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->in_cnt,
+                         h->own_cnt, h->tot_cnt);
+      HIPOK(hipGetLastError());
+      CHECKRC(scan_i32(h, h->tot_cnt, h->nbr_start[nl], n));
+    }
+    if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
+    CHECKRC(place(PlaceGuard{}));
   }
-  if (!cur_zeroed) CHECKRC(fill_i32(h->stream, h->in_cur, n, 0));
-  CarryPtrs cp{};
-  cp.flags = h->flags;
-  cp.old_start = h->nbr_start[h->lt]; cp.old_split = h->nbr_split[h->lt];
-  cp.old_nbr = h->nbr[h->lt]; cp.old_pair_base = h->pair_base[h->lt];
-  cp.pft_old = h->pft[h->pcur];
-  cp.pft_new = h->pft[1 - h->pcur];
-  cp.pfl_old = h->pfl[h->pcur];
-  cp.pfl_new = h->pfl[1 - h->pcur];
-  hipLaunchKernelGGL(k_place, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->own_cnt, h->own_tmp,
-                     nsq || wave_rows ? nullptr : h->own_key, h->pair_base[nl],
-                     h->nbr_start[nl], h->in_cnt, h->nbr_split[nl], h->in_cur, h->nbr[nl],
-                     h->nbr_pid[nl], cp.pfl_new);
-  hipLaunchKernelGGL(k_carry, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->bin_inv, cp,
-                     h->pair_base[nl], h->nbr_split[nl], h->nbr_start[nl], h->nbr[nl]);
   h->flags_set = true;
-  hipLaunchKernelGGL(k_sort_in, dim3(nblk(n)), dim3(kBS), 0, h->stream, n, h->gid[t],
-                     h->nbr_start[nl], h->nbr_split[nl], h->nbr[nl], h->nbr_pid[nl]);
   HIPOK(hipGetLastError());
   h->lt = nl;
   h->pcur = 1 - h->pcur;

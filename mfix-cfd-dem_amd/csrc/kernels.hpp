@@ -3097,7 +3097,9 @@ __global__ __launch_bounds__(kBS) void k_cell_fields(int imax, int jmax, int kma
   constexpr int kRun = kFieldCells + 1, kNodes = 4 * kRun, kPer = (kNodes + kBS - 1) / kBS;
   __shared__ double s_dens[4][kRun];
   const int ncell = ni * nj * nk, plane = nj * ni;
-  const int c0 = blockIdx.x * kFieldCells;
+  // XCD-aware order: the workgroups of one XCD take a contiguous range of
+  // cells, so the node rows a K-neighbour reads again are in that XCD's L2
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * kFieldCells;
   if (FIXED && f.rm_cnt) {
     apply_removals(f.rm_cnt, f.rm_list, f.state, f.radius, blockIdx.x * kBS + threadIdx.x,
                    gridDim.x * kBS);

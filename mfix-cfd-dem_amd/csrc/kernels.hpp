@@ -1893,7 +1893,9 @@ __global__ __launch_bounds__(kBS) void k_own_rows(int n, int S, GridDev G0, Buil
   constexpr int kHits = kRowLanes == 1 ? kHitCap : kHitCap / 2;
   __shared__ int s_lo[kCells][kBS], s_hi[kCells][kBS];
   __shared__ int s_hit[kHits][kBS];
-  const int id = blockIdx.x * kBS + threadIdx.x;
+  // XCD-aware order: the cells a row walks in the next planes of search
+  // cells are read by workgroups of the same XCD, from its L2
+  const int id = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
   const int hp = id / kRowLanes, part = id % kRowLanes;  // position in the search-cell order
   if (hp >= n) return;
   const int h = b.idx ? b.idx[hp] : hp;                   // and slot

@@ -100,6 +100,29 @@ def test_column_inlet_to_outlet_bit_exact(kind):
     o.close()
 
 
+def test_column_hertz_bit_exact():
+    """The column with Hertz contacts (E = 1e7, nu = 0.3, normal and
+    tangential restitution 0.9, walls alike): the contact law, its derived
+    collision time and the wall contact of the particles at the bottom all
+    follow the inlet's seeding bit for bit."""
+    p, box, inlet = column()
+    p.coll_model = abi.HERTZ
+    p.e_young[0], p.v_poisson[0] = 1.0e7, 0.3
+    p.ew_young, p.vw_poisson = 1.0e7, 0.3
+    p.des_et_input[0] = 0.9
+    p.des_et_wall_input[0] = 0.9
+    outlet = abi.dem_bc(abi.BC_MASS_OUTFLOW, abi.PLANE_S, (0.0, box[1], 0.0), box,
+                        vel=[(0.0, 0.8, 0.0)])
+    empty = workloads.make_particles(np.zeros((3, 0)), 0.5 * D, RHO)
+    g, o = start(p, [inlet, outlet], empty)
+    run_compare(g, o, 3000, 500)
+    st = o.get_particles()[3]
+    assert (st == 1).any() and (st == 2).any()
+    assert_same_pairs(g, o)
+    g.close()
+    o.close()
+
+
 def test_side_inlet_two_phases_into_a_bed_bit_exact():
     """Two phases (by mass flow) through an inlet in the x = 0 face into a
     bed of normal particles with holes among them, out through an outflow
